@@ -1,0 +1,144 @@
+/*
+ * sfdx.h -- C ABI of the MI355X-native SFD_EXCMG forward solve (libsfdx.so).
+ *
+ * All arrays are caller-owned host memory; the context owns device memory and
+ * keeps no host pointer across calls.  Return value 0 = success, > 0 = error
+ * (SFDX_ERR_*), < 0 = warning with a valid result (SFDX_WARN_*).  One host
+ * thread per context; calls on one context are not re-entrant.
+ *
+ * Layouts follow the reference exactly (sfd_excmg/src):
+ *   grid widths a(nx), b(ny), c(nz) in metres, nz including the nair air layers
+ *   on top (file_opt.f90:40-52);
+ *   sigma[comp*ncell + ie], comp = xx,yy,zz,xy,xz,yz, ie = (k*ny + j)*nx + i
+ *   (Fortran column-major sigma(ncell,6), file_opt.f90:56-71);
+ *   edge vectors in the reference edge numbering [Ex | Ey | Ez] with i fastest;
+ *   ex[f*2*nl + pol*nl + e] (Fortran ex(2*nl, nf), fdfe_main.f90:1577-1589),
+ *   pol 0 = xy (Ex = 1 at the top), pol 1 = yx.
+ * Complex numbers are interleaved (re, im) doubles, matching Fortran
+ * complex(c_double_complex) and C double _Complex.
+ */
+#ifndef SFDX_H
+#define SFDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFDX_OK 0
+#define SFDX_ERR_ARG 1          /* bad argument / null pointer */
+#define SFDX_ERR_MESH 2         /* dimensions not divisible for the EXCMG levels */
+#define SFDX_ERR_NAN 3          /* NaN/Inf in inputs or iterates */
+#define SFDX_ERR_HIP 4          /* HIP runtime error (message in sfdx_last_error) */
+#define SFDX_ERR_DIRECT 5       /* coarse direct solver unavailable or failed */
+#define SFDX_ERR_BREAKDOWN 6    /* Krylov breakdown */
+#define SFDX_ERR_UNSUPPORTED 7  /* option not supported */
+#define SFDX_WARN_NOCONV (-1)   /* maxit reached; last iterate returned */
+
+#define SFDX_SOLVER_BICGSTAB 1  /* solvers.f90:522-719 */
+#define SFDX_SOLVER_GPBICG 3    /* solvers.f90:137-315 */
+#define SFDX_SOLVER_COCR 5      /* COCR (Sogabe & Zhang 2007) on the symmetric form */
+
+#define SFDX_PREC_FP64 0
+#define SFDX_PREC_MIXED 1       /* complex64 inner solve, FP64 refinement */
+
+#define SFDX_NORM_REFERENCE 0   /* ||b-Ax||/||b||, b incl. Dirichlet rows (solvers.f90:615-625) */
+#define SFDX_NORM_INTERIOR 1    /* interior rows only */
+
+#define SFDX_MAX_LEVELS 8
+
+typedef struct sfdx_ctx sfdx_ctx;
+
+typedef struct {
+    int nx, ny, nz;   /* cells; nz counts the air layers */
+    int nair;         /* air layers on top */
+    const double *a, *b, *c;
+} sfdx_grid;
+
+typedef struct {
+    int level;          /* EXCMG grid levels (>= 3), .fwd header (file_opt.f90:21) */
+    int use_excmg;      /* 1: doublemode_acc, 0: doublemode_nor (global_para.f90:19) */
+    int solver;         /* SFDX_SOLVER_* */
+    double eps;         /* fine-level tolerance */
+    int corr_intv;      /* divergence correction interval, <= 0 disables (global_para.f90:8-9) */
+    double corr_tol;    /* PCG tolerance of the correction (global_para.f90:10) */
+    double ssor_omega;  /* SSOR relaxation (solvers.f90:14) */
+    int precision;      /* SFDX_PREC_* */
+    int norm;           /* SFDX_NORM_* */
+    int maxit_plain;    /* 5000, fdfe_main.f90:993 */
+    int maxit_excmg;    /* 1000, fdfe_main.f90:1414 */
+    double inner_tol;   /* mixed precision: relative tolerance of each inner solve */
+    int max_refine;     /* mixed precision: maximum refinement steps */
+    int verbose;
+} sfdx_opts;
+
+typedef struct {
+    int status;
+    double total_seconds;
+    double fine_seconds;            /* fine-level Krylov time, all frequencies and both polarisations */
+    double fine_bytes;              /* algorithmic HBM bytes of the fine-level kernels */
+    int iters[SFDX_MAX_LEVELS][2];  /* last frequency: Krylov iterations per level and polarisation */
+    double relres[SFDX_MAX_LEVELS][2];
+    int dc_calls[SFDX_MAX_LEVELS][2];
+    int pcg_iters[SFDX_MAX_LEVELS][2];
+    int refine_steps[2];
+    int fine_iters_total;           /* summed over frequencies and polarisations */
+} sfdx_stats;
+
+/* Context: device selection, stream, workspace. */
+int sfdx_create(int device, sfdx_ctx **out);
+int sfdx_destroy(sfdx_ctx *ctx);
+const char *sfdx_last_error(void);
+void sfdx_default_opts(sfdx_opts *o);
+
+/* Replaces fwd3d::fwd_solver (sfd_excmg/src/fdfe_main.f90:1565-1594): all
+ * frequencies, both polarisations; ex caller-allocated with 2*nl*nf entries. */
+int sfdx_fwd_solve(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf, const double *freq,
+                   const sfdx_opts *o, double *ex, sfdx_stats *st);
+
+/* Replaces fwd3d::post_process (sfd_excmg/src/fdfe_main.f90:549-716) and
+ * arho_interp2 (:718-790).  recv(nrec,3) row-major (x,y,z); results and tipper
+ * (nrec*nf, 4) row-major.  out_type 1: Re/Im Zxy, Re/Im Zyx; 2: rho_xy, rho_yx,
+ * phi_xy, phi_yx.  rd = 2 reproduces the reference's station sampling
+ * (arho_interp2 called with rd=2), rd = 1 samples at the station's own cell. */
+int sfdx_post_process(const sfdx_grid *g, int nf, const double *freq, int nrec, const double *recv,
+                      const double *ex, int out_type, int rd, double *results, double *tipper);
+
+/* Replaces mt1d_anl (sfd_excmg/src/model_selec.f90:266-401): conv(nz,6)
+ * column-major; ex, hx: 2*nz+2 complex; imp: 4 complex. */
+int sfdx_mt1d_anl(int nz, const double *conv, const double *c, double freq, double *ex, double *hx, double *imp);
+
+/* Direct solve of one grid level for both polarisations on the host (MKL
+ * PARDISO, complex symmetric): replaces doublemode_nor with mid = 1, 2
+ * (fdfe_main.f90:997-998, 1028-1029) -> pardiso_spd (solvers.f90:765-875).
+ * u: 2*nl complex, xy then yx, Dirichlet values included. */
+int sfdx_direct_solve(const sfdx_grid *g, const double *sigma, double freq, double *u);
+
+/* Kernel-level entry points (parity tests and benchmarks).
+ * y = A_s x: the dual-volume scaled operator of fd_assemb (fdfe_main.f90:226-396,
+ * rows times ve, lines 252/301/350) on interior edges, 0 on boundary edges;
+ * replaces zamux (div_correc.f90:431-449) on the dcfast CSR. */
+int sfdx_apply_operator(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, int precision,
+                        const double *x, double *y);
+/* y = M^-1 x, multicolour SSOR of A_s (replaces ssorfac + lusol,
+ * div_correc.f90:171-220, solvers.f90:721-762); x must vanish on boundary edges. */
+int sfdx_ssor_apply(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, double omega,
+                    int precision, const double *x, double *y);
+/* In-place divergence correction of an edge field (replaces div_correc,
+ * div_correc.f90:11-147).  pcg_iters may be NULL. */
+int sfdx_div_correct(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double *x, double corr_tol,
+                     int *pcg_iters);
+/* EXCMG initial guess on grid g from the two coarser levels (replaces
+ * extint_v, excmg.f90:9-77): u0 on g/4, u1 on g/2, out on g. */
+int sfdx_extint_v(sfdx_ctx *ctx, const sfdx_grid *g, const double *u0, const double *u1, double *out);
+/* Times the fine-level kernels on the context's stream with HIP events:
+ * ms[0] SpMV, ms[1] SSOR apply (12 sweeps), ms[2] COCR vector update,
+ * ms[3] one COCR iteration; averages over reps. */
+int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, int precision,
+                       int reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

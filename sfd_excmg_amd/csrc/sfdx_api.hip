@@ -1,0 +1,851 @@
+// sfdx_api.hip -- host orchestration of the forward solve and the C ABI.
+//
+// fwd_solver (sfd_excmg/src/fdfe_main.f90:1565-1594) loops over frequencies;
+// for each one doublemode_acc (:1258-1563) solves the two coarsest levels
+// directly (u1u2, :1228-1256; here one MKL PARDISO factorisation serves both
+// polarisations), then for levels 3..L builds the EXCMG initial guess on the
+// device (extint_v), imposes the Dirichlet data of the level's 1-D background
+// and runs the preconditioned Krylov solver on the device with the level
+// tolerance eps*10^(i-L) (:1312-1331).  Without EXCMG, doublemode_nor (:892-1063)
+// solves the fine level from x = 0.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+
+#include "../../include/sfdx.h"
+#include "sfdx_host.h"
+#include "sfdx_kernels.h"
+
+using namespace sfdx;
+typedef std::complex<double> zc;
+
+static thread_local std::string g_err;
+
+namespace {
+
+struct Pool {
+    std::map<std::string, std::pair<void *, size_t>> bufs;
+    void *get(const std::string &tag, size_t bytes)
+    {
+        auto it = bufs.find(tag);
+        if (it != bufs.end()) {
+            if (it->second.second >= bytes) return it->second.first;
+            SFDX_CHECK(hipFree(it->second.first));
+            bufs.erase(it);
+        }
+        void *p = nullptr;
+        SFDX_CHECK(hipMalloc(&p, bytes ? bytes : 16));
+        bufs[tag] = {p, bytes};
+        return p;
+    }
+    void release()
+    {
+        for (auto &kv : bufs) (void)hipFree(kv.second.first);
+        bufs.clear();
+    }
+};
+
+struct Timer {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double s() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+}  // namespace
+
+struct sfdx_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    RedBuf rb;
+    Pool pool;
+    double *hslots = nullptr;  // pinned
+};
+
+namespace {
+
+template <typename T>
+T *pool_vec(sfdx_ctx *ctx, const std::string &tag, size_t n)
+{
+    return (T *)ctx->pool.get(tag, n * sizeof(T));
+}
+
+void ensure_partials(sfdx_ctx *ctx, int64_t nblk)
+{
+    if (nblk > ctx->rb.max_blocks) {
+        if (ctx->rb.partials) SFDX_CHECK(hipFree(ctx->rb.partials));
+        SFDX_CHECK(hipMalloc(&ctx->rb.partials, (size_t)nblk * 8 * sizeof(double)));
+        ctx->rb.max_blocks = nblk;
+    }
+}
+
+void read_slots(sfdx_ctx *ctx)
+{
+    SFDX_CHECK(hipMemcpyAsync(ctx->hslots, ctx->rb.slots, SL_COUNT * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    SFDX_CHECK(hipStreamSynchronize(ctx->st));
+}
+
+// Device copy of one grid level in precision R (plus FP64 copies for P assembly).
+template <typename R>
+struct LevelDev {
+    DevGrid<R> g;
+    double *a64 = nullptr, *b64 = nullptr, *c64 = nullptr, *sig64 = nullptr;
+    R *P = nullptr;  // 19 * NP divergence-correction operator
+};
+
+template <typename R>
+LevelDev<R> upload_level(sfdx_ctx *ctx, const HostLevel &L, const std::string &tag)
+{
+    LevelDev<R> D;
+    D.g.d = make_dims(L.nx, L.ny, L.nz);
+    std::vector<R> a(L.a.begin(), L.a.end()), b(L.b.begin(), L.b.end()), c(L.c.begin(), L.c.end()),
+        s(L.sig.begin(), L.sig.end());
+    const std::string p = tag + (sizeof(R) == 8 ? "/d" : "/f");
+    D.g.a = pool_vec<R>(ctx, p + "a", a.size());
+    D.g.b = pool_vec<R>(ctx, p + "b", b.size());
+    D.g.c = pool_vec<R>(ctx, p + "c", c.size());
+    D.g.sig = pool_vec<R>(ctx, p + "s", s.size());
+    SFDX_CHECK(hipMemcpyAsync(D.g.a, a.data(), a.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
+    SFDX_CHECK(hipMemcpyAsync(D.g.b, b.data(), b.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
+    SFDX_CHECK(hipMemcpyAsync(D.g.c, c.data(), c.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
+    SFDX_CHECK(hipMemcpyAsync(D.g.sig, s.data(), s.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
+    D.a64 = pool_vec<double>(ctx, tag + "/a64", L.a.size());
+    D.b64 = pool_vec<double>(ctx, tag + "/b64", L.b.size());
+    D.c64 = pool_vec<double>(ctx, tag + "/c64", L.c.size());
+    D.sig64 = pool_vec<double>(ctx, tag + "/s64", L.sig.size());
+    SFDX_CHECK(hipMemcpyAsync(D.a64, L.a.data(), L.a.size() * 8, hipMemcpyHostToDevice, ctx->st));
+    SFDX_CHECK(hipMemcpyAsync(D.b64, L.b.data(), L.b.size() * 8, hipMemcpyHostToDevice, ctx->st));
+    SFDX_CHECK(hipMemcpyAsync(D.c64, L.c.data(), L.c.size() * 8, hipMemcpyHostToDevice, ctx->st));
+    SFDX_CHECK(hipMemcpyAsync(D.sig64, L.sig.data(), L.sig.size() * 8, hipMemcpyHostToDevice, ctx->st));
+    SFDX_CHECK(hipStreamSynchronize(ctx->st));
+    return D;
+}
+
+template <typename R>
+void ensure_P(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag)
+{
+    if (D.P) return;
+    D.P = pool_vec<R>(ctx, tag + (sizeof(R) == 8 ? "/P64" : "/P32"), (size_t)19 * D.g.d.NP);
+    launch_pbuild<R>(D.g.d, D.a64, D.b64, D.c64, D.sig64, D.P, ctx->st);
+}
+
+struct SolveInfo {
+    int iters = 0, dc_calls = 0, pcg_iters = 0, status = 0;
+    double relres = 0;
+    double bytes = 0;
+};
+
+// ---------------------------------------------------------------- divergence correction
+// x <- x - G phi with P phi = G^T M_sigma x on interior nodes (div_correc,
+// src/div_correc.f90:11-147; Jacobi-PCG as pcg, :343-402, itmax 2000).
+template <typename R>
+int div_correct(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, cx<R> *x, double tol)
+{
+    const Dims &d = D.g.d;
+    ensure_P(ctx, D, tag);
+    const std::string p = tag + (sizeof(R) == 8 ? "/dc64" : "/dc32");
+    cx<R> *w = pool_vec<cx<R>>(ctx, p + "w", d.NL);
+    cx<R> *t = pool_vec<cx<R>>(ctx, p + "t", d.NP);
+    cx<R> *u = pool_vec<cx<R>>(ctx, p + "u", d.NP);
+    cx<R> *r = pool_vec<cx<R>>(ctx, p + "r", d.NP);
+    cx<R> *pp = pool_vec<cx<R>>(ctx, p + "p", d.NP);
+    cx<R> *v = pool_vec<cx<R>>(ctx, p + "v", d.NP);
+    int64_t nb;
+    launch_mass<R>(D.g, x, w, ctx->st);
+    launch_node_div<R>(D.g, w, t, ctx->rb, ctx->st, nb);
+    reduce_final(ctx->rb, nb, 1, SL_TMP, OP_T, ctx->st);
+    launch_pcg<R>(2, d.NP, D.P, ctx->rb, u, r, pp, t, ctx->st, nb);
+    reduce_final(ctx->rb, nb, 1, SL_TMP, OP_PCG_INIT, ctx->st);
+    read_slots(ctx);
+    const double tn = std::sqrt(ctx->hslots[SL_T]);
+    if (!(tn > 0)) return 0;
+    int it = 0;
+    for (; it < 2000; it++) {
+        launch_papply<R>(d, D.P, pp, v, 1, ctx->rb, ctx->st, nb);
+        reduce_final(ctx->rb, nb, 2, SL_TMP, OP_PCG_ALPHA, ctx->st);
+        launch_pcg<R>(0, d.NP, D.P, ctx->rb, u, r, pp, v, ctx->st, nb);
+        reduce_final(ctx->rb, nb, 2, SL_TMP, OP_PCG_BETA, ctx->st);
+        launch_pcg<R>(1, d.NP, D.P, ctx->rb, u, r, pp, v, ctx->st, nb);
+        read_slots(ctx);
+        const double err = std::sqrt(ctx->hslots[SL_PR]) / tn;
+        if (!(err == err)) break;
+        if (err < tol) { it++; break; }
+    }
+    launch_grad_update<R>(D.g, u, x, ctx->st);
+    return it;
+}
+
+// ---------------------------------------------------------------- Krylov
+// Solve A_s x = f on the interior edges (boundary entries of x held fixed; f may
+// be null = 0) to ||r/ve|| <= eps * bnorm.
+template <typename R>
+SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *f, cx<R> *x, double bnorm,
+               double eps, int maxit, int corr_intv, double corr_tol, R omega, int verbose)
+{
+    const Dims &d = D.g.d;
+    const std::string p = tag + (sizeof(R) == 8 ? "/k64" : "/k32");
+    cx<R> *r = pool_vec<cx<R>>(ctx, p + "r", d.NL), *z = pool_vec<cx<R>>(ctx, p + "z", d.NL);
+    cx<R> *Az = pool_vec<cx<R>>(ctx, p + "Az", d.NL), *pv = pool_vec<cx<R>>(ctx, p + "p", d.NL);
+    cx<R> *Ap = pool_vec<cx<R>>(ctx, p + "Ap", d.NL), *q = pool_vec<cx<R>>(ctx, p + "q", d.NL);
+    SliceLaunch L = edge_launch(d);
+    const int64_t nbe = L.nblocks();
+    const double sc = 2.0 * sizeof(R), vecb = (double)d.NL * sc;
+    const double b_spmv = 2 * vecb + (double)d.NC * 6 * sizeof(R), b_ssor = 2 * (b_spmv + vecb);
+    SolveInfo info;
+    bool restart = true;
+    while (true) {
+        if (restart) {
+            launch_spmv<R>(D.g, wmu, x, r, (R)-1, 0, nullptr, ctx->rb, ctx->st);
+            if (f) launch_axpby<R>(d.NL, cx<R>(1), f, cx<R>(1), r, ctx->st);
+            launch_ssor<R>(D.g, wmu, r, z, omega, ctx->st);
+            launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
+            reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_RHO, ctx->st);
+            launch_copy<R>(d.NL, z, pv, ctx->st);
+            launch_copy<R>(d.NL, Az, Ap, ctx->st);
+            launch_vec<R>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
+            reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+            read_slots(ctx);
+            info.bytes += 2 * b_spmv + b_ssor + 6 * vecb;
+            restart = false;
+        }
+        info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
+        if (!(info.relres == info.relres) || std::isinf(info.relres)) { info.status = SFDX_ERR_NAN; break; }
+        if (verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  cocr it %d relres %.3e\n", info.iters, info.relres);
+        if (info.relres < eps) { info.status = 0; break; }
+        if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
+        if (ctx->hslots[SL_BREAK] != 0) { info.status = SFDX_ERR_BREAKDOWN; break; }
+        int64_t nbd;
+        launch_ssor<R>(D.g, wmu, Ap, q, omega, ctx->st);
+        launch_dot<R>(0, d.NL, Ap, q, ctx->rb, ctx->st, nbd);
+        reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
+        launch_vec<R>(0, D.g, ctx->rb, x, pv, Ap, r, q, z, ctx->st);
+        reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+        info.iters++;
+        info.bytes += b_ssor + 2 * vecb + 9 * vecb;
+        if (corr_intv > 0 && info.iters % corr_intv == 0) {
+            info.pcg_iters += div_correct<R>(ctx, D, tag, x, corr_tol);
+            info.dc_calls++;
+            restart = true;
+            continue;
+        }
+        launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
+        reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_BETA, ctx->st);
+        launch_vec<R>(3, D.g, ctx->rb, pv, z, Ap, Az, nullptr, nullptr, ctx->st);
+        info.bytes += b_spmv + 6 * vecb;
+        read_slots(ctx);
+    }
+    return info;
+}
+
+// Right-preconditioned BiCGStab (Bicg_stab_p, src/solvers.f90:522-719) on A_s.
+template <typename R>
+SolveInfo bicgstab(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *f, cx<R> *x,
+                   double bnorm, double eps, int maxit, int corr_intv, double corr_tol, R omega, int verbose)
+{
+    const Dims &d = D.g.d;
+    const std::string p = tag + (sizeof(R) == 8 ? "/b64" : "/b32");
+    cx<R> *r = pool_vec<cx<R>>(ctx, p + "r", d.NL), *rh = pool_vec<cx<R>>(ctx, p + "rh", d.NL);
+    cx<R> *pv = pool_vec<cx<R>>(ctx, p + "p", d.NL), *y = pool_vec<cx<R>>(ctx, p + "y", d.NL);
+    cx<R> *v = pool_vec<cx<R>>(ctx, p + "v", d.NL), *s = pool_vec<cx<R>>(ctx, p + "s", d.NL);
+    cx<R> *z = pool_vec<cx<R>>(ctx, p + "z", d.NL), *t = pool_vec<cx<R>>(ctx, p + "t", d.NL);
+    SliceLaunch L = edge_launch(d);
+    const int64_t nbe = L.nblocks();
+    const double vecb = (double)d.NL * 2.0 * sizeof(R);
+    const double b_spmv = 2 * vecb + (double)d.NC * 6 * sizeof(R), b_ssor = 2 * (b_spmv + vecb);
+    SolveInfo info;
+    bool restart = true;
+    while (true) {
+        if (restart) {
+            launch_spmv<R>(D.g, wmu, x, r, (R)-1, 0, nullptr, ctx->rb, ctx->st);
+            if (f) launch_axpby<R>(d.NL, cx<R>(1), f, cx<R>(1), r, ctx->st);
+            launch_copy<R>(d.NL, r, rh, ctx->st);
+            SFDX_CHECK(hipMemsetAsync(pv, 0, d.NL * sizeof(cx<R>), ctx->st));
+            SFDX_CHECK(hipMemsetAsync(v, 0, d.NL * sizeof(cx<R>), ctx->st));
+            reduce_final(ctx->rb, 0, 0, SL_TMP, OP_BICG_INIT, ctx->st);
+            launch_vec<R>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
+            reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+            read_slots(ctx);
+            restart = false;
+        }
+        info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
+        if (!(info.relres == info.relres) || std::isinf(info.relres)) { info.status = SFDX_ERR_NAN; break; }
+        if (verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  bicgstab it %d relres %.3e\n", info.iters, info.relres);
+        if (info.relres < eps) { info.status = 0; break; }
+        if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
+        if (ctx->hslots[SL_BREAK] != 0) { info.status = SFDX_ERR_BREAKDOWN; break; }
+        if (corr_intv > 0 && info.iters > 0 && info.iters % corr_intv == 0 && info.dc_calls < info.iters / corr_intv) {
+            info.pcg_iters += div_correct<R>(ctx, D, tag, x, corr_tol);
+            info.dc_calls++;
+            restart = true;
+            continue;
+        }
+        int64_t nbd;
+        launch_dot<R>(1, d.NL, rh, r, ctx->rb, ctx->st, nbd);
+        reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_BICG_RHO, ctx->st);
+        launch_vec<R>(4, D.g, ctx->rb, pv, r, v, nullptr, nullptr, nullptr, ctx->st);
+        launch_ssor<R>(D.g, wmu, pv, y, omega, ctx->st);
+        launch_spmv<R>(D.g, wmu, y, v, (R)1, 2, rh, ctx->rb, ctx->st);
+        reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_BICG_ALPHA, ctx->st);
+        launch_vec<R>(1, D.g, ctx->rb, s, r, v, nullptr, nullptr, nullptr, ctx->st);
+        launch_ssor<R>(D.g, wmu, s, z, omega, ctx->st);
+        launch_spmv<R>(D.g, wmu, z, t, (R)1, 3, s, ctx->rb, ctx->st);
+        reduce_final(ctx->rb, nbe, 3, SL_TMP, OP_BICG_OMEGA, ctx->st);
+        launch_vec<R>(2, D.g, ctx->rb, x, y, z, r, s, t, ctx->st);
+        reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+        read_slots(ctx);
+        info.iters++;
+        info.bytes += 2 * b_spmv + 2 * b_ssor + 16 * vecb;
+    }
+    return info;
+}
+
+template <typename R>
+SolveInfo krylov(sfdx_ctx *ctx, int solver, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *f, cx<R> *x,
+                 double bnorm, double eps, int maxit, int corr_intv, double corr_tol, R omega, int verbose)
+{
+    if (solver == SFDX_SOLVER_COCR)
+        return cocr<R>(ctx, D, tag, wmu, f, x, bnorm, eps, maxit, corr_intv, corr_tol, omega, verbose);
+    return bicgstab<R>(ctx, D, tag, wmu, f, x, bnorm, eps, maxit, corr_intv, corr_tol, omega, verbose);
+}
+
+// Right-hand-side norm of a level: ||b_int|| (unscaled, = ||A_ib g / ve||) and
+// ||g||, for the chosen residual convention.
+double rhs_norm(sfdx_ctx *ctx, LevelDev<double> &D, const std::string &tag, const zd *edc, int pol, int norm_mode,
+                double wmu)
+{
+    const Dims &d = D.g.d;
+    zd *g = pool_vec<zd>(ctx, tag + "/bn_g", d.NL), *r = pool_vec<zd>(ctx, tag + "/bn_r", d.NL);
+    launch_dirichlet<double>(d, edc, pol, g, 1, ctx->st);
+    launch_spmv<double>(D.g, wmu, g, r, -1.0, 0, nullptr, ctx->rb, ctx->st);
+    SliceLaunch L = edge_launch(d);
+    launch_vec<double>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
+    reduce_final(ctx->rb, L.nblocks(), 1, SL_TMP, OP_RN, ctx->st);
+    read_slots(ctx);
+    double bi2 = ctx->hslots[SL_RN];
+    if (norm_mode == SFDX_NORM_INTERIOR) return std::sqrt(bi2);
+    int64_t nbd;
+    launch_dot<double>(1, d.NL, g, g, ctx->rb, ctx->st, nbd);
+    reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_NONE, ctx->st);
+    read_slots(ctx);
+    return std::sqrt(bi2 + ctx->hslots[SL_TMP]);
+}
+
+struct LevelSolveResult {
+    SolveInfo info;
+    int refine = 0;
+};
+
+// Solve one level for one polarisation; x (FP64) holds the initial guess on the
+// interior and receives the Dirichlet data first.
+LevelSolveResult solve_level(sfdx_ctx *ctx, const sfdx_opts &o, LevelDev<double> &D64, LevelDev<float> *D32,
+                             const std::string &tag, double freq, const zd *edc, int pol, zd *x, double eps, int maxit)
+{
+    const Dims &d = D64.g.d;
+    const double wmu = 2 * 3.141592653589793 * freq * kMu0;
+    launch_dirichlet<double>(d, edc, pol, x, 0, ctx->st);
+    const double bnorm = rhs_norm(ctx, D64, tag, edc, pol, o.norm, wmu);
+    LevelSolveResult res;
+    if (o.precision == SFDX_PREC_FP64 || !D32) {
+        res.info = krylov<double>(ctx, o.solver, D64, tag, wmu, nullptr, x, bnorm, eps, maxit, o.corr_intv, o.corr_tol,
+                                  o.ssor_omega, o.verbose);
+        return res;
+    }
+    // Mixed precision: FP64 residual and update, complex64 inner solves.
+    zd *r64 = pool_vec<zd>(ctx, tag + "/ir_r", d.NL);
+    zf *r32 = pool_vec<zf>(ctx, tag + "/ir_r32", d.NL), *d32 = pool_vec<zf>(ctx, tag + "/ir_d32", d.NL);
+    SliceLaunch L = edge_launch(d);
+    int total_iters = 0, dc_calls = 0, pcg_iters = 0;
+    double bytes = 0;
+    for (int step = 0;; step++) {
+        if (o.corr_intv > 0 && step > 0) {
+            pcg_iters += div_correct<double>(ctx, D64, tag, x, o.corr_tol);
+            dc_calls++;
+        }
+        launch_spmv<double>(D64.g, wmu, x, r64, -1.0, 0, nullptr, ctx->rb, ctx->st);
+        launch_vec<double>(5, D64.g, ctx->rb, r64, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
+        reduce_final(ctx->rb, L.nblocks(), 1, SL_TMP, OP_RN, ctx->st);
+        read_slots(ctx);
+        const double rel = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
+        bytes += (double)d.NL * 16 * 3 + (double)d.NC * 48;
+        res.info.relres = rel;
+        if (!(rel == rel)) { res.info.status = SFDX_ERR_NAN; break; }
+        if (rel < eps) { res.info.status = 0; break; }
+        if (step >= o.max_refine || total_iters >= maxit) { res.info.status = SFDX_WARN_NOCONV; break; }
+        launch_convert<double, float>(d.NL, r64, r32, ctx->st);
+        SFDX_CHECK(hipMemsetAsync(d32, 0, d.NL * sizeof(zf), ctx->st));
+        const double rn = std::sqrt(ctx->hslots[SL_RN]);
+        const double inner = std::max(o.inner_tol, 0.5 * eps * bnorm / rn);
+        SolveInfo si = krylov<float>(ctx, o.solver, *D32, tag, (float)wmu, r32, d32, rn, inner, maxit - total_iters,
+                                     o.corr_intv, o.corr_tol, (float)o.ssor_omega, o.verbose);
+        total_iters += si.iters;
+        dc_calls += si.dc_calls;
+        pcg_iters += si.pcg_iters;
+        bytes += si.bytes;
+        if (si.status > 0 && si.status != SFDX_ERR_BREAKDOWN) { res.info.status = si.status; break; }
+        launch_add_lo(d.NL, x, d32, ctx->st);
+        res.refine = step + 1;
+        if (o.verbose) fprintf(stderr, "  refine %d: outer %.3e inner its %d (%.2e)\n", step, rel, si.iters, si.relres);
+    }
+    res.info.iters = total_iters;
+    res.info.dc_calls = dc_calls;
+    res.info.pcg_iters = pcg_iters;
+    res.info.bytes = bytes;
+    return res;
+}
+
+// prolongation tables of one fine level on the device
+ProlongDev upload_prolong(sfdx_ctx *ctx, const HostLevel &fine, int m, const std::string &tag)
+{
+    ProlongHost H;
+    prolong_tables(fine, m, H);
+    ProlongDev T;
+    for (int q = 0; q < 3; q++) {
+        const std::string p = tag + "/pt" + std::to_string(m) + std::to_string(q);
+        T.nb[q] = pool_vec<int>(ctx, p + "nb", H.nb[q].size());
+        T.eb[q] = pool_vec<int>(ctx, p + "eb", H.eb[q].size());
+        T.nw[q] = pool_vec<double>(ctx, p + "nw", H.nw[q].size());
+        T.ew[q] = pool_vec<double>(ctx, p + "ew", H.ew[q].size());
+        SFDX_CHECK(hipMemcpyAsync(T.nb[q], H.nb[q].data(), H.nb[q].size() * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+        SFDX_CHECK(hipMemcpyAsync(T.eb[q], H.eb[q].data(), H.eb[q].size() * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+        SFDX_CHECK(hipMemcpyAsync(T.nw[q], H.nw[q].data(), H.nw[q].size() * 8, hipMemcpyHostToDevice, ctx->st));
+        SFDX_CHECK(hipMemcpyAsync(T.ew[q], H.ew[q].data(), H.ew[q].size() * 8, hipMemcpyHostToDevice, ctx->st));
+    }
+    SFDX_CHECK(hipStreamSynchronize(ctx->st));
+    return T;
+}
+
+// extint_v on the device: out(level i) = I4[(1+1/4) u1 - 1/4 I2 u0]
+void extint_device(sfdx_ctx *ctx, const HostLevel &fine, const HostLevel &mid, const zd *u0, const zd *u1, zd *out,
+                   const std::string &tag)
+{
+    const Dims df = make_dims(fine.nx, fine.ny, fine.nz), dm = make_dims(mid.nx, mid.ny, mid.nz);
+    const Dims dcs = make_dims(mid.nx / 2, mid.ny / 2, mid.nz / 2);
+    ProlongDev T2 = upload_prolong(ctx, mid, 2, tag + "/mid");
+    ProlongDev T4 = upload_prolong(ctx, fine, 4, tag + "/fine");
+    zd *w = pool_vec<zd>(ctx, tag + "/extw", dm.NL);
+    launch_prolong<double>(dm, dcs, 2, T2, u0, w, ctx->st);
+    launch_extrap<double>(dm.NL, 1.25, u1, 0.25, w, ctx->st);
+    launch_prolong<double>(df, dm, 4, T4, w, out, ctx->st);
+}
+
+bool mesh_ok(const HostLevel &F, int L)
+{
+    const int f = 1 << L;
+    return F.nx % f == 0 && F.ny % f == 0 && F.nz % f == 0 && F.nair % (1 << (L - 1)) == 0 && F.nx / f >= 1;
+}
+
+HostLevel host_level(const sfdx_grid *g, const double *sigma)
+{
+    HostLevel L;
+    L.nx = g->nx; L.ny = g->ny; L.nz = g->nz; L.nair = g->nair;
+    L.a.assign(g->a, g->a + g->nx);
+    L.b.assign(g->b, g->b + g->ny);
+    L.c.assign(g->c, g->c + g->nz);
+    L.sig.assign(sigma, sigma + 6 * L.ncell());
+    return L;
+}
+
+int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf, const double *freq,
+                   const sfdx_opts *o, double *ex, sfdx_stats *st)
+{
+    Timer total;
+    sfdx_stats S;
+    std::memset(&S, 0, sizeof(S));
+    const HostLevel fine = host_level(g, sigma);
+    for (size_t q = 0; q < fine.sig.size(); q++)
+        if (!std::isfinite(fine.sig[q])) { g_err = "non-finite sigma"; return SFDX_ERR_NAN; }
+    const int Lv = o->use_excmg ? o->level : 1;
+    if (o->use_excmg && (Lv < 3 || Lv > SFDX_MAX_LEVELS || !mesh_ok(fine, Lv))) {
+        g_err = "grid dimensions must be divisible by 2^level and nair by 2^(level-1)";
+        return SFDX_ERR_MESH;
+    }
+    if (o->solver != SFDX_SOLVER_COCR && o->solver != SFDX_SOLVER_BICGSTAB) {
+        g_err = "solver not supported on the device path";
+        return SFDX_ERR_UNSUPPORTED;
+    }
+    // level hierarchy: lev[i] for i = 1..Lv (Lv = fine)
+    std::vector<HostLevel> lev(Lv + 1);
+    lev[Lv] = fine;
+    for (int i = Lv - 1; i >= 1; i--) lev[i] = coarsen(fine, Lv - i);
+    std::vector<LevelDev<double>> D64(Lv + 1);
+    std::vector<LevelDev<float>> D32(Lv + 1);
+    const int first_dev = o->use_excmg ? 3 : 1;
+    for (int i = first_dev; i <= Lv; i++) {
+        D64[i] = upload_level<double>(ctx, lev[i], "L" + std::to_string(i));
+        if (o->precision == SFDX_PREC_MIXED) D32[i] = upload_level<float>(ctx, lev[i], "L" + std::to_string(i));
+    }
+    ensure_partials(ctx, edge_launch(make_dims(fine.nx, fine.ny, fine.nz)).nblocks() + 64);
+    const int64_t NLf = make_dims(fine.nx, fine.ny, fine.nz).NL;
+    int worst = 0;
+    for (int fi = 0; fi < nf; fi++) {
+        const double ff = freq[fi];
+        // device solution history: u[pol][level]
+        std::vector<std::array<zd *, 2>> u(Lv + 1, {nullptr, nullptr});
+        auto ubuf = [&](int i, int pol) {
+            const Dims d = make_dims(lev[i].nx, lev[i].ny, lev[i].nz);
+            return pool_vec<zd>(ctx, "u" + std::to_string(i) + "_" + std::to_string(pol), d.NL);
+        };
+        if (o->use_excmg) {
+            // levels 1 and 2: direct solves (u1u2, src/fdfe_main.f90:1228-1256)
+            for (int i = 1; i <= 2; i++) {
+                const std::vector<zc> edc = background_field(lev[i], ff);
+                CoarseSystem CS;
+                coarse_system(lev[i], ff, edc, CS);
+                std::vector<zc> sol;
+                if (!direct_solve(CS, sol, g_err)) return SFDX_ERR_DIRECT;
+                const Dims d = make_dims(lev[i].nx, lev[i].ny, lev[i].nz);
+                for (int pol = 0; pol < 2; pol++) {
+                    std::vector<zc> full(d.NL, 0.0);
+                    for (int64_t e = 0; e < d.NL; e++) {
+                        int ii, jj, kk;
+                        int oo = edge_decode(d, e, ii, jj, kk);
+                        if (CS.imap[e] >= 0) full[e] = sol[(size_t)pol * CS.n + CS.imap[e]];
+                        else dirichlet_value(d, edc.data(), pol, oo, ii, jj, kk, full[e]);
+                    }
+                    u[i][pol] = ubuf(i, pol);
+                    SFDX_CHECK(hipMemcpyAsync(u[i][pol], full.data(), d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+                }
+                SFDX_CHECK(hipStreamSynchronize(ctx->st));
+            }
+        }
+        double tol = o->eps * std::pow(10.0, 2 - Lv);  // after u1u2 (:1312, :1323)
+        for (int i = first_dev; i <= Lv; i++) {
+            tol *= 10;                                   // level tolerance eps*10^(i-L) (:1331)
+            if (!o->use_excmg) tol = o->eps;
+            const std::vector<zc> edc_h = background_field(lev[i], ff);
+            const Dims d = make_dims(lev[i].nx, lev[i].ny, lev[i].nz);
+            zd *edc = pool_vec<zd>(ctx, "edc" + std::to_string(i), edc_h.size());
+            SFDX_CHECK(hipMemcpyAsync(edc, edc_h.data(), edc_h.size() * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+            const std::string tag = "L" + std::to_string(i);
+            for (int pol = 0; pol < 2; pol++) {
+                zd *x = ubuf(i, pol);
+                if (o->use_excmg) extint_device(ctx, lev[i], lev[i - 1], u[i - 2][pol], u[i - 1][pol], x, tag);
+                else SFDX_CHECK(hipMemsetAsync(x, 0, d.NL * sizeof(zd), ctx->st));
+                const int maxit = o->use_excmg ? o->maxit_excmg : o->maxit_plain;
+                SFDX_CHECK(hipStreamSynchronize(ctx->st));
+                Timer tl;
+                LevelSolveResult R = solve_level(ctx, *o, D64[i], o->precision == SFDX_PREC_MIXED ? &D32[i] : nullptr,
+                                                 tag, ff, edc, pol, x, tol, maxit);
+                SFDX_CHECK(hipStreamSynchronize(ctx->st));
+                const double secs = tl.s();
+                if (i == Lv) {
+                    S.fine_seconds += secs;
+                    S.fine_bytes += R.info.bytes;
+                    S.fine_iters_total += R.info.iters;
+                    S.refine_steps[pol] = R.refine;
+                }
+                S.iters[i - 1][pol] = R.info.iters;
+                S.relres[i - 1][pol] = R.info.relres;
+                S.dc_calls[i - 1][pol] = R.info.dc_calls;
+                S.pcg_iters[i - 1][pol] = R.info.pcg_iters;
+                if (o->verbose)
+                    fprintf(stderr, "f=%g level %d (%dx%dx%d) pol %d: %d its relres %.3e dc %d pcg %d %.3fs\n", ff, i,
+                            d.nx, d.ny, d.nz, pol, R.info.iters, R.info.relres, R.info.dc_calls, R.info.pcg_iters, secs);
+                if (R.info.status > 0) {
+                    g_err = "Krylov solve failed on level " + std::to_string(i);
+                    return R.info.status;
+                }
+                if (R.info.status < 0) worst = R.info.status;
+                u[i][pol] = x;
+            }
+        }
+        for (int pol = 0; pol < 2; pol++)
+            SFDX_CHECK(hipMemcpyAsync(ex + ((size_t)fi * 2 * NLf + (size_t)pol * NLf) * 2, u[Lv][pol],
+                                      NLf * sizeof(zd), hipMemcpyDeviceToHost, ctx->st));
+        SFDX_CHECK(hipStreamSynchronize(ctx->st));
+    }
+    S.total_seconds = total.s();
+    S.status = worst;
+    if (st) *st = S;
+    return worst;
+}
+
+template <typename F>
+int guarded(F &&fn)
+{
+    try {
+        return fn();
+    } catch (const HipError &e) {
+        g_err = e.what;
+        return SFDX_ERR_HIP;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return SFDX_ERR_ARG;
+    }
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+const char *sfdx_last_error(void) { return g_err.c_str(); }
+
+void sfdx_default_opts(sfdx_opts *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->level = 3;
+    o->use_excmg = 1;
+    o->solver = SFDX_SOLVER_BICGSTAB;
+    o->eps = 1e-8;
+    o->corr_intv = 200;
+    o->corr_tol = 1e-3;
+    o->ssor_omega = 1.0;
+    o->precision = SFDX_PREC_FP64;
+    o->norm = SFDX_NORM_REFERENCE;
+    o->maxit_plain = 5000;
+    o->maxit_excmg = 1000;
+    o->inner_tol = 1e-5;
+    o->max_refine = 20;
+    o->verbose = 0;
+}
+
+int sfdx_create(int device, sfdx_ctx **out)
+{
+    if (!out) return SFDX_ERR_ARG;
+    return guarded([&]() {
+        auto *c = new sfdx_ctx();
+        c->device = device;
+        SFDX_CHECK(hipSetDevice(device));
+        SFDX_CHECK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+        SFDX_CHECK(hipMalloc(&c->rb.slots, SL_COUNT * sizeof(double)));
+        SFDX_CHECK(hipMemset(c->rb.slots, 0, SL_COUNT * sizeof(double)));
+        SFDX_CHECK(hipHostMalloc(&c->hslots, SL_COUNT * sizeof(double), hipHostMallocDefault));
+        *out = c;
+        return SFDX_OK;
+    });
+}
+
+int sfdx_destroy(sfdx_ctx *c)
+{
+    if (!c) return SFDX_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->st);
+    c->pool.release();
+    if (c->rb.partials) (void)hipFree(c->rb.partials);
+    if (c->rb.slots) (void)hipFree(c->rb.slots);
+    if (c->hslots) (void)hipHostFree(c->hslots);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return SFDX_OK;
+}
+
+int sfdx_fwd_solve(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf, const double *freq,
+                   const sfdx_opts *o, double *ex, sfdx_stats *st)
+{
+    if (!ctx || !g || !sigma || !freq || !ex || nf <= 0) return SFDX_ERR_ARG;
+    sfdx_opts def;
+    if (!o) { sfdx_default_opts(&def); o = &def; }
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        return fwd_solve_impl(ctx, g, sigma, nf, freq, o, ex, st);
+    });
+}
+
+int sfdx_post_process(const sfdx_grid *g, int nf, const double *freq, int nrec, const double *recv, const double *ex,
+                      int out_type, int rd, double *results, double *tipper)
+{
+    if (!g || !freq || !recv || !ex || !results) return SFDX_ERR_ARG;
+    PostGrid pg{g->nx, g->ny, g->nz, g->nair, g->a, g->b, g->c};
+    int s = post_process(pg, nf, freq, nrec, recv, (const zc *)ex, out_type, rd, results, tipper);
+    if (s) { g_err = "station " + std::to_string(s) + " lies outside the interpolation area"; return SFDX_ERR_ARG; }
+    return SFDX_OK;
+}
+
+int sfdx_direct_solve(const sfdx_grid *g, const double *sigma, double freq, double *u)
+{
+    if (!g || !sigma || !u) return SFDX_ERR_ARG;
+    return guarded([&]() {
+        const HostLevel L = host_level(g, sigma);
+        const std::vector<zc> edc = background_field(L, freq);
+        CoarseSystem CS;
+        coarse_system(L, freq, edc, CS);
+        std::vector<zc> sol;
+        if (!direct_solve(CS, sol, g_err)) return SFDX_ERR_DIRECT;
+        const Dims d = make_dims(L.nx, L.ny, L.nz);
+        zc *out = (zc *)u;
+        for (int pol = 0; pol < 2; pol++)
+            for (int64_t e = 0; e < d.NL; e++) {
+                int ii, jj, kk;
+                int oo = edge_decode(d, e, ii, jj, kk);
+                zc v(0);
+                if (CS.imap[e] >= 0) v = sol[(size_t)pol * CS.n + CS.imap[e]];
+                else dirichlet_value(d, edc.data(), pol, oo, ii, jj, kk, v);
+                out[(size_t)pol * d.NL + e] = v;
+            }
+        return SFDX_OK;
+    });
+}
+
+int sfdx_mt1d_anl(int nz, const double *conv, const double *c, double freq, double *ex, double *hx, double *imp)
+{
+    if (!conv || !c || !ex) return SFDX_ERR_ARG;
+    mt1d_anl(nz, conv, c, freq, (zc *)ex, (zc *)hx, (zc *)imp);
+    return SFDX_OK;
+}
+
+int sfdx_apply_operator(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, int precision,
+                        const double *x, double *y)
+{
+    if (!ctx || !g || !sigma || !x || !y) return SFDX_ERR_ARG;
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        HostLevel L = host_level(g, sigma);
+        const Dims d = make_dims(L.nx, L.ny, L.nz);
+        ensure_partials(ctx, edge_launch(d).nblocks() + 64);
+        const double wmu = 2 * 3.141592653589793 * freq * kMu0;
+        if (precision == SFDX_PREC_FP64) {
+            LevelDev<double> D = upload_level<double>(ctx, L, "op");
+            zd *xd = pool_vec<zd>(ctx, "op/x", d.NL), *yd = pool_vec<zd>(ctx, "op/y", d.NL);
+            SFDX_CHECK(hipMemcpyAsync(xd, x, d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+            launch_spmv<double>(D.g, wmu, xd, yd, 1.0, 0, nullptr, ctx->rb, ctx->st);
+            SFDX_CHECK(hipMemcpyAsync(y, yd, d.NL * sizeof(zd), hipMemcpyDeviceToHost, ctx->st));
+        } else {
+            LevelDev<float> D = upload_level<float>(ctx, L, "op");
+            std::vector<zf> xf(d.NL), yf(d.NL);
+            for (int64_t e = 0; e < d.NL; e++) xf[e] = zf((float)x[2 * e], (float)x[2 * e + 1]);
+            zf *xd = pool_vec<zf>(ctx, "op/xf", d.NL), *yd = pool_vec<zf>(ctx, "op/yf", d.NL);
+            SFDX_CHECK(hipMemcpyAsync(xd, xf.data(), d.NL * sizeof(zf), hipMemcpyHostToDevice, ctx->st));
+            launch_spmv<float>(D.g, (float)wmu, xd, yd, 1.0f, 0, nullptr, ctx->rb, ctx->st);
+            SFDX_CHECK(hipMemcpyAsync(yf.data(), yd, d.NL * sizeof(zf), hipMemcpyDeviceToHost, ctx->st));
+            SFDX_CHECK(hipStreamSynchronize(ctx->st));
+            for (int64_t e = 0; e < d.NL; e++) { y[2 * e] = yf[e].re; y[2 * e + 1] = yf[e].im; }
+        }
+        SFDX_CHECK(hipStreamSynchronize(ctx->st));
+        return SFDX_OK;
+    });
+}
+
+int sfdx_ssor_apply(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, double omega, int precision,
+                    const double *x, double *y)
+{
+    if (!ctx || !g || !sigma || !x || !y) return SFDX_ERR_ARG;
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        HostLevel L = host_level(g, sigma);
+        const Dims d = make_dims(L.nx, L.ny, L.nz);
+        const double wmu = 2 * 3.141592653589793 * freq * kMu0;
+        if (precision == SFDX_PREC_FP64) {
+            LevelDev<double> D = upload_level<double>(ctx, L, "op");
+            zd *xd = pool_vec<zd>(ctx, "op/x", d.NL), *yd = pool_vec<zd>(ctx, "op/y", d.NL);
+            SFDX_CHECK(hipMemcpyAsync(xd, x, d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+            launch_ssor<double>(D.g, wmu, xd, yd, omega, ctx->st);
+            SFDX_CHECK(hipMemcpyAsync(y, yd, d.NL * sizeof(zd), hipMemcpyDeviceToHost, ctx->st));
+            SFDX_CHECK(hipStreamSynchronize(ctx->st));
+        } else {
+            LevelDev<float> D = upload_level<float>(ctx, L, "op");
+            std::vector<zf> xf(d.NL), yf(d.NL);
+            for (int64_t e = 0; e < d.NL; e++) xf[e] = zf((float)x[2 * e], (float)x[2 * e + 1]);
+            zf *xd = pool_vec<zf>(ctx, "op/xf", d.NL), *yd = pool_vec<zf>(ctx, "op/yf", d.NL);
+            SFDX_CHECK(hipMemcpyAsync(xd, xf.data(), d.NL * sizeof(zf), hipMemcpyHostToDevice, ctx->st));
+            launch_ssor<float>(D.g, (float)wmu, xd, yd, (float)omega, ctx->st);
+            SFDX_CHECK(hipMemcpyAsync(yf.data(), yd, d.NL * sizeof(zf), hipMemcpyDeviceToHost, ctx->st));
+            SFDX_CHECK(hipStreamSynchronize(ctx->st));
+            for (int64_t e = 0; e < d.NL; e++) { y[2 * e] = yf[e].re; y[2 * e + 1] = yf[e].im; }
+        }
+        return SFDX_OK;
+    });
+}
+
+int sfdx_div_correct(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double *x, double corr_tol, int *pcg_iters)
+{
+    if (!ctx || !g || !sigma || !x) return SFDX_ERR_ARG;
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        HostLevel L = host_level(g, sigma);
+        const Dims d = make_dims(L.nx, L.ny, L.nz);
+        ensure_partials(ctx, edge_launch(d).nblocks() + 64);
+        LevelDev<double> D = upload_level<double>(ctx, L, "dcop");
+        zd *xd = pool_vec<zd>(ctx, "dcop/x", d.NL);
+        SFDX_CHECK(hipMemcpyAsync(xd, x, d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+        int it = div_correct<double>(ctx, D, "dcop", xd, corr_tol);
+        SFDX_CHECK(hipMemcpyAsync(x, xd, d.NL * sizeof(zd), hipMemcpyDeviceToHost, ctx->st));
+        SFDX_CHECK(hipStreamSynchronize(ctx->st));
+        if (pcg_iters) *pcg_iters = it;
+        return SFDX_OK;
+    });
+}
+
+int sfdx_extint_v(sfdx_ctx *ctx, const sfdx_grid *g, const double *u0, const double *u1, double *out)
+{
+    if (!ctx || !g || !u0 || !u1 || !out) return SFDX_ERR_ARG;
+    if (g->nx % 8 || g->ny % 8 || g->nz % 8 || g->nair % 2) { g_err = "grid not divisible by 8"; return SFDX_ERR_MESH; }
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        HostLevel F;
+        F.nx = g->nx; F.ny = g->ny; F.nz = g->nz; F.nair = g->nair;
+        F.a.assign(g->a, g->a + g->nx); F.b.assign(g->b, g->b + g->ny); F.c.assign(g->c, g->c + g->nz);
+        HostLevel M;
+        M.nx = F.nx / 2; M.ny = F.ny / 2; M.nz = F.nz / 2; M.nair = F.nair / 2;
+        for (int i = 0; i < M.nx; i++) M.a.push_back(F.a[2 * i] + F.a[2 * i + 1]);
+        for (int i = 0; i < M.ny; i++) M.b.push_back(F.b[2 * i] + F.b[2 * i + 1]);
+        for (int i = 0; i < M.nz; i++) M.c.push_back(F.c[2 * i] + F.c[2 * i + 1]);
+        const Dims df = make_dims(F.nx, F.ny, F.nz), dm = make_dims(M.nx, M.ny, M.nz), d0 = make_dims(M.nx / 2, M.ny / 2, M.nz / 2);
+        zd *u0d = pool_vec<zd>(ctx, "ext/u0", d0.NL), *u1d = pool_vec<zd>(ctx, "ext/u1", dm.NL), *od = pool_vec<zd>(ctx, "ext/o", df.NL);
+        SFDX_CHECK(hipMemcpyAsync(u0d, u0, d0.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+        SFDX_CHECK(hipMemcpyAsync(u1d, u1, dm.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+        extint_device(ctx, F, M, u0d, u1d, od, "ext");
+        SFDX_CHECK(hipMemcpyAsync(out, od, df.NL * sizeof(zd), hipMemcpyDeviceToHost, ctx->st));
+        SFDX_CHECK(hipStreamSynchronize(ctx->st));
+        return SFDX_OK;
+    });
+}
+
+int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, int precision, int reps,
+                       double *ms)
+{
+    if (!ctx || !g || !sigma || !ms || reps <= 0) return SFDX_ERR_ARG;
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        HostLevel L = host_level(g, sigma);
+        const Dims d = make_dims(L.nx, L.ny, L.nz);
+        ensure_partials(ctx, edge_launch(d).nblocks() + 64);
+        hipEvent_t e0, e1;
+        SFDX_CHECK(hipEventCreate(&e0));
+        SFDX_CHECK(hipEventCreate(&e1));
+        auto run = [&](auto tagR) {
+            using R = decltype(tagR);
+            LevelDev<R> D = upload_level<R>(ctx, L, "bench");
+            const R wmu = (R)(2 * 3.141592653589793 * freq * kMu0);
+            std::vector<cx<R>> hx(d.NL);
+            for (int64_t e = 0; e < d.NL; e++) hx[e] = cx<R>((R)std::sin(0.001 * e), (R)std::cos(0.002 * e));
+            cx<R> *v[6];
+            for (int q = 0; q < 6; q++) {
+                v[q] = pool_vec<cx<R>>(ctx, "bench/v" + std::to_string(q) + (sizeof(R) == 8 ? "d" : "f"), d.NL);
+                SFDX_CHECK(hipMemcpyAsync(v[q], hx.data(), d.NL * sizeof(cx<R>), hipMemcpyHostToDevice, ctx->st));
+            }
+            float t;
+            // SpMV
+            for (int w = 0; w < 3; w++) launch_spmv<R>(D.g, wmu, v[0], v[1], (R)1, 0, nullptr, ctx->rb, ctx->st);
+            SFDX_CHECK(hipEventRecord(e0, ctx->st));
+            for (int q = 0; q < reps; q++) launch_spmv<R>(D.g, wmu, v[0], v[1], (R)1, 0, nullptr, ctx->rb, ctx->st);
+            SFDX_CHECK(hipEventRecord(e1, ctx->st));
+            SFDX_CHECK(hipEventSynchronize(e1));
+            SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
+            ms[0] = t / reps;
+            // SSOR
+            launch_dirichlet<R>(d, (const cx<R> *)nullptr, 2, v[0], 0, ctx->st);
+            SFDX_CHECK(hipEventRecord(e0, ctx->st));
+            for (int q = 0; q < reps; q++) launch_ssor<R>(D.g, wmu, v[0], v[2], (R)1, ctx->st);
+            SFDX_CHECK(hipEventRecord(e1, ctx->st));
+            SFDX_CHECK(hipEventSynchronize(e1));
+            SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
+            ms[1] = t / reps;
+            // COCR vector update
+            SFDX_CHECK(hipEventRecord(e0, ctx->st));
+            for (int q = 0; q < reps; q++) launch_vec<R>(0, D.g, ctx->rb, v[3], v[4], v[5], v[1], v[2], v[0], ctx->st);
+            SFDX_CHECK(hipEventRecord(e1, ctx->st));
+            SFDX_CHECK(hipEventSynchronize(e1));
+            SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
+            ms[2] = t / reps;
+        };
+        if (precision == SFDX_PREC_FP64) run(0.0); else run(0.0f);
+        ms[3] = ms[0] + ms[1] + ms[2];
+        SFDX_CHECK(hipEventDestroy(e0));
+        SFDX_CHECK(hipEventDestroy(e1));
+        return SFDX_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,51 @@
+// sfdx_host.h -- host-side FP64 helpers (1-D background, coarsening, EXCMG
+// weight tables, coarse direct solve, post-process).
+#pragma once
+#include <array>
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "sfdx_stencil.h"
+
+namespace sfdx {
+
+extern const double kMu0;
+
+struct HostLevel {
+    int nx = 0, ny = 0, nz = 0, nair = 0;
+    std::vector<double> a, b, c;  // widths
+    std::vector<double> sig;      // 6*ncell, sig[comp*ncell + cell]
+    size_t ncell() const { return (size_t)nx * ny * nz; }
+};
+
+struct ProlongHost {
+    std::vector<int> nb[3], eb[3];
+    std::vector<double> nw[3], ew[3];
+};
+
+struct CoarseSystem {
+    int n = 0;
+    std::vector<int64_t> inner;  // interior edge ids
+    std::vector<int> imap;       // edge -> interior row or -1
+    std::vector<int> ia, ja;     // upper-triangular CSR, 0-based
+    std::vector<std::complex<double>> val;
+    std::vector<std::complex<double>> rhs;  // 2*n: xy then yx
+};
+
+struct PostGrid {
+    int nx, ny, nz, nair;
+    const double *a, *b, *c;
+};
+
+void mt1d_anl(int nz, const double *conv, const double *c, double freq, std::complex<double> *ex,
+              std::complex<double> *hx, std::complex<double> *imp);
+std::vector<std::complex<double>> background_field(const HostLevel &L, double freq);
+HostLevel coarsen(const HostLevel &F, int k);
+void prolong_tables(const HostLevel &fine, int m, ProlongHost &T);
+void coarse_system(const HostLevel &L, double freq, const std::vector<std::complex<double>> &edc, CoarseSystem &S);
+bool direct_solve(const CoarseSystem &S, std::vector<std::complex<double>> &sol, std::string &err);
+int post_process(const PostGrid &g, int nf, const double *freq, int nrec, const double *recv,
+                 const std::complex<double> *ex, int out_type, int rd, double *results, double *tipper);
+
+}  // namespace sfdx

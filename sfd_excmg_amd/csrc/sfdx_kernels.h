@@ -1,0 +1,153 @@
+// sfdx_kernels.h -- launcher declarations and the device-resident Krylov scalars.
+#pragma once
+#include "sfdx_internal.h"
+
+namespace sfdx {
+
+// Device scalar slots (doubles).  Reductions land in SL_TMP.. and a scalar
+// step (executed by the last thread of the second-stage reduction) turns them
+// into the Krylov coefficients, so no host round trip is needed per step.
+enum Slot {
+    SL_RHO = 0,      // complex
+    SL_ALPHA = 2,    // complex
+    SL_BETA = 4,     // complex
+    SL_OMEGA = 6,    // complex
+    SL_TMP = 8,      // 8 doubles of reduction output
+    SL_RN = 16,      // |r_unscaled|^2
+    SL_PRHO = 18,    // PCG rho (real)
+    SL_PALPHA = 20,  // complex
+    SL_PBETA = 22,   // complex
+    SL_PR = 24,      // PCG |r|^2
+    SL_T = 26,       // |t|^2 of the DC right-hand side
+    SL_BREAK = 28,   // breakdown flag
+    SL_COUNT = 32
+};
+
+enum ScalarOp {
+    OP_NONE = 0,
+    OP_RN = 1,          // RN = tmp[0]
+    OP_COCR_RHO = 2,    // RHO = tmp[0..1]
+    OP_COCR_ALPHA = 3,  // ALPHA = RHO / tmp
+    OP_COCR_BETA = 4,   // BETA = tmp / RHO ; RHO = tmp
+    OP_BICG_RHO = 5,    // BETA = (tmp/RHO)(ALPHA/OMEGA); RHO = tmp
+    OP_BICG_ALPHA = 6,  // ALPHA = RHO / tmp
+    OP_BICG_OMEGA = 7,  // OMEGA = tmp[0..1] / tmp[2]
+    OP_PCG_INIT = 8,    // PRHO = tmp[0]
+    OP_PCG_ALPHA = 9,   // PALPHA = PRHO / tmp
+    OP_PCG_BETA = 10,   // PBETA = tmp[0]/PRHO ; PRHO = tmp[0] ; PR = tmp[1]
+    OP_T = 11,          // T = tmp[0]
+    OP_BICG_INIT = 12   // RHO = ALPHA = OMEGA = 1
+};
+
+__device__ __forceinline__ void cdiv(double ar, double ai, double br, double bi, double &cr, double &ci)
+{
+    double den = br * br + bi * bi;
+    cr = (ar * br + ai * bi) / den;
+    ci = (ai * br - ar * bi) / den;
+}
+
+__device__ __forceinline__ void scalar_step(double *s, int op)
+{
+    double *t = s + SL_TMP;
+    switch (op) {
+    case OP_RN: s[SL_RN] = t[0]; break;
+    case OP_COCR_RHO: s[SL_RHO] = t[0]; s[SL_RHO + 1] = t[1]; break;
+    case OP_COCR_ALPHA:
+        cdiv(s[SL_RHO], s[SL_RHO + 1], t[0], t[1], s[SL_ALPHA], s[SL_ALPHA + 1]);
+        if (t[0] == 0 && t[1] == 0) s[SL_BREAK] = 1;
+        break;
+    case OP_COCR_BETA:
+        cdiv(t[0], t[1], s[SL_RHO], s[SL_RHO + 1], s[SL_BETA], s[SL_BETA + 1]);
+        s[SL_RHO] = t[0]; s[SL_RHO + 1] = t[1];
+        break;
+    case OP_BICG_RHO: {
+        double qr, qi, ar, ai;
+        cdiv(t[0], t[1], s[SL_RHO], s[SL_RHO + 1], qr, qi);
+        cdiv(s[SL_ALPHA], s[SL_ALPHA + 1], s[SL_OMEGA], s[SL_OMEGA + 1], ar, ai);
+        s[SL_BETA] = qr * ar - qi * ai;
+        s[SL_BETA + 1] = qr * ai + qi * ar;
+        s[SL_RHO] = t[0]; s[SL_RHO + 1] = t[1];
+        if (t[0] == 0 && t[1] == 0) s[SL_BREAK] = 1;
+        break;
+    }
+    case OP_BICG_ALPHA:
+        cdiv(s[SL_RHO], s[SL_RHO + 1], t[0], t[1], s[SL_ALPHA], s[SL_ALPHA + 1]);
+        if (t[0] == 0 && t[1] == 0) s[SL_BREAK] = 1;
+        break;
+    case OP_BICG_OMEGA:
+        s[SL_OMEGA] = t[0] / t[2];
+        s[SL_OMEGA + 1] = t[1] / t[2];
+        if (t[2] == 0) s[SL_BREAK] = 1;
+        break;
+    case OP_PCG_INIT: s[SL_PRHO] = t[0]; break;
+    case OP_PCG_ALPHA:
+        cdiv(s[SL_PRHO], 0.0, t[0], t[1], s[SL_PALPHA], s[SL_PALPHA + 1]);
+        break;
+    case OP_PCG_BETA:
+        s[SL_PBETA] = t[0] / s[SL_PRHO];
+        s[SL_PBETA + 1] = 0;
+        s[SL_PRHO] = t[0];
+        s[SL_PR] = t[1];
+        break;
+    case OP_T: s[SL_T] = t[0]; break;
+    case OP_BICG_INIT:
+        s[SL_RHO] = 1; s[SL_RHO + 1] = 0;
+        s[SL_ALPHA] = 1; s[SL_ALPHA + 1] = 0;
+        s[SL_OMEGA] = 1; s[SL_OMEGA + 1] = 0;
+        s[SL_BREAK] = 0;
+        break;
+    default: break;
+    }
+}
+
+// Prolongation weight tables on the device (per dimension).
+struct ProlongDev {
+    int *nb[3] = {nullptr, nullptr, nullptr};
+    double *nw[3] = {nullptr, nullptr, nullptr};
+    int *eb[3] = {nullptr, nullptr, nullptr};
+    double *ew[3] = {nullptr, nullptr, nullptr};
+};
+
+void reduce_final(RedBuf &rb, int64_t nblk, int K, int dst, int op, hipStream_t st);
+
+template <typename R>
+void launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode, const cx<R> *w,
+                 RedBuf &rb, hipStream_t st);
+template <typename R>
+void launch_mass(const DevGrid<R> &g, const cx<R> *x, cx<R> *w, hipStream_t st);
+template <typename R>
+void launch_ssor(const DevGrid<R> &g, R wmu, const cx<R> *in, cx<R> *out, R omega, hipStream_t st);
+template <typename R>
+void launch_vec(int mode, const DevGrid<R> &g, RedBuf &rb, cx<R> *v0, cx<R> *v1, cx<R> *v2, cx<R> *v3, cx<R> *v4,
+                cx<R> *v5, hipStream_t st);
+template <typename R>
+void launch_dot(int mode, int64_t n, const cx<R> *u, const cx<R> *v, RedBuf &rb, hipStream_t st, int64_t &nblk);
+template <typename R>
+void launch_copy(int64_t n, const cx<R> *s, cx<R> *t, hipStream_t st);
+template <typename RS, typename RD>
+void launch_convert(int64_t n, const cx<RS> *s, cx<RD> *t, hipStream_t st);
+void launch_add_lo(int64_t n, zd *x, const zf *d, hipStream_t st);
+template <typename R>
+void launch_axpby(int64_t n, cx<R> al, const cx<R> *u, cx<R> be, cx<R> *v, hipStream_t st);
+template <typename R>
+void launch_dirichlet(const Dims &d, const cx<R> *edc, int pol, cx<R> *x, int zero_inner, hipStream_t st);
+template <typename R>
+void launch_node_div(const DevGrid<R> &g, const cx<R> *w, cx<R> *t, RedBuf &rb, hipStream_t st, int64_t &nblk);
+template <typename R>
+void launch_pbuild(const Dims &d, const double *a, const double *b, const double *c, const double *sig, R *P,
+                   hipStream_t st);
+template <typename R>
+void launch_papply(const Dims &d, const R *P, const cx<R> *x, cx<R> *y, int dot, RedBuf &rb, hipStream_t st,
+                   int64_t &nblk);
+template <typename R>
+void launch_pcg(int mode, int64_t np, const R *Pdiag, RedBuf &rb, cx<R> *u, cx<R> *r, cx<R> *p, const cx<R> *v,
+                hipStream_t st, int64_t &nblk);
+template <typename R>
+void launch_grad_update(const DevGrid<R> &g, const cx<R> *phi, cx<R> *x, hipStream_t st);
+template <typename R>
+void launch_prolong(const Dims &df, const Dims &dc, int m, const ProlongDev &T, const cx<R> *uc, cx<R> *uf,
+                    hipStream_t st);
+template <typename R>
+void launch_extrap(int64_t n, R c1, const cx<R> *u1, R c2, cx<R> *w, hipStream_t st);
+
+}  // namespace sfdx

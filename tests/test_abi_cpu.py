@@ -1,0 +1,82 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+declared in include/sfdx.h, and its host-side FP64 parts (1-D background,
+post-process, coarse direct solve) agree with the oracle."""
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import oracle as O
+from sfd_excmg_amd import sfdx, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "sfdx.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sfdx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = sfdx.load()
+    names = header_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(sfdx.EXPORTS)
+
+
+def test_default_options_match_reference_parameters():
+    o = sfdx.default_opts()
+    assert o.corr_intv == 200 and o.corr_tol == 1e-3 and o.ssor_omega == 1.0   # global_para.f90:9-10, solvers.f90:14
+    assert o.maxit_plain == 5000 and o.maxit_excmg == 1000                    # fdfe_main.f90:993,1414
+
+
+@pytest.mark.parametrize("aniso", [False, True])
+def test_mt1d_matches_oracle(aniso):
+    rng = np.random.default_rng(4)
+    nz = 30
+    c = rng.uniform(20, 800, nz)
+    conv = np.zeros((nz, 6))
+    conv[:, :3] = 10 ** rng.uniform(-3, -1, (nz, 3))
+    if aniso:
+        conv[:, 3:] = 0.1 * conv[:, :1] * rng.uniform(-0.5, 0.5, (nz, 3))
+    for f in (1e-3, 0.1, 10.0):
+        e1, h1, i1 = sfdx.mt1d_anl(f, conv, c)
+        e2, h2, i2 = O.mt1d_anl(f, conv, c)
+        assert np.allclose(e1, e2, rtol=1e-11, atol=1e-14 * abs(e2).max())
+        assert np.allclose(h1, h2, rtol=1e-11, atol=1e-14 * abs(h2).max())
+        assert np.allclose(i1, i2, rtol=1e-10)
+
+
+@pytest.mark.parametrize("rd,out_type", [(1, 1), (2, 1), (2, 2)])
+def test_post_process_matches_oracle(rd, out_type):
+    m = synth.make_c1()
+    rng = np.random.default_rng(0)
+    ex = rng.standard_normal((2 * m.nl, 2)) + 1j * rng.standard_normal((2 * m.nl, 2))
+    freqs = np.array([1.0, 0.1])
+    r1, t1 = sfdx.post_process(m.a, m.b, m.c, m.nair, freqs, m.recv, ex, out_type=out_type, rd=rd)
+    r2, t2 = O.post_process(m.a, m.b, m.c, m.nair, freqs, m.recv, ex, out_type=out_type, rd=rd)
+    assert np.allclose(r1, r2, rtol=1e-11, atol=1e-12 * abs(r2).max())
+    assert np.allclose(t1, t2, rtol=1e-11, atol=1e-12 * abs(t2).max())
+
+
+def test_direct_solve_matches_superlu():
+    """PARDISO (complex symmetric) on the dual-volume scaled system against
+    SuperLU on the reference's unscaled CSR; pinned by the interior residual."""
+    m = synth.make_c1()
+    a1, b1, c1, s1 = O.mesh_coarsen(m.a, m.b, m.c, m.sigma, 2)
+    try:
+        u = sfdx.direct_solve(a1, b1, c1, s1, 1.0)
+    except sfdx.SfdxError as e:
+        pytest.skip(f"MKL PARDISO unavailable: {e}")
+    S = O.build_system(a1, b1, c1, s1, 1.0)
+    nl = len(S.p1)
+    for pol, rhs in ((0, S.p1), (1, S.p2)):
+        x = u[pol * nl:(pol + 1) * nl]
+        r = rhs - S.A @ x
+        assert np.linalg.norm(r[~S.bnd]) <= 1e-10 * np.linalg.norm(rhs[~S.bnd])
+        assert np.allclose(x[S.bnd], rhs[S.bnd])

@@ -1,0 +1,172 @@
+"""GPU parity of the HIP path against the FP64 oracle, through the C ABI.
+
+Tolerances: FP64 kernels 1e-12 relative (stencil/SSOR/prolongation are the
+same arithmetic in a different order); complex64 kernels 1e-5 relative;
+impedances of full solves 1e-6 relative at an interior residual of 1e-10.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as O
+from sfd_excmg_amd import sfdx, synth
+from test_oracle import tiny_model
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = sfdx.Context(0)
+    yield c
+    c.close()
+
+
+def _rand(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("aniso", [False, True])
+def test_operator_fp64_matches_oracle_csr(ctx, aniso):
+    a, b, c, nair, sig = tiny_model(9, 7, 8, 2, aniso=aniso)
+    f = 0.37
+    S = O.build_system(a, b, c, sig, f)
+    ve = O.row_scale(a, b, c, S.bnd)
+    x = _rand(len(S.p1), 1)
+    x[S.bnd] = 0
+    y = ctx.apply_operator(a, b, c, sig, f, x)
+    yo = ve * (S.A @ x)
+    yo[S.bnd] = 0
+    assert np.abs(y - yo).max() <= 1e-12 * np.abs(yo).max()
+    # boundary coupling: A_s applied to the Dirichlet data alone is -ve * rhs
+    g = np.where(S.bnd, S.p1, 0)
+    y2 = ctx.apply_operator(a, b, c, sig, f, g)
+    ref = -ve * S.p1
+    ref[S.bnd] = 0
+    assert np.abs(y2 - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_operator_fp32_matches_fp64(ctx):
+    m = synth.make_c1()
+    x = _rand(m.nl, 2)
+    y64 = ctx.apply_operator(m.a, m.b, m.c, m.sigma, 1.0, x)
+    y32 = ctx.apply_operator(m.a, m.b, m.c, m.sigma, 1.0, x, precision=sfdx.PREC_MIXED)
+    assert np.abs(y32 - y64).max() <= 1e-5 * np.abs(y64).max()
+
+
+def _colour_order(nx, ny, nz, bnd):
+    """Interior edges sorted by SSOR colour (orientation x parity)."""
+    xnl = nx * (ny + 1) * (nz + 1)
+    ynl = (nx + 1) * ny * (nz + 1)
+    cols = []
+    for e in range(len(bnd)):
+        if e < xnl:
+            i = e % nx; j = (e // nx) % (ny + 1); k = e // (nx * (ny + 1)); col = (j + k) & 1
+        elif e < xnl + ynl:
+            q = e - xnl; i = q % (nx + 1); j = (q // (nx + 1)) % ny; k = q // ((nx + 1) * ny); col = 2 + ((i + k) & 1)
+        else:
+            q = e - xnl - ynl; i = q % (nx + 1); j = (q // (nx + 1)) % (ny + 1); col = 4 + ((i + j) & 1)
+        cols.append(col)
+    cols = np.array(cols)
+    idx = np.arange(len(bnd))[~bnd]
+    return idx[np.argsort(cols[~bnd], kind="stable")], cols
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.2])
+def test_multicolour_ssor_matches_dense(ctx, omega):
+    a, b, c, nair, sig = tiny_model(5, 4, 5, 1, aniso=True)
+    f = 1.3
+    S = O.build_system(a, b, c, sig, f)
+    ve = O.row_scale(a, b, c, S.bnd)
+    As = (sp.diags(ve) @ S.A).toarray()
+    order, cols = _colour_order(len(a), len(b), len(c), S.bnd)
+    Ap = As[np.ix_(order, order)]
+    cp = cols[order]
+    D = np.diag(np.diag(Ap))
+    Lw = np.where(cp[:, None] > cp[None, :], Ap, 0)
+    Uw = np.where(cp[:, None] < cp[None, :], Ap, 0)
+    # within one colour there is no coupling
+    same = (cp[:, None] == cp[None, :]) & ~np.eye(len(cp), dtype=bool)
+    assert np.abs(Ap[same]).max() == 0
+    M = (D / omega + Lw) @ np.linalg.inv(D) @ (D / omega + Uw) * (omega / (2 - omega))
+    x = _rand(len(S.p1), 3)
+    x[S.bnd] = 0
+    y = ctx.ssor_apply(a, b, c, sig, f, x, omega=omega)
+    assert np.all(y[S.bnd] == 0)
+    yo = np.linalg.solve(M, x[order])
+    assert np.abs(y[order] - yo).max() <= 1e-11 * np.abs(yo).max()
+
+
+def test_extint_v_matches_oracle(ctx):
+    rng = np.random.default_rng(5)
+    nx, ny, nz, nair = 16, 24, 16, 4
+    a, b, c = rng.uniform(50, 150, nx), rng.uniform(50, 150, ny), rng.uniform(50, 150, nz)
+    u0 = _rand(O.nedges(nx // 4, ny // 4, nz // 4), 6)
+    u1 = _rand(O.nedges(nx // 2, ny // 2, nz // 2), 7)
+    out = ctx.extint_v(a, b, c, nair, u0, u1)
+    ref = O.extint_v(a, b, c, nair, u0, u1)
+    assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("aniso", [False, True])
+def test_divergence_correction_matches_oracle(ctx, aniso):
+    a, b, c, nair, sig = tiny_model(8, 7, 9, 2, aniso=aniso)
+    x = _rand(O.nedges(len(a), len(b), len(c)), 8)
+    xg, itg = ctx.div_correct(a, b, c, sig, x, corr_tol=1e-11)
+    xo, ito = O.div_correct(a, b, c, sig, x, tol=1e-11)
+    assert itg > 0
+    assert np.abs(xg - xo).max() <= 1e-8 * np.abs(xo).max()
+
+
+def _impedances(m, ex, rd=2):
+    res, _ = O.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, out_type=1, rd=rd)
+    return res[:, 0] + 1j * res[:, 1], res[:, 2] + 1j * res[:, 3]
+
+
+def small_excmg_model(aniso=False):
+    m = synth.make_iso_block_model("s16", 16, 16, 12, 4, seed=11, nblocks=3, freq=1.0, level=3, nstations=6)
+    if aniso:
+        rng = np.random.default_rng(3)
+        e = slice(m.nair * m.nx * m.ny, None)
+        n = m.sigma[e].shape[0]
+        m.sigma[e, 3:] = 0.2 * m.sigma[e, :1] * rng.uniform(-1, 1, (n, 3))
+    return m
+
+
+@pytest.mark.parametrize("solver", [sfdx.SOLVER_BICGSTAB, sfdx.SOLVER_COCR])
+@pytest.mark.parametrize("aniso", [False, True])
+def test_fwd_solve_fp64_matches_oracle(ctx, solver, aniso):
+    m = small_excmg_model(aniso)
+    opts = sfdx.default_opts(level=3, solver=solver, eps=1e-10, corr_intv=50, norm=sfdx.NORM_INTERIOR)
+    ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
+    assert st.status == 0
+    oopt = O.Options(solver=O.SOLVER_COCR, corr_intv=50, norm="interior")
+    exo, _ = O.fwd_solver(m.a, m.b, m.c, m.nair, 3, 1e-11, m.sigma, m.freqs, oopt)
+    z1 = _impedances(m, ex)
+    z2 = _impedances(m, exo)
+    for p, q in zip(z1, z2):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+
+
+def test_fwd_solve_mixed_matches_fp64(ctx):
+    m = small_excmg_model(True)
+    o64 = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=50, norm=sfdx.NORM_INTERIOR)
+    omx = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=50, norm=sfdx.NORM_INTERIOR,
+                            precision=sfdx.PREC_MIXED)
+    ex64, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, o64)
+    exmx, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, omx)
+    assert st.status == 0
+    assert st.refine_steps[0] >= 1
+    for p, q in zip(_impedances(m, exmx), _impedances(m, ex64)):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+
+
+def test_halfspace_on_gpu(ctx):
+    m = synth.make_c1()
+    m.sigma[m.nair * m.nx * m.ny:, :3] = 0.01
+    opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=200, norm=sfdx.NORM_INTERIOR)
+    ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
+    res, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, out_type=2)
+    assert np.allclose(res[:, :2], 100.0, rtol=0.02)
+    assert np.allclose(res[:, 2:], 45.0, atol=0.5)
