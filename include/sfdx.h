@@ -41,7 +41,12 @@ extern "C" {
 #define SFDX_SOLVER_COCR 5      /* COCR (Sogabe & Zhang 2007) on the symmetric form */
 
 #define SFDX_PREC_FP64 0
-#define SFDX_PREC_MIXED 1       /* complex64 inner solve, FP64 refinement */
+#define SFDX_PREC_MIXED 1       /* FP64 Krylov and operator, complex64 preconditioner */
+#define SFDX_PREC_MIXED_IR 2    /* complex64 inner Krylov solves inside FP64 refinement */
+
+#define SFDX_PRECOND_SSOR 0
+#define SFDX_PRECOND_HYBRID 1  /* SSOR + G (G^T A G)^-1 G^T, G^T A G = -i w mu P */
+#define SFDX_PRECOND_MG 2      /* V-cycle over the EXCMG grid hierarchy, hybrid smoothing */
 
 #define SFDX_NORM_REFERENCE 0   /* ||b-Ax||/||b||, b incl. Dirichlet rows (solvers.f90:615-625) */
 #define SFDX_NORM_INTERIOR 1    /* interior rows only */
@@ -71,6 +76,16 @@ typedef struct {
     double inner_tol;   /* mixed precision: relative tolerance of each inner solve */
     int max_refine;     /* mixed precision: maximum refinement steps */
     int verbose;
+    int dc_restart;     /* 1: restart the Krylov recurrences after each correction;
+                           0: recompute r and keep the search directions (the
+                           reference keeps p, v, rho, alpha, omega: solvers.f90:630-646) */
+    int dc_at_start;    /* correct the (prolongated) initial guess before iterating */
+    int inner_maxit;    /* mixed precision: iteration cap of one inner solve */
+    int precond;        /* SFDX_PRECOND_*: 0 multicolour SSOR (the reference's SSOR,
+                           re-ordered), 1 SSOR + auxiliary nodal correction */
+    double sigma_floor32; /* complex64 edge smoother only: diagonal conductivities
+                             below this floor (S/m) are raised to it, bounding the
+                             amplification of near-null air modes; 0 = off */
 } sfdx_opts;
 
 typedef struct {

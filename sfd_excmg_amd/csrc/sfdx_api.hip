@@ -95,12 +95,14 @@ struct LevelDev {
 };
 
 template <typename R>
-LevelDev<R> upload_level(sfdx_ctx *ctx, const HostLevel &L, const std::string &tag)
+LevelDev<R> upload_level(sfdx_ctx *ctx, const HostLevel &L, const std::string &tag, double sigma_floor = 0)
 {
     LevelDev<R> D;
     D.g.d = make_dims(L.nx, L.ny, L.nz);
     std::vector<R> a(L.a.begin(), L.a.end()), b(L.b.begin(), L.b.end()), c(L.c.begin(), L.c.end()),
         s(L.sig.begin(), L.sig.end());
+    if (sigma_floor > 0)
+        for (size_t q = 0; q < 3 * L.ncell(); q++) s[q] = std::max(s[q], (R)sigma_floor);
     const std::string p = tag + (sizeof(R) == 8 ? "/d" : "/f");
     D.g.a = pool_vec<R>(ctx, p + "a", a.size());
     D.g.b = pool_vec<R>(ctx, p + "b", b.size());
@@ -176,14 +178,141 @@ int div_correct(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, cx<R> *x,
     return it;
 }
 
+// ---------------------------------------------------------------- multigrid
+// Geometric V-cycle over the EXCMG grid hierarchy (levels 1..L, each a 2x
+// coarsening of the next), used as the preconditioner of the Krylov solve:
+// forward hybrid Gauss-Seidel pre-smoothing, residual restriction (P^T),
+// recursive coarse correction, prolongation, backward post-smoothing (the
+// transpose of the pre-smoother, so the cycle is symmetric), and a fixed number
+// of smoothing pairs on the coarsest level.  Coarse operators are the same
+// stencil on the volume-averaged conductivity of the coarse grid.
+template <typename R>
+struct MGHier {
+    std::vector<LevelDev<R> *> lev;           // index = level (1-based), lev[0] unused
+    std::vector<std::array<double *, 3>> wl;  // wl[l]: node weights of level l vs l-1
+    int coarse_pairs = 4;
+};
+
+std::vector<double> mg_weights_1d(const std::vector<double> &wf, const std::vector<double> &wc)
+{
+    std::vector<double> xf(wf.size() + 1, 0.0), xc(wc.size() + 1, 0.0), w(wf.size() + 1, 1.0);
+    for (size_t i = 0; i < wf.size(); i++) xf[i + 1] = xf[i] + wf[i];
+    for (size_t i = 0; i < wc.size(); i++) xc[i + 1] = xc[i] + wc[i];
+    for (size_t q = 1; q < xf.size(); q += 2) {
+        const size_t lo = q / 2, hi = lo + 1;
+        w[q] = (xc[hi] - xf[q]) / (xc[hi] - xc[lo]);
+    }
+    return w;
+}
+
+template <typename R>
+void mg_vcycle(sfdx_ctx *ctx, MGHier<R> &H, int l, R wmu, const cx<R> *r, cx<R> *z)
+{
+    LevelDev<R> &D = *H.lev[l];
+    const Dims &d = D.g.d;
+    const std::string tg = "mg" + std::to_string(l) + (sizeof(R) == 8 ? "d" : "f");
+    cx<R> *t = pool_vec<cx<R>>(ctx, tg + "t", d.NL), *w = pool_vec<cx<R>>(ctx, tg + "w", d.NL);
+    cx<R> *tn = pool_vec<cx<R>>(ctx, tg + "tn", d.NP), *ph = pool_vec<cx<R>>(ctx, tg + "ph", d.NP);
+    ensure_P(ctx, D, "L" + std::to_string(l));
+    if (l == 1) {
+        launch_hybrid_gs<R>(D.g, wmu, D.P, r, z, tn, ph, 0, ctx->st);
+        for (int q = 0; q < H.coarse_pairs; q++) {
+            if (q > 0) {
+                launch_spmv<R>(D.g, wmu, z, t, (R)-1, 4, r, ctx->rb, ctx->st);
+                launch_hybrid_gs<R>(D.g, wmu, D.P, t, w, tn, ph, 0, ctx->st);
+                launch_axpby<R>(d.NL, cx<R>(1), w, cx<R>(1), z, ctx->st);
+            }
+            launch_spmv<R>(D.g, wmu, z, t, (R)-1, 4, r, ctx->rb, ctx->st);
+            launch_hybrid_gs<R>(D.g, wmu, D.P, t, w, tn, ph, 1, ctx->st);
+            launch_axpby<R>(d.NL, cx<R>(1), w, cx<R>(1), z, ctx->st);
+        }
+        return;
+    }
+    LevelDev<R> &C = *H.lev[l - 1];
+    const std::string tc = "mg" + std::to_string(l - 1) + (sizeof(R) == 8 ? "d" : "f");
+    cx<R> *rc = pool_vec<cx<R>>(ctx, tc + "r", C.g.d.NL), *zc = pool_vec<cx<R>>(ctx, tc + "z", C.g.d.NL);
+    launch_hybrid_gs<R>(D.g, wmu, D.P, r, z, tn, ph, 0, ctx->st);
+    launch_spmv<R>(D.g, wmu, z, t, (R)-1, 4, r, ctx->rb, ctx->st);
+    launch_restrict<R>(d, C.g.d, H.wl[l].data(), t, rc, ctx->st);
+    mg_vcycle<R>(ctx, H, l - 1, wmu, rc, zc);
+    launch_prolong_add<R>(d, C.g.d, H.wl[l].data(), zc, z, ctx->st);
+    launch_spmv<R>(D.g, wmu, z, t, (R)-1, 4, r, ctx->rb, ctx->st);
+    launch_hybrid_gs<R>(D.g, wmu, D.P, t, w, tn, ph, 1, ctx->st);
+    launch_axpby<R>(d.NL, cx<R>(1), w, cx<R>(1), z, ctx->st);
+}
+
 // ---------------------------------------------------------------- Krylov
+struct KOpts {
+    int corr_intv = 0;
+    double corr_tol = 1e-3;
+    double omega = 1.0;
+    int verbose = 0;
+    int dc_restart = 0;
+    int dc_at_start = 1;
+    int precond = SFDX_PRECOND_HYBRID;
+    LevelDev<float> *lo = nullptr;  // FP64 Krylov with an FP32 preconditioner when set
+    MGHier<double> *mg64 = nullptr;  // multigrid hierarchy (precond == MG)
+    MGHier<float> *mg32 = nullptr;
+    int mg_level = 0;                // level being solved
+};
+
+// out = M^-1 in with the chosen preconditioner
+template <typename R>
+void apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *in, cx<R> *out,
+                const KOpts &ko)
+{
+    if (ko.precond == SFDX_PRECOND_MG) {
+        if (sizeof(R) == 8 && ko.mg32) {
+            // FP64 Krylov, complex64 V-cycle
+            const Dims &d = D.g.d;
+            zf *in32 = pool_vec<zf>(ctx, tag + "/mg_in", d.NL), *z32 = pool_vec<zf>(ctx, tag + "/mg_z", d.NL);
+            launch_convert<double, float>(d.NL, (const zd *)in, in32, ctx->st);
+            mg_vcycle<float>(ctx, *ko.mg32, ko.mg_level, (float)wmu, in32, z32);
+            launch_convert<float, double>(d.NL, z32, (zd *)out, ctx->st);
+        } else if (sizeof(R) == 8) {
+            mg_vcycle<double>(ctx, *ko.mg64, ko.mg_level, (double)wmu, (const zd *)in, (zd *)out);
+        } else {
+            mg_vcycle<float>(ctx, *ko.mg32, ko.mg_level, (float)wmu, (const zf *)in, (zf *)out);
+        }
+        return;
+    }
+    if (sizeof(R) == 8 && ko.lo) {
+        LevelDev<float> &F = *ko.lo;
+        ensure_P(ctx, F, tag);
+        const Dims &d = F.g.d;
+        zf *in32 = pool_vec<zf>(ctx, tag + "/mp_in", d.NL), *z32 = pool_vec<zf>(ctx, tag + "/mp_z", d.NL);
+        zf *tn = pool_vec<zf>(ctx, tag + "/mp_t", d.NP), *ph = pool_vec<zf>(ctx, tag + "/mp_phi", d.NP);
+        if (ko.precond == SFDX_PRECOND_HYBRID) {
+            launch_hybrid_mixed(*(DevGrid<double> *)&D.g, F.g, (double)wmu, F.P, (const zd *)in, (zd *)out, in32, z32,
+                                tn, ph, (float)ko.omega, ctx->st);
+        } else {
+            launch_convert<double, float>(d.NL, (const zd *)in, in32, ctx->st);
+            launch_ssor<float>(F.g, (float)wmu, in32, z32, (float)ko.omega, ctx->st);
+            launch_convert<float, double>(d.NL, z32, (zd *)out, ctx->st);
+        }
+        return;
+    }
+    if (ko.precond == SFDX_PRECOND_HYBRID) {
+        ensure_P(ctx, D, tag);
+        const std::string p = tag + (sizeof(R) == 8 ? "/hy64" : "/hy32");
+        cx<R> *tn = pool_vec<cx<R>>(ctx, p + "t", D.g.d.NP), *ph = pool_vec<cx<R>>(ctx, p + "phi", D.g.d.NP);
+        launch_hybrid<R>(D.g, wmu, D.P, in, out, tn, ph, (R)ko.omega, ctx->st);
+    } else {
+        launch_ssor<R>(D.g, wmu, in, out, (R)ko.omega, ctx->st);
+    }
+}
+
 // Solve A_s x = f on the interior edges (boundary entries of x held fixed; f may
-// be null = 0) to ||r/ve|| <= eps * bnorm.
+// be null = 0) to ||r/ve|| <= eps * bnorm.  COCR (Sogabe & Zhang 2007) with the
+// symmetric multicolour SSOR preconditioner; unconjugated inner products.
+// After a divergence correction either the recurrences restart (dc_restart) or
+// r, z = M^-1 r and A z are recomputed and the search directions are kept.
 template <typename R>
 SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *f, cx<R> *x, double bnorm,
-               double eps, int maxit, int corr_intv, double corr_tol, R omega, int verbose)
+               double eps, int maxit, const KOpts &ko)
 {
     const Dims &d = D.g.d;
+    const R omega = (R)ko.omega;
     const std::string p = tag + (sizeof(R) == 8 ? "/k64" : "/k32");
     cx<R> *r = pool_vec<cx<R>>(ctx, p + "r", d.NL), *z = pool_vec<cx<R>>(ctx, p + "z", d.NL);
     cx<R> *Az = pool_vec<cx<R>>(ctx, p + "Az", d.NL), *pv = pool_vec<cx<R>>(ctx, p + "p", d.NL);
@@ -193,40 +322,50 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     const double sc = 2.0 * sizeof(R), vecb = (double)d.NL * sc;
     const double b_spmv = 2 * vecb + (double)d.NC * 6 * sizeof(R), b_ssor = 2 * (b_spmv + vecb);
     SolveInfo info;
-    bool restart = true;
+    if (ko.dc_at_start) {
+        info.pcg_iters += div_correct<R>(ctx, D, tag, x, ko.corr_tol);
+        info.dc_calls++;
+    }
+    // 0: fresh start, 1: soft restart after DC, 2: iterate
+    int state = 0;
     while (true) {
-        if (restart) {
+        if (state != 2) {
             launch_spmv<R>(D.g, wmu, x, r, (R)-1, 0, nullptr, ctx->rb, ctx->st);
             if (f) launch_axpby<R>(d.NL, cx<R>(1), f, cx<R>(1), r, ctx->st);
-            launch_ssor<R>(D.g, wmu, r, z, omega, ctx->st);
+            apply_prec<R>(ctx, D, tag, wmu, r, z, ko);
             launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
-            reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_RHO, ctx->st);
-            launch_copy<R>(d.NL, z, pv, ctx->st);
-            launch_copy<R>(d.NL, Az, Ap, ctx->st);
+            if (state == 0) {
+                reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_RHO, ctx->st);
+                launch_copy<R>(d.NL, z, pv, ctx->st);
+                launch_copy<R>(d.NL, Az, Ap, ctx->st);
+            } else {
+                reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_BETA, ctx->st);
+                launch_vec<R>(3, D.g, ctx->rb, pv, z, Ap, Az, nullptr, nullptr, ctx->st);
+            }
             launch_vec<R>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
             reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
             read_slots(ctx);
             info.bytes += 2 * b_spmv + b_ssor + 6 * vecb;
-            restart = false;
+            state = 2;
         }
         info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
         if (!(info.relres == info.relres) || std::isinf(info.relres)) { info.status = SFDX_ERR_NAN; break; }
-        if (verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  cocr it %d relres %.3e\n", info.iters, info.relres);
+        if (ko.verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  cocr it %d relres %.3e\n", info.iters, info.relres);
         if (info.relres < eps) { info.status = 0; break; }
         if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
         if (ctx->hslots[SL_BREAK] != 0) { info.status = SFDX_ERR_BREAKDOWN; break; }
         int64_t nbd;
-        launch_ssor<R>(D.g, wmu, Ap, q, omega, ctx->st);
+        apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko);
         launch_dot<R>(0, d.NL, Ap, q, ctx->rb, ctx->st, nbd);
         reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
         launch_vec<R>(0, D.g, ctx->rb, x, pv, Ap, r, q, z, ctx->st);
         reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
         info.iters++;
         info.bytes += b_ssor + 2 * vecb + 9 * vecb;
-        if (corr_intv > 0 && info.iters % corr_intv == 0) {
-            info.pcg_iters += div_correct<R>(ctx, D, tag, x, corr_tol);
+        if (ko.corr_intv > 0 && info.iters % ko.corr_intv == 0) {
+            info.pcg_iters += div_correct<R>(ctx, D, tag, x, ko.corr_tol);
             info.dc_calls++;
-            restart = true;
+            state = ko.dc_restart ? 0 : 1;
             continue;
         }
         launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
@@ -239,11 +378,15 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
 }
 
 // Right-preconditioned BiCGStab (Bicg_stab_p, src/solvers.f90:522-719) on A_s.
+// Divergence correction every corr_intv iterations followed by r = f - Ax,
+// rhat = r; with dc_restart = 0 rho, alpha, omega, p, v carry over as in the
+// reference (src/solvers.f90:630-646).
 template <typename R>
 SolveInfo bicgstab(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *f, cx<R> *x,
-                   double bnorm, double eps, int maxit, int corr_intv, double corr_tol, R omega, int verbose)
+                   double bnorm, double eps, int maxit, const KOpts &ko)
 {
     const Dims &d = D.g.d;
+    const R omega = (R)ko.omega;
     const std::string p = tag + (sizeof(R) == 8 ? "/b64" : "/b32");
     cx<R> *r = pool_vec<cx<R>>(ctx, p + "r", d.NL), *rh = pool_vec<cx<R>>(ctx, p + "rh", d.NL);
     cx<R> *pv = pool_vec<cx<R>>(ctx, p + "p", d.NL), *y = pool_vec<cx<R>>(ctx, p + "y", d.NL);
@@ -254,41 +397,48 @@ SolveInfo bicgstab(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu,
     const double vecb = (double)d.NL * 2.0 * sizeof(R);
     const double b_spmv = 2 * vecb + (double)d.NC * 6 * sizeof(R), b_ssor = 2 * (b_spmv + vecb);
     SolveInfo info;
-    bool restart = true;
+    if (ko.dc_at_start) {
+        info.pcg_iters += div_correct<R>(ctx, D, tag, x, ko.corr_tol);
+        info.dc_calls++;
+    }
+    int state = 0;  // 0 fresh, 1 residual refresh keeping p, v and the scalars, 2 iterate
     while (true) {
-        if (restart) {
+        if (state != 2) {
             launch_spmv<R>(D.g, wmu, x, r, (R)-1, 0, nullptr, ctx->rb, ctx->st);
             if (f) launch_axpby<R>(d.NL, cx<R>(1), f, cx<R>(1), r, ctx->st);
             launch_copy<R>(d.NL, r, rh, ctx->st);
-            SFDX_CHECK(hipMemsetAsync(pv, 0, d.NL * sizeof(cx<R>), ctx->st));
-            SFDX_CHECK(hipMemsetAsync(v, 0, d.NL * sizeof(cx<R>), ctx->st));
-            reduce_final(ctx->rb, 0, 0, SL_TMP, OP_BICG_INIT, ctx->st);
+            if (state == 0) {
+                SFDX_CHECK(hipMemsetAsync(pv, 0, d.NL * sizeof(cx<R>), ctx->st));
+                SFDX_CHECK(hipMemsetAsync(v, 0, d.NL * sizeof(cx<R>), ctx->st));
+                reduce_final(ctx->rb, 0, 0, SL_TMP, OP_BICG_INIT, ctx->st);
+            }
             launch_vec<R>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
             reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
             read_slots(ctx);
-            restart = false;
+            state = 2;
         }
         info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
         if (!(info.relres == info.relres) || std::isinf(info.relres)) { info.status = SFDX_ERR_NAN; break; }
-        if (verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  bicgstab it %d relres %.3e\n", info.iters, info.relres);
+        if (ko.verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  bicgstab it %d relres %.3e\n", info.iters, info.relres);
         if (info.relres < eps) { info.status = 0; break; }
         if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
         if (ctx->hslots[SL_BREAK] != 0) { info.status = SFDX_ERR_BREAKDOWN; break; }
-        if (corr_intv > 0 && info.iters > 0 && info.iters % corr_intv == 0 && info.dc_calls < info.iters / corr_intv) {
-            info.pcg_iters += div_correct<R>(ctx, D, tag, x, corr_tol);
+        if (ko.corr_intv > 0 && info.iters > 0 && info.iters % ko.corr_intv == 0 &&
+            info.dc_calls < info.iters / ko.corr_intv + (ko.dc_at_start ? 1 : 0)) {
+            info.pcg_iters += div_correct<R>(ctx, D, tag, x, ko.corr_tol);
             info.dc_calls++;
-            restart = true;
+            state = ko.dc_restart ? 0 : 1;
             continue;
         }
         int64_t nbd;
         launch_dot<R>(1, d.NL, rh, r, ctx->rb, ctx->st, nbd);
         reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_BICG_RHO, ctx->st);
         launch_vec<R>(4, D.g, ctx->rb, pv, r, v, nullptr, nullptr, nullptr, ctx->st);
-        launch_ssor<R>(D.g, wmu, pv, y, omega, ctx->st);
+        apply_prec<R>(ctx, D, tag, wmu, pv, y, ko);
         launch_spmv<R>(D.g, wmu, y, v, (R)1, 2, rh, ctx->rb, ctx->st);
         reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_BICG_ALPHA, ctx->st);
         launch_vec<R>(1, D.g, ctx->rb, s, r, v, nullptr, nullptr, nullptr, ctx->st);
-        launch_ssor<R>(D.g, wmu, s, z, omega, ctx->st);
+        apply_prec<R>(ctx, D, tag, wmu, s, z, ko);
         launch_spmv<R>(D.g, wmu, z, t, (R)1, 3, s, ctx->rb, ctx->st);
         reduce_final(ctx->rb, nbe, 3, SL_TMP, OP_BICG_OMEGA, ctx->st);
         launch_vec<R>(2, D.g, ctx->rb, x, y, z, r, s, t, ctx->st);
@@ -302,11 +452,23 @@ SolveInfo bicgstab(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu,
 
 template <typename R>
 SolveInfo krylov(sfdx_ctx *ctx, int solver, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *f, cx<R> *x,
-                 double bnorm, double eps, int maxit, int corr_intv, double corr_tol, R omega, int verbose)
+                 double bnorm, double eps, int maxit, const KOpts &ko)
 {
-    if (solver == SFDX_SOLVER_COCR)
-        return cocr<R>(ctx, D, tag, wmu, f, x, bnorm, eps, maxit, corr_intv, corr_tol, omega, verbose);
-    return bicgstab<R>(ctx, D, tag, wmu, f, x, bnorm, eps, maxit, corr_intv, corr_tol, omega, verbose);
+    if (solver == SFDX_SOLVER_COCR) return cocr<R>(ctx, D, tag, wmu, f, x, bnorm, eps, maxit, ko);
+    return bicgstab<R>(ctx, D, tag, wmu, f, x, bnorm, eps, maxit, ko);
+}
+
+KOpts kopts_of(const sfdx_opts &o)
+{
+    KOpts k;
+    k.corr_intv = o.corr_intv;
+    k.corr_tol = o.corr_tol;
+    k.omega = o.ssor_omega;
+    k.verbose = o.verbose;
+    k.dc_restart = o.dc_restart;
+    k.dc_at_start = o.dc_at_start;
+    k.precond = o.precond;
+    return k;
 }
 
 // Right-hand-side norm of a level: ||b_int|| (unscaled, = ||A_ib g / ve||) and
@@ -339,26 +501,40 @@ struct LevelSolveResult {
 // Solve one level for one polarisation; x (FP64) holds the initial guess on the
 // interior and receives the Dirichlet data first.
 LevelSolveResult solve_level(sfdx_ctx *ctx, const sfdx_opts &o, LevelDev<double> &D64, LevelDev<float> *D32,
-                             const std::string &tag, double freq, const zd *edc, int pol, zd *x, double eps, int maxit)
+                             const std::string &tag, double freq, const zd *edc, int pol, zd *x, double eps, int maxit,
+                             MGHier<double> *h64, MGHier<float> *h32, int level)
 {
     const Dims &d = D64.g.d;
     const double wmu = 2 * 3.141592653589793 * freq * kMu0;
     launch_dirichlet<double>(d, edc, pol, x, 0, ctx->st);
     const double bnorm = rhs_norm(ctx, D64, tag, edc, pol, o.norm, wmu);
+    KOpts ko = kopts_of(o);
+    ko.mg64 = h64;
+    ko.mg32 = h32;
+    ko.mg_level = level;
     LevelSolveResult res;
     if (o.precision == SFDX_PREC_FP64 || !D32) {
-        res.info = krylov<double>(ctx, o.solver, D64, tag, wmu, nullptr, x, bnorm, eps, maxit, o.corr_intv, o.corr_tol,
-                                  o.ssor_omega, o.verbose);
+        res.info = krylov<double>(ctx, o.solver, D64, tag, wmu, nullptr, x, bnorm, eps, maxit, ko);
         return res;
     }
-    // Mixed precision: FP64 residual and update, complex64 inner solves.
+    if (o.precision == SFDX_PREC_MIXED) {
+        KOpts km = ko;
+        km.lo = D32;
+        res.info = krylov<double>(ctx, o.solver, D64, tag, wmu, nullptr, x, bnorm, eps, maxit, km);
+        return res;
+    }
+    // Mixed precision: FP64 residual, divergence correction of the iterate and
+    // update; complex64 inner solves of A_s d = r whose own corrections keep
+    // G^T M d = 0 (valid because the FP64 iterate is corrected first).
     zd *r64 = pool_vec<zd>(ctx, tag + "/ir_r", d.NL);
     zf *r32 = pool_vec<zf>(ctx, tag + "/ir_r32", d.NL), *d32 = pool_vec<zf>(ctx, tag + "/ir_d32", d.NL);
     SliceLaunch L = edge_launch(d);
     int total_iters = 0, dc_calls = 0, pcg_iters = 0;
     double bytes = 0;
+    KOpts kin = ko;
+    kin.dc_at_start = 0;
     for (int step = 0;; step++) {
-        if (o.corr_intv > 0 && step > 0) {
+        if (o.corr_intv > 0 && (step > 0 || o.dc_at_start)) {
             pcg_iters += div_correct<double>(ctx, D64, tag, x, o.corr_tol);
             dc_calls++;
         }
@@ -376,8 +552,8 @@ LevelSolveResult solve_level(sfdx_ctx *ctx, const sfdx_opts &o, LevelDev<double>
         SFDX_CHECK(hipMemsetAsync(d32, 0, d.NL * sizeof(zf), ctx->st));
         const double rn = std::sqrt(ctx->hslots[SL_RN]);
         const double inner = std::max(o.inner_tol, 0.5 * eps * bnorm / rn);
-        SolveInfo si = krylov<float>(ctx, o.solver, *D32, tag, (float)wmu, r32, d32, rn, inner, maxit - total_iters,
-                                     o.corr_intv, o.corr_tol, (float)o.ssor_omega, o.verbose);
+        const int cap = std::min(maxit - total_iters, o.inner_maxit > 0 ? o.inner_maxit : maxit);
+        SolveInfo si = krylov<float>(ctx, o.solver, *D32, tag, (float)wmu, r32, d32, rn, inner, cap, kin);
         total_iters += si.iters;
         dc_calls += si.dc_calls;
         pcg_iters += si.pcg_iters;
@@ -455,8 +631,9 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     const HostLevel fine = host_level(g, sigma);
     for (size_t q = 0; q < fine.sig.size(); q++)
         if (!std::isfinite(fine.sig[q])) { g_err = "non-finite sigma"; return SFDX_ERR_NAN; }
-    const int Lv = o->use_excmg ? o->level : 1;
-    if (o->use_excmg && (Lv < 3 || Lv > SFDX_MAX_LEVELS || !mesh_ok(fine, Lv))) {
+    const bool use_mg = o->precond == SFDX_PRECOND_MG;
+    const int Lv = (o->use_excmg || use_mg) ? o->level : 1;
+    if ((o->use_excmg || use_mg) && (Lv < (o->use_excmg ? 3 : 2) || Lv > SFDX_MAX_LEVELS || !mesh_ok(fine, Lv))) {
         g_err = "grid dimensions must be divisible by 2^level and nair by 2^(level-1)";
         return SFDX_ERR_MESH;
     }
@@ -470,10 +647,35 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     for (int i = Lv - 1; i >= 1; i--) lev[i] = coarsen(fine, Lv - i);
     std::vector<LevelDev<double>> D64(Lv + 1);
     std::vector<LevelDev<float>> D32(Lv + 1);
-    const int first_dev = o->use_excmg ? 3 : 1;
-    for (int i = first_dev; i <= Lv; i++) {
+    const int first_dev = o->use_excmg ? 3 : Lv;  // levels solved by the Krylov method
+    const int first_up = use_mg ? 1 : first_dev;  // levels resident on the device
+    for (int i = first_up; i <= Lv; i++) {
         D64[i] = upload_level<double>(ctx, lev[i], "L" + std::to_string(i));
-        if (o->precision == SFDX_PREC_MIXED) D32[i] = upload_level<float>(ctx, lev[i], "L" + std::to_string(i));
+        if (o->precision != SFDX_PREC_FP64)
+            D32[i] = upload_level<float>(ctx, lev[i], "L" + std::to_string(i), o->sigma_floor32);
+    }
+    MGHier<double> H64;
+    MGHier<float> H32;
+    if (use_mg) {
+        H64.lev.assign(Lv + 1, nullptr);
+        H32.lev.assign(Lv + 1, nullptr);
+        H64.wl.assign(Lv + 1, {nullptr, nullptr, nullptr});
+        for (int i = 1; i <= Lv; i++) {
+            H64.lev[i] = &D64[i];
+            if (o->precision != SFDX_PREC_FP64) H32.lev[i] = &D32[i];
+            if (i >= 2) {
+                const std::vector<double> *wf[3] = {&lev[i].a, &lev[i].b, &lev[i].c};
+                const std::vector<double> *wc[3] = {&lev[i - 1].a, &lev[i - 1].b, &lev[i - 1].c};
+                for (int q = 0; q < 3; q++) {
+                    const std::vector<double> w = mg_weights_1d(*wf[q], *wc[q]);
+                    double *dv = pool_vec<double>(ctx, "mgwl" + std::to_string(i) + std::to_string(q), w.size());
+                    SFDX_CHECK(hipMemcpyAsync(dv, w.data(), w.size() * 8, hipMemcpyHostToDevice, ctx->st));
+                    H64.wl[i][q] = dv;
+                }
+            }
+        }
+        H32.wl = H64.wl;
+        SFDX_CHECK(hipStreamSynchronize(ctx->st));
     }
     ensure_partials(ctx, edge_launch(make_dims(fine.nx, fine.ny, fine.nz)).nblocks() + 64);
     const int64_t NLf = make_dims(fine.nx, fine.ny, fine.nz).NL;
@@ -525,8 +727,9 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 const int maxit = o->use_excmg ? o->maxit_excmg : o->maxit_plain;
                 SFDX_CHECK(hipStreamSynchronize(ctx->st));
                 Timer tl;
-                LevelSolveResult R = solve_level(ctx, *o, D64[i], o->precision == SFDX_PREC_MIXED ? &D32[i] : nullptr,
-                                                 tag, ff, edc, pol, x, tol, maxit);
+                LevelSolveResult R = solve_level(ctx, *o, D64[i], o->precision != SFDX_PREC_FP64 ? &D32[i] : nullptr,
+                                                 tag, ff, edc, pol, x, tol, maxit, use_mg ? &H64 : nullptr,
+                                                 (use_mg && o->precision != SFDX_PREC_FP64) ? &H32 : nullptr, i);
                 SFDX_CHECK(hipStreamSynchronize(ctx->st));
                 const double secs = tl.s();
                 if (i == Lv) {
@@ -599,6 +802,10 @@ void sfdx_default_opts(sfdx_opts *o)
     o->inner_tol = 1e-5;
     o->max_refine = 20;
     o->verbose = 0;
+    o->dc_restart = 0;
+    o->dc_at_start = 0;
+    o->inner_maxit = 400;
+    o->precond = SFDX_PRECOND_HYBRID;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)

@@ -148,6 +148,20 @@ template <typename R>
 void launch_prolong(const Dims &df, const Dims &dc, int m, const ProlongDev &T, const cx<R> *uc, cx<R> *uf,
                     hipStream_t st);
 template <typename R>
+void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode, cx<R> *phi,
+                   R omega, hipStream_t st);
+void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *P32,
+                         const zd *in, zd *out, zf *in32, zf *z32, zf *tnode, zf *phi, float omega, hipStream_t st);
+template <typename R>
+void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode, cx<R> *phi,
+                      int dir, hipStream_t st);
+template <typename R>
+void launch_restrict(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *rf, cx<R> *rc,
+                     hipStream_t st);
+template <typename R>
+void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *zc, cx<R> *zf,
+                        hipStream_t st);
+template <typename R>
 void launch_extrap(int64_t n, R c1, const cx<R> *u1, R c2, cx<R> *w, hipStream_t st);
 
 }  // namespace sfdx

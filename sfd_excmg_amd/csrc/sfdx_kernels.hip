@@ -201,6 +201,7 @@ __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, c
             Row<R> r;
             row_coeffs(d, a, b, c, sig, S.o, i, j, S.k, r);
             out = alpha * row_apply(r, wmu, x, x[e]);
+            if (DOT == 4) out = w[e] + out;
         }
         y[e] = out;
         if (DOT == 1) {  // x^T y
@@ -218,7 +219,7 @@ __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, c
             acc[2] = (double)out.re * out.re + (double)out.im * out.im;
         }
     }
-    if (DOT) block_reduce_store<4>(acc, partials);
+    if (DOT >= 1 && DOT <= 3) block_reduce_store<4>(acc, partials);
 }
 
 // w = M_sigma x on interior edges
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(TPB) void k_ssor(Dims d, const R *__restrict__ a, c
     if (!colour_decode<O>(d, PAR, k, p, i, j)) return;
     const int64_t e = (O == 0) ? ex_id(d, i, j, k) : (O == 1) ? ey_id(d, i, j, k) : ez_id(d, i, j, k);
     if (!edge_interior(d, O, i, j, k)) {
-        if (DIR == 0) out[e] = cx<R>(0);
+        if (DIR != 1) out[e] = cx<R>(0);
         return;
     }
     Row<R> r;
@@ -286,9 +287,11 @@ __global__ __launch_bounds__(TPB) void k_ssor(Dims d, const R *__restrict__ a, c
     const cx<R> Dinv(D.re / den, -D.im / den);
     if (DIR == 0) {
         out[e] = omega * ((in[e] - s) * Dinv);
-    } else {
+    } else if (DIR == 1) {
         const cx<R> wv = ((R)(2 - omega) / omega) * (out[e] * D);
         out[e] = omega * ((wv - s) * Dinv);
+    } else {  // backward Gauss-Seidel: (D + U) out = in
+        out[e] = (in[e] - s) * Dinv;
     }
 }
 
@@ -609,6 +612,196 @@ __global__ __launch_bounds__(TPB) void k_grad_update(Dims d, const R *__restrict
     x[e] = x[e] - (phi[n1] - phi[n0]) * ((R)1 / l);
 }
 
+// ------------------------------------------------------------------ K7 auxiliary nodal space
+// Hybrid (Hiptmair-type) preconditioner pieces.  The gradient fields G phi are
+// the near-null space of A_s where sigma is tiny (the air); their Galerkin
+// operator is G^T A_s G = -kappa P exactly (K G = 0), so the preconditioner
+// adds G [(-1/kappa) SSOR_P^-1] G^T to the edge SSOR.
+// Nodal colours (i&1) + 2(j&1) + 4(k&1): the 19-point P never couples two
+// nodes of one colour.
+template <typename R, int COL, int DIR>
+__global__ __launch_bounds__(TPB) void k_pssor(Dims d, const R *__restrict__ P, const cx<R> *__restrict__ t,
+                                               cx<R> *__restrict__ phi)
+{
+    constexpr int ci = COL & 1, cj = (COL >> 1) & 1, ck = (COL >> 2) & 1;
+    const int k = ck + 2 * (int)blockIdx.y;
+    const int W = (d.nx + 2) / 2;
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    const int r = p / W, q = p - r * W;
+    const int i = ci + 2 * q, j = cj + 2 * r;
+    if (k > d.nz || i > d.nx || j > d.ny) return;
+    const int nI = d.nx + 1, nJ = d.ny + 1;
+    const int64_t n = ((int64_t)k * nJ + j) * nI + i;
+    if (node_boundary(d, i, j, k)) {
+        if (DIR != 1) phi[n] = cx<R>(0);
+        return;
+    }
+    const int64_t sy = nI, sz = (int64_t)nI * nJ;
+    cx<R> s(0);
+#pragma unroll
+    for (int q2 = 1; q2 < 19; q2++) {
+        const int dx = c_p19[q2][0], dy = c_p19[q2][1], dz = c_p19[q2][2];
+        const int cm = COL ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
+        const bool take = (DIR == 0) ? (cm < COL) : (cm > COL);  // DIR 1, 2: upper colours
+        if (take) s = s + P[(int64_t)q2 * d.NP + n] * phi[n + dx + dy * sy + dz * sz];
+    }
+    const R dinv = (R)1 / P[n];
+    if (DIR == 0 || DIR == 2) phi[n] = (t[n] - s) * dinv;
+    else phi[n] = phi[n] - s * dinv;
+}
+
+// t = G^T r on interior nodes (0 on the boundary), no reduction
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_gradT(Dims d, const R *__restrict__ a, const R *__restrict__ b,
+                                               const R *__restrict__ c, const cx<R> *__restrict__ w,
+                                               cx<R> *__restrict__ t)
+{
+    const int k = blockIdx.y;
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    const int nI = d.nx + 1, nJ = d.ny + 1;
+    if (p >= nI * nJ) return;
+    const int j = p / nI, i = p - j * nI;
+    const int64_t n = (int64_t)k * nI * nJ + p;
+    cx<R> out(0);
+    if (!node_boundary(d, i, j, k)) {
+        out = w[ex_id(d, i - 1, j, k)] * ((R)1 / a[i - 1]) - w[ex_id(d, i, j, k)] * ((R)1 / a[i]);
+        out = out + w[ey_id(d, i, j - 1, k)] * ((R)1 / b[j - 1]) - w[ey_id(d, i, j, k)] * ((R)1 / b[j]);
+        out = out + w[ez_id(d, i, j, k - 1)] * ((R)1 / c[k - 1]) - w[ez_id(d, i, j, k)] * ((R)1 / c[k]);
+    }
+    t[n] = out;
+}
+
+// z_e += s * (G phi)_e
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_grad_add(Dims d, const R *__restrict__ a, const R *__restrict__ b,
+                                                  const R *__restrict__ c, cx<R> s, const cx<R> *__restrict__ phi,
+                                                  cx<R> *__restrict__ z)
+{
+    const Slice S = edge_slice(d, blockIdx.y);
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    if (p >= S.nI * S.nJ) return;
+    const int j = p / S.nI, i = p - j * S.nI;
+    const int64_t e = S.base + p;
+    int64_t n0, n1;
+    const R l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
+    const cx<R> g = (phi[n1] - phi[n0]) * ((R)1 / l);
+    z[e] = fma_c(z[e], s, g);
+}
+
+// Mixed-precision recombination of the hybrid preconditioner:
+// z64_e = z32_e + s * (G phi32)_e, with G applied in FP64 so that later FP64
+// operator applications see the gradient part exactly.
+__global__ __launch_bounds__(TPB) void k_hyb_combine(Dims d, const double *__restrict__ a, const double *__restrict__ b,
+                                                     const double *__restrict__ c, zd s, const zf *__restrict__ phi,
+                                                     const zf *__restrict__ z32, zd *__restrict__ z64)
+{
+    const Slice S = edge_slice(d, blockIdx.y);
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    if (p >= S.nI * S.nJ) return;
+    const int j = p / S.nI, i = p - j * S.nI;
+    const int64_t e = S.base + p;
+    int64_t n0, n1;
+    const double l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
+    const zf f1 = phi[n1], f0 = phi[n0], zz = z32[e];
+    const zd g(((double)f1.re - (double)f0.re) / l, ((double)f1.im - (double)f0.im) / l);
+    z64[e] = fma_c(zd((double)zz.re, (double)zz.im), s, g);
+}
+
+// ------------------------------------------------------------------ K8 multigrid transfers
+// Edge prolongation between nested grids (fine = 2 x coarse in every
+// direction): the tangential field is constant along a coarse edge and linear
+// across it, so a fine edge takes the coarse edge containing it (along) with
+// linear weights over the two bracketing coarse nodes in each transverse
+// direction.  wl[d][q] is the weight of the lower coarse node for fine node q
+// of dimension d (1 for q even).  Restriction is the exact transpose.
+struct MGTables {
+    const double *wl[3];
+};
+
+__device__ __forceinline__ void node_split(const double *wl, int q, int &lo, int &hi, double &w)
+{
+    lo = q >> 1;
+    if (q & 1) { hi = lo + 1; w = wl[q]; }
+    else { hi = lo; w = 1.0; }
+}
+
+// zf_e += (P zc)_e on interior fine edges
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_prolong_add(Dims df, Dims dc, MGTables T, const cx<R> *__restrict__ zc,
+                                                     cx<R> *__restrict__ zf)
+{
+    const Slice S = edge_slice(df, blockIdx.y);
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    if (p >= S.nI * S.nJ) return;
+    const int j = p / S.nI, i = p - j * S.nI, k = S.k;
+    if (!edge_interior(df, S.o, i, j, k)) return;
+    const int64_t e = S.base + p;
+    const int idx[3] = {i, j, k};
+    const int o = S.o, t1 = (o == 0) ? 1 : 0, t2 = (o == 2) ? 1 : 2;
+    int l1, h1, l2, h2;
+    double w1, w2;
+    node_split(T.wl[t1], idx[t1], l1, h1, w1);
+    node_split(T.wl[t2], idx[t2], l2, h2, w2);
+    const int al = idx[o] >> 1;
+    double sre = 0, sim = 0;
+    for (int a1 = 0; a1 < 2; a1++)
+        for (int a2 = 0; a2 < 2; a2++) {
+            const double w = (a1 ? 1.0 - w1 : w1) * (a2 ? 1.0 - w2 : w2);
+            if (w == 0) continue;
+            int ci[3];
+            ci[o] = al;
+            ci[t1] = a1 ? h1 : l1;
+            ci[t2] = a2 ? h2 : l2;
+            const int64_t ce = (o == 0) ? ex_id(dc, ci[0], ci[1], ci[2]) : (o == 1) ? ey_id(dc, ci[0], ci[1], ci[2])
+                                                                               : ez_id(dc, ci[0], ci[1], ci[2]);
+            if (!edge_interior(dc, o, ci[0], ci[1], ci[2])) continue;
+            const cx<R> v = zc[ce];
+            sre += w * (double)v.re;
+            sim += w * (double)v.im;
+        }
+    zf[e] = zf[e] + cx<R>((R)sre, (R)sim);
+}
+
+// rc = P^T rf on interior coarse edges (0 on the coarse boundary)
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_restrict(Dims df, Dims dc, MGTables T, const cx<R> *__restrict__ rf,
+                                                  cx<R> *__restrict__ rc)
+{
+    const Slice S = edge_slice(dc, blockIdx.y);
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    if (p >= S.nI * S.nJ) return;
+    const int J = p / S.nI, I = p - J * S.nI, K = S.k;
+    const int64_t ec = S.base + p;
+    if (!edge_interior(dc, S.o, I, J, K)) { rc[ec] = cx<R>(0); return; }
+    const int idx[3] = {I, J, K};
+    const int o = S.o, t1 = (o == 0) ? 1 : 0, t2 = (o == 2) ? 1 : 2;
+    const int nf[3] = {df.nx, df.ny, df.nz};
+    double sre = 0, sim = 0;
+    for (int f1 = -1; f1 <= 1; f1++) {
+        const int q1 = 2 * idx[t1] + f1;
+        if (q1 < 0 || q1 > nf[t1]) continue;
+        const double w1 = f1 == 0 ? 1.0 : (f1 < 0 ? 1.0 - T.wl[t1][q1] : T.wl[t1][q1]);
+        for (int f2 = -1; f2 <= 1; f2++) {
+            const int q2 = 2 * idx[t2] + f2;
+            if (q2 < 0 || q2 > nf[t2]) continue;
+            const double w2 = f2 == 0 ? 1.0 : (f2 < 0 ? 1.0 - T.wl[t2][q2] : T.wl[t2][q2]);
+            for (int fa = 0; fa < 2; fa++) {
+                int fi[3];
+                fi[o] = 2 * idx[o] + fa;
+                fi[t1] = q1;
+                fi[t2] = q2;
+                if (!edge_interior(df, o, fi[0], fi[1], fi[2])) continue;
+                const int64_t fe = (o == 0) ? ex_id(df, fi[0], fi[1], fi[2]) : (o == 1) ? ey_id(df, fi[0], fi[1], fi[2])
+                                                                                      : ez_id(df, fi[0], fi[1], fi[2]);
+                const cx<R> v = rf[fe];
+                sre += w1 * w2 * (double)v.re;
+                sim += w1 * w2 * (double)v.im;
+            }
+        }
+    }
+    rc[ec] = cx<R>((R)sre, (R)sim);
+}
+
 // ------------------------------------------------------------------ K4 prolongation
 // Fine edge value = sum over the block's coarse edges of separable Lagrange
 // weights: along-edge table (m weights) x two transverse node tables (m+1).
@@ -684,7 +877,8 @@ void launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, 
     case 0: hipLaunchKernelGGL((k_spmv<R, 0>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials); break;
     case 1: hipLaunchKernelGGL((k_spmv<R, 1>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials); break;
     case 2: hipLaunchKernelGGL((k_spmv<R, 2>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials); break;
-    default: hipLaunchKernelGGL((k_spmv<R, 3>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials); break;
+    case 3: hipLaunchKernelGGL((k_spmv<R, 3>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials); break;
+    default: hipLaunchKernelGGL((k_spmv<R, 4>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials); break;
     }
 }
 
@@ -835,6 +1029,143 @@ void launch_extrap(int64_t n, R c1, const cx<R> *u1, R c2, cx<R> *w, hipStream_t
     hipLaunchKernelGGL((k_extrap<R>), grid1(n), dim3(TPB), 0, st, n, c1, u1, c2, w);
 }
 
+template <typename R, int COL, int DIR>
+static void pssor_one(const Dims &d, const R *P, const cx<R> *t, cx<R> *phi, hipStream_t st)
+{
+    const int W = (d.nx + 2) / 2, H = (d.ny + 2) / 2;
+    dim3 gr((unsigned)(((int64_t)W * H + TPB - 1) / TPB), (unsigned)((d.nz + 2) / 2), 1);
+    hipLaunchKernelGGL((k_pssor<R, COL, DIR>), gr, dim3(TPB), 0, st, d, P, t, phi);
+}
+
+// out = SSOR_e(in) + s * G SSOR_P(G^T in), s = -1/kappa
+template <typename R>
+void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode,
+                   cx<R> *phi, R omega, hipStream_t st)
+{
+    launch_ssor<R>(g, wmu, in, out, omega, st);
+    SliceLaunch Ln = node_launch(g.d);
+    hipLaunchKernelGGL((k_gradT<R>), Ln.grid, Ln.block, 0, st, g.d, g.a, g.b, g.c, in, tnode);
+    pssor_one<R, 0, 0>(g.d, P, tnode, phi, st);
+    pssor_one<R, 1, 0>(g.d, P, tnode, phi, st);
+    pssor_one<R, 2, 0>(g.d, P, tnode, phi, st);
+    pssor_one<R, 3, 0>(g.d, P, tnode, phi, st);
+    pssor_one<R, 4, 0>(g.d, P, tnode, phi, st);
+    pssor_one<R, 5, 0>(g.d, P, tnode, phi, st);
+    pssor_one<R, 6, 0>(g.d, P, tnode, phi, st);
+    pssor_one<R, 7, 0>(g.d, P, tnode, phi, st);
+    pssor_one<R, 7, 1>(g.d, P, tnode, phi, st);
+    pssor_one<R, 6, 1>(g.d, P, tnode, phi, st);
+    pssor_one<R, 5, 1>(g.d, P, tnode, phi, st);
+    pssor_one<R, 4, 1>(g.d, P, tnode, phi, st);
+    pssor_one<R, 3, 1>(g.d, P, tnode, phi, st);
+    pssor_one<R, 2, 1>(g.d, P, tnode, phi, st);
+    pssor_one<R, 1, 1>(g.d, P, tnode, phi, st);
+    pssor_one<R, 0, 1>(g.d, P, tnode, phi, st);
+    // s = -1/kappa = -1/(i wmu) = i/wmu
+    const cx<R> s((R)0, (R)1 / wmu);
+    SliceLaunch Le = edge_launch(g.d);
+    hipLaunchKernelGGL((k_grad_add<R>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out);
+}
+
+// out64 = SSOR32(in) + s * G SSOR_P32(G^T in) with the FP32 pieces on a
+// complex64 copy of the input and the recombination in FP64.
+void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *P32,
+                         const zd *in, zd *out, zf *in32, zf *z32, zf *tnode, zf *phi, float omega, hipStream_t st)
+{
+    const int64_t nl = g64.d.NL;
+    launch_convert<double, float>(nl, in, in32, st);
+    launch_ssor<float>(g32, (float)wmu, in32, z32, omega, st);
+    SliceLaunch Ln = node_launch(g32.d);
+    hipLaunchKernelGGL((k_gradT<float>), Ln.grid, Ln.block, 0, st, g32.d, g32.a, g32.b, g32.c, in32, tnode);
+    pssor_one<float, 0, 0>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 1, 0>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 2, 0>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 3, 0>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 4, 0>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 5, 0>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 6, 0>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 7, 0>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 7, 1>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 6, 1>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 5, 1>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 4, 1>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 3, 1>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 2, 1>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 1, 1>(g32.d, P32, tnode, phi, st);
+    pssor_one<float, 0, 1>(g32.d, P32, tnode, phi, st);
+    const zd s(0.0, 1.0 / wmu);
+    SliceLaunch Le = edge_launch(g64.d);
+    hipLaunchKernelGGL(k_hyb_combine, Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, out);
+}
+
+// One Gauss-Seidel smoothing step of the hybrid smoother:
+//   out = GS_e(in) + s * G GS_n(G^T in),  forward (dir 0) or backward (dir 1)
+// sweeps, so that the backward step is the transpose of the forward one.
+template <typename R>
+void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode, cx<R> *phi,
+                      int dir, hipStream_t st)
+{
+    if (dir == 0) {
+        ssor_one<R, 0, 0, 0>(g, wmu, in, out, (R)1, st);
+        ssor_one<R, 0, 1, 0>(g, wmu, in, out, (R)1, st);
+        ssor_one<R, 1, 0, 0>(g, wmu, in, out, (R)1, st);
+        ssor_one<R, 1, 1, 0>(g, wmu, in, out, (R)1, st);
+        ssor_one<R, 2, 0, 0>(g, wmu, in, out, (R)1, st);
+        ssor_one<R, 2, 1, 0>(g, wmu, in, out, (R)1, st);
+    } else {
+        ssor_one<R, 2, 1, 2>(g, wmu, in, out, (R)1, st);
+        ssor_one<R, 2, 0, 2>(g, wmu, in, out, (R)1, st);
+        ssor_one<R, 1, 1, 2>(g, wmu, in, out, (R)1, st);
+        ssor_one<R, 1, 0, 2>(g, wmu, in, out, (R)1, st);
+        ssor_one<R, 0, 1, 2>(g, wmu, in, out, (R)1, st);
+        ssor_one<R, 0, 0, 2>(g, wmu, in, out, (R)1, st);
+    }
+    SliceLaunch Ln = node_launch(g.d);
+    hipLaunchKernelGGL((k_gradT<R>), Ln.grid, Ln.block, 0, st, g.d, g.a, g.b, g.c, in, tnode);
+    if (dir == 0) {
+        pssor_one<R, 0, 0>(g.d, P, tnode, phi, st);
+        pssor_one<R, 1, 0>(g.d, P, tnode, phi, st);
+        pssor_one<R, 2, 0>(g.d, P, tnode, phi, st);
+        pssor_one<R, 3, 0>(g.d, P, tnode, phi, st);
+        pssor_one<R, 4, 0>(g.d, P, tnode, phi, st);
+        pssor_one<R, 5, 0>(g.d, P, tnode, phi, st);
+        pssor_one<R, 6, 0>(g.d, P, tnode, phi, st);
+        pssor_one<R, 7, 0>(g.d, P, tnode, phi, st);
+    } else {
+        pssor_one<R, 7, 2>(g.d, P, tnode, phi, st);
+        pssor_one<R, 6, 2>(g.d, P, tnode, phi, st);
+        pssor_one<R, 5, 2>(g.d, P, tnode, phi, st);
+        pssor_one<R, 4, 2>(g.d, P, tnode, phi, st);
+        pssor_one<R, 3, 2>(g.d, P, tnode, phi, st);
+        pssor_one<R, 2, 2>(g.d, P, tnode, phi, st);
+        pssor_one<R, 1, 2>(g.d, P, tnode, phi, st);
+        pssor_one<R, 0, 2>(g.d, P, tnode, phi, st);
+    }
+    const cx<R> s((R)0, (R)1 / wmu);
+    SliceLaunch Le = edge_launch(g.d);
+    hipLaunchKernelGGL((k_grad_add<R>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out);
+}
+
+template <typename R>
+void launch_restrict(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *rf, cx<R> *rc,
+                     hipStream_t st)
+{
+    MGTables T;
+    for (int q = 0; q < 3; q++) T.wl[q] = wl[q];
+    SliceLaunch L = edge_launch(dc);
+    hipLaunchKernelGGL((k_restrict<R>), L.grid, L.block, 0, st, df, dc, T, rf, rc);
+}
+
+template <typename R>
+void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *zc, cx<R> *zf,
+                        hipStream_t st)
+{
+    MGTables T;
+    for (int q = 0; q < 3; q++) T.wl[q] = wl[q];
+    SliceLaunch L = edge_launch(df);
+    hipLaunchKernelGGL((k_prolong_add<R>), L.grid, L.block, 0, st, df, dc, T, zc, zf);
+}
+
 // explicit instantiations
 #define SFDX_INST(R)                                                                                              \
     template void launch_spmv<R>(const DevGrid<R> &, R, const cx<R> *, cx<R> *, R, int, const cx<R> *, RedBuf &,  \
@@ -858,7 +1189,15 @@ void launch_extrap(int64_t n, R c1, const cx<R> *u1, R c2, cx<R> *w, hipStream_t
     template void launch_grad_update<R>(const DevGrid<R> &, const cx<R> *, cx<R> *, hipStream_t);                 \
     template void launch_prolong<R>(const Dims &, const Dims &, int, const ProlongDev &, const cx<R> *, cx<R> *,   \
                                     hipStream_t);                                                                 \
-    template void launch_extrap<R>(int64_t, R, const cx<R> *, R, cx<R> *, hipStream_t);
+    template void launch_extrap<R>(int64_t, R, const cx<R> *, R, cx<R> *, hipStream_t);                        \
+    template void launch_hybrid<R>(const DevGrid<R> &, R, const R *, const cx<R> *, cx<R> *, cx<R> *, cx<R> *, R, \
+                                   hipStream_t);                                                                  \
+    template void launch_hybrid_gs<R>(const DevGrid<R> &, R, const R *, const cx<R> *, cx<R> *, cx<R> *, cx<R> *,  \
+                                      int, hipStream_t);                                                          \
+    template void launch_restrict<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *, cx<R> *,   \
+                                     hipStream_t);                                                                \
+    template void launch_prolong_add<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *,         \
+                                        cx<R> *, hipStream_t);
 
 SFDX_INST(double)
 SFDX_INST(float)

@@ -22,8 +22,9 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "lib", "libsfdx.so")
 
 SOLVER_BICGSTAB, SOLVER_GPBICG, SOLVER_COCR = 1, 3, 5
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_IR = 0, 1, 2
 NORM_REFERENCE, NORM_INTERIOR = 0, 1
+PRECOND_SSOR, PRECOND_HYBRID, PRECOND_MG = 0, 1, 2
 MAX_LEVELS = 8
 WARN_NOCONV = -1
 
@@ -44,7 +45,9 @@ class Opts(ctypes.Structure):
                 ("eps", ctypes.c_double), ("corr_intv", ctypes.c_int), ("corr_tol", ctypes.c_double),
                 ("ssor_omega", ctypes.c_double), ("precision", ctypes.c_int), ("norm", ctypes.c_int),
                 ("maxit_plain", ctypes.c_int), ("maxit_excmg", ctypes.c_int), ("inner_tol", ctypes.c_double),
-                ("max_refine", ctypes.c_int), ("verbose", ctypes.c_int)]
+                ("max_refine", ctypes.c_int), ("verbose", ctypes.c_int), ("dc_restart", ctypes.c_int),
+                ("dc_at_start", ctypes.c_int), ("inner_maxit", ctypes.c_int), ("precond", ctypes.c_int),
+                ("sigma_floor32", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
