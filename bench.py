@@ -2,19 +2,24 @@
 
 Workload (default): 128x128x96 cells (16 air + 80 earth layers), full
 anisotropic sigma in 16 seeded blocks over 100 ohm-m, 0.1 Hz, both
-polarisations, 4 EXCMG levels, COCR + multicolour SSOR, divergence correction
-every 50 iterations, fine-level relative residual 1e-8 (interior rows).
+polarisations, 4 EXCMG levels (levels 1-2 direct, 3-4 Krylov), COCR to a
+fine-level relative residual of 1e-8 (interior rows).
 
 A step is one full forward solve (all levels, both polarisations).  `value`
-is the fine-grid solve time of the mixed-precision path (complex64 inner
-solves inside FP64 refinement) averaged over the timed steps; the plain FP64
-GPU path is timed in the same run.  The roofline object is for the stencil
-kernel k_spmv, timed with HIP events on the library's stream.  The CPU
-baseline is the FP64 oracle (CSR, lexicographic SSOR, COCR) on one core, timed
-on a bounded number of fine-level iterations and extrapolated to the FP64 GPU
-iteration count.
+is the fine-grid solve time (both polarisations) of the faster converged GPU
+path, averaged over the timed steps:
+  * FP64: complex128 COCR with the hybrid preconditioner (multicolour SSOR plus
+    the nodal auxiliary-space correction, in complex128);
+  * mixed: complex128 COCR, operator and residual, with the hybrid
+    preconditioner applied in complex64 and restarts from the true residual
+    after every 1e-3 drop.
+The roofline object is for the stencil SpMV (k_spmv), timed with HIP events on
+the library's stream.  The CPU baseline is the FP64 oracle (CSR, lexicographic
+SSOR, COCR) on one core, timed on a bounded number of fine-level iterations and
+extrapolated to the FP64 GPU iteration count.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--small]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--small] [--no-cpu]
+    python bench.py --kernels-only     # stencil/SSOR/copy kernels only (profiling)
 """
 from __future__ import annotations
 
@@ -30,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+RESTART_DROP = 1e-3
 
 
 def spmv_bytes(nl, ncell, sc, sr, nsig=6):
@@ -38,19 +44,23 @@ def spmv_bytes(nl, ncell, sc, sr, nsig=6):
     return 2 * nl * sc + ncell * nsig * sr
 
 
-def run_gpu(m, steps, warmup, precision, eps, verbose=0, sigma_floor=1e-4, ctx=None):
+def solve_opts(m, precision, eps, verbose=0, precond=None):
     from sfd_excmg_amd import sfdx
 
-    ctx = ctx or sfdx.Context(0)
-    # The auxiliary-space (hybrid) preconditioner corrects the gradient (divergence)
-    # components in every iteration, so the periodic divergence correction of the
-    # reference (every 50 iterations in this workload) is not applied: it breaks
-    # the COCR recurrences (measured: divergence with kept directions, 2.3x more
+    # The hybrid preconditioner corrects the gradient (divergence) components in
+    # every iteration, so the periodic divergence correction of the reference
+    # (every 50 iterations in this workload) is not applied: it breaks the COCR
+    # recurrences (measured: divergence when the directions are kept, 2.3x more
     # iterations with restarts).
-    opts = sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0,
+    return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0,
                              norm=sfdx.NORM_INTERIOR, precision=precision, verbose=verbose,
-                             precond=sfdx.PRECOND_HYBRID, sigma_floor32=sigma_floor, maxit_excmg=4000)
+                             precond=sfdx.PRECOND_HYBRID if precond is None else precond,
+                             sigma_floor32=0.0, restart_drop=RESTART_DROP, maxit_excmg=4000)
+
+
+def run_gpu(ctx, m, steps, warmup, opts):
     walls, fine, iters, bytes_, stats = [], [], [], [], None
+    ex = None
     for s in range(warmup + steps):
         t0 = time.perf_counter()
         ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
@@ -61,8 +71,8 @@ def run_gpu(m, steps, warmup, precision, eps, verbose=0, sigma_floor=1e-4, ctx=N
             iters.append(st.fine_iters_total)
             bytes_.append(st.fine_bytes)
             stats = st.to_dict(m.level)
-    return ctx, ex, dict(status=int(stats["status"]) if stats else -9,wall=float(np.mean(walls)), fine=float(np.mean(fine)), fine_iters=int(np.mean(iters)),
-                         fine_bytes=float(np.mean(bytes_)), stats=stats)
+    return ex, dict(status=int(stats["status"]) if stats else -9, wall=float(np.mean(walls)), fine=float(np.mean(fine)),
+                    fine_iters=int(np.mean(iters)), fine_bytes=float(np.mean(bytes_)), stats=stats)
 
 
 def cpu_sample(m, gpu_iters, max_iters=6):
@@ -81,6 +91,21 @@ def cpu_sample(m, gpu_iters, max_iters=6):
     return dict(per_iter=per_it, iters=st.iters, assembly=t_asm, extrapolated=per_it * gpu_iters)
 
 
+def impedances(m, ex):
+    from sfd_excmg_amd import sfdx
+
+    r, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex)
+    return r[:, [0, 2]] + 1j * r[:, [1, 3]]
+
+
+def kernel_times(ctx, m, reps=50):
+    from sfd_excmg_amd import sfdx
+
+    f = float(m.freqs[0])
+    return (ctx.bench_kernels(m.a, m.b, m.c, m.sigma, f, sfdx.PREC_FP64, reps=reps),
+            ctx.bench_kernels(m.a, m.b, m.c, m.sigma, f, sfdx.PREC_MIXED, reps=reps))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,8 +113,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--small", action="store_true", help="64x64x48 variant for quick checks")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true")
     ap.add_argument("--verbose", type=int, default=0)
-    ap.add_argument("--sigma-floor", type=float, default=1e-4)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -101,23 +126,25 @@ def main():
         m = synth.make_c2(nx=64, nz_earth=40, nair=8, nstations=20)
     else:
         m = synth.make_c2()
+    ctx = sfdx.Context(0)
+    nl, nc = m.nl, m.ncell
+    b64, b32 = spmv_bytes(nl, nc, 16, 8), spmv_bytes(nl, nc, 8, 4)
+    if args.kernels_only:
+        ms64, ms32 = kernel_times(ctx, m, reps=20)
+        print(json.dumps({"spmv_ms": {"fp64": ms64[0], "fp32": ms32[0]}, "copy_ms": {"fp64": ms64[4], "fp32": ms32[4]},
+                          "copy_bytes": {"fp64": 2 * nl * 16, "fp32": 2 * nl * 8},
+                          "spmv_bytes": {"fp64": b64, "fp32": b32}}))
+        return
     eps = 1e-8
-    ctx, ex_64, f64 = run_gpu(m, args.steps, args.warmup, sfdx.PREC_FP64, eps, args.verbose)
-    _, ex_mx, mx = run_gpu(m, args.steps, 0, sfdx.PREC_MIXED, eps, args.verbose, args.sigma_floor, ctx)
+    ex_64, f64 = run_gpu(ctx, m, args.steps, args.warmup, solve_opts(m, sfdx.PREC_FP64, eps, args.verbose))
+    print(f"fp64 done: fine {f64['fine']:.3f} s, {f64['fine_iters']} iterations", file=sys.stderr, flush=True)
+    ex_mx, mx = run_gpu(ctx, m, args.steps, 0, solve_opts(m, sfdx.PREC_MIXED, eps, args.verbose))
+    print(f"mixed done: fine {mx['fine']:.3f} s, {mx['fine_iters']} iterations", file=sys.stderr, flush=True)
     mixed_ok = mx["status"] == 0
     head = mx if (mixed_ok and mx["fine"] < f64["fine"]) else f64
-    # impedance agreement mixed vs FP64
-    r1, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex_mx)
-    r2, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex_64)
-    z1 = r1[:, [0, 2]] + 1j * r1[:, [1, 3]]
-    z2 = r2[:, [0, 2]] + 1j * r2[:, [1, 3]]
-    zerr = float(np.max(np.abs(z1 - z2) / np.abs(z2)))
-    # kernel roofline (SpMV) in both precisions
-    ms64 = ctx.bench_kernels(m.a, m.b, m.c, m.sigma, float(m.freqs[0]), sfdx.PREC_FP64, reps=50)
-    ms32 = ctx.bench_kernels(m.a, m.b, m.c, m.sigma, float(m.freqs[0]), sfdx.PREC_MIXED, reps=50)
-    nl, nc = m.nl, m.ncell
-    b64 = spmv_bytes(nl, nc, 16, 8)
-    b32 = spmv_bytes(nl, nc, 8, 4)
+    z64, zmx = impedances(m, ex_64), impedances(m, ex_mx)
+    zerr = float(np.max(np.abs(zmx - z64) / np.abs(z64)))
+    ms64, ms32 = kernel_times(ctx, m)
     ach64 = b64 / (ms64[0] * 1e-3) / 1e9
     ach32 = b32 / (ms32[0] * 1e-3) / 1e9
     cpu = None
@@ -134,25 +161,29 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("complex128 Krylov + complex64 preconditioner" if head is mx else "complex128"),
+        "dtype": ("complex128 Krylov/operator + complex64 preconditioner" if head is mx else "complex128"),
         "data": "synthetic (seeded anisotropic block model)",
-        "config": {"workload": "mt3d_fwd_128x128x96_aniso_0.1Hz_2pol_excmg4_cocr_dc50" if not args.small
-                   else "mt3d_fwd_64x64x48_aniso_small", "cells": [m.nx, m.ny, m.nz], "edges": nl,
-                   "levels": m.level, "residual": "interior, 1e-8"},
+        "config": {"workload": ("mt3d_fwd_128x128x96_aniso_0.1Hz_2pol_excmg4_cocr" if not args.small
+                                else "mt3d_fwd_64x64x48_aniso_small"), "cells": [m.nx, m.ny, m.nz], "edges": nl,
+                   "levels": m.level, "residual": "interior rows, relative 1e-8", "preconditioner": "hybrid"},
         "fp64_gpu": {"fine_s": round(f64["fine"], 4), "wall_s": round(f64["wall"], 3), "fine_iters": f64["fine_iters"],
                      "status": f64["status"]},
         "mixed_gpu": {"fine_s": round(mx["fine"], 4), "wall_s": round(mx["wall"], 3), "fine_iters": mx["fine_iters"],
-                      "status": mx["status"], "sigma_floor32": args.sigma_floor},
+                      "status": mx["status"], "restarts": mx["stats"]["refine_steps"] if mx["stats"] else None},
         "speedup_mixed_vs_fp64": round(f64["fine"] / mx["fine"], 3) if mx["fine"] > 0 else None,
         "max_rel_impedance_err_mixed_vs_fp64": zerr,
         "fine_solve_hbm_gbs_algorithmic": {"mixed": round(mx["fine_bytes"] / mx["fine"] / 1e9, 1) if mx["fine"] else None,
                                           "fp64": round(f64["fine_bytes"] / f64["fine"] / 1e9, 1) if f64["fine"] else None},
-        "roofline": {"kernel": "k_spmv<float> (complex64 stencil)", "bound": "hbm", "achieved": round(ach32, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "avg_ms": round(ms32[0], 5), "algorithmic_bytes": b32,
-                     "fp64": {"achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
-                              "avg_ms": round(ms64[0], 5), "algorithmic_bytes": b64},
-                     "ssor_ms": {"fp32": round(ms32[1], 4), "fp64": round(ms64[1], 4)}},
+        "roofline": {"kernel": "k_spmv<double,0> (complex128 stencil)", "bound": "hbm", "achieved": round(ach64, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_ms": round(ms64[0], 5), "algorithmic_bytes": b64,
+                     "complex64": {"kernel": "k_spmv<float,0>", "achieved": round(ach32, 1),
+                                   "frac": round(ach32 / HBM_PEAK_GBS, 4), "avg_ms": round(ms32[0], 5),
+                                   "algorithmic_bytes": b32},
+                     "ssor_apply_ms": {"fp32": round(ms32[1], 4), "fp64": round(ms64[1], 4)},
+                     "hybrid_apply_ms": {"fp32": round(ms32[5], 4), "fp64": round(ms64[5], 4)},
+                     "copy_ms": {"fp32": round(ms32[4], 5), "fp64": round(ms64[4], 5),
+                                 "gbs_fp64": round(2 * nl * 16 / (ms64[4] * 1e-3) / 1e9, 1)}},
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["extrapolated"], 2), "unit": "s", "cores": 1, "kind": "port",
             "sample": f"FP64 oracle COCR+lexicographic SSOR, {cpu['iters']} fine-level iterations timed "

@@ -82,10 +82,18 @@ typedef struct {
     int dc_at_start;    /* correct the (prolongated) initial guess before iterating */
     int inner_maxit;    /* mixed precision: iteration cap of one inner solve */
     int precond;        /* SFDX_PRECOND_*: 0 multicolour SSOR (the reference's SSOR,
-                           re-ordered), 1 SSOR + auxiliary nodal correction */
+                           re-ordered), 1 SSOR + auxiliary nodal correction,
+                           2 geometric multigrid V-cycle with hybrid smoothing */
     double sigma_floor32; /* complex64 edge smoother only: diagonal conductivities
                              below this floor (S/m) are raised to it, bounding the
                              amplification of near-null air modes; 0 = off */
+    int mg_levels;      /* multigrid: grids in the hierarchy counted from the fine
+                           grid (0 = as many as the dimensions allow); never fewer
+                           than the EXCMG levels */
+    int mg_coarse_pairs; /* multigrid: symmetric smoothing pairs on the coarsest grid */
+    double restart_drop; /* SFDX_PREC_MIXED: restart the FP64 Krylov recurrences from
+                            the true residual whenever the residual has dropped by
+                            this factor since the last restart (0 = never) */
 } sfdx_opts;
 
 typedef struct {
@@ -147,9 +155,11 @@ int sfdx_div_correct(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, dou
 /* EXCMG initial guess on grid g from the two coarser levels (replaces
  * extint_v, excmg.f90:9-77): u0 on g/4, u1 on g/2, out on g. */
 int sfdx_extint_v(sfdx_ctx *ctx, const sfdx_grid *g, const double *u0, const double *u1, double *out);
-/* Times the fine-level kernels on the context's stream with HIP events:
- * ms[0] SpMV, ms[1] SSOR apply (12 sweeps), ms[2] COCR vector update,
- * ms[3] one COCR iteration; averages over reps. */
+/* Times the fine-level kernels on the context's stream with HIP events,
+ * averaged over reps: ms[0] SpMV, ms[1] SSOR apply (12 colour sweeps),
+ * ms[2] COCR vector update, ms[3] = ms[0]+ms[1]+ms[2], ms[4] plain copy of
+ * one edge vector, ms[5] hybrid preconditioner apply.  ms holds 6 doubles.
+ * No counterpart in the reference (measurement only). */
 int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, int precision,
                        int reps, double *ms);
 

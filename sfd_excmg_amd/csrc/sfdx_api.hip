@@ -133,7 +133,7 @@ void ensure_P(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag)
 }
 
 struct SolveInfo {
-    int iters = 0, dc_calls = 0, pcg_iters = 0, status = 0;
+    int iters = 0, dc_calls = 0, pcg_iters = 0, status = 0, restarts = 0;
     double relres = 0;
     double bytes = 0;
 };
@@ -186,6 +186,13 @@ int div_correct(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, cx<R> *x,
 // transpose of the pre-smoother, so the cycle is symmetric), and a fixed number
 // of smoothing pairs on the coarsest level.  Coarse operators are the same
 // stencil on the volume-averaged conductivity of the coarse grid.
+//
+// Split form: every level's correction is held as z + (i/wmu) G psi with an
+// edge part z and a nodal part psi.  In resistive cells (air) psi is many
+// orders of magnitude larger than z; applying the curl-curl stencil to G psi
+// would cancel to rounding noise of that size (K G = 0), so residuals use
+// A G psi = -i wmu M G psi instead, and the edge/nodal prolongations commute
+// (P_e G_c = G_f P_n) so psi is carried through the hierarchy as nodes.
 template <typename R>
 struct MGHier {
     std::vector<LevelDev<R> *> lev;           // index = level (1-based), lev[0] unused
@@ -205,40 +212,43 @@ std::vector<double> mg_weights_1d(const std::vector<double> &wf, const std::vect
     return w;
 }
 
+// (z, psi) ~= A^-1 r on level l (split form, see above)
 template <typename R>
-void mg_vcycle(sfdx_ctx *ctx, MGHier<R> &H, int l, R wmu, const cx<R> *r, cx<R> *z)
+void mg_vcycle(sfdx_ctx *ctx, MGHier<R> &H, int l, R wmu, const cx<R> *r, cx<R> *z, cx<R> *psi)
 {
     LevelDev<R> &D = *H.lev[l];
     const Dims &d = D.g.d;
     const std::string tg = "mg" + std::to_string(l) + (sizeof(R) == 8 ? "d" : "f");
     cx<R> *t = pool_vec<cx<R>>(ctx, tg + "t", d.NL), *w = pool_vec<cx<R>>(ctx, tg + "w", d.NL);
+    cx<R> *gt = pool_vec<cx<R>>(ctx, tg + "g", d.NL);
     cx<R> *tn = pool_vec<cx<R>>(ctx, tg + "tn", d.NP), *ph = pool_vec<cx<R>>(ctx, tg + "ph", d.NP);
-    ensure_P(ctx, D, "L" + std::to_string(l));
+    ensure_P(ctx, D, tg);
+    hipStream_t st = ctx->st;
+    // pre-smoothing from zero: z = GS_e r, psi = GS_n G^T r
+    launch_hybrid_gs<R>(D.g, wmu, D.P, r, z, tn, psi, 0, st, false);
+    auto smooth_add = [&](int dir) {
+        launch_resid_split<R>(D.g, wmu, z, psi, r, gt, t, st);
+        launch_hybrid_gs<R>(D.g, wmu, D.P, t, w, tn, ph, dir, st, false);
+        launch_axpby<R>(d.NL, cx<R>(1), w, cx<R>(1), z, st);
+        launch_axpby<R>(d.NP, cx<R>(1), ph, cx<R>(1), psi, st);
+    };
     if (l == 1) {
-        launch_hybrid_gs<R>(D.g, wmu, D.P, r, z, tn, ph, 0, ctx->st);
         for (int q = 0; q < H.coarse_pairs; q++) {
-            if (q > 0) {
-                launch_spmv<R>(D.g, wmu, z, t, (R)-1, 4, r, ctx->rb, ctx->st);
-                launch_hybrid_gs<R>(D.g, wmu, D.P, t, w, tn, ph, 0, ctx->st);
-                launch_axpby<R>(d.NL, cx<R>(1), w, cx<R>(1), z, ctx->st);
-            }
-            launch_spmv<R>(D.g, wmu, z, t, (R)-1, 4, r, ctx->rb, ctx->st);
-            launch_hybrid_gs<R>(D.g, wmu, D.P, t, w, tn, ph, 1, ctx->st);
-            launch_axpby<R>(d.NL, cx<R>(1), w, cx<R>(1), z, ctx->st);
+            if (q > 0) smooth_add(0);
+            smooth_add(1);
         }
         return;
     }
     LevelDev<R> &C = *H.lev[l - 1];
     const std::string tc = "mg" + std::to_string(l - 1) + (sizeof(R) == 8 ? "d" : "f");
     cx<R> *rc = pool_vec<cx<R>>(ctx, tc + "r", C.g.d.NL), *zc = pool_vec<cx<R>>(ctx, tc + "z", C.g.d.NL);
-    launch_hybrid_gs<R>(D.g, wmu, D.P, r, z, tn, ph, 0, ctx->st);
-    launch_spmv<R>(D.g, wmu, z, t, (R)-1, 4, r, ctx->rb, ctx->st);
-    launch_restrict<R>(d, C.g.d, H.wl[l].data(), t, rc, ctx->st);
-    mg_vcycle<R>(ctx, H, l - 1, wmu, rc, zc);
-    launch_prolong_add<R>(d, C.g.d, H.wl[l].data(), zc, z, ctx->st);
-    launch_spmv<R>(D.g, wmu, z, t, (R)-1, 4, r, ctx->rb, ctx->st);
-    launch_hybrid_gs<R>(D.g, wmu, D.P, t, w, tn, ph, 1, ctx->st);
-    launch_axpby<R>(d.NL, cx<R>(1), w, cx<R>(1), z, ctx->st);
+    cx<R> *pc = pool_vec<cx<R>>(ctx, tc + "p", C.g.d.NP);
+    launch_resid_split<R>(D.g, wmu, z, psi, r, gt, t, st);
+    launch_restrict<R>(d, C.g.d, H.wl[l].data(), t, rc, st);
+    mg_vcycle<R>(ctx, H, l - 1, wmu, rc, zc, pc);
+    launch_prolong_add<R>(d, C.g.d, H.wl[l].data(), zc, z, st);
+    launch_prolong_node_add<R>(d, C.g.d, H.wl[l].data(), pc, psi, st);
+    smooth_add(1);
 }
 
 // ---------------------------------------------------------------- Krylov
@@ -254,6 +264,7 @@ struct KOpts {
     MGHier<double> *mg64 = nullptr;  // multigrid hierarchy (precond == MG)
     MGHier<float> *mg32 = nullptr;
     int mg_level = 0;                // level being solved
+    double restart_drop = 0;         // COCR: restart from the true residual after this drop
 };
 
 // out = M^-1 in with the chosen preconditioner
@@ -262,17 +273,19 @@ void apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
                 const KOpts &ko)
 {
     if (ko.precond == SFDX_PRECOND_MG) {
+        const Dims &d = D.g.d;
         if (sizeof(R) == 8 && ko.mg32) {
-            // FP64 Krylov, complex64 V-cycle
-            const Dims &d = D.g.d;
+            // FP64 Krylov, complex64 V-cycle; the recombination z + (i/wmu) G psi is done in FP64
             zf *in32 = pool_vec<zf>(ctx, tag + "/mg_in", d.NL), *z32 = pool_vec<zf>(ctx, tag + "/mg_z", d.NL);
+            zf *p32 = pool_vec<zf>(ctx, tag + "/mg_p", d.NP);
             launch_convert<double, float>(d.NL, (const zd *)in, in32, ctx->st);
-            mg_vcycle<float>(ctx, *ko.mg32, ko.mg_level, (float)wmu, in32, z32);
-            launch_convert<float, double>(d.NL, z32, (zd *)out, ctx->st);
-        } else if (sizeof(R) == 8) {
-            mg_vcycle<double>(ctx, *ko.mg64, ko.mg_level, (double)wmu, (const zd *)in, (zd *)out);
+            mg_vcycle<float>(ctx, *ko.mg32, ko.mg_level, (float)wmu, in32, z32, p32);
+            launch_hyb_combine(*(const DevGrid<double> *)&D.g, zd(0.0, 1.0 / (double)wmu), p32, z32, (zd *)out, ctx->st);
         } else {
-            mg_vcycle<float>(ctx, *ko.mg32, ko.mg_level, (float)wmu, (const zf *)in, (zf *)out);
+            MGHier<R> &H = *(MGHier<R> *)(sizeof(R) == 8 ? (void *)ko.mg64 : (void *)ko.mg32);
+            cx<R> *pn = pool_vec<cx<R>>(ctx, tag + "/mg_pn", d.NP);
+            mg_vcycle<R>(ctx, H, ko.mg_level, wmu, in, out, pn);
+            launch_grad_add<R>(D.g, cx<R>((R)0, (R)1 / wmu), pn, out, ctx->st);
         }
         return;
     }
@@ -328,8 +341,11 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     }
     // 0: fresh start, 1: soft restart after DC, 2: iterate
     int state = 0;
+    bool fresh = false;
+    double rel0 = 0;
     while (true) {
         if (state != 2) {
+            fresh = true;
             launch_spmv<R>(D.g, wmu, x, r, (R)-1, 0, nullptr, ctx->rb, ctx->st);
             if (f) launch_axpby<R>(d.NL, cx<R>(1), f, cx<R>(1), r, ctx->st);
             apply_prec<R>(ctx, D, tag, wmu, r, z, ko);
@@ -351,9 +367,18 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
         info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
         if (!(info.relres == info.relres) || std::isinf(info.relres)) { info.status = SFDX_ERR_NAN; break; }
         if (ko.verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  cocr it %d relres %.3e\n", info.iters, info.relres);
+        if (fresh) { rel0 = info.relres; fresh = false; }
         if (info.relres < eps) { info.status = 0; break; }
         if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
         if (ctx->hslots[SL_BREAK] != 0) { info.status = SFDX_ERR_BREAKDOWN; break; }
+        if (ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0) {
+            // the preconditioner is applied in complex64: its rounding is not a
+            // fixed linear operator, so the short recurrences drift; restart
+            // from the true FP64 residual
+            state = 0;
+            info.restarts++;
+            continue;
+        }
         int64_t nbd;
         apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko);
         launch_dot<R>(0, d.NL, Ap, q, ctx->rb, ctx->st, nbd);
@@ -365,7 +390,10 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
         if (ko.corr_intv > 0 && info.iters % ko.corr_intv == 0) {
             info.pcg_iters += div_correct<R>(ctx, D, tag, x, ko.corr_tol);
             info.dc_calls++;
-            state = ko.dc_restart ? 0 : 1;
+            // COCR's coupled recurrences assume r = b - A x along one Krylov
+            // sequence; after the correction only a full restart is consistent
+            // (keeping p and Ap diverges with the hybrid preconditioner)
+            state = (ko.dc_restart || ko.precond != SFDX_PRECOND_SSOR) ? 0 : 1;
             continue;
         }
         launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
@@ -520,7 +548,9 @@ LevelSolveResult solve_level(sfdx_ctx *ctx, const sfdx_opts &o, LevelDev<double>
     if (o.precision == SFDX_PREC_MIXED) {
         KOpts km = ko;
         km.lo = D32;
+        km.restart_drop = o.restart_drop;
         res.info = krylov<double>(ctx, o.solver, D64, tag, wmu, nullptr, x, bnorm, eps, maxit, km);
+        res.refine = res.info.restarts;
         return res;
     }
     // Mixed precision: FP64 residual, divergence correction of the iterate and
@@ -632,8 +662,8 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     for (size_t q = 0; q < fine.sig.size(); q++)
         if (!std::isfinite(fine.sig[q])) { g_err = "non-finite sigma"; return SFDX_ERR_NAN; }
     const bool use_mg = o->precond == SFDX_PRECOND_MG;
-    const int Lv = (o->use_excmg || use_mg) ? o->level : 1;
-    if ((o->use_excmg || use_mg) && (Lv < (o->use_excmg ? 3 : 2) || Lv > SFDX_MAX_LEVELS || !mesh_ok(fine, Lv))) {
+    const int Lv = o->use_excmg ? o->level : 1;
+    if (o->use_excmg && (Lv < 3 || Lv > SFDX_MAX_LEVELS || !mesh_ok(fine, Lv))) {
         g_err = "grid dimensions must be divisible by 2^level and nair by 2^(level-1)";
         return SFDX_ERR_MESH;
     }
@@ -654,24 +684,51 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         if (o->precision != SFDX_PREC_FP64)
             D32[i] = upload_level<float>(ctx, lev[i], "L" + std::to_string(i), o->sigma_floor32);
     }
+    // Multigrid hierarchy mg[1..Lmg] (mg[Lmg] = fine); the EXCMG level i is
+    // mg[i + off].  Grids coarser than the EXCMG levels are uploaded here.
+    int Lmg = Lv;
+    if (use_mg) {
+        int m = 1;
+        while (m < SFDX_MAX_LEVELS && fine.nx % (1 << m) == 0 && fine.ny % (1 << m) == 0 && fine.nz % (1 << m) == 0 &&
+               std::min(std::min(fine.nx, fine.ny), fine.nz) >> m >= 2)
+            m++;
+        Lmg = o->mg_levels > 0 ? std::min(o->mg_levels, m) : m;
+        Lmg = std::max(Lmg, Lv);
+    }
+    const int off = Lmg - Lv;
+    std::vector<HostLevel> mgh(Lmg + 1);
+    std::vector<LevelDev<double>> X64(off + 1);
+    std::vector<LevelDev<float>> X32(off + 1);
     MGHier<double> H64;
     MGHier<float> H32;
     if (use_mg) {
-        H64.lev.assign(Lv + 1, nullptr);
-        H32.lev.assign(Lv + 1, nullptr);
-        H64.wl.assign(Lv + 1, {nullptr, nullptr, nullptr});
-        for (int i = 1; i <= Lv; i++) {
-            H64.lev[i] = &D64[i];
-            if (o->precision != SFDX_PREC_FP64) H32.lev[i] = &D32[i];
-            if (i >= 2) {
-                const std::vector<double> *wf[3] = {&lev[i].a, &lev[i].b, &lev[i].c};
-                const std::vector<double> *wc[3] = {&lev[i - 1].a, &lev[i - 1].b, &lev[i - 1].c};
-                for (int q = 0; q < 3; q++) {
-                    const std::vector<double> w = mg_weights_1d(*wf[q], *wc[q]);
-                    double *dv = pool_vec<double>(ctx, "mgwl" + std::to_string(i) + std::to_string(q), w.size());
-                    SFDX_CHECK(hipMemcpyAsync(dv, w.data(), w.size() * 8, hipMemcpyHostToDevice, ctx->st));
-                    H64.wl[i][q] = dv;
+        H64.lev.assign(Lmg + 1, nullptr);
+        H32.lev.assign(Lmg + 1, nullptr);
+        H64.wl.assign(Lmg + 1, {nullptr, nullptr, nullptr});
+        H64.coarse_pairs = H32.coarse_pairs = o->mg_coarse_pairs > 0 ? o->mg_coarse_pairs : 4;
+        for (int j = 1; j <= Lmg; j++) {
+            if (j > off) {
+                mgh[j] = lev[j - off];
+                H64.lev[j] = &D64[j - off];
+                if (o->precision != SFDX_PREC_FP64) H32.lev[j] = &D32[j - off];
+            } else {
+                mgh[j] = coarsen(fine, Lmg - j);
+                X64[j] = upload_level<double>(ctx, mgh[j], "M" + std::to_string(j));
+                H64.lev[j] = &X64[j];
+                if (o->precision != SFDX_PREC_FP64) {
+                    X32[j] = upload_level<float>(ctx, mgh[j], "M" + std::to_string(j), o->sigma_floor32);
+                    H32.lev[j] = &X32[j];
                 }
+            }
+        }
+        for (int j = 2; j <= Lmg; j++) {
+            const std::vector<double> *wf[3] = {&mgh[j].a, &mgh[j].b, &mgh[j].c};
+            const std::vector<double> *wc[3] = {&mgh[j - 1].a, &mgh[j - 1].b, &mgh[j - 1].c};
+            for (int q = 0; q < 3; q++) {
+                const std::vector<double> w = mg_weights_1d(*wf[q], *wc[q]);
+                double *dv = pool_vec<double>(ctx, "mgwl" + std::to_string(j) + "_" + std::to_string(q), w.size());
+                SFDX_CHECK(hipMemcpyAsync(dv, w.data(), w.size() * 8, hipMemcpyHostToDevice, ctx->st));
+                H64.wl[j][q] = dv;
             }
         }
         H32.wl = H64.wl;
@@ -729,7 +786,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 Timer tl;
                 LevelSolveResult R = solve_level(ctx, *o, D64[i], o->precision != SFDX_PREC_FP64 ? &D32[i] : nullptr,
                                                  tag, ff, edc, pol, x, tol, maxit, use_mg ? &H64 : nullptr,
-                                                 (use_mg && o->precision != SFDX_PREC_FP64) ? &H32 : nullptr, i);
+                                                 (use_mg && o->precision != SFDX_PREC_FP64) ? &H32 : nullptr, i + off);
                 SFDX_CHECK(hipStreamSynchronize(ctx->st));
                 const double secs = tl.s();
                 if (i == Lv) {
@@ -806,6 +863,9 @@ void sfdx_default_opts(sfdx_opts *o)
     o->dc_at_start = 0;
     o->inner_maxit = 400;
     o->precond = SFDX_PRECOND_HYBRID;
+    o->mg_levels = 0;
+    o->mg_coarse_pairs = 4;
+    o->restart_drop = 1e-3;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)
@@ -1046,6 +1106,24 @@ int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, d
             SFDX_CHECK(hipEventSynchronize(e1));
             SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
             ms[2] = t / reps;
+            // plain copy of one edge vector (2 * NL * sizeof(cx) bytes): calibrates
+            // hardware byte counters against a kernel of known traffic
+            SFDX_CHECK(hipEventRecord(e0, ctx->st));
+            for (int q = 0; q < reps; q++) launch_copy<R>(d.NL, v[3], v[4], ctx->st);
+            SFDX_CHECK(hipEventRecord(e1, ctx->st));
+            SFDX_CHECK(hipEventSynchronize(e1));
+            SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
+            ms[4] = t / reps;
+            // hybrid preconditioner (SSOR + nodal auxiliary correction)
+            ensure_P(ctx, D, "bench");
+            cx<R> *tn = pool_vec<cx<R>>(ctx, std::string("bench/tn") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
+            cx<R> *ph = pool_vec<cx<R>>(ctx, std::string("bench/ph") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
+            SFDX_CHECK(hipEventRecord(e0, ctx->st));
+            for (int q = 0; q < reps; q++) launch_hybrid<R>(D.g, wmu, D.P, v[0], v[2], tn, ph, (R)1, ctx->st);
+            SFDX_CHECK(hipEventRecord(e1, ctx->st));
+            SFDX_CHECK(hipEventSynchronize(e1));
+            SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
+            ms[5] = t / reps;
         };
         if (precision == SFDX_PREC_FP64) run(0.0); else run(0.0f);
         ms[3] = ms[0] + ms[1] + ms[2];

@@ -152,9 +152,22 @@ void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R
                    R omega, hipStream_t st);
 void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *P32,
                          const zd *in, zd *out, zf *in32, zf *z32, zf *tnode, zf *phi, float omega, hipStream_t st);
+// One hybrid Gauss-Seidel step: out = GS_e(in), phi = GS_n(G^T in); with
+// add_grad the nodal part is folded in (out += (i/wmu) G phi).
 template <typename R>
 void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode, cx<R> *phi,
-                      int dir, hipStream_t st);
+                      int dir, hipStream_t st, bool add_grad = true);
+// t = r - A z - M (G psi): residual of z + (i/wmu) G psi, using A G = -i wmu M G
+// so that the (possibly huge) gradient part never meets the curl-curl term.
+template <typename R>
+void launch_resid_split(const DevGrid<R> &g, R wmu, const cx<R> *z, const cx<R> *psi, const cx<R> *r, cx<R> *gtmp,
+                        cx<R> *t, hipStream_t st);
+template <typename R>
+void launch_prolong_node_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *pc, cx<R> *pf,
+                             hipStream_t st);
+template <typename R>
+void launch_grad_add(const DevGrid<R> &g, cx<R> s, const cx<R> *phi, cx<R> *z, hipStream_t st);
+void launch_hyb_combine(const DevGrid<double> &g64, zd s, const zf *phi, const zf *z32, zd *z64, hipStream_t st);
 template <typename R>
 void launch_restrict(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *rf, cx<R> *rc,
                      hipStream_t st);

@@ -188,7 +188,8 @@ template <typename R, int DOT>
 __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, const R *__restrict__ b,
                                               const R *__restrict__ c, const R *__restrict__ sig, R wmu,
                                               const cx<R> *__restrict__ x, cx<R> *__restrict__ y, R alpha,
-                                              const cx<R> *__restrict__ w, double *__restrict__ partials)
+                                              const cx<R> *__restrict__ w, double *__restrict__ partials,
+                                              const cx<R> *__restrict__ g2)
 {
     const Slice S = edge_slice(d, blockIdx.y);
     const int p = blockIdx.x * TPB + threadIdx.x;
@@ -202,6 +203,13 @@ __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, c
             row_coeffs(d, a, b, c, sig, S.o, i, j, S.k, r);
             out = alpha * row_apply(r, wmu, x, x[e]);
             if (DOT == 4) out = w[e] + out;
+            if (DOT == 5) {
+                // split residual w - A x - M g2 (g2 = G psi, A G psi = -i wmu M G psi)
+                cx<R> mg = r.kmd * g2[e];
+#pragma unroll
+                for (int q = 4; q < 12; q++) mg = mg + r.km[q] * g2[r.nb[q]];
+                out = w[e] + out - mg;
+            }
         }
         y[e] = out;
         if (DOT == 1) {  // x^T y
@@ -688,6 +696,21 @@ __global__ __launch_bounds__(TPB) void k_grad_add(Dims d, const R *__restrict__ 
     z[e] = fma_c(z[e], s, g);
 }
 
+// g_e = (G phi)_e on every edge (0 on boundary edges when phi is 0 on boundary nodes)
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_grad(Dims d, const R *__restrict__ a, const R *__restrict__ b,
+                                              const R *__restrict__ c, const cx<R> *__restrict__ phi,
+                                              cx<R> *__restrict__ g)
+{
+    const Slice S = edge_slice(d, blockIdx.y);
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    if (p >= S.nI * S.nJ) return;
+    const int j = p / S.nI, i = p - j * S.nI;
+    int64_t n0, n1;
+    const R l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
+    g[S.base + p] = (phi[n1] - phi[n0]) * ((R)1 / l);
+}
+
 // Mixed-precision recombination of the hybrid preconditioner:
 // z64_e = z32_e + s * (G phi32)_e, with G applied in FP64 so that later FP64
 // operator applications see the gradient part exactly.
@@ -802,6 +825,37 @@ __global__ __launch_bounds__(TPB) void k_restrict(Dims df, Dims dc, MGTables T, 
     rc[ec] = cx<R>((R)sre, (R)sim);
 }
 
+// psi_f += P_n psi_c on interior fine nodes: trilinear nodal interpolation with
+// the same node weights as the edge transfers, so that P_e G_c = G_f P_n.
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_prolong_node_add(Dims df, Dims dc, MGTables T, const cx<R> *__restrict__ pc,
+                                                          cx<R> *__restrict__ pf)
+{
+    const int k = blockIdx.y;
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    const int nI = df.nx + 1, nJ = df.ny + 1;
+    if (p >= nI * nJ) return;
+    const int j = p / nI, i = p - j * nI;
+    if (node_boundary(df, i, j, k)) return;
+    int lo[3], hi[3];
+    double w[3];
+    node_split(T.wl[0], i, lo[0], hi[0], w[0]);
+    node_split(T.wl[1], j, lo[1], hi[1], w[1]);
+    node_split(T.wl[2], k, lo[2], hi[2], w[2]);
+    const int cI = dc.nx + 1, cJ = dc.ny + 1;
+    double sre = 0, sim = 0;
+    for (int q = 0; q < 8; q++) {
+        const int ci = (q & 1) ? hi[0] : lo[0], cj = (q & 2) ? hi[1] : lo[1], ck = (q & 4) ? hi[2] : lo[2];
+        const double wq = ((q & 1) ? 1.0 - w[0] : w[0]) * ((q & 2) ? 1.0 - w[1] : w[1]) * ((q & 4) ? 1.0 - w[2] : w[2]);
+        if (wq == 0 || node_boundary(dc, ci, cj, ck)) continue;
+        const cx<R> v = pc[((int64_t)ck * cJ + cj) * cI + ci];
+        sre += wq * (double)v.re;
+        sim += wq * (double)v.im;
+    }
+    const int64_t n = (int64_t)k * nI * nJ + p;
+    pf[n] = pf[n] + cx<R>((R)sre, (R)sim);
+}
+
 // ------------------------------------------------------------------ K4 prolongation
 // Fine edge value = sum over the block's coarse edges of separable Lagrange
 // weights: along-edge table (m weights) x two transverse node tables (m+1).
@@ -874,12 +928,23 @@ void launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, 
 {
     SliceLaunch L = edge_launch(g.d);
     switch (dot_mode) {
-    case 0: hipLaunchKernelGGL((k_spmv<R, 0>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials); break;
-    case 1: hipLaunchKernelGGL((k_spmv<R, 1>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials); break;
-    case 2: hipLaunchKernelGGL((k_spmv<R, 2>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials); break;
-    case 3: hipLaunchKernelGGL((k_spmv<R, 3>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials); break;
-    default: hipLaunchKernelGGL((k_spmv<R, 4>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials); break;
+    case 0: hipLaunchKernelGGL((k_spmv<R, 0>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
+    case 1: hipLaunchKernelGGL((k_spmv<R, 1>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
+    case 2: hipLaunchKernelGGL((k_spmv<R, 2>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
+    case 3: hipLaunchKernelGGL((k_spmv<R, 3>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
+    default: hipLaunchKernelGGL((k_spmv<R, 4>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
     }
+}
+
+// t = r - A z - M (G psi): residual of z + (i/wmu) G psi without forming A G psi
+template <typename R>
+void launch_resid_split(const DevGrid<R> &g, R wmu, const cx<R> *z, const cx<R> *psi, const cx<R> *r, cx<R> *gtmp,
+                        cx<R> *t, hipStream_t st)
+{
+    SliceLaunch L = edge_launch(g.d);
+    hipLaunchKernelGGL((k_grad<R>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, psi, gtmp);
+    hipLaunchKernelGGL((k_spmv<R, 5>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, z, t, (R)-1, r,
+                       (double *)nullptr, (const cx<R> *)gtmp);
 }
 
 template <typename R>
@@ -1103,7 +1168,7 @@ void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, 
 // sweeps, so that the backward step is the transpose of the forward one.
 template <typename R>
 void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode, cx<R> *phi,
-                      int dir, hipStream_t st)
+                      int dir, hipStream_t st, bool add_grad)
 {
     if (dir == 0) {
         ssor_one<R, 0, 0, 0>(g, wmu, in, out, (R)1, st);
@@ -1141,6 +1206,7 @@ void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, c
         pssor_one<R, 1, 2>(g.d, P, tnode, phi, st);
         pssor_one<R, 0, 2>(g.d, P, tnode, phi, st);
     }
+    if (!add_grad) return;  // caller keeps the nodal part phi (split form)
     const cx<R> s((R)0, (R)1 / wmu);
     SliceLaunch Le = edge_launch(g.d);
     hipLaunchKernelGGL((k_grad_add<R>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out);
@@ -1154,6 +1220,29 @@ void launch_restrict(const Dims &df, const Dims &dc, const double *const wl[3], 
     for (int q = 0; q < 3; q++) T.wl[q] = wl[q];
     SliceLaunch L = edge_launch(dc);
     hipLaunchKernelGGL((k_restrict<R>), L.grid, L.block, 0, st, df, dc, T, rf, rc);
+}
+
+template <typename R>
+void launch_prolong_node_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *pc, cx<R> *pf,
+                             hipStream_t st)
+{
+    MGTables T;
+    for (int q = 0; q < 3; q++) T.wl[q] = wl[q];
+    SliceLaunch L = node_launch(df);
+    hipLaunchKernelGGL((k_prolong_node_add<R>), L.grid, L.block, 0, st, df, dc, T, pc, pf);
+}
+
+template <typename R>
+void launch_grad_add(const DevGrid<R> &g, cx<R> s, const cx<R> *phi, cx<R> *z, hipStream_t st)
+{
+    SliceLaunch L = edge_launch(g.d);
+    hipLaunchKernelGGL((k_grad_add<R>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, s, phi, z);
+}
+
+void launch_hyb_combine(const DevGrid<double> &g64, zd s, const zf *phi, const zf *z32, zd *z64, hipStream_t st)
+{
+    SliceLaunch L = edge_launch(g64.d);
+    hipLaunchKernelGGL(k_hyb_combine, L.grid, L.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, z64);
 }
 
 template <typename R>
@@ -1193,7 +1282,12 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
     template void launch_hybrid<R>(const DevGrid<R> &, R, const R *, const cx<R> *, cx<R> *, cx<R> *, cx<R> *, R, \
                                    hipStream_t);                                                                  \
     template void launch_hybrid_gs<R>(const DevGrid<R> &, R, const R *, const cx<R> *, cx<R> *, cx<R> *, cx<R> *,  \
-                                      int, hipStream_t);                                                          \
+                                      int, hipStream_t, bool);                                                    \
+    template void launch_resid_split<R>(const DevGrid<R> &, R, const cx<R> *, const cx<R> *, const cx<R> *,       \
+                                        cx<R> *, cx<R> *, hipStream_t);                                           \
+    template void launch_prolong_node_add<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *,    \
+                                             cx<R> *, hipStream_t);                                               \
+    template void launch_grad_add<R>(const DevGrid<R> &, cx<R>, const cx<R> *, cx<R> *, hipStream_t);            \
     template void launch_restrict<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *, cx<R> *,   \
                                      hipStream_t);                                                                \
     template void launch_prolong_add<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *,         \

@@ -4,7 +4,8 @@ module sfdx_iface
   implicit none
 
   integer(c_int), parameter :: SFDX_SOLVER_BICGSTAB = 1, SFDX_SOLVER_GPBICG = 3, SFDX_SOLVER_COCR = 5
-  integer(c_int), parameter :: SFDX_PREC_FP64 = 0, SFDX_PREC_MIXED = 1
+  integer(c_int), parameter :: SFDX_PREC_FP64 = 0, SFDX_PREC_MIXED = 1, SFDX_PREC_MIXED_IR = 2
+  integer(c_int), parameter :: SFDX_PRECOND_SSOR = 0, SFDX_PRECOND_HYBRID = 1, SFDX_PRECOND_MG = 2
   integer(c_int), parameter :: SFDX_NORM_REFERENCE = 0, SFDX_NORM_INTERIOR = 1
 
   type, bind(C) :: sfdx_grid_t
@@ -21,6 +22,8 @@ module sfdx_iface
     real(c_double) :: inner_tol
     integer(c_int) :: max_refine, verbose, dc_restart, dc_at_start, inner_maxit, precond
     real(c_double) :: sigma_floor32
+    integer(c_int) :: mg_levels, mg_coarse_pairs
+    real(c_double) :: restart_drop
   end type
 
   type, bind(C) :: sfdx_stats_t

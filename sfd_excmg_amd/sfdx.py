@@ -47,7 +47,8 @@ class Opts(ctypes.Structure):
                 ("maxit_plain", ctypes.c_int), ("maxit_excmg", ctypes.c_int), ("inner_tol", ctypes.c_double),
                 ("max_refine", ctypes.c_int), ("verbose", ctypes.c_int), ("dc_restart", ctypes.c_int),
                 ("dc_at_start", ctypes.c_int), ("inner_maxit", ctypes.c_int), ("precond", ctypes.c_int),
-                ("sigma_floor32", ctypes.c_double)]
+                ("sigma_floor32", ctypes.c_double), ("mg_levels", ctypes.c_int), ("mg_coarse_pairs", ctypes.c_int),
+                ("restart_drop", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -210,7 +211,7 @@ class Context:
 
     def bench_kernels(self, a, b, c, sigma, freq, precision=PREC_FP64, reps=20):
         G = _GridHolder(a, b, c)
-        ms = np.zeros(4)
+        ms = np.zeros(6)
         _check(self.lib.sfdx_bench_kernels(self.h, ctypes.byref(G.g), _ptr(sigma_flat(sigma)), float(freq),
                                            int(precision), int(reps), _ptr(ms)))
         return ms

@@ -149,16 +149,33 @@ def test_fwd_solve_fp64_matches_oracle(ctx, solver, aniso):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
-def test_fwd_solve_mixed_matches_fp64(ctx):
+@pytest.mark.parametrize("precond", [sfdx.PRECOND_HYBRID, sfdx.PRECOND_MG])
+def test_fwd_solve_mixed_matches_fp64(ctx, precond):
+    """complex64 preconditioner inside FP64 COCR with restarts from the true
+    residual: same impedances as the FP64 path to 1e-6."""
     m = small_excmg_model(True)
-    o64 = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=50, norm=sfdx.NORM_INTERIOR)
-    omx = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=50, norm=sfdx.NORM_INTERIOR,
-                            precision=sfdx.PREC_MIXED)
-    ex64, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, o64)
-    exmx, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, omx)
+    kw = dict(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR, precond=precond)
+    ex64, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(**kw))
+    exmx, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs,
+                             sfdx.default_opts(precision=sfdx.PREC_MIXED, restart_drop=1e-3, sigma_floor32=0.0, **kw))
     assert st.status == 0
     assert st.refine_steps[0] >= 1
     for p, q in zip(_impedances(m, exmx), _impedances(m, ex64)):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+
+
+@pytest.mark.parametrize("use_excmg,mg_levels", [(1, 0), (0, 2), (0, 0)])
+def test_fwd_solve_multigrid_matches_oracle(ctx, use_excmg, mg_levels):
+    """Multigrid-preconditioned COCR (split edge/nodal V-cycle) reaches the
+    oracle's solution; the hierarchy may be deeper than the EXCMG levels."""
+    m = small_excmg_model(True)
+    opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR,
+                             precond=sfdx.PRECOND_MG, use_excmg=use_excmg, mg_levels=mg_levels)
+    ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
+    assert st.status == 0
+    oopt = O.Options(solver=O.SOLVER_COCR, corr_intv=50, norm="interior")
+    exo, _ = O.fwd_solver(m.a, m.b, m.c, m.nair, 3, 1e-11, m.sigma, m.freqs, oopt)
+    for p, q in zip(_impedances(m, ex), _impedances(m, exo)):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
