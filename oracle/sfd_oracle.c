@@ -29,6 +29,29 @@
  *     on both T and P (src/div_correc.f90:593-615,709-750), hence the same phi
  *     correction; the reference's anisotropic branch (src/div_correc.f90:616-664,
  *     751-819) is not restated because its P and T disagree in sign and scale.
+ *
+ * How each routine is pinned (tests/test_oracle.py, tests/test_abi_cpu.py):
+ *   orc_build_system   row-scaled operator complex symmetric; curl(grad) = 0 at
+ *                      sigma = 0; layered model = 1-D FD solution, O(h^2) to the
+ *                      analytic field; full-tensor rotation equivariance (pins
+ *                      the off-diagonal conductivity averaging: a flipped xy sign
+ *                      or swapped xz/yz moves the result by 0.24 / 0.11 relative)
+ *   orc_ssor_apply     dense (D+L) D^-1 (D+U) factorisation
+ *   orc_bicgstab       iterates equal SciPy's BiCGStab; dense solution reached
+ *   orc_gpbicg         dense solution reached -- iterate-level parity unpinned
+ *   orc_cocr           dense solution reached (new solver: the reference's COCR
+ *                      cannot run, so there is nothing to be at parity with)
+ *   orc_dc_matrix      isotropic P, T equal the reference formulas x 8; DC drives
+ *                      G^T M x to the PCG tolerance
+ *   orc_mt1d_anl       plane wave in a uniform earth; axial anisotropy decouples
+ *                      the modes; two layers against a fine 1-D FD solve
+ *   orc_extint_v       polynomial reproduction of I2 / I4 on nonuniform nodes;
+ *                      extrapolation constants from the text
+ *   orc_mesh_coarsen   conductance integral preserved
+ *   post_process       rd = 1 exact on quadratic fields; half-space gives 100
+ *                      ohm-m and 45 degrees.  rd = 2 replicates the reference's
+ *                      transposed station indexing: parity unpinned (no
+ *                      reference fixture exists to confirm it)
  */
 #include <complex.h>
 #include <math.h>

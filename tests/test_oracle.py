@@ -436,3 +436,57 @@ def test_halfspace_apparent_resistivity_and_phase():
     res, _ = O.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, out_type=2)
     assert np.allclose(res[:, :2], 100.0, rtol=0.02)
     assert np.allclose(res[:, 2:], 45.0, atol=0.5)
+
+
+def _rotated_model_pair(seed=5):
+    """A symmetric grid and a full-tensor block model, plus the same model
+    rotated by +90 degrees about z: sigma' = R sigma R^T at the rotated cells."""
+    rng = np.random.default_rng(seed)
+    a = synth.padded_axis(4, 100.0, 2, 1.3)
+    c = synth.with_air(synth.earth_layers(3, 60.0, 2, 1.3), 2)
+    n, nz = len(a), len(c)
+    sig = np.zeros((n * n * nz, 6))
+    sig[:, :3] = 0.01                       # horizontally isotropic background
+    for k in range(3, 6):
+        for j in range(2, 6):
+            for i in range(3, 7):
+                S = synth.euler_tensor(*(10 ** rng.uniform(-2.5, -0.5, 3)), *rng.uniform(0, np.pi, 3))
+                sig[(k * n + j) * n + i] = synth.tensor_to_cols(S)
+    synth.set_air(sig, n, n, 2, 1e-6)
+    R = np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    rot = np.zeros_like(sig)
+    for k in range(nz):
+        for jp in range(n):
+            for ip in range(n):
+                s = sig[(k * n + (n - 1 - ip)) * n + jp]     # original cell (i=j', j=n-1-i')
+                S = np.array([[s[0], s[3], s[4]], [s[3], s[1], s[5]], [s[4], s[5], s[2]]])
+                rot[(k * n + jp) * n + ip] = synth.tensor_to_cols(R @ S @ R.T)
+    return a, c, sig, rot
+
+
+def test_full_tensor_operator_is_rotation_equivariant():
+    """Pins the anisotropic off-diagonal conductivity averaging of fd_assemb
+    (src/fdfe_main.f90:257-260,305-308,354-357) without retyping it: on a grid
+    symmetric under a 90-degree rotation about z, the discrete solution of the
+    rotated full-tensor model must be the rotated solution of the original
+    (E'_pol1 = -R E_pol2, E'_pol2 = R E_pol1), which fails for a wrong cell
+    index, a dropped coupling or a sign error in any xy/xz/yz term."""
+    a, c, sig, rot = _rotated_model_pair()
+    n, nz = len(a), len(c)
+    f = 1.0
+    S0 = O.build_system(a, a, c, sig, f)
+    S1 = O.build_system(a, a, c, rot, f)
+    x1, x2 = O.direct_solve(S0)
+    y1, y2 = O.direct_solve(S1)
+    xn = n * (n + 1) * (nz + 1)
+    ex = lambda v: v[:xn].reshape(nz + 1, n + 1, n)          # [k, j, i]
+    ey = lambda v: v[xn:2 * xn].reshape(nz + 1, n, n + 1)    # [k, j, i]
+    ez = lambda v: v[2 * xn:].reshape(nz, n + 1, n + 1)
+    flipT = lambda v: np.transpose(v[:, ::-1, :], (0, 2, 1))
+    # E'(x') = R E(R^T x'):  E'_x[k,j',i'] = -E_y[k, n-1-i', j'],  E'_y[k,j',i'] = E_x[k, n-i', j'],
+    # E'_z[k,j',i'] = E_z[k, n-i', j'];  E'_pol1 = -R E_pol2 and E'_pol2 = +R E_pol1
+    for yp, xs, sgn in ((y1, x2, -1.0), (y2, x1, 1.0)):
+        scale = np.abs(yp).max()
+        assert np.abs(ex(yp) - sgn * -flipT(ey(xs))).max() / scale < 1e-6
+        assert np.abs(ey(yp) - sgn * flipT(ex(xs))).max() / scale < 1e-6
+        assert np.abs(ez(yp) - sgn * flipT(ez(xs))).max() / scale < 1e-6
