@@ -478,11 +478,90 @@ SolveInfo bicgstab(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu,
     return info;
 }
 
+// Right-preconditioned GPBiCG (pgpbicg, src/solvers.f90:137-315; Zhang 1997) on
+// A_s with conjugated inner products as the reference's dot_product.  All
+// scalars stay on the device; the host reads the residual once per iteration.
+// A divergence correction every corr_intv iterations is followed by a full
+// restart (the reference resets r and r0 only, src/solvers.f90:223-235).
+template <typename R>
+SolveInfo gpbicg(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *f, cx<R> *x,
+                 double bnorm, double eps, int maxit, const KOpts &ko)
+{
+    const Dims &d = D.g.d;
+    const std::string pf = tag + (sizeof(R) == 8 ? "/g64" : "/g32");
+    GPVecs<R> v;
+    cx<R> **all[] = {&v.p, &v.rp, &v.u, &v.w, &v.at, &v.ap, &v.y, &v.t, &v.r, &v.q, &v.tp, &v.tmp, &v.z, &v.r0};
+    for (int q = 0; q < 14; q++) *all[q] = pool_vec<cx<R>>(ctx, pf + std::to_string(q), d.NL);
+    v.x = x;
+    SliceLaunch L = edge_launch(d);
+    const int64_t nbe = L.nblocks();
+    const double vecb = (double)d.NL * 2.0 * sizeof(R);
+    const double b_spmv = 2 * vecb + (double)d.NC * 6 * sizeof(R), b_ssor = 2 * (b_spmv + vecb);
+    SolveInfo info;
+    if (ko.dc_at_start) {
+        info.pcg_iters += div_correct<R>(ctx, D, tag, x, ko.corr_tol);
+        info.dc_calls++;
+    }
+    bool start = true;
+    int step = 0;  // iterations since the last (re)start
+    while (true) {
+        if (start) {
+            launch_spmv<R>(D.g, wmu, x, v.r, (R)-1, 0, nullptr, ctx->rb, ctx->st);
+            if (f) launch_axpby<R>(d.NL, cx<R>(1), f, cx<R>(1), v.r, ctx->st);
+            launch_copy<R>(d.NL, v.r, v.r0, ctx->st);
+            cx<R> *zero[] = {v.p, v.u, v.w, v.at, v.ap, v.t, v.tp, v.z};
+            for (cx<R> *zv : zero) SFDX_CHECK(hipMemsetAsync(zv, 0, d.NL * sizeof(cx<R>), ctx->st));
+            int64_t nbd;
+            launch_dot<R>(1, d.NL, v.r0, v.r, ctx->rb, ctx->st, nbd);
+            reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_GP_INIT, ctx->st);
+            launch_vec<R>(5, D.g, ctx->rb, v.r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
+            reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+            read_slots(ctx);
+            start = false;
+            step = 0;
+            info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
+            if (info.relres >= eps) apply_prec<R>(ctx, D, tag, wmu, v.r, v.rp, ko);
+        } else {
+            info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
+        }
+        if (!(info.relres == info.relres) || std::isinf(info.relres)) { info.status = SFDX_ERR_NAN; break; }
+        if (ko.verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  gpbicg it %d relres %.3e\n", info.iters, info.relres);
+        if (info.relres < eps) { info.status = 0; break; }
+        if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
+        if (ctx->hslots[SL_BREAK] != 0) { info.status = SFDX_ERR_BREAKDOWN; break; }
+        if (ko.corr_intv > 0 && info.iters > 0 && step > 0 && info.iters % ko.corr_intv == 0) {
+            info.pcg_iters += div_correct<R>(ctx, D, tag, x, ko.corr_tol);
+            info.dc_calls++;
+            start = true;
+            continue;
+        }
+        launch_gpbicg<R>(0, D.g, ctx->rb, v, ctx->st);                         // w, p
+        launch_spmv<R>(D.g, wmu, v.p, v.ap, (R)1, 2, v.r0, ctx->rb, ctx->st);  // ap, (r0,ap)
+        reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_BICG_ALPHA, ctx->st);         // alpha = rho/(r0,ap)
+        launch_gpbicg<R>(1, D.g, ctx->rb, v, ctx->st);                         // y, t
+        apply_prec<R>(ctx, D, tag, wmu, v.ap, v.tmp, ko);                      // tmp = M ap
+        launch_gpbicg<R>(2, D.g, ctx->rb, v, ctx->st);                         // q, tp
+        launch_spmv<R>(D.g, wmu, v.tp, v.at, (R)1, 0, nullptr, ctx->rb, ctx->st);
+        launch_gpbicg<R>(3, D.g, ctx->rb, v, ctx->st);                         // 5 inner products
+        reduce_final(ctx->rb, nbe, 8, SL_TMP, step == 0 ? OP_GP_OMEGA1 : OP_GP_OMEGA, ctx->st);
+        launch_gpbicg<R>(4, D.g, ctx->rb, v, ctx->st);                         // u, z, x, r
+        reduce_final(ctx->rb, nbe, 3, SL_TMP, OP_GP_BETA, ctx->st);
+        read_slots(ctx);
+        info.iters++;
+        step++;
+        info.bytes += 2 * b_spmv + 2 * b_ssor + 30 * vecb;
+        const double rel = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
+        if (rel >= eps && ctx->hslots[SL_BREAK] == 0) apply_prec<R>(ctx, D, tag, wmu, v.r, v.rp, ko);
+    }
+    return info;
+}
+
 template <typename R>
 SolveInfo krylov(sfdx_ctx *ctx, int solver, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *f, cx<R> *x,
                  double bnorm, double eps, int maxit, const KOpts &ko)
 {
     if (solver == SFDX_SOLVER_COCR) return cocr<R>(ctx, D, tag, wmu, f, x, bnorm, eps, maxit, ko);
+    if (solver == SFDX_SOLVER_GPBICG) return gpbicg<R>(ctx, D, tag, wmu, f, x, bnorm, eps, maxit, ko);
     return bicgstab<R>(ctx, D, tag, wmu, f, x, bnorm, eps, maxit, ko);
 }
 
@@ -667,7 +746,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         g_err = "grid dimensions must be divisible by 2^level and nair by 2^(level-1)";
         return SFDX_ERR_MESH;
     }
-    if (o->solver != SFDX_SOLVER_COCR && o->solver != SFDX_SOLVER_BICGSTAB) {
+    if (o->solver != SFDX_SOLVER_COCR && o->solver != SFDX_SOLVER_BICGSTAB && o->solver != SFDX_SOLVER_GPBICG) {
         g_err = "solver not supported on the device path";
         return SFDX_ERR_UNSUPPORTED;
     }

@@ -20,6 +20,7 @@ enum Slot {
     SL_PR = 24,      // PCG |r|^2
     SL_T = 26,       // |t|^2 of the DC right-hand side
     SL_BREAK = 28,   // breakdown flag
+    SL_ETA = 30,     // complex (GPBiCG)
     SL_COUNT = 32
 };
 
@@ -36,7 +37,11 @@ enum ScalarOp {
     OP_PCG_ALPHA = 9,   // PALPHA = PRHO / tmp
     OP_PCG_BETA = 10,   // PBETA = tmp[0]/PRHO ; PRHO = tmp[0] ; PR = tmp[1]
     OP_T = 11,          // T = tmp[0]
-    OP_BICG_INIT = 12   // RHO = ALPHA = OMEGA = 1
+    OP_BICG_INIT = 12,  // RHO = ALPHA = OMEGA = 1
+    OP_GP_INIT = 13,    // RHO = tmp[0..1]; BETA = 0
+    OP_GP_OMEGA1 = 14,  // first GPBiCG step: OMEGA = (at,t)/(at,at); ETA = 0
+    OP_GP_OMEGA = 15,   // OMEGA, ETA from the 2x2 minimisation (tmp[0..7])
+    OP_GP_BETA = 16     // RN = tmp[0]; BETA = (ALPHA/OMEGA) tmp[1..2]/RHO; RHO = tmp[1..2]
 };
 
 __device__ __forceinline__ void cdiv(double ar, double ai, double br, double bi, double &cr, double &ci)
@@ -90,6 +95,41 @@ __device__ __forceinline__ void scalar_step(double *s, int op)
         s[SL_PR] = t[1];
         break;
     case OP_T: s[SL_T] = t[0]; break;
+    case OP_GP_INIT:
+        s[SL_RHO] = t[0]; s[SL_RHO + 1] = t[1];
+        s[SL_BETA] = 0; s[SL_BETA + 1] = 0;
+        s[SL_BREAK] = 0;
+        break;
+    case OP_GP_OMEGA1:
+        // tmp: [0] (y,y) [1] (at,at) [2..3] (at,t) [4..5] (y,t) [6..7] (at,y), conj first argument
+        s[SL_OMEGA] = t[2] / t[1];
+        s[SL_OMEGA + 1] = t[3] / t[1];
+        s[SL_ETA] = 0; s[SL_ETA + 1] = 0;
+        if (t[1] == 0) s[SL_BREAK] = 1;
+        break;
+    case OP_GP_OMEGA: {
+        // den = (at,at)(y,y) - (y,at)(at,y) = (at,at)(y,y) - |(at,y)|^2  (real)
+        const double den = t[1] * t[0] - (t[6] * t[6] + t[7] * t[7]);
+        // omega = ((y,y)(at,t) - (y,t)(at,y)) / den
+        s[SL_OMEGA] = (t[0] * t[2] - (t[4] * t[6] - t[5] * t[7])) / den;
+        s[SL_OMEGA + 1] = (t[0] * t[3] - (t[4] * t[7] + t[5] * t[6])) / den;
+        // eta = ((at,at)(y,t) - (y,at)(at,t)) / den with (y,at) = conj((at,y))
+        s[SL_ETA] = (t[1] * t[4] - (t[6] * t[2] + t[7] * t[3])) / den;
+        s[SL_ETA + 1] = (t[1] * t[5] - (t[6] * t[3] - t[7] * t[2])) / den;
+        if (den == 0) s[SL_BREAK] = 1;
+        break;
+    }
+    case OP_GP_BETA: {
+        s[SL_RN] = t[0];
+        double qr, qi, ar, ai;
+        cdiv(t[1], t[2], s[SL_RHO], s[SL_RHO + 1], qr, qi);
+        cdiv(s[SL_ALPHA], s[SL_ALPHA + 1], s[SL_OMEGA], s[SL_OMEGA + 1], ar, ai);
+        s[SL_BETA] = qr * ar - qi * ai;
+        s[SL_BETA + 1] = qr * ai + qi * ar;
+        s[SL_RHO] = t[1]; s[SL_RHO + 1] = t[2];
+        if (t[1] == 0 && t[2] == 0) s[SL_BREAK] = 1;
+        break;
+    }
     case OP_BICG_INIT:
         s[SL_RHO] = 1; s[SL_RHO + 1] = 0;
         s[SL_ALPHA] = 1; s[SL_ALPHA + 1] = 0;
@@ -174,6 +214,20 @@ void launch_restrict(const Dims &df, const Dims &dc, const double *const wl[3], 
 template <typename R>
 void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *zc, cx<R> *zf,
                         hipStream_t st);
+// GPBiCG work vectors (pgpbicg, src/solvers.f90:137-315)
+template <typename R>
+struct GPVecs {
+    cx<R> *p, *rp, *u, *w, *at, *ap, *y, *t, *r, *q, *tp, *tmp, *z, *x, *r0;
+};
+// mode 0: w = at + beta ap; p = rp + beta (p - u)
+// mode 1: y = t - r - alpha w + alpha ap; t = r - alpha ap
+// mode 2: q = tp - rp + beta u; tp = rp - alpha tmp
+// mode 3: partial sums of (y,y) (at,at) (at,t) (y,t) (at,y) (conjugated first argument)
+// mode 4: u = omega tmp + eta q; z = omega rp + eta z - alpha u; x += alpha p + z;
+//         r = t - eta y - omega at; partial sums |r/ve|^2 (interior) and (r0,r)
+template <typename R>
+void launch_gpbicg(int mode, const DevGrid<R> &g, RedBuf &rb, const GPVecs<R> &v, hipStream_t st);
+
 template <typename R>
 void launch_extrap(int64_t n, R c1, const cx<R> *u1, R c2, cx<R> *w, hipStream_t st);
 

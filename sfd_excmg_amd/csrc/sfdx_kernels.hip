@@ -368,6 +368,67 @@ __global__ __launch_bounds__(TPB) void k_vec(Dims d, const R *__restrict__ a, co
     if (MODE == 0 || MODE == 2 || MODE == 5) block_reduce_store<4>(acc, partials);
 }
 
+// GPBiCG fused vector kernels (modes documented at launch_gpbicg)
+template <typename R, int MODE>
+__global__ __launch_bounds__(TPB) void k_gpbicg(Dims d, const R *__restrict__ a, const R *__restrict__ b,
+                                                const R *__restrict__ c, const double *__restrict__ slots,
+                                                GPVecs<R> v, double *__restrict__ partials)
+{
+    const Slice S = edge_slice(d, blockIdx.y);
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (p < S.nI * S.nJ) {
+        const int j = p / S.nI, i = p - j * S.nI;
+        const int64_t e = S.base + p;
+        const cx<R> al((R)slots[SL_ALPHA], (R)slots[SL_ALPHA + 1]);
+        const cx<R> be((R)slots[SL_BETA], (R)slots[SL_BETA + 1]);
+        const cx<R> om((R)slots[SL_OMEGA], (R)slots[SL_OMEGA + 1]);
+        const cx<R> et((R)slots[SL_ETA], (R)slots[SL_ETA + 1]);
+        if (MODE == 0) {
+            v.w[e] = v.at[e] + be * v.ap[e];
+            v.p[e] = v.rp[e] + be * (v.p[e] - v.u[e]);
+        } else if (MODE == 1) {
+            const cx<R> r = v.r[e], ap = v.ap[e];
+            v.y[e] = v.t[e] - r - al * v.w[e] + al * ap;
+            v.t[e] = r - al * ap;
+        } else if (MODE == 2) {
+            const cx<R> rp = v.rp[e];
+            v.q[e] = v.tp[e] - rp + be * v.u[e];
+            v.tp[e] = rp - al * v.tmp[e];
+        } else if (MODE == 3) {
+            const cx<R> y = v.y[e], at = v.at[e], t = v.t[e];
+            acc[0] = (double)y.re * y.re + (double)y.im * y.im;
+            acc[1] = (double)at.re * at.re + (double)at.im * at.im;
+            acc[2] = (double)at.re * t.re + (double)at.im * t.im;   // conj(at) t
+            acc[3] = (double)at.re * t.im - (double)at.im * t.re;
+            acc[4] = (double)y.re * t.re + (double)y.im * t.im;     // conj(y) t
+            acc[5] = (double)y.re * t.im - (double)y.im * t.re;
+            acc[6] = (double)at.re * y.re + (double)at.im * y.im;   // conj(at) y
+            acc[7] = (double)at.re * y.im - (double)at.im * y.re;
+        } else {
+            const cx<R> u = om * v.tmp[e] + et * v.q[e];
+            v.u[e] = u;
+            const cx<R> z = om * v.rp[e] + et * v.z[e] - al * u;
+            v.z[e] = z;
+            v.x[e] = v.x[e] + al * v.p[e] + z;
+            const cx<R> r = v.t[e] - et * v.y[e] - om * v.at[e];
+            v.r[e] = r;
+            if (edge_interior(d, S.o, i, j, S.k)) {
+                const double ve = (double)dual_volume(a, b, c, S.o, i, j, S.k);
+                acc[0] = ((double)r.re * r.re + (double)r.im * r.im) / (ve * ve);
+            }
+            const cx<R> r0 = v.r0[e];
+            acc[1] = (double)r0.re * r.re + (double)r0.im * r.im;
+            acc[2] = (double)r0.re * r.im - (double)r0.im * r.re;
+        }
+    }
+    if (MODE == 3) block_reduce_store<8>(acc, partials);
+    if (MODE == 4) {
+        double a3[3] = {acc[0], acc[1], acc[2]};
+        block_reduce_store<3>(a3, partials);
+    }
+}
+
 // generic dots over all edges: mode 0: u^T v (unconjugated); 1: conj(u)^T v
 template <typename R, int MODE>
 __global__ __launch_bounds__(TPB) void k_dot(int64_t n, const cx<R> *__restrict__ u, const cx<R> *__restrict__ v,
@@ -1223,6 +1284,21 @@ void launch_restrict(const Dims &df, const Dims &dc, const double *const wl[3], 
 }
 
 template <typename R>
+void launch_gpbicg(int mode, const DevGrid<R> &g, RedBuf &rb, const GPVecs<R> &v, hipStream_t st)
+{
+    SliceLaunch L = edge_launch(g.d);
+#define SFDX_GP(M) hipLaunchKernelGGL((k_gpbicg<R, M>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, rb.slots, v, rb.partials)
+    switch (mode) {
+    case 0: SFDX_GP(0); break;
+    case 1: SFDX_GP(1); break;
+    case 2: SFDX_GP(2); break;
+    case 3: SFDX_GP(3); break;
+    default: SFDX_GP(4); break;
+    }
+#undef SFDX_GP
+}
+
+template <typename R>
 void launch_prolong_node_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *pc, cx<R> *pf,
                              hipStream_t st)
 {
@@ -1288,6 +1364,7 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
     template void launch_prolong_node_add<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *,    \
                                              cx<R> *, hipStream_t);                                               \
     template void launch_grad_add<R>(const DevGrid<R> &, cx<R>, const cx<R> *, cx<R> *, hipStream_t);            \
+    template void launch_gpbicg<R>(int, const DevGrid<R> &, RedBuf &, const GPVecs<R> &, hipStream_t);           \
     template void launch_restrict<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *, cx<R> *,   \
                                      hipStream_t);                                                                \
     template void launch_prolong_add<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *,         \

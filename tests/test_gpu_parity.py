@@ -4,6 +4,8 @@ Tolerances: FP64 kernels 1e-12 relative (stencil/SSOR/prolongation are the
 same arithmetic in a different order); complex64 kernels 1e-5 relative;
 impedances of full solves 1e-6 relative at an interior residual of 1e-10.
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -11,6 +13,8 @@ import scipy.sparse as sp
 import oracle as O
 from sfd_excmg_amd import sfdx, synth
 from test_oracle import tiny_model
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 pytestmark = pytest.mark.gpu
 
@@ -134,7 +138,7 @@ def small_excmg_model(aniso=False):
     return m
 
 
-@pytest.mark.parametrize("solver", [sfdx.SOLVER_BICGSTAB, sfdx.SOLVER_COCR])
+@pytest.mark.parametrize("solver", [sfdx.SOLVER_BICGSTAB, sfdx.SOLVER_GPBICG, sfdx.SOLVER_COCR])
 @pytest.mark.parametrize("aniso", [False, True])
 def test_fwd_solve_fp64_matches_oracle(ctx, solver, aniso):
     m = small_excmg_model(aniso)
@@ -187,3 +191,31 @@ def test_halfspace_on_gpu(ctx):
     res, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, out_type=2)
     assert np.allclose(res[:, :2], 100.0, rtol=0.02)
     assert np.allclose(res[:, 2:], 45.0, atol=0.5)
+
+
+def test_fortran_driver_matches_library(ctx, tmp_path):
+    """The drop-in Fortran driver (reference command line and file formats)
+    reads <name>.fwd/.dat, solves through the C-ABI and writes the same
+    apparent resistivities/phases as the library called directly."""
+    import subprocess
+    drv = os.path.join(ROOT, "sfd_excmg_amd", "bin", "sfd_excmg_gpu")
+    assert os.path.exists(drv), "Fortran driver not built"
+    m0 = small_excmg_model(True)
+    m0.out_type = 2
+    prefix = str(tmp_path / "drv")
+    synth.write_fwd(m0, prefix)
+    synth.write_dat(m0, prefix)
+    m = synth.read_fwd_dat(prefix, sigma_air=1e-8)
+    env = dict(os.environ, SFDX_FWD_ACC="1", SFDX_SOLVER="5", SFDX_NORM="interior", SFDX_RES_FULL="1",
+               SFDX_SIGMA_AIR="1e-8", SFDX_CORR_INTV="0")
+    print("running", drv, prefix, flush=True)
+    out = subprocess.run([drv, prefix, "3", "1e-10"], cwd=str(tmp_path), env=env, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    got = np.loadtxt(prefix + ".resx")
+    opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR)
+    ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
+    assert st.status == 0
+    ref, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, out_type=2, rd=2)
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref) / np.abs(ref)) < 1e-6
