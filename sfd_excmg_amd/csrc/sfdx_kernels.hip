@@ -31,6 +31,34 @@ HipError::HipError(hipError_t c, const char *expr, const char *file, int line) :
 
 static constexpr int TPB = 256;
 
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8 XCDs
+// (linear block id mod 8), and each XCD has its own L2.  The launchers pad every
+// 2-D grid to a multiple of 8 blocks, and the kernels renumber their block so
+// that XCD x runs the contiguous tile range [x n/8, (x+1) n/8).  Tiles are
+// ordered plane by plane (and, for edges, orientation within a plane), so the
+// k-1/k+1 planes and the other orientations a stencil row reads are fetched
+// into the same L2 instead of into several.
+struct Tile {
+    int x, y;
+};
+__device__ __forceinline__ Tile tile_id()
+{
+    const unsigned gx = gridDim.x, n = gx * gridDim.y;
+    const unsigned lin = blockIdx.y * gx + blockIdx.x;
+    const unsigned t = (lin & 7u) * (n >> 3) + (lin >> 3);
+    Tile T;
+    T.y = (int)(t / gx);
+    T.x = (int)(t - (unsigned)T.y * gx);
+    return T;
+}
+// smallest gy' >= gy with gx * gy' a multiple of 8
+static unsigned pad_y(unsigned gx, unsigned gy)
+{
+    while ((gx * gy) % 8u) gy++;
+    return gy;
+}
+static constexpr int kNoTile = 1 << 30;  // thread index of a padding tile: fails every range test
+
 // ------------------------------------------------------------------ slices
 struct Slice {
     int o, k, nI, nJ;
@@ -39,15 +67,22 @@ struct Slice {
 
 __device__ __forceinline__ Slice edge_slice(const Dims &d, int s)
 {
+    // slice s = 3k + o: the three orientations of plane k are adjacent tiles
     Slice q;
-    if (s <= d.nz) {
-        q.o = 0; q.k = s; q.nI = d.nx; q.nJ = d.ny + 1;
+    q.k = s / 3;
+    q.o = s - 3 * q.k;
+    if (q.k > d.nz || (q.o == 2 && q.k == d.nz)) {  // padding tile
+        q.o = 0; q.k = 0; q.nI = 0; q.nJ = 0; q.base = 0;
+        return q;
+    }
+    if (q.o == 0) {
+        q.nI = d.nx; q.nJ = d.ny + 1;
         q.base = (int64_t)q.k * q.nI * q.nJ;
-    } else if (s <= 2 * d.nz + 1) {
-        q.o = 1; q.k = s - (d.nz + 1); q.nI = d.nx + 1; q.nJ = d.ny;
+    } else if (q.o == 1) {
+        q.nI = d.nx + 1; q.nJ = d.ny;
         q.base = d.XN + (int64_t)q.k * q.nI * q.nJ;
     } else {
-        q.o = 2; q.k = s - 2 * (d.nz + 1); q.nI = d.nx + 1; q.nJ = d.ny + 1;
+        q.nI = d.nx + 1; q.nJ = d.ny + 1;
         q.base = d.XN + d.YN + (int64_t)q.k * q.nI * q.nJ;
     }
     return q;
@@ -58,7 +93,8 @@ SliceLaunch edge_launch(const Dims &d)
     SliceLaunch L;
     int64_t plane = (int64_t)(d.nx + 1) * (d.ny + 1);
     L.nslices = 3 * d.nz + 2;
-    L.grid = dim3((unsigned)((plane + TPB - 1) / TPB), (unsigned)L.nslices, 1);
+    const unsigned gx = (unsigned)((plane + TPB - 1) / TPB);
+    L.grid = dim3(gx, pad_y(gx, (unsigned)L.nslices), 1);
     L.block = dim3(TPB, 1, 1);
     return L;
 }
@@ -68,7 +104,8 @@ SliceLaunch node_launch(const Dims &d)
     SliceLaunch L;
     int64_t plane = (int64_t)(d.nx + 1) * (d.ny + 1);
     L.nslices = d.nz + 1;
-    L.grid = dim3((unsigned)((plane + TPB - 1) / TPB), (unsigned)L.nslices, 1);
+    const unsigned gx = (unsigned)((plane + TPB - 1) / TPB);
+    L.grid = dim3(gx, pad_y(gx, (unsigned)L.nslices), 1);
     L.block = dim3(TPB, 1, 1);
     return L;
 }
@@ -85,7 +122,8 @@ SliceLaunch colour_launch(const Dims &d, int colour)
     else { plane = (int64_t)(d.ny + 1) * W; ns = d.nz; }
     SliceLaunch L;
     L.nslices = ns;
-    L.grid = dim3((unsigned)((plane + TPB - 1) / TPB), (unsigned)ns, 1);
+    const unsigned gx = (unsigned)((plane + TPB - 1) / TPB);
+    L.grid = dim3(gx, pad_y(gx, (unsigned)ns), 1);
     L.block = dim3(TPB, 1, 1);
     return L;
 }
@@ -191,8 +229,9 @@ __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, c
                                               const cx<R> *__restrict__ w, double *__restrict__ partials,
                                               const cx<R> *__restrict__ g2)
 {
-    const Slice S = edge_slice(d, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     double acc[4] = {0, 0, 0, 0};
     if (p < S.nI * S.nJ) {
         const int j = p / S.nI, i = p - j * S.nI;
@@ -236,8 +275,9 @@ __global__ __launch_bounds__(TPB) void k_mass(Dims d, const R *__restrict__ a, c
                                               const R *__restrict__ c, const R *__restrict__ sig,
                                               const cx<R> *__restrict__ x, cx<R> *__restrict__ w)
 {
-    const Slice S = edge_slice(d, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     if (p >= S.nI * S.nJ) return;
     const int j = p / S.nI, i = p - j * S.nI;
     const int64_t e = S.base + p;
@@ -260,8 +300,9 @@ __global__ __launch_bounds__(TPB) void k_ssor(Dims d, const R *__restrict__ a, c
                                               const R *__restrict__ c, const R *__restrict__ sig, R wmu,
                                               const cx<R> *__restrict__ in, cx<R> *__restrict__ out, R omega)
 {
-    const int k = blockIdx.y;
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const int k = TL.y;
+    const int p = (k < (O == 2 ? d.nz : d.nz + 1)) ? TL.x * TPB + threadIdx.x : kNoTile;
     int i, j;
     if (!colour_decode<O>(d, PAR, k, p, i, j)) return;
     const int64_t e = (O == 0) ? ex_id(d, i, j, k) : (O == 1) ? ey_id(d, i, j, k) : ez_id(d, i, j, k);
@@ -317,8 +358,9 @@ __global__ __launch_bounds__(TPB) void k_vec(Dims d, const R *__restrict__ a, co
                                              cx<R> *v0, cx<R> *v1, cx<R> *v2, cx<R> *v3, cx<R> *v4, cx<R> *v5,
                                              double *__restrict__ partials)
 {
-    const Slice S = edge_slice(d, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     double acc[4] = {0, 0, 0, 0};
     if (p < S.nI * S.nJ) {
         const int j = p / S.nI, i = p - j * S.nI;
@@ -374,8 +416,9 @@ __global__ __launch_bounds__(TPB) void k_gpbicg(Dims d, const R *__restrict__ a,
                                                 const R *__restrict__ c, const double *__restrict__ slots,
                                                 GPVecs<R> v, double *__restrict__ partials)
 {
-    const Slice S = edge_slice(d, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (p < S.nI * S.nJ) {
         const int j = p / S.nI, i = p - j * S.nI;
@@ -488,8 +531,9 @@ template <typename R>
 __global__ __launch_bounds__(TPB) void k_dirichlet(Dims d, const cx<R> *__restrict__ edc, int pol,
                                                    cx<R> *__restrict__ x, int zero_inner)
 {
-    const Slice S = edge_slice(d, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     if (p >= S.nI * S.nJ) return;
     const int j = p / S.nI, i = p - j * S.nI;
     const int64_t e = S.base + p;
@@ -510,8 +554,9 @@ __global__ __launch_bounds__(TPB) void k_node_div(Dims d, const R *__restrict__ 
                                                   const R *__restrict__ c, const cx<R> *__restrict__ w,
                                                   cx<R> *__restrict__ t, double *__restrict__ partials)
 {
-    const int k = blockIdx.y;
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const int k = TL.y;
+    const int p = (k <= d.nz) ? TL.x * TPB + threadIdx.x : kNoTile;
     const int nI = d.nx + 1, nJ = d.ny + 1;
     double acc[2] = {0, 0};
     if (p < nI * nJ) {
@@ -542,8 +587,9 @@ __global__ __launch_bounds__(TPB) void k_pbuild(Dims d, const double *__restrict
                                                 const double *__restrict__ c, const double *__restrict__ sig,
                                                 R *__restrict__ P)
 {
-    const int k = blockIdx.y;
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const int k = TL.y;
+    const int p = (k <= d.nz) ? TL.x * TPB + threadIdx.x : kNoTile;
     const int nI = d.nx + 1, nJ = d.ny + 1;
     if (p >= nI * nJ) return;
     const int j = p / nI, i = p - j * nI;
@@ -600,8 +646,9 @@ template <typename R, int DOT>
 __global__ __launch_bounds__(TPB) void k_papply(Dims d, const R *__restrict__ P, const cx<R> *__restrict__ x,
                                                 cx<R> *__restrict__ y, double *__restrict__ partials)
 {
-    const int k = blockIdx.y;
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const int k = TL.y;
+    const int p = (k <= d.nz) ? TL.x * TPB + threadIdx.x : kNoTile;
     const int nI = d.nx + 1, nJ = d.ny + 1;
     double acc[2] = {0, 0};
     if (p < nI * nJ) {
@@ -668,8 +715,9 @@ __global__ __launch_bounds__(TPB) void k_grad_update(Dims d, const R *__restrict
                                                      const R *__restrict__ c, const cx<R> *__restrict__ phi,
                                                      cx<R> *__restrict__ x)
 {
-    const Slice S = edge_slice(d, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     if (p >= S.nI * S.nJ) return;
     const int j = p / S.nI, i = p - j * S.nI;
     const int64_t e = S.base + p;
@@ -693,9 +741,10 @@ __global__ __launch_bounds__(TPB) void k_pssor(Dims d, const R *__restrict__ P, 
                                                cx<R> *__restrict__ phi)
 {
     constexpr int ci = COL & 1, cj = (COL >> 1) & 1, ck = (COL >> 2) & 1;
-    const int k = ck + 2 * (int)blockIdx.y;
+    const Tile TL = tile_id();
+    const int k = ck + 2 * TL.y;
     const int W = (d.nx + 2) / 2;
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const int p = TL.x * TPB + threadIdx.x;
     const int r = p / W, q = p - r * W;
     const int i = ci + 2 * q, j = cj + 2 * r;
     if (k > d.nz || i > d.nx || j > d.ny) return;
@@ -725,8 +774,9 @@ __global__ __launch_bounds__(TPB) void k_gradT(Dims d, const R *__restrict__ a, 
                                                const R *__restrict__ c, const cx<R> *__restrict__ w,
                                                cx<R> *__restrict__ t)
 {
-    const int k = blockIdx.y;
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const int k = TL.y;
+    const int p = (k <= d.nz) ? TL.x * TPB + threadIdx.x : kNoTile;
     const int nI = d.nx + 1, nJ = d.ny + 1;
     if (p >= nI * nJ) return;
     const int j = p / nI, i = p - j * nI;
@@ -746,8 +796,9 @@ __global__ __launch_bounds__(TPB) void k_grad_add(Dims d, const R *__restrict__ 
                                                   const R *__restrict__ c, cx<R> s, const cx<R> *__restrict__ phi,
                                                   cx<R> *__restrict__ z)
 {
-    const Slice S = edge_slice(d, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     if (p >= S.nI * S.nJ) return;
     const int j = p / S.nI, i = p - j * S.nI;
     const int64_t e = S.base + p;
@@ -763,8 +814,9 @@ __global__ __launch_bounds__(TPB) void k_grad(Dims d, const R *__restrict__ a, c
                                               const R *__restrict__ c, const cx<R> *__restrict__ phi,
                                               cx<R> *__restrict__ g)
 {
-    const Slice S = edge_slice(d, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     if (p >= S.nI * S.nJ) return;
     const int j = p / S.nI, i = p - j * S.nI;
     int64_t n0, n1;
@@ -779,8 +831,9 @@ __global__ __launch_bounds__(TPB) void k_hyb_combine(Dims d, const double *__res
                                                      const double *__restrict__ c, zd s, const zf *__restrict__ phi,
                                                      const zf *__restrict__ z32, zd *__restrict__ z64)
 {
-    const Slice S = edge_slice(d, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     if (p >= S.nI * S.nJ) return;
     const int j = p / S.nI, i = p - j * S.nI;
     const int64_t e = S.base + p;
@@ -814,8 +867,9 @@ template <typename R>
 __global__ __launch_bounds__(TPB) void k_prolong_add(Dims df, Dims dc, MGTables T, const cx<R> *__restrict__ zc,
                                                      cx<R> *__restrict__ zf)
 {
-    const Slice S = edge_slice(df, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(df, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     if (p >= S.nI * S.nJ) return;
     const int j = p / S.nI, i = p - j * S.nI, k = S.k;
     if (!edge_interior(df, S.o, i, j, k)) return;
@@ -851,8 +905,9 @@ template <typename R>
 __global__ __launch_bounds__(TPB) void k_restrict(Dims df, Dims dc, MGTables T, const cx<R> *__restrict__ rf,
                                                   cx<R> *__restrict__ rc)
 {
-    const Slice S = edge_slice(dc, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(dc, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     if (p >= S.nI * S.nJ) return;
     const int J = p / S.nI, I = p - J * S.nI, K = S.k;
     const int64_t ec = S.base + p;
@@ -892,8 +947,9 @@ template <typename R>
 __global__ __launch_bounds__(TPB) void k_prolong_node_add(Dims df, Dims dc, MGTables T, const cx<R> *__restrict__ pc,
                                                           cx<R> *__restrict__ pf)
 {
-    const int k = blockIdx.y;
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const int k = TL.y;
+    const int p = (k <= df.nz) ? TL.x * TPB + threadIdx.x : kNoTile;
     const int nI = df.nx + 1, nJ = df.ny + 1;
     if (p >= nI * nJ) return;
     const int j = p / nI, i = p - j * nI;
@@ -932,8 +988,9 @@ template <typename R>
 __global__ __launch_bounds__(TPB) void k_prolong(Dims df, Dims dc, int m, ProlongTables T,
                                                  const cx<R> *__restrict__ uc, cx<R> *__restrict__ uf)
 {
-    const Slice S = edge_slice(df, blockIdx.y);
-    const int p = blockIdx.x * TPB + threadIdx.x;
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(df, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
     if (p >= S.nI * S.nJ) return;
     const int j = p / S.nI, i = p - j * S.nI, k = S.k;
     const int64_t e = S.base + p;
@@ -1159,7 +1216,8 @@ template <typename R, int COL, int DIR>
 static void pssor_one(const Dims &d, const R *P, const cx<R> *t, cx<R> *phi, hipStream_t st)
 {
     const int W = (d.nx + 2) / 2, H = (d.ny + 2) / 2;
-    dim3 gr((unsigned)(((int64_t)W * H + TPB - 1) / TPB), (unsigned)((d.nz + 2) / 2), 1);
+    const unsigned gx = (unsigned)(((int64_t)W * H + TPB - 1) / TPB);
+    dim3 gr(gx, pad_y(gx, (unsigned)((d.nz + 2) / 2)), 1);
     hipLaunchKernelGGL((k_pssor<R, COL, DIR>), gr, dim3(TPB), 0, st, d, P, t, phi);
 }
 
