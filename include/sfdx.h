@@ -94,6 +94,8 @@ typedef struct {
     double restart_drop; /* SFDX_PREC_MIXED: restart the FP64 Krylov recurrences from
                             the true residual whenever the residual has dropped by
                             this factor since the last restart (0 = never) */
+    int use_graph;      /* 1: replay the steady-state COCR iteration as a captured HIP
+                           graph (only without periodic divergence correction) */
 } sfdx_opts;
 
 typedef struct {

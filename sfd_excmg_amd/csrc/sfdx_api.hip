@@ -61,6 +61,8 @@ struct sfdx_ctx {
     RedBuf rb;
     Pool pool;
     double *hslots = nullptr;  // pinned
+    double *hlag = nullptr;    // pinned, 2 x SL_COUNT: lagged convergence test
+    hipEvent_t lag_ev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -84,6 +86,20 @@ void read_slots(sfdx_ctx *ctx)
 {
     SFDX_CHECK(hipMemcpyAsync(ctx->hslots, ctx->rb.slots, SL_COUNT * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
     SFDX_CHECK(hipStreamSynchronize(ctx->st));
+}
+
+// asynchronous copy of the scalar slots into pinned buffer `slot` (0 or 1)
+void post_slots(sfdx_ctx *ctx, int slot)
+{
+    SFDX_CHECK(hipMemcpyAsync(ctx->hlag + slot * SL_COUNT, ctx->rb.slots, SL_COUNT * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->st));
+    SFDX_CHECK(hipEventRecord(ctx->lag_ev[slot], ctx->st));
+}
+
+const double *wait_slots(sfdx_ctx *ctx, int slot)
+{
+    SFDX_CHECK(hipEventSynchronize(ctx->lag_ev[slot]));
+    return ctx->hlag + slot * SL_COUNT;
 }
 
 // Device copy of one grid level in precision R (plus FP64 copies for P assembly).
@@ -251,6 +267,27 @@ void mg_vcycle(sfdx_ctx *ctx, MGHier<R> &H, int l, R wmu, const cx<R> *r, cx<R> 
     smooth_add(1);
 }
 
+// One captured HIP graph of a fixed launch sequence on a stream (kernel nodes
+// only: no memset or host-synchronising call is issued inside the capture).
+struct StreamGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    template <typename F>
+    void capture(hipStream_t st, F &&enqueue)
+    {
+        SFDX_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        enqueue();
+        SFDX_CHECK(hipStreamEndCapture(st, &g));
+        SFDX_CHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    }
+    void launch(hipStream_t st) { SFDX_CHECK(hipGraphLaunch(exec, st)); }
+    ~StreamGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
 // ---------------------------------------------------------------- Krylov
 struct KOpts {
     int corr_intv = 0;
@@ -265,6 +302,7 @@ struct KOpts {
     MGHier<float> *mg32 = nullptr;
     int mg_level = 0;                // level being solved
     double restart_drop = 0;         // COCR: restart from the true residual after this drop
+    int use_graph = 1;               // replay the steady-state iteration as a HIP graph
 };
 
 // out = M^-1 in with the chosen preconditioner
@@ -343,9 +381,18 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     int state = 0;
     bool fresh = false;
     double rel0 = 0;
+    // Lagged convergence test: the scalars of iteration i are copied to pinned
+    // memory asynchronously and examined only after iteration i+1 has been
+    // queued, so the host never drains the stream between iterations.  A stop
+    // decision taken on lagged values is re-checked on the newest values.
+    double cur[SL_COUNT];
+    bool pending = false;
+    int pend_slot = 0;
+    StreamGraph graph;
     while (true) {
         if (state != 2) {
             fresh = true;
+            pending = false;
             launch_spmv<R>(D.g, wmu, x, r, (R)-1, 0, nullptr, ctx->rb, ctx->st);
             if (f) launch_axpby<R>(d.NL, cx<R>(1), f, cx<R>(1), r, ctx->st);
             apply_prec<R>(ctx, D, tag, wmu, r, z, ko);
@@ -361,17 +408,24 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             launch_vec<R>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
             reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
             read_slots(ctx);
+            std::memcpy(cur, ctx->hslots, sizeof(cur));
             info.bytes += 2 * b_spmv + b_ssor + 6 * vecb;
             state = 2;
         }
-        info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
-        if (!(info.relres == info.relres) || std::isinf(info.relres)) { info.status = SFDX_ERR_NAN; break; }
-        if (ko.verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  cocr it %d relres %.3e\n", info.iters, info.relres);
+        info.relres = std::sqrt(cur[SL_RN]) / bnorm;
+        const bool bad = !(info.relres == info.relres) || std::isinf(info.relres);
         if (fresh) { rel0 = info.relres; fresh = false; }
-        if (info.relres < eps) { info.status = 0; break; }
-        if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
-        if (ctx->hslots[SL_BREAK] != 0) { info.status = SFDX_ERR_BREAKDOWN; break; }
-        if (ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0) {
+        const bool restart = ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0;
+        if (bad || info.relres < eps || info.iters >= maxit || cur[SL_BREAK] != 0 || restart) {
+            if (pending) {  // decide on the newest iteration instead
+                std::memcpy(cur, wait_slots(ctx, pend_slot), sizeof(cur));
+                pending = false;
+                continue;
+            }
+            if (bad) { info.status = SFDX_ERR_NAN; break; }
+            if (info.relres < eps) { info.status = 0; break; }
+            if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
+            if (cur[SL_BREAK] != 0) { info.status = SFDX_ERR_BREAKDOWN; break; }
             // the preconditioner is applied in complex64: its rounding is not a
             // fixed linear operator, so the short recurrences drift; restart
             // from the true FP64 residual
@@ -379,12 +433,34 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             info.restarts++;
             continue;
         }
-        int64_t nbd;
-        apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko);
-        launch_dot<R>(0, d.NL, Ap, q, ctx->rb, ctx->st, nbd);
-        reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
-        launch_vec<R>(0, D.g, ctx->rb, x, pv, Ap, r, q, z, ctx->st);
-        reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+        if (ko.verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  cocr it %d relres %.3e\n", info.iters, info.relres);
+        auto part1 = [&]() {
+            int64_t nbd;
+            apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko);
+            launch_dot<R>(0, d.NL, Ap, q, ctx->rb, ctx->st, nbd);
+            reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
+            launch_vec<R>(0, D.g, ctx->rb, x, pv, Ap, r, q, z, ctx->st);
+            reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+        };
+        auto part2 = [&]() {
+            launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
+            reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_BETA, ctx->st);
+            launch_vec<R>(3, D.g, ctx->rb, pv, z, Ap, Az, nullptr, nullptr, ctx->st);
+        };
+        if (ko.use_graph && ko.corr_intv <= 0 && pending) {
+            // steady state: the whole iteration is one captured graph, replayed
+            // (every buffer and the nodal operator exist after the first iteration)
+            if (!graph.exec) graph.capture(ctx->st, [&]() { part1(); part2(); });
+            graph.launch(ctx->st);
+            info.iters++;
+            info.bytes += b_ssor + 2 * vecb + 9 * vecb + b_spmv + 6 * vecb;
+            const int slot = info.iters & 1;
+            post_slots(ctx, slot);
+            std::memcpy(cur, wait_slots(ctx, pend_slot), sizeof(cur));
+            pend_slot = slot;
+            continue;
+        }
+        part1();
         info.iters++;
         info.bytes += b_ssor + 2 * vecb + 9 * vecb;
         if (ko.corr_intv > 0 && info.iters % ko.corr_intv == 0) {
@@ -396,11 +472,13 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             state = (ko.dc_restart || ko.precond != SFDX_PRECOND_SSOR) ? 0 : 1;
             continue;
         }
-        launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
-        reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_BETA, ctx->st);
-        launch_vec<R>(3, D.g, ctx->rb, pv, z, Ap, Az, nullptr, nullptr, ctx->st);
+        part2();
         info.bytes += b_spmv + 6 * vecb;
-        read_slots(ctx);
+        const int slot = info.iters & 1;
+        post_slots(ctx, slot);
+        if (pending) std::memcpy(cur, wait_slots(ctx, pend_slot), sizeof(cur));
+        pending = true;
+        pend_slot = slot;
     }
     return info;
 }
@@ -575,6 +653,7 @@ KOpts kopts_of(const sfdx_opts &o)
     k.dc_restart = o.dc_restart;
     k.dc_at_start = o.dc_at_start;
     k.precond = o.precond;
+    k.use_graph = o.use_graph;
     return k;
 }
 
@@ -945,6 +1024,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->mg_levels = 0;
     o->mg_coarse_pairs = 4;
     o->restart_drop = 1e-3;
+    o->use_graph = 1;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)
@@ -958,6 +1038,8 @@ int sfdx_create(int device, sfdx_ctx **out)
         SFDX_CHECK(hipMalloc(&c->rb.slots, SL_COUNT * sizeof(double)));
         SFDX_CHECK(hipMemset(c->rb.slots, 0, SL_COUNT * sizeof(double)));
         SFDX_CHECK(hipHostMalloc(&c->hslots, SL_COUNT * sizeof(double), hipHostMallocDefault));
+        SFDX_CHECK(hipHostMalloc(&c->hlag, 2 * SL_COUNT * sizeof(double), hipHostMallocDefault));
+        for (int q = 0; q < 2; q++) SFDX_CHECK(hipEventCreateWithFlags(&c->lag_ev[q], hipEventDisableTiming));
         *out = c;
         return SFDX_OK;
     });
@@ -972,6 +1054,9 @@ int sfdx_destroy(sfdx_ctx *c)
     if (c->rb.partials) (void)hipFree(c->rb.partials);
     if (c->rb.slots) (void)hipFree(c->rb.slots);
     if (c->hslots) (void)hipHostFree(c->hslots);
+    if (c->hlag) (void)hipHostFree(c->hlag);
+    for (int q = 0; q < 2; q++)
+        if (c->lag_ev[q]) (void)hipEventDestroy(c->lag_ev[q]);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return SFDX_OK;

@@ -171,17 +171,18 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[K], double *parti
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+        const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+        const int64_t nblk = (int64_t)gridDim.x * gridDim.y;
 #pragma unroll
         for (int q = 0; q < K; q++) {
             double t = 0;
             for (int w = 0; w < TPB / 64; w++) t += sm[q][w];
-            partials[blk * 8 + q] = t;
+            partials[q * nblk + blk] = t;  // component-major: the final pass reads contiguously
         }
     }
 }
 
-// Sum partials[nblk][8] (first K columns) into slots[dst..dst+K) then apply the
+// Sum partials[K][nblk] (component-major) into slots[dst..dst+K) then apply the
 // scalar step `op` (single thread).  1024 threads, fixed summation order.
 __global__ __launch_bounds__(1024) void k_reduce_final(const double *__restrict__ partials, int64_t nblk, int K,
                                                        double *slots, int dst, int op)
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(1024) void k_reduce_final(const double *__restrict_
     __shared__ double sm[8][16];
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x)
-        for (int q = 0; q < K; q++) acc[q] += partials[b * 8 + q];
+        for (int q = 0; q < K; q++) acc[q] += partials[q * nblk + b];
     for (int q = 0; q < K; q++) {
         double t = acc[q];
         for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
@@ -367,7 +368,10 @@ __global__ __launch_bounds__(TPB) void k_vec(Dims d, const R *__restrict__ a, co
         const int64_t e = S.base + p;
         const bool inner = edge_interior(d, S.o, i, j, S.k);
         if (MODE == 0) {
-            const cx<R> al((R)slots[SL_ALPHA], (R)slots[SL_ALPHA + 1]);
+            // after a breakdown (flag set by the scalar step) the update is a no-op,
+            // so an iteration queued before the host saw the flag leaves x intact
+            const bool brk = slots[SL_BREAK] != 0;
+            const cx<R> al(brk ? (R)0 : (R)slots[SL_ALPHA], brk ? (R)0 : (R)slots[SL_ALPHA + 1]);
             cx<R> ap = v2[e], q = v4[e];
             v0[e] = fma_c(v0[e], al, v1[e]);
             cx<R> rr = v3[e] - al * ap;

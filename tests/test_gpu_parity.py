@@ -219,3 +219,18 @@ def test_fortran_driver_matches_library(ctx, tmp_path):
     ref, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, out_type=2, rd=2)
     assert got.shape == ref.shape
     assert np.max(np.abs(got - ref) / np.abs(ref)) < 1e-6
+
+
+@pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
+def test_graph_replay_matches_eager(ctx, precision):
+    """The steady-state COCR iteration replayed as a captured HIP graph gives
+    bit-identical iterates to eager launches (the reductions are deterministic),
+    across hundreds of replays interleaved with eager restarts and copies."""
+    m = small_excmg_model(True)
+    kw = dict(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR,
+              precision=precision, sigma_floor32=0.0)
+    ex0, st0 = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(use_graph=0, **kw))
+    ex1, st1 = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(use_graph=1, **kw))
+    assert st0.status == 0 and st1.status == 0
+    assert st0.fine_iters_total == st1.fine_iters_total
+    assert np.array_equal(ex0, ex1)
