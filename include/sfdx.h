@@ -134,6 +134,29 @@ int sfdx_post_process(const sfdx_grid *g, int nf, const double *freq, int nrec, 
  * column-major; ex, hx: 2*nz+2 complex; imp: 4 complex. */
 int sfdx_mt1d_anl(int nz, const double *conv, const double *c, double freq, double *ex, double *hx, double *imp);
 
+/* One model of a batched solve: grid, conductivity sigma(ncell,6), its nf
+ * frequencies and the caller-allocated ex(2*nl, nf) it receives (same layouts
+ * as sfdx_fwd_solve).  level > 0 overrides opts.level for this model. */
+typedef struct {
+    sfdx_grid grid;
+    const double *sigma;
+    int nf;
+    const double *freq;
+    double *ex;
+    int level;
+} sfdx_job;
+
+/* Batched forward solves: every (model, frequency) pair of `jobs` (both
+ * polarisations each) is an independent work item; items run concurrently on
+ * `nstreams` HIP streams (0 = 8, at most 16), longest first, each stream with
+ * its own workspace, so small uneven models fill the GPU together.  Results are
+ * identical to calling sfdx_fwd_solve per model.  st (optional) receives one
+ * sfdx_stats per job (times and iterations summed over its frequencies).
+ * Replaces the serial frequency loop of fwd_solver (fdfe_main.f90:1580-1592)
+ * repeated over models; the reference has no batched entry point. */
+int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_opts *o, int nstreams,
+                     sfdx_stats *st);
+
 /* Direct solve of one grid level for both polarisations on the host (MKL
  * PARDISO, complex symmetric): replaces doublemode_nor with mid = 1, 2
  * (fdfe_main.f90:997-998, 1028-1029) -> pardiso_spd (solvers.f90:765-875).

@@ -12,8 +12,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
 #include <map>
 #include <memory>
+#include <thread>
 
 #include "../../include/sfdx.h"
 #include "sfdx_host.h"
@@ -63,6 +66,7 @@ struct sfdx_ctx {
     double *hslots = nullptr;  // pinned
     double *hlag = nullptr;    // pinned, 2 x SL_COUNT: lagged convergence test
     hipEvent_t lag_ev[2] = {nullptr, nullptr};
+    std::vector<sfdx_ctx *> workers;  // per-stream contexts of sfdx_solve_batch
 };
 
 namespace {
@@ -1048,6 +1052,8 @@ int sfdx_create(int device, sfdx_ctx **out)
 int sfdx_destroy(sfdx_ctx *c)
 {
     if (!c) return SFDX_OK;
+    for (sfdx_ctx *w : c->workers) sfdx_destroy(w);
+    c->workers.clear();
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->st);
     c->pool.release();
@@ -1071,6 +1077,89 @@ int sfdx_fwd_solve(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     return guarded([&]() {
         SFDX_CHECK(hipSetDevice(ctx->device));
         return fwd_solve_impl(ctx, g, sigma, nf, freq, o, ex, st);
+    });
+}
+
+int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_opts *o, int nstreams,
+                     sfdx_stats *st)
+{
+    if (!ctx || njobs <= 0 || !jobs) return SFDX_ERR_ARG;
+    for (int j = 0; j < njobs; j++)
+        if (!jobs[j].sigma || !jobs[j].freq || !jobs[j].ex || jobs[j].nf <= 0 || !jobs[j].grid.a || !jobs[j].grid.b ||
+            !jobs[j].grid.c)
+            return SFDX_ERR_ARG;
+    sfdx_opts def;
+    if (!o) { sfdx_default_opts(&def); o = &def; }
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        struct Item {
+            int j, f;
+            double cost;
+        };
+        std::vector<Item> items;
+        for (int j = 0; j < njobs; j++) {
+            const sfdx_grid &g = jobs[j].grid;
+            const double nl = (double)make_dims(g.nx, g.ny, g.nz).NL;
+            for (int f = 0; f < jobs[j].nf; f++) items.push_back({j, f, nl});
+        }
+        // longest processing time first
+        std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
+        const int ns = std::max(1, std::min({nstreams > 0 ? nstreams : 8, 16, (int)items.size()}));
+        while ((int)ctx->workers.size() < ns) {
+            sfdx_ctx *w = nullptr;
+            const int rc = sfdx_create(ctx->device, &w);
+            if (rc != SFDX_OK) return rc;
+            ctx->workers.push_back(w);
+        }
+        std::vector<int> rcs(items.size(), 0);
+        std::vector<std::string> errs(items.size());
+        std::vector<sfdx_stats> sts(items.size());
+        std::atomic<int> next{0};
+        auto worker = [&](int w) {
+            (void)hipSetDevice(ctx->device);
+            for (int k = next++; k < (int)items.size(); k = next++) {
+                const Item &it = items[k];
+                const sfdx_job &J = jobs[it.j];
+                sfdx_opts oj = *o;
+                if (J.level > 0) oj.level = J.level;
+                const int64_t nl = make_dims(J.grid.nx, J.grid.ny, J.grid.nz).NL;
+                double *exf = J.ex + (size_t)it.f * 2 * nl * 2;
+                std::memset(&sts[k], 0, sizeof(sfdx_stats));
+                rcs[k] = guarded([&]() { return fwd_solve_impl(ctx->workers[w], &J.grid, J.sigma, 1, J.freq + it.f, &oj, exf, &sts[k]); });
+                if (rcs[k] > 0) errs[k] = g_err;
+            }
+        };
+        std::vector<std::thread> th;
+        for (int w = 0; w < ns; w++) th.emplace_back(worker, w);
+        for (auto &t : th) t.join();
+        int worst = SFDX_OK;
+        for (size_t k = 0; k < items.size(); k++) {
+            if (rcs[k] > 0) {
+                g_err = "job " + std::to_string(items[k].j) + " frequency " + std::to_string(items[k].f) + ": " + errs[k];
+                return rcs[k];
+            }
+            if (rcs[k] < 0) worst = rcs[k];
+        }
+        if (st) {
+            for (int j = 0; j < njobs; j++) std::memset(&st[j], 0, sizeof(sfdx_stats));
+            for (size_t k = 0; k < items.size(); k++) {
+                sfdx_stats &S = st[items[k].j];
+                const sfdx_stats &T = sts[k];
+                S.total_seconds += T.total_seconds;
+                S.fine_seconds += T.fine_seconds;
+                S.fine_bytes += T.fine_bytes;
+                S.fine_iters_total += T.fine_iters_total;
+                if (T.status < 0) S.status = T.status;
+                if (items[k].f == jobs[items[k].j].nf - 1) {
+                    std::memcpy(S.iters, T.iters, sizeof(S.iters));
+                    std::memcpy(S.relres, T.relres, sizeof(S.relres));
+                    std::memcpy(S.dc_calls, T.dc_calls, sizeof(S.dc_calls));
+                    std::memcpy(S.pcg_iters, T.pcg_iters, sizeof(S.pcg_iters));
+                    std::memcpy(S.refine_steps, T.refine_steps, sizeof(S.refine_steps));
+                }
+            }
+        }
+        return worst;
     });
 }
 

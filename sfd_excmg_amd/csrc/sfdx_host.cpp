@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 
 namespace sfdx {
@@ -271,6 +272,8 @@ typedef void (*pardiso_fn)(void *, const int *, const int *, const int *, const 
 
 static pardiso_fn load_pardiso(std::string &err)
 {
+    static std::mutex mu;  // batched solves call this from several threads
+    std::lock_guard<std::mutex> lock(mu);
     static pardiso_fn fn = nullptr;
     static std::string cached_err;
     static bool tried = false;

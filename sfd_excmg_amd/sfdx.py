@@ -69,9 +69,14 @@ class Stats(ctypes.Structure):
                     fine_iters_total=self.fine_iters_total)
 
 
+class Job(ctypes.Structure):
+    _fields_ = [("grid", Grid), ("sigma", ctypes.c_void_p), ("nf", ctypes.c_int), ("freq", ctypes.c_void_p),
+                ("ex", ctypes.c_void_p), ("level", ctypes.c_int)]
+
+
 EXPORTS = ["sfdx_create", "sfdx_destroy", "sfdx_last_error", "sfdx_default_opts", "sfdx_fwd_solve",
            "sfdx_post_process", "sfdx_mt1d_anl", "sfdx_apply_operator", "sfdx_ssor_apply", "sfdx_div_correct",
-           "sfdx_extint_v", "sfdx_bench_kernels", "sfdx_direct_solve"]
+           "sfdx_extint_v", "sfdx_bench_kernels", "sfdx_direct_solve", "sfdx_solve_batch"]
 
 _lib = None
 
@@ -98,6 +103,7 @@ def load(path: str = LIB_PATH):
     L.sfdx_extint_v.argtypes = [P, ctypes.POINTER(Grid), P, P, P]
     L.sfdx_bench_kernels.argtypes = [P, ctypes.POINTER(Grid), P, D, I, I, P]
     L.sfdx_direct_solve.argtypes = [ctypes.POINTER(Grid), P, D, P]
+    L.sfdx_solve_batch.argtypes = [P, I, ctypes.POINTER(Job), ctypes.POINTER(Opts), I, ctypes.POINTER(Stats)]
     _lib = L
     return L
 
@@ -176,6 +182,33 @@ class Context:
                                        _ptr(ex), ctypes.byref(st))
         _check(code)
         return ex.T.copy(), st   # ex(2*nl, nf) like the reference
+
+    def solve_batch(self, models, opts: Opts | None = None, nstreams: int = 0):
+        """Batched forward solves (sfdx_solve_batch): every (model, frequency)
+        pair runs as an independent item on one of `nstreams` concurrent HIP
+        streams.  `models`: objects with a, b, c, nair, sigma, freqs and an
+        optional level.  Returns ([ex(2*nl, nf) per model], [Stats per model])."""
+        models = list(models)
+        n = len(models)
+        jobs = (Job * n)()
+        keep, outs = [], []
+        for k, m in enumerate(models):
+            G = _GridHolder(m.a, m.b, m.c, m.nair)
+            sf = sigma_flat(m.sigma)
+            fr = np.ascontiguousarray(np.atleast_1d(m.freqs), np.float64)
+            ex = np.zeros((len(fr), 2 * nedges(len(G.a), len(G.b), len(G.c))), np.complex128)
+            jobs[k].grid = G.g
+            jobs[k].sigma = _ptr(sf)
+            jobs[k].nf = len(fr)
+            jobs[k].freq = _ptr(fr)
+            jobs[k].ex = _ptr(ex)
+            jobs[k].level = int(getattr(m, "level", 0) or 0)
+            keep.append((G, sf, fr))
+            outs.append(ex)
+        st = (Stats * n)()
+        o = opts if opts is not None else default_opts()
+        _check(self.lib.sfdx_solve_batch(self.h, n, jobs, ctypes.byref(o), int(nstreams), st))
+        return [ex.T.copy() for ex in outs], list(st)
 
     def apply_operator(self, a, b, c, sigma, freq, x, precision=PREC_FP64):
         G = _GridHolder(a, b, c)

@@ -234,3 +234,31 @@ def test_graph_replay_matches_eager(ctx, precision):
     assert st0.status == 0 and st1.status == 0
     assert st0.fine_iters_total == st1.fine_iters_total
     assert np.array_equal(ex0, ex1)
+
+
+@pytest.mark.parametrize("use_excmg", [0, 1])
+def test_batched_solve_matches_single_solves(ctx, use_excmg):
+    """sfdx_solve_batch over models of uneven sizes and several frequencies,
+    run concurrently on several streams, returns what one-model sfdx_fwd_solve
+    calls return.  Without EXCMG the whole solve is device arithmetic and the
+    fields are bit-identical.  With EXCMG the two coarsest levels are host
+    PARDISO solves, which run in parallel threads and may round differently;
+    the converged solutions then agree at the solver tolerance, checked on the
+    impedances."""
+    ms = [synth.make_iso_block_model("b0", 16, 16, 12, 4, seed=21, nblocks=2, freq=1.0, level=3, nstations=4),
+          synth.make_iso_block_model("b1", 24, 16, 16, 8, seed=22, nblocks=2, freq=1.0, level=3, nstations=4),
+          small_excmg_model(True)]
+    ms[0].freqs = np.array([0.1, 10.0])
+    ms[1].freqs = np.array([1.0, 3.0, 30.0])
+    opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR,
+                             use_excmg=use_excmg)
+    exb, stb = ctx.solve_batch(ms, opts, nstreams=4)
+    for m, ex, st in zip(ms, exb, stb):
+        ex1, st1 = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
+        assert st.status == 0 and st1.status == 0
+        assert ex.shape == ex1.shape
+        if not use_excmg:
+            assert np.array_equal(ex, ex1)
+        # rd=1: the reference's transposed station sampling (rd=2) needs nx == ny
+        for p, q in zip(_impedances(m, ex, rd=1), _impedances(m, ex1, rd=1)):
+            assert np.max(np.abs(p - q) / np.abs(q)) < 1e-7
