@@ -107,8 +107,11 @@ typedef struct {
     double relres[SFDX_MAX_LEVELS][2];
     int dc_calls[SFDX_MAX_LEVELS][2];
     int pcg_iters[SFDX_MAX_LEVELS][2];
-    int refine_steps[2];
+    int refine_steps[2];            /* fine level: restarts (SFDX_PREC_MIXED) or refinement steps */
     int fine_iters_total;           /* summed over frequencies and polarisations */
+    int fallback_fp64;              /* solves (any level, frequency, polarisation) that switched from
+                                       the complex64 to the complex128 preconditioner after repeated
+                                       restarts without progress */
 } sfdx_stats;
 
 /* Context: device selection, stream, workspace. */

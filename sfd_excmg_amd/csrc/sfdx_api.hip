@@ -153,7 +153,7 @@ void ensure_P(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag)
 }
 
 struct SolveInfo {
-    int iters = 0, dc_calls = 0, pcg_iters = 0, status = 0, restarts = 0;
+    int iters = 0, dc_calls = 0, pcg_iters = 0, status = 0, restarts = 0, fallback = 0;
     double relres = 0;
     double bytes = 0;
 };
@@ -285,11 +285,14 @@ struct StreamGraph {
         SFDX_CHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
     }
     void launch(hipStream_t st) { SFDX_CHECK(hipGraphLaunch(exec, st)); }
-    ~StreamGraph()
+    void reset()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (g) (void)hipGraphDestroy(g);
+        exec = nullptr;
+        g = nullptr;
     }
+    ~StreamGraph() { reset(); }
 };
 
 // ---------------------------------------------------------------- Krylov
@@ -364,8 +367,9 @@ void apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
 // r, z = M^-1 r and A z are recomputed and the search directions are kept.
 template <typename R>
 SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *f, cx<R> *x, double bnorm,
-               double eps, int maxit, const KOpts &ko)
+               double eps, int maxit, const KOpts &ko_in)
 {
+    KOpts ko = ko_in;  // the complex64 preconditioner may be dropped below
     const Dims &d = D.g.d;
     const R omega = (R)ko.omega;
     const std::string p = tag + (sizeof(R) == 8 ? "/k64" : "/k32");
@@ -393,6 +397,14 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     bool pending = false;
     int pend_slot = 0;
     StreamGraph graph;
+    // Safeguards of the complex64 preconditioner (restart_drop > 0): restart on
+    // divergence (residual 100x above the best since the last restart) or on
+    // stagnation (no new best for 200 iterations); after 3 restarts in a row
+    // that did not lower the best residual by 2x, continue with the complex128
+    // preconditioner, which is a fixed linear operator.
+    double best = 1e300, best_at_restart = 1e300;
+    int since_best = 0, idle_restarts = 0;
+    const bool guard = ko.restart_drop > 0;
     while (true) {
         if (state != 2) {
             fresh = true;
@@ -418,8 +430,11 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
         }
         info.relres = std::sqrt(cur[SL_RN]) / bnorm;
         const bool bad = !(info.relres == info.relres) || std::isinf(info.relres);
-        if (fresh) { rel0 = info.relres; fresh = false; }
-        const bool restart = ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0;
+        if (fresh) { rel0 = info.relres; fresh = false; best = info.relres; since_best = 0; }
+        if (info.relres < best) { best = info.relres; since_best = 0; }
+        const bool diverging = guard && info.relres > 100 * best;
+        const bool stalled = guard && since_best > 200;
+        const bool restart = (ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0) || diverging || stalled;
         if (bad || info.relres < eps || info.iters >= maxit || cur[SL_BREAK] != 0 || restart) {
             if (pending) {  // decide on the newest iteration instead
                 std::memcpy(cur, wait_slots(ctx, pend_slot), sizeof(cur));
@@ -433,6 +448,17 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             // the preconditioner is applied in complex64: its rounding is not a
             // fixed linear operator, so the short recurrences drift; restart
             // from the true FP64 residual
+            if (guard) {
+                idle_restarts = (best < 0.5 * best_at_restart) ? 0 : idle_restarts + 1;
+                best_at_restart = std::min(best_at_restart, best);
+                if (idle_restarts >= 3 && (ko.lo || ko.mg32)) {
+                    ko.lo = nullptr;
+                    ko.mg32 = nullptr;
+                    ko.restart_drop = 0;
+                    info.fallback = 1;
+                    graph.reset();  // the captured iteration used the complex64 preconditioner
+                }
+            }
             state = 0;
             info.restarts++;
             continue;
@@ -451,6 +477,7 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_BETA, ctx->st);
             launch_vec<R>(3, D.g, ctx->rb, pv, z, Ap, Az, nullptr, nullptr, ctx->st);
         };
+        since_best++;
         if (ko.use_graph && ko.corr_intv <= 0 && pending) {
             // steady state: the whole iteration is one captured graph, replayed
             // (every buffer and the nodal operator exist after the first iteration)
@@ -957,6 +984,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                     S.fine_iters_total += R.info.iters;
                     S.refine_steps[pol] = R.refine;
                 }
+                S.fallback_fp64 += R.info.fallback;
                 S.iters[i - 1][pol] = R.info.iters;
                 S.relres[i - 1][pol] = R.info.relres;
                 S.dc_calls[i - 1][pol] = R.info.dc_calls;
@@ -1149,6 +1177,7 @@ int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_
                 S.fine_seconds += T.fine_seconds;
                 S.fine_bytes += T.fine_bytes;
                 S.fine_iters_total += T.fine_iters_total;
+                S.fallback_fp64 += T.fallback_fp64;
                 if (T.status < 0) S.status = T.status;
                 if (items[k].f == jobs[items[k].j].nf - 1) {
                     std::memcpy(S.iters, T.iters, sizeof(S.iters));
