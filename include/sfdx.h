@@ -160,6 +160,32 @@ typedef struct {
 int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_opts *o, int nstreams,
                      sfdx_stats *st);
 
+/* Data misfit of the fields ex(2*nl, nf) (host only, FP64):
+ *   misfit = 1/2 sum_{f, station, xy|yx} (log10 rho_a - log10 rho_obs)^2 + (phi - phi_obs)^2
+ * with phi in radians and rho_a, phi computed as post_process does (rd as
+ * there); obs holds nf*nrec rows (rho_xy, rho_yx, phi_xy, phi_yx) in ohm-m and
+ * degrees, i.e. post_process out_type 2 output of the observed data.  If w is
+ * non-null it receives the field gradient (2*nl*nf complex) with
+ * d misfit = Re(w^T d ex).  No counterpart in the reference (inversion support). */
+int sfdx_misfit(const sfdx_grid *g, int nf, const double *freq, int nrec, const double *recv, int rd,
+                const double *obs, const double *ex, double *misfit, double *w);
+
+/* Misfit and its gradient with respect to log(s_c), s_c a scale factor on the
+ * conductivity tensor of every cell c (for isotropic sigma: d/d log sigma_c).
+ * Per frequency: forward solve (as sfdx_fwd_solve), field gradient
+ * (sfdx_misfit), adjoint solves A_s^T lam = w with the forward kernels (A_s is
+ * complex symmetric) and homogeneous Dirichlet data, then
+ * grad_c += Re(i omega mu lam^T M_c e) summed over both polarisations, M_c the
+ * cell's corner-lumped edge mass.  Only one frequency's fields live on the
+ * device at a time.  The Dirichlet data of the 1-D background are held fixed
+ * (their dependence on the corner column, fdfe_main.f90:938-942, is not
+ * differentiated).  grad: ncell doubles; ex (optional): the forward fields
+ * 2*nl*nf; st (optional): forward-solve statistics.  No counterpart in the
+ * reference. */
+int sfdx_adjoint(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf, const double *freq, int nrec,
+                 const double *recv, int rd, const double *obs, const sfdx_opts *o, double *misfit, double *grad,
+                 double *ex, sfdx_stats *st);
+
 /* Direct solve of one grid level for both polarisations on the host (MKL
  * PARDISO, complex symmetric): replaces doublemode_nor with mid = 1, 2
  * (fdfe_main.f90:997-998, 1028-1029) -> pardiso_spd (solvers.f90:765-875).

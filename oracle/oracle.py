@@ -548,3 +548,44 @@ def post_process(a, b, c, nair, freqs, recv, ex_all, out_type=1, rd=2):
         ty = (ar1[:, 2] * ar2[:, 4] - ar2[:, 2] * ar1[:, 4]) / tmp
         tip[sl] = np.stack([tx.real, tx.imag, ty.real, ty.imag], 1)
     return res, tip
+
+
+# ---------------------------------------------------------------- misfit and its sensitivity
+def misfit(a, b, c, nair, freqs, recv, ex_all, obs, rd=2):
+    """1/2 sum (log10 rho_a - log10 rho_obs)^2 + (phi - phi_obs)^2, phase in
+    radians, rho_a and phi from post_process out_type 2 (src/fdfe_main.f90:
+    686-697); obs rows (rho_xy, rho_yx, phi_xy, phi_yx) in ohm-m and degrees."""
+    res, _ = post_process(a, b, c, nair, freqs, recv, ex_all, out_type=2, rd=rd)
+    rr = np.log10(res[:, :2]) - np.log10(obs[:, :2])
+    rp = np.deg2rad(res[:, 2:] - obs[:, 2:])
+    return 0.5 * float(np.sum(rr ** 2) + np.sum(rp ** 2))
+
+
+def misfit_sensitivity(a, b, c, nair, sigma, freqs, recv, obs, cells, rd=2, h=1e-4):
+    """d misfit / d log(s_c), s_c a scale factor on cell c's conductivity
+    tensor, by tangent-linear sensitivities on the CPU in FP64: the assembled
+    operator (fd_assemb + dcfast) is affine in s_c, so A(2 sigma_c) - A(sigma_c)
+    is its exact derivative; the field sensitivity de = -A^-1 dA e comes from
+    SuperLU solves, and the misfit derivative along de from a central
+    difference of the (smooth) post-process.  The 1-D Dirichlet data are held
+    fixed, as in the device adjoint."""
+    freqs = np.atleast_1d(freqs)
+    nrec = len(recv)
+    out = np.zeros(len(cells))
+    for fi, f in enumerate(freqs):
+        S = build_system(a, b, c, sigma, f)
+        lu = spla.splu(S.A.tocsc())
+        e1, e2 = lu.solve(S.p1), lu.solve(S.p2)
+        ex = np.stack([np.concatenate([e1, e2])], axis=1)
+        ob = obs[fi * nrec:(fi + 1) * nrec]
+        for q, cell in enumerate(cells):
+            sig2 = np.array(sigma, dtype=np.float64, copy=True)
+            sig2[cell] *= 2.0
+            S2 = build_system(a, b, c, sig2, f, e_dc=S.e_dc)
+            d1 = -lu.solve(S2.A @ e1 - S2.p1)
+            d2 = -lu.solve(S2.A @ e2 - S2.p2)
+            de = np.stack([np.concatenate([d1, d2])], axis=1)
+            fp = misfit(a, b, c, nair, [f], recv, ex + h * de, ob, rd)
+            fm = misfit(a, b, c, nair, [f], recv, ex - h * de, ob, rd)
+            out[q] += (fp - fm) / (2 * h)
+    return out

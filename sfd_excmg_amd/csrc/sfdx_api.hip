@@ -1011,6 +1011,86 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     return worst;
 }
 
+// Misfit gradient with respect to log cell-tensor scale factors (sfdx_adjoint).
+int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf, const double *freq, int nrec,
+                 const double *recv, int rd, const double *obs, const sfdx_opts *o, double *misfit, double *grad,
+                 double *ex_out, sfdx_stats *st)
+{
+    const HostLevel fine = host_level(g, sigma);
+    const Dims d = make_dims(fine.nx, fine.ny, fine.nz);
+    const PostGrid pg{g->nx, g->ny, g->nz, g->nair, g->a, g->b, g->c};
+    LevelDev<double> D64 = upload_level<double>(ctx, fine, "ADJ");
+    LevelDev<float> D32;
+    if (o->precision == SFDX_PREC_MIXED) D32 = upload_level<float>(ctx, fine, "ADJ", o->sigma_floor32);
+    ensure_partials(ctx, edge_launch(d).nblocks() + 64);
+    double *gdev = pool_vec<double>(ctx, "adj/g", d.NC);
+    SFDX_CHECK(hipMemsetAsync(gdev, 0, d.NC * sizeof(double), ctx->st));
+    zd *ed = pool_vec<zd>(ctx, "adj/e", 2 * d.NL), *wd = pool_vec<zd>(ctx, "adj/w", 2 * d.NL);
+    zd *lam = pool_vec<zd>(ctx, "adj/lam", d.NL);
+    std::vector<zc> exf((size_t)2 * d.NL), wf((size_t)2 * d.NL);
+    // Adjoint solves: no divergence correction (the adjoint source is not
+    // divergence-free); the multigrid option falls back to the hybrid smoother.
+    KOpts ko = kopts_of(*o);
+    ko.corr_intv = 0;
+    ko.dc_at_start = 0;
+    if (ko.precond == SFDX_PRECOND_MG) ko.precond = SFDX_PRECOND_HYBRID;
+    if (o->precision == SFDX_PREC_MIXED) {
+        ko.lo = &D32;
+        ko.restart_drop = o->restart_drop;
+    }
+    sfdx_stats S;
+    std::memset(&S, 0, sizeof(S));
+    int worst = SFDX_OK;
+    double phi = 0;
+    SliceLaunch L = edge_launch(d);
+    for (int f = 0; f < nf; f++) {
+        sfdx_stats sf;
+        const int rc = fwd_solve_impl(ctx, g, sigma, 1, freq + f, o, (double *)exf.data(), &sf);
+        if (rc > 0) return rc;
+        if (rc < 0) worst = rc;
+        S.total_seconds += sf.total_seconds;
+        S.fine_seconds += sf.fine_seconds;
+        S.fine_iters_total += sf.fine_iters_total;
+        S.fallback_fp64 += sf.fallback_fp64;
+        if (ex_out) std::memcpy(ex_out + (size_t)f * 2 * d.NL * 2, exf.data(), (size_t)2 * d.NL * sizeof(zc));
+        double phf = 0;
+        const int bad = misfit_fields(pg, 1, freq + f, nrec, recv, rd, obs + (size_t)f * nrec * 4, exf.data(), phf,
+                                      wf.data());
+        if (bad) {
+            g_err = "station " + std::to_string(bad) + " lies outside the interpolation area";
+            return SFDX_ERR_ARG;
+        }
+        phi += phf;
+        SFDX_CHECK(hipMemcpyAsync(ed, exf.data(), (size_t)2 * d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+        SFDX_CHECK(hipMemcpyAsync(wd, wf.data(), (size_t)2 * d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+        const double wmu = 2 * 3.141592653589793 * freq[f] * kMu0;
+        for (int pol = 0; pol < 2; pol++) {
+            zd *w = wd + (size_t)pol * d.NL;
+            launch_zero_boundary(d, w, ctx->st);  // the Dirichlet values are data, not unknowns
+            launch_vec<double>(5, D64.g, ctx->rb, w, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
+            reduce_final(ctx->rb, L.nblocks(), 1, SL_TMP, OP_RN, ctx->st);
+            read_slots(ctx);
+            const double bnorm = std::sqrt(ctx->hslots[SL_RN]);
+            if (!(bnorm > 0)) continue;
+            SFDX_CHECK(hipMemsetAsync(lam, 0, d.NL * sizeof(zd), ctx->st));
+            SolveInfo si = krylov<double>(ctx, o->solver, D64, "ADJ", wmu, w, lam, bnorm, o->eps, o->maxit_plain, ko);
+            S.fallback_fp64 += si.fallback;
+            if (si.status > 0) {
+                g_err = "adjoint solve failed (frequency " + std::to_string(f) + ")";
+                return si.status;
+            }
+            if (si.status < 0) worst = si.status;
+            launch_cell_grad(D64.g, wmu, ed + (size_t)pol * d.NL, lam, gdev, ctx->st);
+        }
+    }
+    SFDX_CHECK(hipMemcpyAsync(grad, gdev, d.NC * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    SFDX_CHECK(hipStreamSynchronize(ctx->st));
+    *misfit = phi;
+    S.status = worst;
+    if (st) *st = S;
+    return worst;
+}
+
 template <typename F>
 int guarded(F &&fn)
 {
@@ -1189,6 +1269,34 @@ int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_
             }
         }
         return worst;
+    });
+}
+
+int sfdx_misfit(const sfdx_grid *g, int nf, const double *freq, int nrec, const double *recv, int rd,
+                const double *obs, const double *ex, double *misfit, double *w)
+{
+    if (!g || !freq || !recv || !obs || !ex || !misfit || nf <= 0 || nrec <= 0) return SFDX_ERR_ARG;
+    return guarded([&]() {
+        const PostGrid pg{g->nx, g->ny, g->nz, g->nair, g->a, g->b, g->c};
+        const int bad = misfit_fields(pg, nf, freq, nrec, recv, rd, obs, (const zc *)ex, *misfit, (zc *)w);
+        if (bad) {
+            g_err = "station " + std::to_string(bad) + " lies outside the interpolation area";
+            return (int)SFDX_ERR_ARG;
+        }
+        return (int)SFDX_OK;
+    });
+}
+
+int sfdx_adjoint(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf, const double *freq, int nrec,
+                 const double *recv, int rd, const double *obs, const sfdx_opts *o, double *misfit, double *grad,
+                 double *ex, sfdx_stats *st)
+{
+    if (!ctx || !g || !sigma || !freq || !recv || !obs || !misfit || !grad || nf <= 0 || nrec <= 0) return SFDX_ERR_ARG;
+    sfdx_opts def;
+    if (!o) { sfdx_default_opts(&def); o = &def; }
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        return adjoint_impl(ctx, g, sigma, nf, freq, nrec, recv, rd, obs, o, misfit, grad, ex, st);
     });
 }
 

@@ -453,4 +453,156 @@ int post_process(const PostGrid &g, int nf, const double *freq, int nrec, const 
     return 0;
 }
 
+// ------------------------------------------------------------------ misfit and its field gradient
+// Transposes of surface_fields / station_interp: coefficients of dPhi = Re(sum
+// g dfield) are scattered back onto the edge values with the same (real or
+// complex) weights, unconjugated.
+static void station_interp_T(const PostGrid &g, int rd, const double *recv, int nrec, const std::vector<zc> &gs,
+                             std::vector<zc> (&gsurf)[5])
+{
+    const int nx = g.nx, ny = g.ny;
+    std::vector<double> ax(nx), by(ny);
+    double sa = 0, sb = 0;
+    for (int i = 0; i < nx; i++) sa += g.a[i];
+    for (int i = 0; i < ny; i++) sb += g.b[i];
+    double acc = 0;
+    for (int i = 0; i < nx; i++) { ax[i] = -sa / 2 + acc + g.a[i] / 2; acc += g.a[i]; }
+    acc = 0;
+    for (int i = 0; i < ny; i++) { by[i] = -sb / 2 + acc + g.b[i] / 2; acc += g.b[i]; }
+    for (int q = 0; q < 5; q++) gsurf[q].assign((size_t)nx * ny, 0.0);
+    for (int r = 0; r < nrec; r++) {
+        const double xr = recv[(size_t)r * 3], yr = recv[(size_t)r * 3 + 1];
+        int ir = -1, jr = -1;
+        for (int q = 0; q + 1 < nx; q++)
+            if (xr >= ax[q] && xr <= ax[q + 1]) { ir = q; break; }
+        for (int q = 0; q + 1 < ny; q++)
+            if (yr >= by[q] && yr <= by[q + 1]) { jr = q; break; }
+        double w1[3], w2[3];
+        quad3(xr, &ax[ir - 1], w1);
+        quad3(yr, &by[jr - 1], w2);
+        for (int mm = 0; mm < 3; mm++)
+            for (int nn = 0; nn < 3; nn++) {
+                const double wt = w1[mm] * w2[nn];
+                int I = ir - 1 + mm, J = jr - 1 + nn;
+                if (rd == 2) { I = jr - 1 + mm; J = ir - 1 + nn; }
+                const int Ih = ir - 1 + mm, Jh = jr - 1 + nn;
+                for (int q = 0; q < 4; q++) gsurf[q][(size_t)I + (size_t)nx * J] += wt * gs[(size_t)r * 5 + q];
+                gsurf[4][(size_t)Ih + (size_t)nx * Jh] += wt * gs[(size_t)r * 5 + 4];
+            }
+    }
+}
+
+static void surface_fields_T(const PostGrid &g, double freq, const std::vector<zc> (&gs)[5], zc *w)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz, m = g.nair;
+    const int64_t n1 = (int64_t)nx * (ny + 1) * (nz + 1), n2 = (int64_t)ny * (nz + 1) * (nx + 1);
+    const zc kap(0.0, 2 * kPi * freq * kMu0);
+    const int64_t P = (int64_t)nx * (ny + 1), Q = (int64_t)ny * (nx + 1);
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const size_t s = (size_t)i + (size_t)nx * j;
+            const int64_t idx = (int64_t)m * nx * (ny + 1) + (int64_t)j * nx + i;
+            const int64_t idx2 = n1 + (int64_t)m * ny * (nx + 1) + (int64_t)j * (nx + 1) + i;
+            const int64_t idx3 = n1 + n2 + (int64_t)m * (nx + 1) * (ny + 1) + (int64_t)j * (nx + 1) + i;
+            const double cz = g.c[m], ai = g.a[i], bj = g.b[j];
+            // coefficients of the intermediate quantities
+            const zc gHx = gs[2][s] / kap, gHy = gs[3][s] / kap, gHz = gs[4][s] / kap;
+            const zc g_ezy = gHx, g_eyz = -gHx, g_exz = gHy, g_ezx = -gHy, g_exy = gHz, g_eyx = -gHz;
+            const zc g_ex = gs[0][s] - g_exz / cz;   // ex enters ex and exz
+            const zc g_ey = gs[1][s] - g_eyz / cz;   // ey enters ey and eyz
+            // ex = (x[idx] + x[idx+nx]) / 2
+            w[idx] += g_ex / 2.0;
+            w[idx + nx] += g_ex / 2.0;
+            // ey = (x[idx2] + x[idx2+1]) / 2
+            w[idx2] += g_ey / 2.0;
+            w[idx2 + 1] += g_ey / 2.0;
+            // exz = ((x[idx+P] + x[idx+P+nx]) / 2 - ex) / cz
+            w[idx + P] += g_exz / (2.0 * cz);
+            w[idx + P + nx] += g_exz / (2.0 * cz);
+            // ezx = ((x[idx3+1] + x[idx3+nx+2]) / 2 - (x[idx3+nx+1] + x[idx3]) / 2) / a_i
+            w[idx3 + 1] += g_ezx / (2.0 * ai);
+            w[idx3 + nx + 2] += g_ezx / (2.0 * ai);
+            w[idx3 + nx + 1] -= g_ezx / (2.0 * ai);
+            w[idx3] -= g_ezx / (2.0 * ai);
+            // eyz = ((x[idx2+Q] + x[idx2+Q+1]) / 2 - ey) / cz
+            w[idx2 + Q] += g_eyz / (2.0 * cz);
+            w[idx2 + Q + 1] += g_eyz / (2.0 * cz);
+            // ezy = (-(x[idx3] + x[idx3+1]) / 2 + (x[idx3+nx+1] + x[idx3+nx+2]) / 2) / b_j
+            w[idx3] -= g_ezy / (2.0 * bj);
+            w[idx3 + 1] -= g_ezy / (2.0 * bj);
+            w[idx3 + nx + 1] += g_ezy / (2.0 * bj);
+            w[idx3 + nx + 2] += g_ezy / (2.0 * bj);
+            // exy = (x[idx+nx] - x[idx]) / b_j ; eyx = (x[idx2+1] - x[idx2]) / a_i
+            w[idx + nx] += g_exy / bj;
+            w[idx] -= g_exy / bj;
+            w[idx2 + 1] += g_eyx / ai;
+            w[idx2] -= g_eyx / ai;
+        }
+}
+
+int misfit_fields(const PostGrid &g, int nf, const double *freq, int nrec, const double *recv, int rd,
+                  const double *obs, const zc *ex, double &misfit, zc *w)
+{
+    const Dims d = make_dims(g.nx, g.ny, g.nz);
+    const double ln10 = std::log(10.0);
+    misfit = 0;
+    if (w) std::fill(w, w + (size_t)nf * 2 * d.NL, zc(0.0));
+    for (int k = 0; k < nf; k++) {
+        const zc *x1 = ex + (size_t)k * 2 * d.NL;
+        const zc *x2 = x1 + d.NL;
+        Surface S;
+        surface_fields(g, freq[k], x1, 0, S);
+        surface_fields(g, freq[k], x2, 1, S);
+        std::vector<zc> s1, s2;
+        int st = station_interp(g, S, 0, rd, recv, nrec, s1);
+        if (!st) st = station_interp(g, S, 1, rd, recv, nrec, s2);
+        if (st) return st;
+        const double om = 2 * kPi * freq[k] * kMu0;
+        std::vector<zc> g1((size_t)nrec * 5, 0.0), g2((size_t)nrec * 5, 0.0);
+        for (int r = 0; r < nrec; r++) {
+            const zc *A1 = &s1[(size_t)r * 5], *A2 = &s2[(size_t)r * 5];
+            const zc Ex1 = A1[0], Ey1 = A1[1], Hx1 = A1[2], Hy1 = A1[3];
+            const zc Ex2 = A2[0], Ey2 = A2[1], Hx2 = A2[2], Hy2 = A2[3];
+            const zc t = Hx1 * Hy2 - Hx2 * Hy1;
+            const zc N = Ex2 * Hx1 - Ex1 * Hx2, M = Ey1 * Hy2 - Ey2 * Hy1;
+            const zc zxy = N / t, zyx = M / t;
+            const double *ob = obs + ((size_t)k * nrec + r) * 4;
+            zc cz[2];
+            const zc Z[2] = {zxy, zyx};
+            for (int q = 0; q < 2; q++) {
+                const double rho = std::norm(Z[q]) / om;
+                const double phi = -std::atan(Z[q].imag() / Z[q].real());
+                const double rr = std::log10(rho) - std::log10(ob[q]);
+                const double rp = phi - ob[2 + q] * kPi / 180.0;
+                misfit += 0.5 * (rr * rr + rp * rp);
+                // d log10 rho = (2/ln10) Re(dZ/Z), d phi = -Im(dZ/Z)
+                cz[q] = zc(2.0 * rr / ln10, rp) / Z[q];
+            }
+            if (!w) continue;
+            const zc t2 = t * t;
+            // d zxy / d(Ex1, Ex2, Hx1, Hx2, Hy1, Hy2); d zyx / d(Ey1, Ey2, Hx1, Hx2, Hy1, Hy2)
+            zc *G1 = &g1[(size_t)r * 5], *G2 = &g2[(size_t)r * 5];
+            G1[0] += cz[0] * (-Hx2 / t);
+            G2[0] += cz[0] * (Hx1 / t);
+            G1[2] += cz[0] * (Ex2 / t - N * Hy2 / t2);
+            G2[2] += cz[0] * (-Ex1 / t + N * Hy1 / t2);
+            G1[3] += cz[0] * (N * Hx2 / t2);
+            G2[3] += cz[0] * (-N * Hx1 / t2);
+            G1[1] += cz[1] * (Hy2 / t);
+            G2[1] += cz[1] * (-Hy1 / t);
+            G1[2] += cz[1] * (-M * Hy2 / t2);
+            G2[2] += cz[1] * (M * Hy1 / t2);
+            G1[3] += cz[1] * (-Ey2 / t + M * Hx2 / t2);
+            G2[3] += cz[1] * (Ey1 / t - M * Hx1 / t2);
+        }
+        if (!w) continue;
+        std::vector<zc> gsurf[5];
+        station_interp_T(g, rd, recv, nrec, g1, gsurf);
+        surface_fields_T(g, freq[k], gsurf, w + (size_t)k * 2 * d.NL);
+        station_interp_T(g, rd, recv, nrec, g2, gsurf);
+        surface_fields_T(g, freq[k], gsurf, w + (size_t)k * 2 * d.NL + d.NL);
+    }
+    return 0;
+}
+
 }  // namespace sfdx

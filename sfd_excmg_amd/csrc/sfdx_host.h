@@ -47,5 +47,12 @@ void coarse_system(const HostLevel &L, double freq, const std::vector<std::compl
 bool direct_solve(const CoarseSystem &S, std::vector<std::complex<double>> &sol, std::string &err);
 int post_process(const PostGrid &g, int nf, const double *freq, int nrec, const double *recv,
                  const std::complex<double> *ex, int out_type, int rd, double *results, double *tipper);
+// Misfit 1/2 sum over frequencies, stations and the xy/yx modes of
+// (log10 rho_a - log10 rho_obs)^2 + (phi - phi_obs)^2 (phase in radians), obs
+// given as post_process out_type 2 rows (rho_xy, rho_yx, phi_xy, phi_yx in
+// ohm-m and degrees).  If w is non-null it receives dPhi = Re(w^T dex) for the
+// fields ex (same layout as ex).  Returns a non-zero station number on error.
+int misfit_fields(const PostGrid &g, int nf, const double *freq, int nrec, const double *recv, int rd,
+                  const double *obs, const std::complex<double> *ex, double &misfit, std::complex<double> *w);
 
 }  // namespace sfdx

@@ -228,6 +228,10 @@ struct GPVecs {
 template <typename R>
 void launch_gpbicg(int mode, const DevGrid<R> &g, RedBuf &rb, const GPVecs<R> &v, hipStream_t st);
 
+// adjoint gradient accumulation over cells (see k_cell_grad) and boundary zeroing
+void launch_cell_grad(const DevGrid<double> &g, double wmu, const zd *e, const zd *lam, double *grad, hipStream_t st);
+void launch_zero_boundary(const Dims &d, zd *v, hipStream_t st);
+
 template <typename R>
 void launch_extrap(int64_t n, R c1, const cx<R> *u1, R c2, cx<R> *w, hipStream_t st);
 

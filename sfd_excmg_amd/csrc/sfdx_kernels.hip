@@ -476,6 +476,54 @@ __global__ __launch_bounds__(TPB) void k_gpbicg(Dims d, const R *__restrict__ a,
     }
 }
 
+// g_c += Re(i wmu lam^T M_c e): derivative of the misfit with respect to the log
+// of a scale factor on cell c's conductivity tensor.  M_c is the cell's
+// corner-lumped contribution to the row-scaled edge mass: at each of its 8
+// corners the x/y/z edges meeting there couple through (V_c/8) sigma_c, so
+// that the operator's mass term is sum_c M_c and dA_s/dlog(s_c) = -i wmu M_c.
+// One thread per cell; the 12 edge values of e and lam are read once.
+__global__ __launch_bounds__(TPB) void k_cell_grad(Dims d, const double *__restrict__ a, const double *__restrict__ b,
+                                                   const double *__restrict__ c, const double *__restrict__ sig,
+                                                   double wmu, const zd *__restrict__ e, const zd *__restrict__ lam,
+                                                   double *__restrict__ g)
+{
+    const Tile TL = tile_id();
+    const int k = TL.y;
+    const int p = (k < d.nz) ? TL.x * TPB + threadIdx.x : kNoTile;
+    if (p >= d.nx * d.ny) return;
+    const int j = p / d.nx, i = p - j * d.nx;
+    const int64_t cell = (int64_t)k * d.nx * d.ny + p;
+    const double sxx = sig[cell], syy = sig[d.NC + cell], szz = sig[2 * d.NC + cell];
+    const double sxy = sig[3 * d.NC + cell], sxz = sig[4 * d.NC + cell], syz = sig[5 * d.NC + cell];
+    double sre = 0, sim = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int ci = q & 1, cj = (q >> 1) & 1, ck = (q >> 2) & 1;
+        const int64_t kx = ex_id(d, i, j + cj, k + ck), ky = ey_id(d, i + ci, j, k + ck), kz = ez_id(d, i + ci, j + cj, k);
+        const zd ex = e[kx], ey = e[ky], ez = e[kz];
+        const zd lx = lam[kx], ly = lam[ky], lz = lam[kz];
+        const zd sx = sxx * ex + sxy * ey + sxz * ez;
+        const zd sy = sxy * ex + syy * ey + syz * ez;
+        const zd sz = sxz * ex + syz * ey + szz * ez;
+        const zd t = lx * sx + ly * sy + lz * sz;
+        sre += t.re;
+        sim += t.im;
+    }
+    const double V = a[i] * b[j] * c[k];
+    g[cell] += -wmu * (V / 8.0) * sim;  // Re(i wmu (V/8) sum)
+}
+
+// v_e = 0 on boundary edges
+__global__ __launch_bounds__(TPB) void k_zero_boundary(Dims d, zd *__restrict__ v)
+{
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
+    if (p >= S.nI * S.nJ) return;
+    const int j = p / S.nI, i = p - j * S.nI;
+    if (!edge_interior(d, S.o, i, j, S.k)) v[S.base + p] = zd(0);
+}
+
 // generic dots over all edges: mode 0: u^T v (unconjugated); 1: conj(u)^T v
 template <typename R, int MODE>
 __global__ __launch_bounds__(TPB) void k_dot(int64_t n, const cx<R> *__restrict__ u, const cx<R> *__restrict__ v,
@@ -1343,6 +1391,20 @@ void launch_restrict(const Dims &df, const Dims &dc, const double *const wl[3], 
     for (int q = 0; q < 3; q++) T.wl[q] = wl[q];
     SliceLaunch L = edge_launch(dc);
     hipLaunchKernelGGL((k_restrict<R>), L.grid, L.block, 0, st, df, dc, T, rf, rc);
+}
+
+void launch_cell_grad(const DevGrid<double> &g, double wmu, const zd *e, const zd *lam, double *grad, hipStream_t st)
+{
+    const int64_t plane = (int64_t)g.d.nx * g.d.ny;
+    const unsigned gx = (unsigned)((plane + TPB - 1) / TPB);
+    hipLaunchKernelGGL(k_cell_grad, dim3(gx, pad_y(gx, (unsigned)g.d.nz), 1), dim3(TPB), 0, st, g.d, g.a, g.b, g.c,
+                       g.sig, wmu, e, lam, grad);
+}
+
+void launch_zero_boundary(const Dims &d, zd *v, hipStream_t st)
+{
+    SliceLaunch L = edge_launch(d);
+    hipLaunchKernelGGL(k_zero_boundary, L.grid, L.block, 0, st, d, v);
 }
 
 template <typename R>

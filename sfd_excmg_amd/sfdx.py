@@ -76,7 +76,8 @@ class Job(ctypes.Structure):
 
 EXPORTS = ["sfdx_create", "sfdx_destroy", "sfdx_last_error", "sfdx_default_opts", "sfdx_fwd_solve",
            "sfdx_post_process", "sfdx_mt1d_anl", "sfdx_apply_operator", "sfdx_ssor_apply", "sfdx_div_correct",
-           "sfdx_extint_v", "sfdx_bench_kernels", "sfdx_direct_solve", "sfdx_solve_batch"]
+           "sfdx_extint_v", "sfdx_bench_kernels", "sfdx_direct_solve", "sfdx_solve_batch", "sfdx_misfit",
+           "sfdx_adjoint"]
 
 _lib = None
 
@@ -104,6 +105,9 @@ def load(path: str = LIB_PATH):
     L.sfdx_bench_kernels.argtypes = [P, ctypes.POINTER(Grid), P, D, I, I, P]
     L.sfdx_direct_solve.argtypes = [ctypes.POINTER(Grid), P, D, P]
     L.sfdx_solve_batch.argtypes = [P, I, ctypes.POINTER(Job), ctypes.POINTER(Opts), I, ctypes.POINTER(Stats)]
+    L.sfdx_misfit.argtypes = [ctypes.POINTER(Grid), I, P, I, P, I, P, P, ctypes.POINTER(D), P]
+    L.sfdx_adjoint.argtypes = [P, ctypes.POINTER(Grid), P, I, P, I, P, I, P, ctypes.POINTER(Opts), ctypes.POINTER(D), P,
+                               P, ctypes.POINTER(Stats)]
     _lib = L
     return L
 
@@ -210,6 +214,29 @@ class Context:
         _check(self.lib.sfdx_solve_batch(self.h, n, jobs, ctypes.byref(o), int(nstreams), st))
         return [ex.T.copy() for ex in outs], list(st)
 
+    def adjoint(self, a, b, c, nair, sigma, freqs, recv, obs, opts: Opts | None = None, rd: int = 2,
+                return_fields: bool = False):
+        """Misfit and d misfit / d log(s_c) for every cell (sfdx_adjoint).
+        obs: (nf*nrec, 4) rows rho_xy, rho_yx, phi_xy, phi_yx (ohm-m, degrees)."""
+        G = _GridHolder(a, b, c, nair)
+        sf = sigma_flat(sigma)
+        fr = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
+        rv = np.ascontiguousarray(recv, np.float64)
+        ob = np.ascontiguousarray(obs, np.float64)
+        assert ob.shape == (len(fr) * len(rv), 4)
+        ncell = len(G.a) * len(G.b) * len(G.c)
+        grad = np.zeros(ncell)
+        phi = ctypes.c_double(0.0)
+        ex = np.zeros((len(fr), 2 * nedges(len(G.a), len(G.b), len(G.c))), np.complex128) if return_fields else None
+        st = Stats()
+        o = opts if opts is not None else default_opts()
+        _check(self.lib.sfdx_adjoint(self.h, ctypes.byref(G.g), _ptr(sf), len(fr), _ptr(fr), len(rv), _ptr(rv), int(rd),
+                                     _ptr(ob), ctypes.byref(o), ctypes.byref(phi), _ptr(grad),
+                                     _ptr(ex) if ex is not None else None, ctypes.byref(st)))
+        if return_fields:
+            return phi.value, grad, ex.T.copy(), st
+        return phi.value, grad, st
+
     def apply_operator(self, a, b, c, sigma, freq, x, precision=PREC_FP64):
         G = _GridHolder(a, b, c)
         x = np.ascontiguousarray(x, np.complex128)
@@ -248,6 +275,21 @@ class Context:
         _check(self.lib.sfdx_bench_kernels(self.h, ctypes.byref(G.g), _ptr(sigma_flat(sigma)), float(freq),
                                            int(precision), int(reps), _ptr(ms)))
         return ms
+
+
+def misfit(a, b, c, nair, freqs, recv, ex, obs, rd=2, with_grad=False):
+    """Data misfit of the fields ex(2*nl, nf) and optionally its field gradient
+    w (d misfit = Re(w^T d ex)), host FP64 (sfdx_misfit)."""
+    G = _GridHolder(a, b, c, nair)
+    fr = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
+    rv = np.ascontiguousarray(recv, np.float64)
+    ob = np.ascontiguousarray(obs, np.float64)
+    e = np.ascontiguousarray(np.asarray(ex).T, np.complex128)   # (nf, 2*nl)
+    w = np.zeros_like(e) if with_grad else None
+    phi = ctypes.c_double(0.0)
+    _check(load().sfdx_misfit(ctypes.byref(G.g), len(fr), _ptr(fr), len(rv), _ptr(rv), int(rd), _ptr(ob), _ptr(e),
+                              ctypes.byref(phi), _ptr(w) if w is not None else None))
+    return (phi.value, w.T.copy()) if with_grad else phi.value
 
 
 # ---------------------------------------------------------------- reference-shaped API

@@ -80,3 +80,32 @@ def test_direct_solve_matches_superlu():
         r = rhs - S.A @ x
         assert np.linalg.norm(r[~S.bnd]) <= 1e-10 * np.linalg.norm(rhs[~S.bnd])
         assert np.allclose(x[S.bnd], rhs[S.bnd])
+
+
+@pytest.mark.parametrize("rd", [1, 2])
+def test_misfit_field_gradient_matches_directional_difference(rd):
+    """sfdx_misfit's field gradient w (the transposed post-process) against a
+    central difference of the misfit along a random complex direction; the
+    misfit itself against the oracle's."""
+    m = synth.make_iso_block_model("mf", 16, 16, 12, 4, seed=31, nblocks=3, freq=1.0, level=3, nstations=6)
+    freqs = np.array([0.5, 5.0])
+    cols = []
+    for f in freqs:
+        S = O.build_system(m.a, m.b, m.c, m.sigma, f)
+        x1, x2 = O.direct_solve(S)
+        cols.append(np.concatenate([x1, x2]))
+    ex = np.stack(cols, axis=1)
+    res, _ = O.post_process(m.a, m.b, m.c, m.nair, freqs, m.recv, ex, out_type=2, rd=rd)
+    obs = res.copy()
+    obs[:, :2] *= 1.2
+    obs[:, 2:] += 3.0
+    phi, w = sfdx.misfit(m.a, m.b, m.c, m.nair, freqs, m.recv, ex, obs, rd=rd, with_grad=True)
+    assert phi == pytest.approx(O.misfit(m.a, m.b, m.c, m.nair, freqs, m.recv, ex, obs, rd=rd), rel=1e-12)
+    rng = np.random.default_rng(7)
+    dirn = rng.standard_normal(ex.shape) + 1j * rng.standard_normal(ex.shape)
+    h = 1e-6 * np.abs(ex).max() / np.abs(dirn).max()
+    fp = sfdx.misfit(m.a, m.b, m.c, m.nair, freqs, m.recv, ex + h * dirn, obs, rd=rd)
+    fm = sfdx.misfit(m.a, m.b, m.c, m.nair, freqs, m.recv, ex - h * dirn, obs, rd=rd)
+    fd = (fp - fm) / (2 * h)
+    an = float(np.real(np.sum(w * dirn)))
+    assert abs(fd - an) <= 1e-6 * abs(an)

@@ -262,3 +262,56 @@ def test_batched_solve_matches_single_solves(ctx, use_excmg):
         # rd=1: the reference's transposed station sampling (rd=2) needs nx == ny
         for p, q in zip(_impedances(m, ex, rd=1), _impedances(m, ex1, rd=1)):
             assert np.max(np.abs(p - q) / np.abs(q)) < 1e-7
+
+
+def _adjoint_case():
+    m = small_excmg_model(True)
+    m.freqs = np.array([1.0, 10.0])
+    res, _ = O.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv,
+                            O.fwd_solver(m.a, m.b, m.c, m.nair, 3, 1e-11, m.sigma, m.freqs,
+                                         O.Options(solver=O.SOLVER_COCR, corr_intv=50, norm="interior"))[0],
+                            out_type=2)
+    obs = res * np.array([1.15, 0.9, 1.0, 1.0]) + np.array([0, 0, 2.0, -1.5])
+    nx, ny = m.nx, m.ny
+    cells = [(m.nair + 1) * nx * ny + 8 * nx + 8, (m.nair + 2) * nx * ny + 6 * nx + 9,
+             (m.nair + 4) * nx * ny + 9 * nx + 6]
+    return m, obs, cells
+
+
+def test_adjoint_gradient_matches_oracle_sensitivity_and_fd(ctx):
+    """d misfit / d log(s_c) from the device adjoint (forward + adjoint COCR
+    solves with the forward kernels, cell kernel over the 12 edges) against the
+    FP64 CPU tangent-linear sensitivity of the oracle and against central
+    differences of device forward solves."""
+    m, obs, cells = _adjoint_case()
+    opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-12, corr_intv=0, norm=sfdx.NORM_INTERIOR,
+                             maxit_plain=5000, maxit_excmg=5000)
+    phi, grad, st = ctx.adjoint(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, obs, opts)
+    assert st.status == 0
+    sens = O.misfit_sensitivity(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, obs, cells)
+    scale = np.max(np.abs(sens))
+    assert np.max(np.abs(grad[cells] - sens)) <= 1e-4 * scale
+    h = 1e-3
+    for q, cell in enumerate(cells[:2]):
+        sp_, sm_ = m.sigma.copy(), m.sigma.copy()
+        sp_[cell] *= np.exp(h)
+        sm_[cell] *= np.exp(-h)
+        exp_, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, sp_, m.freqs, opts)
+        exm_, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, sm_, m.freqs, opts)
+        fd = (sfdx.misfit(m.a, m.b, m.c, m.nair, m.freqs, m.recv, exp_, obs)
+              - sfdx.misfit(m.a, m.b, m.c, m.nair, m.freqs, m.recv, exm_, obs)) / (2 * h)
+        assert abs(grad[cell] - fd) <= 1e-3 * scale
+
+
+def test_autograd_function_returns_adjoint_gradient(ctx):
+    import torch
+    from sfd_excmg_amd.autograd import mt_misfit
+
+    m, obs, cells = _adjoint_case()
+    opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR)
+    ls = torch.zeros(m.nx * m.ny * m.nz, dtype=torch.float64, requires_grad=True)
+    phi = mt_misfit(ls, m, obs, opts, context=ctx)
+    phi.backward()
+    phi2, grad, _ = ctx.adjoint(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, obs, opts)
+    assert float(phi.detach()) == pytest.approx(phi2, rel=1e-12)
+    assert np.allclose(ls.grad.numpy(), grad, rtol=0, atol=1e-12 * np.abs(grad).max())
