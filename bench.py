@@ -139,6 +139,60 @@ def run_c4(ctx, args):
                       "dtype": "complex128 Krylov/operator + complex64 preconditioner"}))
 
 
+def run_c5(ctx, args):
+    """C5: misfit gradient with respect to log sigma over all cells of the
+    96x96x64 isotropic random-field model, 6 frequencies, 20x20 stations, data
+    from the seed-6 model; central differences (step 1e-3 in log sigma) on
+    seeded random earth cells under the station area."""
+    from sfd_excmg_amd import sfdx, synth
+
+    kw = dict(nx=48, ny=48, nz_earth=24, nstations=8) if args.small else {}
+    m = synth.make_c5(**kw)
+    d = synth.make_c5(seed=6, **kw)
+    opts = sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0,
+                             norm=sfdx.NORM_INTERIOR, maxit_excmg=5000, maxit_plain=5000)
+    exd, _ = ctx.fwd_solve(d.a, d.b, d.c, d.nair, d.sigma, d.freqs, opts)
+    obs, _ = sfdx.post_process(d.a, d.b, d.c, d.nair, d.freqs, d.recv, exd, out_type=2)
+    ctx.adjoint(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, obs, opts)   # warm-up
+    t0 = time.perf_counter()
+    phi, grad, st = ctx.adjoint(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, obs, opts)
+    t_adj = time.perf_counter() - t0
+    print(f"adjoint done {t_adj:.2f} s", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(7)
+    nx, ny = m.nx, m.ny
+    npad = nx // 8
+    cells = []
+    while len(cells) < args.fd_cells:
+        i, j = (int(v) for v in rng.integers(npad + 2, nx - npad - 2, 2))
+        k = int(rng.integers(m.nair, m.nair + 10))
+        cells.append((k * ny + j) * nx + i)
+    h = 1e-3
+    errs, fds = [], []
+    t0 = time.perf_counter()
+    for cell in cells:
+        sp_, sm_ = m.sigma.copy(), m.sigma.copy()
+        sp_[cell] *= np.exp(h)
+        sm_[cell] *= np.exp(-h)
+        exp_, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, sp_, m.freqs, opts)
+        exm_, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, sm_, m.freqs, opts)
+        fd = (sfdx.misfit(m.a, m.b, m.c, m.nair, m.freqs, m.recv, exp_, obs)
+              - sfdx.misfit(m.a, m.b, m.c, m.nair, m.freqs, m.recv, exm_, obs)) / (2 * h)
+        fds.append(fd)
+        errs.append(abs(grad[cell] - fd))
+        print(f"cell {cell}: adjoint {grad[cell]:.6e} fd {fd:.6e}", file=sys.stderr, flush=True)
+    t_fd = time.perf_counter() - t0
+    scale = float(np.max(np.abs(grad)))
+    print(json.dumps({"metric": "adjoint misfit gradient (C5): wall time for d misfit / d log sigma over all cells",
+                      "value": round(t_adj, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+                      "cells": [m.nx, m.ny, m.nz], "ncell": int(m.nx * m.ny * m.nz), "frequencies": len(m.freqs),
+                      "stations": int(len(m.recv)), "misfit": phi, "status": int(st.status),
+                      "forward_fine_iters": int(st.fine_iters_total),
+                      "fd_cells": cells, "fd_step_log_sigma": h, "fd_values": fds,
+                      "adjoint_values": [float(grad[c]) for c in cells],
+                      "max_abs_err_vs_fd_over_max_grad": float(max(errs) / scale), "fd_wall_s": round(t_fd, 2),
+                      "dtype": "complex128"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +204,9 @@ def main():
     ap.add_argument("--c4", action="store_true", help="batched workload: 16 frequencies on the 128x128x96 model "
                     "plus 8 isotropic models of uneven sizes, batched vs one stream")
     ap.add_argument("--nstreams", type=int, default=8)
+    ap.add_argument("--c5", action="store_true", help="adjoint gradient on the 96x96x64 random-field model with a "
+                    "central-difference check on --fd-cells random cells")
+    ap.add_argument("--fd-cells", type=int, default=4)
     ap.add_argument("--verbose", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -167,6 +224,9 @@ def main():
     b64, b32 = spmv_bytes(nl, nc, 16, 8), spmv_bytes(nl, nc, 8, 4)
     if args.c4:
         run_c4(ctx, args)
+        return
+    if args.c5:
+        run_c5(ctx, args)
         return
     if args.kernels_only:
         ms64, ms32 = kernel_times(ctx, m, reps=20)

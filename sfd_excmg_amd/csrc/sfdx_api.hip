@@ -397,11 +397,15 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     bool pending = false;
     int pend_slot = 0;
     StreamGraph graph;
-    // Safeguards of the complex64 preconditioner (restart_drop > 0): restart on
-    // divergence (residual 100x above the best since the last restart) or on
-    // stagnation (no new best for 200 iterations); after 3 restarts in a row
-    // that did not lower the best residual by 2x, continue with the complex128
-    // preconditioner, which is a fixed linear operator.
+    // Safeguards: restart from the true residual on a breakdown (non-finite
+    // scalars; the update of x is skipped on the device), on stagnation (no new
+    // best residual for 200 iterations: the recurrence residual has reached its
+    // attainable accuracy or the recurrences drift) and, with the complex64
+    // preconditioner (restart_drop > 0), on divergence (residual 100x above the
+    // best since the last restart).  After 3 such restarts in a row that did not
+    // halve the best residual, the complex64 preconditioner is replaced by the
+    // complex128 one (a fixed linear operator); without it the solve stops with
+    // SFDX_WARN_NOCONV and the last iterate.
     double best = 1e300, best_at_restart = 1e300;
     int since_best = 0, idle_restarts = 0;
     const bool guard = ko.restart_drop > 0;
@@ -433,7 +437,7 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
         if (fresh) { rel0 = info.relres; fresh = false; best = info.relres; since_best = 0; }
         if (info.relres < best) { best = info.relres; since_best = 0; }
         const bool diverging = guard && info.relres > 100 * best;
-        const bool stalled = guard && since_best > 200;
+        const bool stalled = since_best > 200;
         const bool restart = (ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0) || diverging || stalled;
         if (bad || info.relres < eps || info.iters >= maxit || cur[SL_BREAK] != 0 || restart) {
             if (pending) {  // decide on the newest iteration instead
@@ -441,23 +445,31 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
                 pending = false;
                 continue;
             }
-            if (bad) { info.status = SFDX_ERR_NAN; break; }
-            if (info.relres < eps) { info.status = 0; break; }
+            if (!bad && info.relres < eps) { info.status = 0; break; }
             if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
-            if (cur[SL_BREAK] != 0) { info.status = SFDX_ERR_BREAKDOWN; break; }
-            // the preconditioner is applied in complex64: its rounding is not a
-            // fixed linear operator, so the short recurrences drift; restart
-            // from the true FP64 residual
-            if (guard) {
+            const bool planned = !bad && cur[SL_BREAK] == 0 && !stalled && !diverging;
+            if (!planned) {
+                // no progress since the last restart: count it
                 idle_restarts = (best < 0.5 * best_at_restart) ? 0 : idle_restarts + 1;
                 best_at_restart = std::min(best_at_restart, best);
-                if (idle_restarts >= 3 && (ko.lo || ko.mg32)) {
-                    ko.lo = nullptr;
-                    ko.mg32 = nullptr;
-                    ko.restart_drop = 0;
-                    info.fallback = 1;
-                    graph.reset();  // the captured iteration used the complex64 preconditioner
+                if (idle_restarts >= 3) {
+                    if (ko.lo || ko.mg32) {
+                        ko.lo = nullptr;
+                        ko.mg32 = nullptr;
+                        ko.restart_drop = 0;
+                        info.fallback = 1;
+                        idle_restarts = 0;
+                        graph.reset();  // the captured iteration used the complex64 preconditioner
+                    } else {
+                        info.status = SFDX_WARN_NOCONV;
+                        break;
+                    }
                 }
+            } else {
+                // the preconditioner is applied in complex64: its rounding is not a
+                // fixed linear operator, so the short recurrences drift; restart
+                // from the true FP64 residual after each restart_drop reduction
+                best_at_restart = std::min(best_at_restart, best);
             }
             state = 0;
             info.restarts++;

@@ -56,14 +56,18 @@ __device__ __forceinline__ void scalar_step(double *s, int op)
     double *t = s + SL_TMP;
     switch (op) {
     case OP_RN: s[SL_RN] = t[0]; break;
-    case OP_COCR_RHO: s[SL_RHO] = t[0]; s[SL_RHO + 1] = t[1]; break;
+    case OP_COCR_RHO:
+        s[SL_RHO] = t[0]; s[SL_RHO + 1] = t[1];
+        s[SL_BREAK] = 0;  // fresh start of the recurrences
+        break;
     case OP_COCR_ALPHA:
         cdiv(s[SL_RHO], s[SL_RHO + 1], t[0], t[1], s[SL_ALPHA], s[SL_ALPHA + 1]);
-        if (t[0] == 0 && t[1] == 0) s[SL_BREAK] = 1;
+        if (!isfinite(s[SL_ALPHA]) || !isfinite(s[SL_ALPHA + 1])) s[SL_BREAK] = 1;
         break;
     case OP_COCR_BETA:
         cdiv(t[0], t[1], s[SL_RHO], s[SL_RHO + 1], s[SL_BETA], s[SL_BETA + 1]);
         s[SL_RHO] = t[0]; s[SL_RHO + 1] = t[1];
+        if (!isfinite(s[SL_BETA]) || !isfinite(s[SL_BETA + 1])) s[SL_BREAK] = 1;
         break;
     case OP_BICG_RHO: {
         double qr, qi, ar, ai;
