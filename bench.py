@@ -44,6 +44,9 @@ def spmv_bytes(nl, ncell, sc, sr, nsig=6):
     return 2 * nl * sc + ncell * nsig * sr
 
 
+USE_GRAPH = 1
+
+
 def solve_opts(m, precision, eps, verbose=0, precond=None):
     from sfd_excmg_amd import sfdx
 
@@ -55,7 +58,7 @@ def solve_opts(m, precision, eps, verbose=0, precond=None):
     return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0,
                              norm=sfdx.NORM_INTERIOR, precision=precision, verbose=verbose,
                              precond=sfdx.PRECOND_HYBRID if precond is None else precond,
-                             sigma_floor32=0.0, restart_drop=RESTART_DROP, maxit_excmg=4000)
+                             sigma_floor32=0.0, restart_drop=RESTART_DROP, maxit_excmg=4000, use_graph=USE_GRAPH)
 
 
 def run_gpu(ctx, m, steps, warmup, opts):
@@ -208,7 +211,11 @@ def main():
                     "central-difference check on --fd-cells random cells")
     ap.add_argument("--fd-cells", type=int, default=4)
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--no-graph", action="store_true", help="launch every kernel eagerly (no HIP graph replay); "
+                    "used for profiler runs")
     args = ap.parse_args()
+    global USE_GRAPH
+    USE_GRAPH = 0 if args.no_graph else 1
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(frequencies/polarisations/models would shard across GPUs without communication).")

@@ -132,6 +132,17 @@ LevelDev<R> upload_level(sfdx_ctx *ctx, const HostLevel &L, const std::string &t
     SFDX_CHECK(hipMemcpyAsync(D.g.b, b.data(), b.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
     SFDX_CHECK(hipMemcpyAsync(D.g.c, c.data(), c.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
     SFDX_CHECK(hipMemcpyAsync(D.g.sig, s.data(), s.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
+    // reciprocal widths (computed in FP64, stored in R) for the stencil rows
+    std::vector<R> ra(L.a.size()), rb(L.b.size()), rc(L.c.size());
+    for (size_t q = 0; q < ra.size(); q++) ra[q] = (R)(1.0 / L.a[q]);
+    for (size_t q = 0; q < rb.size(); q++) rb[q] = (R)(1.0 / L.b[q]);
+    for (size_t q = 0; q < rc.size(); q++) rc[q] = (R)(1.0 / L.c[q]);
+    D.g.ra = pool_vec<R>(ctx, p + "ra", ra.size());
+    D.g.rb = pool_vec<R>(ctx, p + "rb", rb.size());
+    D.g.rc = pool_vec<R>(ctx, p + "rc", rc.size());
+    SFDX_CHECK(hipMemcpyAsync(D.g.ra, ra.data(), ra.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
+    SFDX_CHECK(hipMemcpyAsync(D.g.rb, rb.data(), rb.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
+    SFDX_CHECK(hipMemcpyAsync(D.g.rc, rc.data(), rc.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
     D.a64 = pool_vec<double>(ctx, tag + "/a64", L.a.size());
     D.b64 = pool_vec<double>(ctx, tag + "/b64", L.b.size());
     D.c64 = pool_vec<double>(ctx, tag + "/c64", L.c.size());
@@ -1479,53 +1490,45 @@ int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, d
             const R wmu = (R)(2 * 3.141592653589793 * freq * kMu0);
             std::vector<cx<R>> hx(d.NL);
             for (int64_t e = 0; e < d.NL; e++) hx[e] = cx<R>((R)std::sin(0.001 * e), (R)std::cos(0.002 * e));
-            cx<R> *v[6];
-            for (int q = 0; q < 6; q++) {
+            // 12 edge vectors in rotation (0.9 GB in complex128): every timed call
+            // reads and writes vectors that the previous calls did not touch, so
+            // the 256 MB last-level cache cannot serve them as it would a
+            // repeated call on the same data
+            constexpr int NV = 12;
+            cx<R> *v[NV];
+            for (int q = 0; q < NV; q++) {
                 v[q] = pool_vec<cx<R>>(ctx, "bench/v" + std::to_string(q) + (sizeof(R) == 8 ? "d" : "f"), d.NL);
                 SFDX_CHECK(hipMemcpyAsync(v[q], hx.data(), d.NL * sizeof(cx<R>), hipMemcpyHostToDevice, ctx->st));
+                launch_dirichlet<R>(d, (const cx<R> *)nullptr, 2, v[q], 0, ctx->st);
             }
-            float t;
+            auto V = [&](int q) { return v[((q % NV) + NV) % NV]; };
+            auto timed = [&](auto &&fn) {
+                for (int q = 0; q < 3; q++) fn(q);  // warm-up
+                SFDX_CHECK(hipEventRecord(e0, ctx->st));
+                for (int q = 0; q < reps; q++) fn(q);
+                SFDX_CHECK(hipEventRecord(e1, ctx->st));
+                SFDX_CHECK(hipEventSynchronize(e1));
+                float t;
+                SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
+                return (double)t / reps;
+            };
             // SpMV
-            for (int w = 0; w < 3; w++) launch_spmv<R>(D.g, wmu, v[0], v[1], (R)1, 0, nullptr, ctx->rb, ctx->st);
-            SFDX_CHECK(hipEventRecord(e0, ctx->st));
-            for (int q = 0; q < reps; q++) launch_spmv<R>(D.g, wmu, v[0], v[1], (R)1, 0, nullptr, ctx->rb, ctx->st);
-            SFDX_CHECK(hipEventRecord(e1, ctx->st));
-            SFDX_CHECK(hipEventSynchronize(e1));
-            SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
-            ms[0] = t / reps;
-            // SSOR
-            launch_dirichlet<R>(d, (const cx<R> *)nullptr, 2, v[0], 0, ctx->st);
-            SFDX_CHECK(hipEventRecord(e0, ctx->st));
-            for (int q = 0; q < reps; q++) launch_ssor<R>(D.g, wmu, v[0], v[2], (R)1, ctx->st);
-            SFDX_CHECK(hipEventRecord(e1, ctx->st));
-            SFDX_CHECK(hipEventSynchronize(e1));
-            SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
-            ms[1] = t / reps;
-            // COCR vector update
-            SFDX_CHECK(hipEventRecord(e0, ctx->st));
-            for (int q = 0; q < reps; q++) launch_vec<R>(0, D.g, ctx->rb, v[3], v[4], v[5], v[1], v[2], v[0], ctx->st);
-            SFDX_CHECK(hipEventRecord(e1, ctx->st));
-            SFDX_CHECK(hipEventSynchronize(e1));
-            SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
-            ms[2] = t / reps;
+            ms[0] = timed([&](int q) { launch_spmv<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, 0, nullptr, ctx->rb, ctx->st); });
+            // SSOR (12 colour sweeps)
+            ms[1] = timed([&](int q) { launch_ssor<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, ctx->st); });
+            // COCR vector update (reads 6 vectors, writes 3)
+            ms[2] = timed([&](int q) {
+                launch_vec<R>(0, D.g, ctx->rb, V(6 * q), V(6 * q + 1), V(6 * q + 2), V(6 * q + 3), V(6 * q + 4),
+                              V(6 * q + 5), ctx->st);
+            });
             // plain copy of one edge vector (2 * NL * sizeof(cx) bytes): calibrates
             // hardware byte counters against a kernel of known traffic
-            SFDX_CHECK(hipEventRecord(e0, ctx->st));
-            for (int q = 0; q < reps; q++) launch_copy<R>(d.NL, v[3], v[4], ctx->st);
-            SFDX_CHECK(hipEventRecord(e1, ctx->st));
-            SFDX_CHECK(hipEventSynchronize(e1));
-            SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
-            ms[4] = t / reps;
+            ms[4] = timed([&](int q) { launch_copy<R>(d.NL, V(2 * q), V(2 * q + 1), ctx->st); });
             // hybrid preconditioner (SSOR + nodal auxiliary correction)
             ensure_P(ctx, D, "bench");
             cx<R> *tn = pool_vec<cx<R>>(ctx, std::string("bench/tn") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
             cx<R> *ph = pool_vec<cx<R>>(ctx, std::string("bench/ph") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
-            SFDX_CHECK(hipEventRecord(e0, ctx->st));
-            for (int q = 0; q < reps; q++) launch_hybrid<R>(D.g, wmu, D.P, v[0], v[2], tn, ph, (R)1, ctx->st);
-            SFDX_CHECK(hipEventRecord(e1, ctx->st));
-            SFDX_CHECK(hipEventSynchronize(e1));
-            SFDX_CHECK(hipEventElapsedTime(&t, e0, e1));
-            ms[5] = t / reps;
+            ms[5] = timed([&](int q) { launch_hybrid<R>(D.g, wmu, D.P, V(2 * q), V(2 * q + 1), tn, ph, (R)1, ctx->st); });
         };
         if (precision == SFDX_PREC_FP64) run(0.0); else run(0.0f);
         ms[3] = ms[0] + ms[1] + ms[2];

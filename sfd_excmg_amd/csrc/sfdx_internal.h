@@ -57,8 +57,9 @@ struct HipError {
 template <typename R>
 struct DevGrid {
     Dims d;
-    R *a = nullptr, *b = nullptr, *c = nullptr;  // widths (device)
-    R *sig = nullptr;                            // 6*NC (device)
+    R *a = nullptr, *b = nullptr, *c = nullptr;     // widths (device)
+    R *ra = nullptr, *rb = nullptr, *rc = nullptr;  // reciprocal widths (device)
+    R *sig = nullptr;                               // 6*NC (device)
 };
 
 // Reduction scratch: partial sums per block, final slots.

@@ -225,7 +225,9 @@ __device__ __forceinline__ cx<R> row_apply(const Row<R> &r, R wmu, const cx<R> *
 
 template <typename R, int DOT>
 __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, const R *__restrict__ b,
-                                              const R *__restrict__ c, const R *__restrict__ sig, R wmu,
+                                              const R *__restrict__ c, const R *__restrict__ ra,
+                                              const R *__restrict__ rb, const R *__restrict__ rc,
+                                              const R *__restrict__ sig, R wmu,
                                               const cx<R> *__restrict__ x, cx<R> *__restrict__ y, R alpha,
                                               const cx<R> *__restrict__ w, double *__restrict__ partials,
                                               const cx<R> *__restrict__ g2)
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, c
         cx<R> out(0);
         if (edge_interior(d, S.o, i, j, S.k)) {
             Row<R> r;
-            row_coeffs(d, a, b, c, sig, S.o, i, j, S.k, r);
+            row_coeffs<R, true>(d, a, b, c, sig, S.o, i, j, S.k, r, ra, rb, rc);
             out = alpha * row_apply(r, wmu, x, x[e]);
             if (DOT == 4) out = w[e] + out;
             if (DOT == 5) {
@@ -273,7 +275,9 @@ __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, c
 // w = M_sigma x on interior edges
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_mass(Dims d, const R *__restrict__ a, const R *__restrict__ b,
-                                              const R *__restrict__ c, const R *__restrict__ sig,
+                                              const R *__restrict__ c, const R *__restrict__ ra,
+                                              const R *__restrict__ rb, const R *__restrict__ rc,
+                                              const R *__restrict__ sig,
                                               const cx<R> *__restrict__ x, cx<R> *__restrict__ w)
 {
     const Tile TL = tile_id();
@@ -285,7 +289,7 @@ __global__ __launch_bounds__(TPB) void k_mass(Dims d, const R *__restrict__ a, c
     cx<R> out(0);
     if (edge_interior(d, S.o, i, j, S.k)) {
         Row<R> r;
-        row_coeffs(d, a, b, c, sig, S.o, i, j, S.k, r);
+        row_coeffs<R, true>(d, a, b, c, sig, S.o, i, j, S.k, r, ra, rb, rc);
         out = r.kmd * x[e];
 #pragma unroll
         for (int s = 4; s < 12; s++) out = out + r.km[s] * x[r.nb[s]];
@@ -298,7 +302,9 @@ __global__ __launch_bounds__(TPB) void k_mass(Dims d, const R *__restrict__ a, c
 // Backward (DIR=1): out_e = omega*(out_e*D_e*(2-omega)/omega - sum_{upper} A_en out_n)/D_e
 template <typename R, int O, int PAR, int DIR>
 __global__ __launch_bounds__(TPB) void k_ssor(Dims d, const R *__restrict__ a, const R *__restrict__ b,
-                                              const R *__restrict__ c, const R *__restrict__ sig, R wmu,
+                                              const R *__restrict__ c, const R *__restrict__ ra,
+                                              const R *__restrict__ rb, const R *__restrict__ rc,
+                                              const R *__restrict__ sig, R wmu,
                                               const cx<R> *__restrict__ in, cx<R> *__restrict__ out, R omega)
 {
     const Tile TL = tile_id();
@@ -312,7 +318,7 @@ __global__ __launch_bounds__(TPB) void k_ssor(Dims d, const R *__restrict__ a, c
         return;
     }
     Row<R> r;
-    row_coeffs(d, a, b, c, sig, O, i, j, k, r);
+    row_coeffs<R, true>(d, a, b, c, sig, O, i, j, k, r, ra, rb, rc);
     // group orientations: O=0 -> (1,2); O=1 -> (0,2); O=2 -> (0,1)
     constexpr int OA = (O == 0) ? 1 : 0;
     constexpr int OB = (O == 2) ? 1 : 2;
@@ -333,8 +339,8 @@ __global__ __launch_bounds__(TPB) void k_ssor(Dims d, const R *__restrict__ a, c
         for (int q = 8; q < 12; q++) s = fma_c(s, coef_of(r.kg[q], r.km[q], wmu), out[r.nb[q]]);
     }
     const cx<R> D = coef_of(r.kgd, r.kmd, wmu);
-    const R den = D.re * D.re + D.im * D.im;
-    const cx<R> Dinv(D.re / den, -D.im / den);
+    const R rden = (R)1 / (D.re * D.re + D.im * D.im);
+    const cx<R> Dinv(D.re * rden, -D.im * rden);
     if (DIR == 0) {
         out[e] = omega * ((in[e] - s) * Dinv);
     } else if (DIR == 1) {
@@ -1098,11 +1104,11 @@ void launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, 
 {
     SliceLaunch L = edge_launch(g.d);
     switch (dot_mode) {
-    case 0: hipLaunchKernelGGL((k_spmv<R, 0>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
-    case 1: hipLaunchKernelGGL((k_spmv<R, 1>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
-    case 2: hipLaunchKernelGGL((k_spmv<R, 2>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
-    case 3: hipLaunchKernelGGL((k_spmv<R, 3>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
-    default: hipLaunchKernelGGL((k_spmv<R, 4>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
+    case 0: hipLaunchKernelGGL((k_spmv<R, 0>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
+    case 1: hipLaunchKernelGGL((k_spmv<R, 1>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
+    case 2: hipLaunchKernelGGL((k_spmv<R, 2>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
+    case 3: hipLaunchKernelGGL((k_spmv<R, 3>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
+    default: hipLaunchKernelGGL((k_spmv<R, 4>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
     }
 }
 
@@ -1113,7 +1119,7 @@ void launch_resid_split(const DevGrid<R> &g, R wmu, const cx<R> *z, const cx<R> 
 {
     SliceLaunch L = edge_launch(g.d);
     hipLaunchKernelGGL((k_grad<R>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, psi, gtmp);
-    hipLaunchKernelGGL((k_spmv<R, 5>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, z, t, (R)-1, r,
+    hipLaunchKernelGGL((k_spmv<R, 5>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, z, t, (R)-1, r,
                        (double *)nullptr, (const cx<R> *)gtmp);
 }
 
@@ -1121,14 +1127,15 @@ template <typename R>
 void launch_mass(const DevGrid<R> &g, const cx<R> *x, cx<R> *w, hipStream_t st)
 {
     SliceLaunch L = edge_launch(g.d);
-    hipLaunchKernelGGL((k_mass<R>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, x, w);
+    hipLaunchKernelGGL((k_mass<R>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, x, w);
 }
 
 template <typename R, int O, int PAR, int DIR>
 static void ssor_one(const DevGrid<R> &g, R wmu, const cx<R> *in, cx<R> *out, R omega, hipStream_t st)
 {
     SliceLaunch L = colour_launch(g.d, 2 * O + PAR);
-    hipLaunchKernelGGL((k_ssor<R, O, PAR, DIR>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.sig, wmu, in, out, omega);
+    hipLaunchKernelGGL((k_ssor<R, O, PAR, DIR>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, in,
+                       out, omega);
 }
 
 template <typename R>

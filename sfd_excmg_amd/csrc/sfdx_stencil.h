@@ -107,18 +107,26 @@ struct Row {
 };
 
 // sigma accessor: sig[comp*NC + cell]
-template <typename R>
+// With RCP the reciprocal widths ra, rb, rc (1/a, 1/b, 1/c) replace the four
+// divisions of a row by multiplications (device kernels); without, the widths
+// are divided (host assembly).
+template <typename R, bool RCP = false>
 SFDX_HD void row_coeffs(const Dims &d, const R *__restrict__ a, const R *__restrict__ b,
                         const R *__restrict__ c, const R *__restrict__ sig, int o, int i, int j, int k,
-                        Row<R> &r)
+                        Row<R> &r, const R *__restrict__ ra = nullptr, const R *__restrict__ rb = nullptr,
+                        const R *__restrict__ rc = nullptr)
 {
+    auto ia = [&](int q) { return RCP ? ra[q] : (R)1 / a[q]; };
+    auto ib = [&](int q) { return RCP ? rb[q] : (R)1 / b[q]; };
+    auto ic = [&](int q) { return RCP ? rc[q] : (R)1 / c[q]; };
     const int64_t NC = d.NC;
     if (o == 0) {
         const R ai = a[i], bm = b[j - 1], bp = b[j], cm = c[k - 1], cp = c[k];
         const int64_t C00 = cell_id(d, i, j - 1, k - 1), C10 = cell_id(d, i, j, k - 1);
         const int64_t C01 = cell_id(d, i, j - 1, k), C11 = cell_id(d, i, j, k);
-        const R wkm = ai * (bm + bp) / (2 * cm), wkp = ai * (bm + bp) / (2 * cp);
-        const R wjm = ai * (cm + cp) / (2 * bm), wjp = ai * (cm + cp) / (2 * bp);
+        const R hb = (R)0.5 * ai * (bm + bp), hc = (R)0.5 * ai * (cm + cp);
+        const R wkm = hb * ic(k - 1), wkp = hb * ic(k);
+        const R wjm = hc * ib(j - 1), wjp = hc * ib(j);
         const R hy = (cm + cp) / 2, hz = (bm + bp) / 2;
         r.nb[0] = ex_id(d, i, j, k - 1); r.kg[0] = -wkm;
         r.nb[1] = ex_id(d, i, j - 1, k); r.kg[1] = -wjm;
@@ -144,8 +152,9 @@ SFDX_HD void row_coeffs(const Dims &d, const R *__restrict__ a, const R *__restr
         const R bj = b[j], am = a[i - 1], ap = a[i], cm = c[k - 1], cp = c[k];
         const int64_t D00 = cell_id(d, i - 1, j, k - 1), D10 = cell_id(d, i, j, k - 1);
         const int64_t D01 = cell_id(d, i - 1, j, k), D11 = cell_id(d, i, j, k);
-        const R wkm = bj * (am + ap) / (2 * cm), wkp = bj * (am + ap) / (2 * cp);
-        const R wim = bj * (cm + cp) / (2 * am), wip = bj * (cm + cp) / (2 * ap);
+        const R ha = (R)0.5 * bj * (am + ap), hc = (R)0.5 * bj * (cm + cp);
+        const R wkm = ha * ic(k - 1), wkp = ha * ic(k);
+        const R wim = hc * ia(i - 1), wip = hc * ia(i);
         const R hx = (cm + cp) / 2, hz = (am + ap) / 2;
         r.nb[0] = ey_id(d, i, j, k - 1); r.kg[0] = -wkm;
         r.nb[1] = ey_id(d, i - 1, j, k); r.kg[1] = -wim;
@@ -171,8 +180,9 @@ SFDX_HD void row_coeffs(const Dims &d, const R *__restrict__ a, const R *__restr
         const R ck = c[k], am = a[i - 1], ap = a[i], bm = b[j - 1], bp = b[j];
         const int64_t F00 = cell_id(d, i - 1, j - 1, k), F01 = cell_id(d, i - 1, j, k);
         const int64_t F10 = cell_id(d, i, j - 1, k), F11 = cell_id(d, i, j, k);
-        const R wjm = ck * (am + ap) / (2 * bm), wjp = ck * (am + ap) / (2 * bp);
-        const R wim = ck * (bm + bp) / (2 * am), wip = ck * (bm + bp) / (2 * ap);
+        const R ha = (R)0.5 * ck * (am + ap), hb = (R)0.5 * ck * (bm + bp);
+        const R wjm = ha * ib(j - 1), wjp = ha * ib(j);
+        const R wim = hb * ia(i - 1), wip = hb * ia(i);
         const R hx = (bm + bp) / 2, hy = (am + ap) / 2;
         r.nb[0] = ez_id(d, i, j - 1, k); r.kg[0] = -wjm;
         r.nb[1] = ez_id(d, i - 1, j, k); r.kg[1] = -wim;
