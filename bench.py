@@ -38,6 +38,21 @@ HBM_PEAK_GBS = 8000.0
 RESTART_DROP = 1e-3
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` measured by two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE, calibrated against a plain copy) on this workload,
+    as recorded in profiles/k_spmv_c2_128x128x96_kernels.md; None if absent."""
+    path = os.path.join(ROOT, "profiles", "k_spmv_c2_128x128x96_kernels.md")
+    if not os.path.exists(path):
+        return None
+    key = f"traffic_bytes {kernel} = "
+    with open(path) as f:
+        for line in f:
+            if line.startswith(key):
+                return int(line[len(key):].strip())
+    return None
+
+
 def spmv_bytes(nl, ncell, sc, sr, nsig=6):
     """Canonical algorithmic bytes of one stencil application: read x, write y,
     read the model once."""
@@ -142,6 +157,34 @@ def run_c4(ctx, args):
                       "dtype": "complex128 Krylov/operator + complex64 preconditioner"}))
 
 
+def run_c3(ctx, args):
+    """C3: the 128x128x96 anisotropic model with undulating topography (cells
+    above the ground become air), FP64 and mixed-precision solves to 1e-8;
+    per-station, per-component impedance errors of the mixed path against the
+    FP64 path."""
+    from sfd_excmg_amd import sfdx, synth
+
+    m = synth.make_c3() if not args.small else synth.make_c3(nx=64, nz_earth=40, nair=8, nstations=20)
+    out = {}
+    exs = {}
+    for name, prec in (("fp64", sfdx.PREC_FP64), ("mixed", sfdx.PREC_MIXED)):
+        ex, r = run_gpu(ctx, m, 1, 1, solve_opts(m, prec, 1e-8))
+        exs[name] = ex
+        out[name] = {"fine_s": round(r["fine"], 4), "wall_s": round(r["wall"], 3), "fine_iters": r["fine_iters"],
+                     "status": r["status"], "relres": r["stats"]["relres"][m.level - 1] if r["stats"] else None}
+        print(f"{name} done {r['fine']:.3f} s", file=sys.stderr, flush=True)
+    z64, zmx = impedances(m, exs["fp64"]), impedances(m, exs["mixed"])
+    err = np.abs(zmx - z64) / np.abs(z64)
+    print(json.dumps({"metric": "mixed vs FP64 on the topography model (C3): fine-grid solve s and impedance error",
+                      "value": out["mixed"]["fine_s"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+                      "fp64": out["fp64"], "mixed": out["mixed"],
+                      "speedup_mixed_vs_fp64": round(out["fp64"]["fine_s"] / out["mixed"]["fine_s"], 3),
+                      "impedance_rel_err": {"max": float(err.max()), "p99": float(np.quantile(err, 0.99)),
+                                            "median": float(np.median(err)), "count": int(err.size)},
+                      "topography_max_depth_m": m.meta.get("topography_max_depth"),
+                      "dtype": "complex128 Krylov/operator + complex64 preconditioner"}))
+
+
 def run_c5(ctx, args):
     """C5: misfit gradient with respect to log sigma over all cells of the
     96x96x64 isotropic random-field model, 6 frequencies, 20x20 stations, data
@@ -210,6 +253,8 @@ def main():
     ap.add_argument("--c5", action="store_true", help="adjoint gradient on the 96x96x64 random-field model with a "
                     "central-difference check on --fd-cells random cells")
     ap.add_argument("--fd-cells", type=int, default=4)
+    ap.add_argument("--c3", action="store_true", help="128x128x96 anisotropic model with topography: mixed vs FP64, "
+                    "per-impedance errors")
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel eagerly (no HIP graph replay); "
                     "used for profiler runs")
@@ -234,6 +279,9 @@ def main():
         return
     if args.c5:
         run_c5(ctx, args)
+        return
+    if args.c3:
+        run_c3(ctx, args)
         return
     if args.kernels_only:
         ms64, ms32 = kernel_times(ctx, m, reps=20)
@@ -281,9 +329,11 @@ def main():
         "fine_solve_hbm_gbs_algorithmic": {"mixed": round(mx["fine_bytes"] / mx["fine"] / 1e9, 1) if mx["fine"] else None,
                                           "fp64": round(f64["fine_bytes"] / f64["fine"] / 1e9, 1) if f64["fine"] else None},
         "roofline": {"kernel": "k_spmv<double,0> (complex128 stencil)", "bound": "hbm", "achieved": round(ach64, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),
+                     "traffic": None if args.small else profiled_traffic("k_spmv<double,0>"),
                      "avg_ms": round(ms64[0], 5), "algorithmic_bytes": b64,
                      "complex64": {"kernel": "k_spmv<float,0>", "achieved": round(ach32, 1),
+                                   "traffic": None if args.small else profiled_traffic("k_spmv<float,0>"),
                                    "frac": round(ach32 / HBM_PEAK_GBS, 4), "avg_ms": round(ms32[0], 5),
                                    "algorithmic_bytes": b32},
                      "ssor_apply_ms": {"fp32": round(ms32[1], 4), "fp64": round(ms64[1], 4)},
