@@ -12,7 +12,7 @@ path, averaged over the timed steps:
     the nodal auxiliary-space correction, in complex128);
   * mixed: complex128 COCR, operator and residual, with the hybrid
     preconditioner applied in complex64 and restarts from the true residual
-    after every 1e-3 drop.
+    after every 1e-4 drop (and a final check of the true residual).
 The roofline object is for the stencil SpMV (k_spmv), timed with HIP events on
 the library's stream.  The CPU baseline is the FP64 oracle (CSR, lexicographic
 SSOR, COCR) on one core, timed on a bounded number of fine-level iterations and
@@ -35,7 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-RESTART_DROP = 1e-3
+RESTART_DROP = 1e-4
 
 
 def profiled_traffic(kernel):
@@ -159,28 +159,35 @@ def run_c4(ctx, args):
 
 def run_c3(ctx, args):
     """C3: the 128x128x96 anisotropic model with undulating topography (cells
-    above the ground become air), FP64 and mixed-precision solves to 1e-8;
-    per-station, per-component impedance errors of the mixed path against the
-    FP64 path."""
+    above the ground become air).  FP64 and mixed-precision solves to 1e-8, and
+    an FP64 solve to 1e-11 as the reference: per-station, per-component
+    impedance errors of both 1e-8 solves against it, so the error that mixed
+    precision adds is separated from the error the 1e-8 tolerance leaves."""
     from sfd_excmg_amd import sfdx, synth
 
     m = synth.make_c3() if not args.small else synth.make_c3(nx=64, nz_earth=40, nair=8, nstations=20)
+    ref_opts = solve_opts(m, sfdx.PREC_FP64, 1e-11)
+    ref_opts.maxit_excmg = 8000
+    ex_ref, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, ref_opts)
+    z_ref = impedances(m, ex_ref)
     out = {}
-    exs = {}
     for name, prec in (("fp64", sfdx.PREC_FP64), ("mixed", sfdx.PREC_MIXED)):
         ex, r = run_gpu(ctx, m, 1, 1, solve_opts(m, prec, 1e-8))
-        exs[name] = ex
+        err = np.abs(impedances(m, ex) - z_ref) / np.abs(z_ref)
         out[name] = {"fine_s": round(r["fine"], 4), "wall_s": round(r["wall"], 3), "fine_iters": r["fine_iters"],
-                     "status": r["status"], "relres": r["stats"]["relres"][m.level - 1] if r["stats"] else None}
+                     "status": r["status"],
+                     "impedance_rel_err_vs_fp64_1e-11": {"max": float(err.max()), "p99": float(np.quantile(err, 0.99)),
+                                                          "median": float(np.median(err)), "count": int(err.size)}}
+        if name == "fp64":
+            z64 = impedances(m, ex)
+        else:
+            d = np.abs(impedances(m, ex) - z64) / np.abs(z64)
+            out[name]["impedance_rel_err_vs_fp64_1e-8"] = {"max": float(d.max()), "median": float(np.median(d))}
         print(f"{name} done {r['fine']:.3f} s", file=sys.stderr, flush=True)
-    z64, zmx = impedances(m, exs["fp64"]), impedances(m, exs["mixed"])
-    err = np.abs(zmx - z64) / np.abs(z64)
     print(json.dumps({"metric": "mixed vs FP64 on the topography model (C3): fine-grid solve s and impedance error",
                       "value": out["mixed"]["fine_s"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
                       "fp64": out["fp64"], "mixed": out["mixed"],
                       "speedup_mixed_vs_fp64": round(out["fp64"]["fine_s"] / out["mixed"]["fine_s"], 3),
-                      "impedance_rel_err": {"max": float(err.max()), "p99": float(np.quantile(err, 0.99)),
-                                            "median": float(np.median(err)), "count": int(err.size)},
                       "topography_max_depth_m": m.meta.get("topography_max_depth"),
                       "dtype": "complex128 Krylov/operator + complex64 preconditioner"}))
 
