@@ -96,6 +96,8 @@ typedef struct {
                             this factor since the last restart (0 = never) */
     int use_graph;      /* 1: replay the steady-state COCR iteration as a captured HIP
                            graph (only without periodic divergence correction) */
+    int nodal_smoother; /* nodal part of the hybrid preconditioner: 0 8-colour SSOR on
+                           P = G^T M G, 1 two damped Jacobi steps (one pass over P) */
 } sfdx_opts;
 
 typedef struct {

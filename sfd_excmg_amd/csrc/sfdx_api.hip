@@ -321,6 +321,7 @@ struct KOpts {
     int mg_level = 0;                // level being solved
     double restart_drop = 0;         // COCR: restart from the true residual after this drop
     int use_graph = 1;               // replay the steady-state iteration as a HIP graph
+    int nodal = 0;                   // hybrid preconditioner: 0 nodal SSOR, 1 two-step Jacobi
 };
 
 // out = M^-1 in with the chosen preconditioner
@@ -353,7 +354,7 @@ void apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
         zf *tn = pool_vec<zf>(ctx, tag + "/mp_t", d.NP), *ph = pool_vec<zf>(ctx, tag + "/mp_phi", d.NP);
         if (ko.precond == SFDX_PRECOND_HYBRID) {
             launch_hybrid_mixed(*(DevGrid<double> *)&D.g, F.g, (double)wmu, F.P, (const zd *)in, (zd *)out, in32, z32,
-                                tn, ph, (float)ko.omega, ctx->st);
+                                tn, ph, (float)ko.omega, ctx->st, ko.nodal);
         } else {
             launch_convert<double, float>(d.NL, (const zd *)in, in32, ctx->st);
             launch_ssor<float>(F.g, (float)wmu, in32, z32, (float)ko.omega, ctx->st);
@@ -365,7 +366,7 @@ void apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
         ensure_P(ctx, D, tag);
         const std::string p = tag + (sizeof(R) == 8 ? "/hy64" : "/hy32");
         cx<R> *tn = pool_vec<cx<R>>(ctx, p + "t", D.g.d.NP), *ph = pool_vec<cx<R>>(ctx, p + "phi", D.g.d.NP);
-        launch_hybrid<R>(D.g, wmu, D.P, in, out, tn, ph, (R)ko.omega, ctx->st);
+        launch_hybrid<R>(D.g, wmu, D.P, in, out, tn, ph, (R)ko.omega, ctx->st, ko.nodal);
     } else {
         launch_ssor<R>(D.g, wmu, in, out, (R)ko.omega, ctx->st);
     }
@@ -718,6 +719,7 @@ KOpts kopts_of(const sfdx_opts &o)
     k.dc_at_start = o.dc_at_start;
     k.precond = o.precond;
     k.use_graph = o.use_graph;
+    k.nodal = o.nodal_smoother;
     return k;
 }
 
@@ -1170,6 +1172,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->mg_coarse_pairs = 4;
     o->restart_drop = 1e-4;
     o->use_graph = 1;
+    o->nodal_smoother = 0;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)

@@ -193,9 +193,10 @@ void launch_prolong(const Dims &df, const Dims &dc, int m, const ProlongDev &T, 
                     hipStream_t st);
 template <typename R>
 void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode, cx<R> *phi,
-                   R omega, hipStream_t st);
+                   R omega, hipStream_t st, int nodal = 0);
 void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *P32,
-                         const zd *in, zd *out, zf *in32, zf *z32, zf *tnode, zf *phi, float omega, hipStream_t st);
+                         const zd *in, zd *out, zf *in32, zf *z32, zf *tnode, zf *phi, float omega, hipStream_t st,
+                         int nodal = 0);
 // One hybrid Gauss-Seidel step: out = GS_e(in), phi = GS_n(G^T in); with
 // add_grad the nodal part is folded in (out += (i/wmu) G phi).
 template <typename R>
