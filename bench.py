@@ -8,10 +8,11 @@ fine-level relative residual of 1e-8 (interior rows).
 A step is one full forward solve (all levels, both polarisations).  `value`
 is the fine-grid solve time (both polarisations) of the faster converged GPU
 path, averaged over the timed steps:
-  * FP64: complex128 COCR with the hybrid preconditioner (multicolour SSOR plus
-    the nodal auxiliary-space correction, in complex128);
+  * FP64: complex128 COCR with the hybrid preconditioner (multicolour edge SSOR
+    plus the nodal auxiliary-space correction by a two-step Jacobi polynomial,
+    in complex128);
   * mixed: complex128 COCR, operator and residual, with the hybrid
-    preconditioner applied in complex64 and restarts from the true residual
+    preconditioner (edge SSOR + nodal SSOR) applied in complex64 and restarts from the true residual
     after every 1e-4 drop (and a final check of the true residual).
 The roofline object is for the stencil SpMV (k_spmv), timed with HIP events on
 the library's stream.  The CPU baseline is the FP64 oracle (CSR, lexicographic
@@ -70,10 +71,16 @@ def solve_opts(m, precision, eps, verbose=0, precond=None):
     # (every 50 iterations in this workload) is not applied: it breaks the COCR
     # recurrences (measured: divergence when the directions are kept, 2.3x more
     # iterations with restarts).
+    # Nodal part of the hybrid preconditioner: the one-pass Jacobi polynomial is
+    # the faster choice in complex128 (1.85 s against 2.20 s with the nodal SSOR
+    # on this workload); with the complex64 preconditioner the nodal SSOR needs
+    # fewer iterations (2491 against 3101) and is faster.
+    nodal = 1 if precision == sfdx.PREC_FP64 else 0
     return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0,
                              norm=sfdx.NORM_INTERIOR, precision=precision, verbose=verbose,
                              precond=sfdx.PRECOND_HYBRID if precond is None else precond,
-                             sigma_floor32=0.0, restart_drop=RESTART_DROP, maxit_excmg=4000, use_graph=USE_GRAPH)
+                             sigma_floor32=0.0, restart_drop=RESTART_DROP, maxit_excmg=4000, use_graph=USE_GRAPH,
+                             nodal_smoother=nodal)
 
 
 def run_gpu(ctx, m, steps, warmup, opts):

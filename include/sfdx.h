@@ -97,7 +97,8 @@ typedef struct {
     int use_graph;      /* 1: replay the steady-state COCR iteration as a captured HIP
                            graph (only without periodic divergence correction) */
     int nodal_smoother; /* nodal part of the hybrid preconditioner: 0 8-colour SSOR on
-                           P = G^T M G, 1 two damped Jacobi steps (one pass over P) */
+                           P = G^T M G, 1 (default) two damped Jacobi steps (one pass
+                           over P instead of 16 dependent colour sweeps) */
 } sfdx_opts;
 
 typedef struct {

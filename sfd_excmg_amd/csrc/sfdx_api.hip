@@ -1172,7 +1172,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->mg_coarse_pairs = 4;
     o->restart_drop = 1e-4;
     o->use_graph = 1;
-    o->nodal_smoother = 0;
+    o->nodal_smoother = 1;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)
