@@ -11,9 +11,10 @@ path, averaged over the timed steps:
   * FP64: complex128 COCR with the hybrid preconditioner (multicolour edge SSOR
     plus the nodal auxiliary-space correction by a two-step Jacobi polynomial,
     in complex128);
-  * mixed: complex128 COCR, operator and residual, with the hybrid
-    preconditioner (edge SSOR + nodal SSOR) applied in complex64 and restarts from the true residual
-    after every 1e-4 drop (and a final check of the true residual).
+  * mixed: complex128 COCR, operator and residual; the edge SSOR of the
+    hybrid preconditioner in complex64, its nodal part in complex128; restarts
+    from the true residual after every 1e-6 drop and on stagnation, and a
+    final check of the true residual.
 The roofline object is for the stencil SpMV (k_spmv), timed with HIP events on
 the library's stream.  The CPU baseline is the FP64 oracle (CSR, lexicographic
 SSOR, COCR) on one core, timed on a bounded number of fine-level iterations and
@@ -36,7 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-RESTART_DROP = 1e-4
+RESTART_DROP = 1e-6
 
 
 def profiled_traffic(kernel):
@@ -71,11 +72,11 @@ def solve_opts(m, precision, eps, verbose=0, precond=None):
     # (every 50 iterations in this workload) is not applied: it breaks the COCR
     # recurrences (measured: divergence when the directions are kept, 2.3x more
     # iterations with restarts).
-    # Nodal part of the hybrid preconditioner: the one-pass Jacobi polynomial is
-    # the faster choice in complex128 (1.85 s against 2.20 s with the nodal SSOR
-    # on this workload); with the complex64 preconditioner the nodal SSOR needs
-    # fewer iterations (2491 against 3101) and is faster.
-    nodal = 1 if precision == sfdx.PREC_FP64 else 0
+    # Nodal part of the hybrid preconditioner: the one-pass Jacobi polynomial
+    # (1.76 s against 2.20 s with the nodal SSOR on this workload in complex128).
+    # In the mixed path it runs in complex128 while the edge SSOR runs in
+    # complex64 (fewer iterations than an all-complex64 preconditioner).
+    nodal = 1
     return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0,
                              norm=sfdx.NORM_INTERIOR, precision=precision, verbose=verbose,
                              precond=sfdx.PRECOND_HYBRID if precond is None else precond,

@@ -326,8 +326,10 @@ struct KOpts {
 
 // out = M^-1 in with the chosen preconditioner
 template <typename R>
-void apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *in, cx<R> *out,
-                const KOpts &ko)
+// With dotw the partial sums of dotw^T out are produced by the last kernel when
+// the preconditioner allows it (hybrid); returns whether it did.
+bool apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *in, cx<R> *out,
+                const KOpts &ko, const cx<R> *dotw = nullptr)
 {
     if (ko.precond == SFDX_PRECOND_MG) {
         const Dims &d = D.g.d;
@@ -344,7 +346,7 @@ void apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
             mg_vcycle<R>(ctx, H, ko.mg_level, wmu, in, out, pn);
             launch_grad_add<R>(D.g, cx<R>((R)0, (R)1 / wmu), pn, out, ctx->st);
         }
-        return;
+        return false;
     }
     if (sizeof(R) == 8 && ko.lo) {
         LevelDev<float> &F = *ko.lo;
@@ -352,24 +354,38 @@ void apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
         const Dims &d = F.g.d;
         zf *in32 = pool_vec<zf>(ctx, tag + "/mp_in", d.NL), *z32 = pool_vec<zf>(ctx, tag + "/mp_z", d.NL);
         zf *tn = pool_vec<zf>(ctx, tag + "/mp_t", d.NP), *ph = pool_vec<zf>(ctx, tag + "/mp_phi", d.NP);
-        if (ko.precond == SFDX_PRECOND_HYBRID) {
-            launch_hybrid_mixed(*(DevGrid<double> *)&D.g, F.g, (double)wmu, F.P, (const zd *)in, (zd *)out, in32, z32,
-                                tn, ph, (float)ko.omega, ctx->st, ko.nodal);
-        } else {
+        if (ko.precond == SFDX_PRECOND_HYBRID && ko.nodal == 1) {
+            // edge SSOR in complex64; the nodal part (one Jacobi pass over P, where
+            // the air's large potentials live) in complex128
+            LevelDev<double> &D64 = *(LevelDev<double> *)&D;
+            ensure_P(ctx, D64, tag);
+            zd *tn64 = pool_vec<zd>(ctx, tag + "/mp_t64", d.NP), *ph64 = pool_vec<zd>(ctx, tag + "/mp_phi64", d.NP);
             launch_convert<double, float>(d.NL, (const zd *)in, in32, ctx->st);
             launch_ssor<float>(F.g, (float)wmu, in32, z32, (float)ko.omega, ctx->st);
             launch_convert<float, double>(d.NL, z32, (zd *)out, ctx->st);
+            launch_nodal_correction(D64.g, (double)wmu, D64.P, (const zd *)in, (zd *)out, tn64, ph64, ctx->st,
+                                    (const zd *)dotw, ctx->rb.partials);
+            return dotw != nullptr;
         }
-        return;
+        if (ko.precond == SFDX_PRECOND_HYBRID) {
+            launch_hybrid_mixed(*(DevGrid<double> *)&D.g, F.g, (double)wmu, F.P, (const zd *)in, (zd *)out, in32, z32,
+                                tn, ph, (float)ko.omega, ctx->st, ko.nodal, (const zd *)dotw, ctx->rb.partials);
+            return dotw != nullptr;
+        }
+        launch_convert<double, float>(d.NL, (const zd *)in, in32, ctx->st);
+        launch_ssor<float>(F.g, (float)wmu, in32, z32, (float)ko.omega, ctx->st);
+        launch_convert<float, double>(d.NL, z32, (zd *)out, ctx->st);
+        return false;
     }
     if (ko.precond == SFDX_PRECOND_HYBRID) {
         ensure_P(ctx, D, tag);
         const std::string p = tag + (sizeof(R) == 8 ? "/hy64" : "/hy32");
         cx<R> *tn = pool_vec<cx<R>>(ctx, p + "t", D.g.d.NP), *ph = pool_vec<cx<R>>(ctx, p + "phi", D.g.d.NP);
-        launch_hybrid<R>(D.g, wmu, D.P, in, out, tn, ph, (R)ko.omega, ctx->st, ko.nodal);
-    } else {
-        launch_ssor<R>(D.g, wmu, in, out, (R)ko.omega, ctx->st);
+        launch_hybrid<R>(D.g, wmu, D.P, in, out, tn, ph, (R)ko.omega, ctx->st, ko.nodal, dotw, ctx->rb.partials);
+        return dotw != nullptr;
     }
+    launch_ssor<R>(D.g, wmu, in, out, (R)ko.omega, ctx->st);
+    return false;
 }
 
 // Solve A_s x = f on the interior edges (boundary entries of x held fixed; f may
@@ -499,9 +515,9 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
         }
         if (ko.verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  cocr it %d relres %.3e\n", info.iters, info.relres);
         auto part1 = [&]() {
-            int64_t nbd;
-            apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko);
-            launch_dot<R>(0, d.NL, Ap, q, ctx->rb, ctx->st, nbd);
+            int64_t nbd = nbe;
+            // q = M Ap, with (Ap, q) fused into the preconditioner's last kernel when possible
+            if (!apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko, Ap)) launch_dot<R>(0, d.NL, Ap, q, ctx->rb, ctx->st, nbd);
             reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
             launch_vec<R>(0, D.g, ctx->rb, x, pv, Ap, r, q, z, ctx->st);
             reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
@@ -1170,7 +1186,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->precond = SFDX_PRECOND_HYBRID;
     o->mg_levels = 0;
     o->mg_coarse_pairs = 4;
-    o->restart_drop = 1e-4;
+    o->restart_drop = 1e-6;
     o->use_graph = 1;
     o->nodal_smoother = 1;
 }

@@ -189,7 +189,19 @@ __global__ __launch_bounds__(1024) void k_reduce_final(const double *__restrict_
 {
     __shared__ double sm[8][16];
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x)
+    // four independent loads in flight per component, summed in increasing b
+    const int64_t step = blockDim.x;
+    int64_t b = threadIdx.x;
+    for (; b + 3 * step < nblk; b += 4 * step)
+        for (int q = 0; q < K; q++) {
+            const double *pq = partials + q * nblk;
+            const double v0 = pq[b], v1 = pq[b + step], v2 = pq[b + 2 * step], v3 = pq[b + 3 * step];
+            acc[q] += v0;
+            acc[q] += v1;
+            acc[q] += v2;
+            acc[q] += v3;
+        }
+    for (; b < nblk; b += step)
         for (int q = 0; q < K; q++) acc[q] += partials[q * nblk + b];
     for (int q = 0; q < K; q++) {
         double t = acc[q];
@@ -848,22 +860,32 @@ __global__ __launch_bounds__(TPB) void k_gradT(Dims d, const R *__restrict__ a, 
     t[n] = out;
 }
 
-// z_e += s * (G phi)_e
-template <typename R>
+// z_e += s * (G phi)_e; with DOT also the partial sums of w^T z (unconjugated)
+template <typename R, bool DOT = false>
 __global__ __launch_bounds__(TPB) void k_grad_add(Dims d, const R *__restrict__ a, const R *__restrict__ b,
                                                   const R *__restrict__ c, cx<R> s, const cx<R> *__restrict__ phi,
-                                                  cx<R> *__restrict__ z)
+                                                  cx<R> *__restrict__ z, const cx<R> *__restrict__ w,
+                                                  double *__restrict__ partials)
 {
     const Tile TL = tile_id();
     const Slice S = edge_slice(d, TL.y);
     const int p = TL.x * TPB + threadIdx.x;
-    if (p >= S.nI * S.nJ) return;
-    const int j = p / S.nI, i = p - j * S.nI;
-    const int64_t e = S.base + p;
-    int64_t n0, n1;
-    const R l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
-    const cx<R> g = (phi[n1] - phi[n0]) * ((R)1 / l);
-    z[e] = fma_c(z[e], s, g);
+    double acc[2] = {0, 0};
+    if (p < S.nI * S.nJ) {
+        const int j = p / S.nI, i = p - j * S.nI;
+        const int64_t e = S.base + p;
+        int64_t n0, n1;
+        const R l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
+        const cx<R> g = (phi[n1] - phi[n0]) * ((R)1 / l);
+        const cx<R> zn = fma_c(z[e], s, g);
+        z[e] = zn;
+        if (DOT) {
+            const cx<R> wv = w[e];
+            acc[0] = (double)wv.re * zn.re - (double)wv.im * zn.im;
+            acc[1] = (double)wv.re * zn.im + (double)wv.im * zn.re;
+        }
+    }
+    if (DOT) block_reduce_store<2>(acc, partials);
 }
 
 // g_e = (G phi)_e on every edge (0 on boundary edges when phi is 0 on boundary nodes)
@@ -885,21 +907,32 @@ __global__ __launch_bounds__(TPB) void k_grad(Dims d, const R *__restrict__ a, c
 // Mixed-precision recombination of the hybrid preconditioner:
 // z64_e = z32_e + s * (G phi32)_e, with G applied in FP64 so that later FP64
 // operator applications see the gradient part exactly.
+template <bool DOT = false>
 __global__ __launch_bounds__(TPB) void k_hyb_combine(Dims d, const double *__restrict__ a, const double *__restrict__ b,
                                                      const double *__restrict__ c, zd s, const zf *__restrict__ phi,
-                                                     const zf *__restrict__ z32, zd *__restrict__ z64)
+                                                     const zf *__restrict__ z32, zd *__restrict__ z64,
+                                                     const zd *__restrict__ w, double *__restrict__ partials)
 {
     const Tile TL = tile_id();
     const Slice S = edge_slice(d, TL.y);
     const int p = TL.x * TPB + threadIdx.x;
-    if (p >= S.nI * S.nJ) return;
-    const int j = p / S.nI, i = p - j * S.nI;
-    const int64_t e = S.base + p;
-    int64_t n0, n1;
-    const double l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
-    const zf f1 = phi[n1], f0 = phi[n0], zz = z32[e];
-    const zd g(((double)f1.re - (double)f0.re) / l, ((double)f1.im - (double)f0.im) / l);
-    z64[e] = fma_c(zd((double)zz.re, (double)zz.im), s, g);
+    double acc[2] = {0, 0};
+    if (p < S.nI * S.nJ) {
+        const int j = p / S.nI, i = p - j * S.nI;
+        const int64_t e = S.base + p;
+        int64_t n0, n1;
+        const double l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
+        const zf f1 = phi[n1], f0 = phi[n0], zz = z32[e];
+        const zd g(((double)f1.re - (double)f0.re) / l, ((double)f1.im - (double)f0.im) / l);
+        const zd zn = fma_c(zd((double)zz.re, (double)zz.im), s, g);
+        z64[e] = zn;
+        if (DOT) {
+            const zd wv = w[e];
+            acc[0] = wv.re * zn.re - wv.im * zn.im;
+            acc[1] = wv.re * zn.im + wv.im * zn.re;
+        }
+    }
+    if (DOT) block_reduce_store<2>(acc, partials);
 }
 
 // ------------------------------------------------------------------ K8 multigrid transfers
@@ -1147,7 +1180,9 @@ void launch_ssor(const DevGrid<R> &g, R wmu, const cx<R> *in, cx<R> *out, R omeg
     ssor_one<R, 1, 1, 0>(g, wmu, in, out, omega, st);
     ssor_one<R, 2, 0, 0>(g, wmu, in, out, omega, st);
     ssor_one<R, 2, 1, 0>(g, wmu, in, out, omega, st);
-    ssor_one<R, 2, 1, 1>(g, wmu, in, out, omega, st);
+    // the backward step of the last colour has no upper neighbours: it only
+    // scales by (2 - omega), which is the identity for omega = 1
+    if (omega != (R)1) ssor_one<R, 2, 1, 1>(g, wmu, in, out, omega, st);
     ssor_one<R, 2, 0, 1>(g, wmu, in, out, omega, st);
     ssor_one<R, 1, 1, 1>(g, wmu, in, out, omega, st);
     ssor_one<R, 1, 0, 1>(g, wmu, in, out, omega, st);
@@ -1339,7 +1374,7 @@ static void nodal_smooth(const Dims &d, const R *P, const cx<R> *t, cx<R> *phi, 
 // (nodal 0: 8-colour SSOR on P; 1: two-step Jacobi polynomial)
 template <typename R>
 void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode,
-                   cx<R> *phi, R omega, hipStream_t st, int nodal)
+                   cx<R> *phi, R omega, hipStream_t st, int nodal, const cx<R> *dotw, double *partials)
 {
     launch_ssor<R>(g, wmu, in, out, omega, st);
     SliceLaunch Ln = node_launch(g.d);
@@ -1348,14 +1383,37 @@ void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R
     // s = -1/kappa = -1/(i wmu) = i/wmu
     const cx<R> s((R)0, (R)1 / wmu);
     SliceLaunch Le = edge_launch(g.d);
-    hipLaunchKernelGGL((k_grad_add<R>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out);
+    if (dotw)
+        hipLaunchKernelGGL((k_grad_add<R, true>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out, dotw,
+                           partials);
+    else
+        hipLaunchKernelGGL((k_grad_add<R, false>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out,
+                           (const cx<R> *)nullptr, (double *)nullptr);
+}
+
+// out += s * G N_P(G^T in) with the one-pass Jacobi nodal smoother, complex128
+// (the nodal half of the hybrid preconditioner on its own), optional fused dot
+void launch_nodal_correction(const DevGrid<double> &g, double wmu, const double *P, const zd *in, zd *out, zd *tnode,
+                             zd *phi, hipStream_t st, const zd *dotw, double *partials)
+{
+    SliceLaunch Ln = node_launch(g.d);
+    hipLaunchKernelGGL((k_gradT<double>), Ln.grid, Ln.block, 0, st, g.d, g.a, g.b, g.c, in, tnode);
+    nodal_smooth<double>(g.d, P, tnode, phi, 1, st);
+    const zd s(0.0, 1.0 / wmu);
+    SliceLaunch Le = edge_launch(g.d);
+    if (dotw)
+        hipLaunchKernelGGL((k_grad_add<double, true>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out, dotw,
+                           partials);
+    else
+        hipLaunchKernelGGL((k_grad_add<double, false>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out,
+                           (const zd *)nullptr, (double *)nullptr);
 }
 
 // out64 = SSOR32(in) + s * G SSOR_P32(G^T in) with the FP32 pieces on a
 // complex64 copy of the input and the recombination in FP64.
 void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *P32,
                          const zd *in, zd *out, zf *in32, zf *z32, zf *tnode, zf *phi, float omega, hipStream_t st,
-                         int nodal)
+                         int nodal, const zd *dotw, double *partials)
 {
     const int64_t nl = g64.d.NL;
     launch_convert<double, float>(nl, in, in32, st);
@@ -1365,7 +1423,12 @@ void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, 
     nodal_smooth<float>(g32.d, P32, tnode, phi, nodal, st);
     const zd s(0.0, 1.0 / wmu);
     SliceLaunch Le = edge_launch(g64.d);
-    hipLaunchKernelGGL(k_hyb_combine, Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, out);
+    if (dotw)
+        hipLaunchKernelGGL(k_hyb_combine<true>, Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, out,
+                           dotw, partials);
+    else
+        hipLaunchKernelGGL(k_hyb_combine<false>, Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, out,
+                           (const zd *)nullptr, (double *)nullptr);
 }
 
 // One Gauss-Seidel smoothing step of the hybrid smoother:
@@ -1414,7 +1477,8 @@ void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, c
     if (!add_grad) return;  // caller keeps the nodal part phi (split form)
     const cx<R> s((R)0, (R)1 / wmu);
     SliceLaunch Le = edge_launch(g.d);
-    hipLaunchKernelGGL((k_grad_add<R>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out);
+    hipLaunchKernelGGL((k_grad_add<R, false>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out,
+                       (const cx<R> *)nullptr, (double *)nullptr);
 }
 
 template <typename R>
@@ -1470,13 +1534,15 @@ template <typename R>
 void launch_grad_add(const DevGrid<R> &g, cx<R> s, const cx<R> *phi, cx<R> *z, hipStream_t st)
 {
     SliceLaunch L = edge_launch(g.d);
-    hipLaunchKernelGGL((k_grad_add<R>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, s, phi, z);
+    hipLaunchKernelGGL((k_grad_add<R, false>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, s, phi, z,
+                       (const cx<R> *)nullptr, (double *)nullptr);
 }
 
 void launch_hyb_combine(const DevGrid<double> &g64, zd s, const zf *phi, const zf *z32, zd *z64, hipStream_t st)
 {
     SliceLaunch L = edge_launch(g64.d);
-    hipLaunchKernelGGL(k_hyb_combine, L.grid, L.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, z64);
+    hipLaunchKernelGGL(k_hyb_combine<false>, L.grid, L.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, z64,
+                       (const zd *)nullptr, (double *)nullptr);
 }
 
 template <typename R>
@@ -1514,7 +1580,7 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
                                     hipStream_t);                                                                 \
     template void launch_extrap<R>(int64_t, R, const cx<R> *, R, cx<R> *, hipStream_t);                        \
     template void launch_hybrid<R>(const DevGrid<R> &, R, const R *, const cx<R> *, cx<R> *, cx<R> *, cx<R> *, R, \
-                                   hipStream_t, int);                                                             \
+                                   hipStream_t, int, const cx<R> *, double *);                                   \
     template void launch_hybrid_gs<R>(const DevGrid<R> &, R, const R *, const cx<R> *, cx<R> *, cx<R> *, cx<R> *,  \
                                       int, hipStream_t, bool);                                                    \
     template void launch_resid_split<R>(const DevGrid<R> &, R, const cx<R> *, const cx<R> *, const cx<R> *,       \
