@@ -445,13 +445,15 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             if (f) launch_axpby<R>(d.NL, cx<R>(1), f, cx<R>(1), r, ctx->st);
             apply_prec<R>(ctx, D, tag, wmu, r, z, ko);
             launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
+            // The direction p is kept one step behind: the update kernel (mode 6)
+            // forms p = z + beta p itself, so here only Ap is advanced.
             if (state == 0) {
-                reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_RHO, ctx->st);
+                reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_RHO, ctx->st);  // also beta = 0
                 launch_copy<R>(d.NL, z, pv, ctx->st);
                 launch_copy<R>(d.NL, Az, Ap, ctx->st);
             } else {
                 reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_BETA, ctx->st);
-                launch_vec<R>(3, D.g, ctx->rb, pv, z, Ap, Az, nullptr, nullptr, ctx->st);
+                launch_vec<R>(7, D.g, ctx->rb, nullptr, nullptr, Ap, Az, nullptr, nullptr, ctx->st);
             }
             launch_vec<R>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
             reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
@@ -519,13 +521,13 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             // q = M Ap, with (Ap, q) fused into the preconditioner's last kernel when possible
             if (!apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko, Ap)) launch_dot<R>(0, d.NL, Ap, q, ctx->rb, ctx->st, nbd);
             reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
-            launch_vec<R>(0, D.g, ctx->rb, x, pv, Ap, r, q, z, ctx->st);
+            launch_vec<R>(6, D.g, ctx->rb, x, pv, Ap, r, q, z, ctx->st);
             reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
         };
         auto part2 = [&]() {
             launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
             reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_BETA, ctx->st);
-            launch_vec<R>(3, D.g, ctx->rb, pv, z, Ap, Az, nullptr, nullptr, ctx->st);
+            launch_vec<R>(7, D.g, ctx->rb, nullptr, nullptr, Ap, Az, nullptr, nullptr, ctx->st);
         };
         since_best++;
         if (ko.use_graph && ko.corr_intv <= 0 && pending) {

@@ -58,6 +58,7 @@ __device__ __forceinline__ void scalar_step(double *s, int op)
     case OP_RN: s[SL_RN] = t[0]; break;
     case OP_COCR_RHO:
         s[SL_RHO] = t[0]; s[SL_RHO + 1] = t[1];
+        s[SL_BETA] = 0; s[SL_BETA + 1] = 0;  // p = z at a fresh start (direction folded one step behind)
         s[SL_BREAK] = 0;  // fresh start of the recurrences
         break;
     case OP_COCR_ALPHA:

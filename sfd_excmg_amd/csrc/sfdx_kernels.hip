@@ -427,9 +427,35 @@ __global__ __launch_bounds__(TPB) void k_vec(Dims d, const R *__restrict__ a, co
                 double ve = (double)dual_volume(a, b, c, S.o, i, j, S.k);
                 acc[0] = ((double)rr.re * rr.re + (double)rr.im * rr.im) / (ve * ve);
             }
+        } else if (MODE == 6) {
+            // COCR update with the direction folded in (p is kept one step behind):
+            // v0 = x, v1 = p, v2 = Ap, v3 = r, v4 = q, v5 = z
+            //   p = z + beta p; x += alpha p; r -= alpha Ap; z -= alpha q
+            // after a breakdown nothing is written, so x, p, r and z stay finite
+            const bool brk = slots[SL_BREAK] != 0;
+            cx<R> rr = v3[e];
+            if (!brk) {
+                const cx<R> al((R)slots[SL_ALPHA], (R)slots[SL_ALPHA + 1]);
+                const cx<R> be((R)slots[SL_BETA], (R)slots[SL_BETA + 1]);
+                const cx<R> zv = v5[e];
+                const cx<R> pn = fma_c(zv, be, v1[e]);
+                v1[e] = pn;
+                v0[e] = fma_c(v0[e], al, pn);
+                rr = rr - al * v2[e];
+                v3[e] = rr;
+                v5[e] = zv - al * v4[e];
+            }
+            if (inner) {
+                double ve = (double)dual_volume(a, b, c, S.o, i, j, S.k);
+                acc[0] = ((double)rr.re * rr.re + (double)rr.im * rr.im) / (ve * ve);
+            }
+        } else if (MODE == 7) {
+            // Ap = Az + beta Ap  (v2 = Ap, v3 = Az)
+            const cx<R> be((R)slots[SL_BETA], (R)slots[SL_BETA + 1]);
+            v2[e] = fma_c(v3[e], be, v2[e]);
         }
     }
-    if (MODE == 0 || MODE == 2 || MODE == 5) block_reduce_store<4>(acc, partials);
+    if (MODE == 0 || MODE == 2 || MODE == 5 || MODE == 6) block_reduce_store<4>(acc, partials);
 }
 
 // GPBiCG fused vector kernels (modes documented at launch_gpbicg)
@@ -1202,7 +1228,9 @@ void launch_vec(int mode, const DevGrid<R> &g, RedBuf &rb, cx<R> *v0, cx<R> *v1,
     case 2: SFDX_VEC(2); break;
     case 3: SFDX_VEC(3); break;
     case 4: SFDX_VEC(4); break;
-    default: SFDX_VEC(5); break;
+    case 5: SFDX_VEC(5); break;
+    case 6: SFDX_VEC(6); break;
+    default: SFDX_VEC(7); break;
     }
 #undef SFDX_VEC
 }
