@@ -133,14 +133,14 @@ def kernel_times(ctx, m, reps=50):
 
 
 def run_c4(ctx, args):
-    """C4: the 128x128x96 anisotropic model at 16 frequencies (1e-3..1e3 Hz)
+    """Batched workload: the 128x128x96 anisotropic model at 16 frequencies (1e-3..1e3 Hz)
     x 2 polarisations plus 8 isotropic models of uneven sizes at 1 Hz, solved
     (a) as one batch over concurrent streams and (b) one item at a time."""
     from sfd_excmg_amd import sfdx, synth
 
-    big = synth.make_c2() if not args.small else synth.make_c2(nx=64, nz_earth=40, nair=8, nstations=20)
-    big.freqs = synth.c4_frequencies()
-    small = synth.make_c4_models()
+    big = synth.make_aniso_blocks_128() if not args.small else synth.make_aniso_blocks_128(nx=64, nz_earth=40, nair=8, nstations=20)
+    big.freqs = synth.sounding_frequencies_16()
+    small = synth.make_uneven_iso_models()
     if args.small:
         small = small[:3]
         big.freqs = big.freqs[::4]
@@ -155,7 +155,7 @@ def run_c4(ctx, args):
     exs, sts = ctx.solve_batch(models, opts, nstreams=1)
     t_seq = time.perf_counter() - t0
     diff = max(float(np.max(np.abs(p - q)) / np.max(np.abs(q))) for p, q in zip(exb, exs))
-    print(json.dumps({"metric": "batched forward solves (C4): right-hand sides per second", "value": round(nrhs / t_batch, 3),
+    print(json.dumps({"metric": "batched forward solves (16 frequencies x 128x128x96 + 8 uneven models): right-hand sides per second", "value": round(nrhs / t_batch, 3),
                       "unit": "RHS/s", "n_gpus": 1, "higher_is_better": True, "nstreams": args.nstreams,
                       "rhs": nrhs, "models": [[m.nx, m.ny, m.nz] for m in models],
                       "batch_s": round(t_batch, 3), "one_stream_s": round(t_seq, 3),
@@ -166,14 +166,14 @@ def run_c4(ctx, args):
 
 
 def run_c3(ctx, args):
-    """C3: the 128x128x96 anisotropic model with undulating topography (cells
+    """Topography workload: the 128x128x96 anisotropic model with undulating topography (cells
     above the ground become air).  FP64 and mixed-precision solves to 1e-8, and
     an FP64 solve to 1e-11 as the reference: per-station, per-component
     impedance errors of both 1e-8 solves against it, so the error that mixed
     precision adds is separated from the error the 1e-8 tolerance leaves."""
     from sfd_excmg_amd import sfdx, synth
 
-    m = synth.make_c3() if not args.small else synth.make_c3(nx=64, nz_earth=40, nair=8, nstations=20)
+    m = synth.make_aniso_topography_128() if not args.small else synth.make_aniso_topography_128(nx=64, nz_earth=40, nair=8, nstations=20)
     ref_opts = solve_opts(m, sfdx.PREC_FP64, 1e-11)
     ref_opts.maxit_excmg = 8000
     ex_ref, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, ref_opts)
@@ -192,7 +192,7 @@ def run_c3(ctx, args):
             d = np.abs(impedances(m, ex) - z64) / np.abs(z64)
             out[name]["impedance_rel_err_vs_fp64_1e-8"] = {"max": float(d.max()), "median": float(np.median(d))}
         print(f"{name} done {r['fine']:.3f} s", file=sys.stderr, flush=True)
-    print(json.dumps({"metric": "mixed vs FP64 on the topography model (C3): fine-grid solve s and impedance error",
+    print(json.dumps({"metric": "mixed vs FP64 on the 128x128x96 topography model: fine-grid solve s and impedance error",
                       "value": out["mixed"]["fine_s"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
                       "fp64": out["fp64"], "mixed": out["mixed"],
                       "speedup_mixed_vs_fp64": round(out["fp64"]["fine_s"] / out["mixed"]["fine_s"], 3),
@@ -201,15 +201,15 @@ def run_c3(ctx, args):
 
 
 def run_c5(ctx, args):
-    """C5: misfit gradient with respect to log sigma over all cells of the
+    """Adjoint workload: misfit gradient with respect to log sigma over all cells of the
     96x96x64 isotropic random-field model, 6 frequencies, 20x20 stations, data
     from the seed-6 model; central differences (step 1e-3 in log sigma) on
     seeded random earth cells under the station area."""
     from sfd_excmg_amd import sfdx, synth
 
     kw = dict(nx=48, ny=48, nz_earth=24, nstations=8) if args.small else {}
-    m = synth.make_c5(**kw)
-    d = synth.make_c5(seed=6, **kw)
+    m = synth.make_random_field_96(**kw)
+    d = synth.make_random_field_96(seed=6, **kw)
     opts = sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0,
                              norm=sfdx.NORM_INTERIOR, maxit_excmg=5000, maxit_plain=5000)
     exd, _ = ctx.fwd_solve(d.a, d.b, d.c, d.nair, d.sigma, d.freqs, opts)
@@ -243,7 +243,7 @@ def run_c5(ctx, args):
         print(f"cell {cell}: adjoint {grad[cell]:.6e} fd {fd:.6e}", file=sys.stderr, flush=True)
     t_fd = time.perf_counter() - t0
     scale = float(np.max(np.abs(grad)))
-    print(json.dumps({"metric": "adjoint misfit gradient (C5): wall time for d misfit / d log sigma over all cells",
+    print(json.dumps({"metric": "adjoint misfit gradient (96x96x64, 6 frequencies): wall time for d misfit / d log sigma over all cells",
                       "value": round(t_adj, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
                       "cells": [m.nx, m.ny, m.nz], "ncell": int(m.nx * m.ny * m.nz), "frequencies": len(m.freqs),
                       "stations": int(len(m.recv)), "misfit": phi, "status": int(st.status),
@@ -262,13 +262,13 @@ def main():
     ap.add_argument("--small", action="store_true", help="64x64x48 variant for quick checks")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true")
-    ap.add_argument("--c4", action="store_true", help="batched workload: 16 frequencies on the 128x128x96 model "
+    ap.add_argument("--batch", "--c4", dest="c4", action="store_true", help="batched workload: 16 frequencies on the 128x128x96 model "
                     "plus 8 isotropic models of uneven sizes, batched vs one stream")
     ap.add_argument("--nstreams", type=int, default=8)
-    ap.add_argument("--c5", action="store_true", help="adjoint gradient on the 96x96x64 random-field model with a "
+    ap.add_argument("--adjoint", "--c5", dest="c5", action="store_true", help="adjoint gradient on the 96x96x64 random-field model with a "
                     "central-difference check on --fd-cells random cells")
     ap.add_argument("--fd-cells", type=int, default=4)
-    ap.add_argument("--c3", action="store_true", help="128x128x96 anisotropic model with topography: mixed vs FP64, "
+    ap.add_argument("--topography", "--c3", dest="c3", action="store_true", help="128x128x96 anisotropic model with topography: mixed vs FP64, "
                     "per-impedance errors")
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel eagerly (no HIP graph replay); "
@@ -277,15 +277,15 @@ def main():
     global USE_GRAPH
     USE_GRAPH = 0 if args.no_graph else 1
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(frequencies/polarisations/models would shard across GPUs without communication).")
         sys.exit(2)
     from sfd_excmg_amd import sfdx, synth
 
     if args.small:
-        m = synth.make_c2(nx=64, nz_earth=40, nair=8, nstations=20)
+        m = synth.make_aniso_blocks_128(nx=64, nz_earth=40, nair=8, nstations=20)
     else:
-        m = synth.make_c2()
+        m = synth.make_aniso_blocks_128()
     ctx = sfdx.Context(0)
     nl, nc = m.nl, m.ncell
     b64, b32 = spmv_bytes(nl, nc, 16, 8), spmv_bytes(nl, nc, 8, 4)

@@ -7,7 +7,7 @@
   air), frequencies and stations (x, y, z).
 * ``write_fwd`` / ``read_fwd`` / ``write_dat`` / ``read_dat`` follow the text
   formats of ``file_opt.f90:8-95`` and ``input.py:6-118``.
-* ``make_c1`` ... ``make_c5`` build the five benchmark workloads with fixed
+* ``make_block_halfspace_32`` ... ``make_random_field_96`` build the five benchmark workloads with fixed
   seeds (numpy ``default_rng``); the air thicknesses mirror the deepest earth
   layers as ``file_opt.f90:52`` does.
 * ``euler_tensor`` restates the Euler-rotated conductivity tensor of
@@ -120,7 +120,7 @@ def set_air(sig, nx, ny, nair, sigma_air):
 
 
 # ----------------------------------------------------------------- configs
-def make_c1(seed: int = 1, sigma_air: float = SIGMA_AIR_DEFAULT) -> Model:
+def make_block_halfspace_32(seed: int = 1, sigma_air: float = SIGMA_AIR_DEFAULT) -> Model:
     """32^3 cells, 3 levels: 100 ohm-m half-space, one 10 ohm-m 8x8x6 block at a
     seeded position under the surface, 1 Hz, BiCGStab+SSOR to 1e-6, 10x10
     stations."""
@@ -138,7 +138,7 @@ def make_c1(seed: int = 1, sigma_air: float = SIGMA_AIR_DEFAULT) -> Model:
         for j in range(j0, j0 + 8):
             sig[k * nx * ny + j * nx + i0: k * nx * ny + j * nx + i0 + 8, :3] = 1.0 / 10.0
     set_air(sig, nx, ny, nair, sigma_air)
-    return Model("c1", a, b, c, nair, sig, np.array([1.0]), station_grid(10, 750.0), level=3, eps=1e-6,
+    return Model("block_halfspace_32", a, b, c, nair, sig, np.array([1.0]), station_grid(10, 750.0), level=3, eps=1e-6,
                  solver=1, corr_intv=200, meta={"block": (i0, j0, k0)})
 
 
@@ -160,7 +160,7 @@ def _anisotropic_blocks(rng, sig, nx, ny, nair, nz_earth, core_lo, core_n, nbloc
     return blocks
 
 
-def make_c2(seed: int = 2, sigma_air: float = SIGMA_AIR_DEFAULT, nx: int = 128, nz_earth: int = 80,
+def make_aniso_blocks_128(seed: int = 2, sigma_air: float = SIGMA_AIR_DEFAULT, nx: int = 128, nz_earth: int = 80,
             nair: int = 16, nstations: int = 40) -> Model:
     """128x128x96 cells (16 air + 80 earth), full anisotropic tensor: 16 seeded
     blocks with principal resistivities log-uniform in [1,1000] ohm-m and Euler
@@ -177,7 +177,7 @@ def make_c2(seed: int = 2, sigma_air: float = SIGMA_AIR_DEFAULT, nx: int = 128, 
     blocks = _anisotropic_blocks(rng, sig, nx, ny, nair, nz_earth, npad, ncore, 16)
     set_air(sig, nx, ny, nair, sigma_air)
     half = ncore * 100.0 / 2 - 2 * 100.0
-    return Model("c2", a, b, c, nair, sig, np.array([0.1]), station_grid(nstations, half), level=4, eps=1e-8,
+    return Model("aniso_blocks_128", a, b, c, nair, sig, np.array([0.1]), station_grid(nstations, half), level=4, eps=1e-8,
                  solver=5, corr_intv=50, meta={"blocks": blocks})
 
 
@@ -197,10 +197,10 @@ def topography_depth(rng, x, y, amplitude=500.0):
     return h.max() - h
 
 
-def make_c3(seed: int = 3, sigma_air: float = SIGMA_AIR_DEFAULT, **kw) -> Model:
-    """C2 plus undulating topography: earth cells whose centre lies above the
+def make_aniso_topography_128(seed: int = 3, sigma_air: float = SIGMA_AIR_DEFAULT, **kw) -> Model:
+    """The 128x128x96 anisotropic block model plus undulating topography: earth cells whose centre lies above the
     ground surface become air."""
-    m = make_c2(sigma_air=sigma_air, **kw)
+    m = make_aniso_blocks_128(sigma_air=sigma_air, **kw)
     rng = np.random.default_rng(seed)
     xc, yc = cell_centres(m.a), cell_centres(m.b)
     depth = topography_depth(rng, xc, yc)
@@ -214,7 +214,7 @@ def make_c3(seed: int = 3, sigma_air: float = SIGMA_AIR_DEFAULT, **kw) -> Model:
         mask = air.T.reshape(-1)                              # index j*nx + i
         sl[mask, :3] = sigma_air
         sl[mask, 3:] = 0
-    m.name = "c3"
+    m.name = "aniso_topography_128"
     m.meta["topography_max_depth"] = float(depth.max())
     return m
 
@@ -248,7 +248,7 @@ def make_iso_block_model(name, nx, ny, nz_earth, nair, seed, nblocks=6, freq=1.0
                  eps=1e-6, solver=1)
 
 
-C4_SHAPES = [(48, 48, 48), (64, 64, 64), (64, 64, 48), (80, 80, 80), (96, 96, 64), (56, 56, 56), (72, 72, 72), (96, 96, 96)]
+UNEVEN_ISO_SHAPES = [(48, 48, 48), (64, 64, 64), (64, 64, 48), (80, 80, 80), (96, 96, 64), (56, 56, 56), (72, 72, 72), (96, 96, 96)]
 
 
 def max_levels(nx, ny, nz, nair):
@@ -262,19 +262,19 @@ def max_levels(nx, ny, nz, nair):
         L += 1
 
 
-def make_c4_models(seed: int = 4, sigma_air: float = SIGMA_AIR_DEFAULT):
+def make_uneven_iso_models(seed: int = 4, sigma_air: float = SIGMA_AIR_DEFAULT):
     rng = np.random.default_rng(seed)
     out = []
-    for idx, (nx, ny, nz) in enumerate(C4_SHAPES):
+    for idx, (nx, ny, nz) in enumerate(UNEVEN_ISO_SHAPES):
         nair = 8
         lv = 4 if (nx % 16 == 0 and ny % 16 == 0 and nz % 16 == 0) else 3
-        m = make_iso_block_model(f"c4_{nx}x{ny}x{nz}", nx, ny, nz - nair, nair, int(rng.integers(1 << 30)),
+        m = make_iso_block_model(f"iso_{nx}x{ny}x{nz}", nx, ny, nz - nair, nair, int(rng.integers(1 << 30)),
                                  freq=1.0, level=lv, sigma_air=sigma_air)
         out.append(m)
     return out
 
 
-def c4_frequencies():
+def sounding_frequencies_16():
     return np.logspace(-3, 3, 16)
 
 
@@ -291,7 +291,7 @@ def gaussian_random_field(shape, corr_len, seed):
     return (f - f.mean()) / f.std()
 
 
-def make_c5(seed: int = 5, sigma_air: float = SIGMA_AIR_DEFAULT, nx=96, ny=96, nz_earth=56, nair=8,
+def make_random_field_96(seed: int = 5, sigma_air: float = SIGMA_AIR_DEFAULT, nx=96, ny=96, nz_earth=56, nair=8,
             nstations=20) -> Model:
     """96x96x64 isotropic model, log10 rho = Gaussian random field (mean 2, sd
     0.5, correlation length 8 cells), 6 frequencies 1e-2..1e2 Hz, 20x20
@@ -308,7 +308,7 @@ def make_c5(seed: int = 5, sigma_air: float = SIGMA_AIR_DEFAULT, nx=96, ny=96, n
     sig[nair * nx * ny:, :3] = (10.0 ** -logrho).reshape(-1)[:, None]
     set_air(sig, nx, ny, nair, sigma_air)
     half = ncore * 100.0 / 2 - 200.0
-    return Model("c5", a, b, c, nair, sig, np.logspace(-2, 2, 6), station_grid(nstations, half), level=4,
+    return Model("random_field_96", a, b, c, nair, sig, np.logspace(-2, 2, 6), station_grid(nstations, half), level=4,
                  eps=1e-8, solver=5)
 
 

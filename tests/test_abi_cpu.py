@@ -54,7 +54,7 @@ def test_mt1d_matches_oracle(aniso):
 
 @pytest.mark.parametrize("rd,out_type", [(1, 1), (2, 1), (2, 2)])
 def test_post_process_matches_oracle(rd, out_type):
-    m = synth.make_c1()
+    m = synth.make_block_halfspace_32()
     rng = np.random.default_rng(0)
     ex = rng.standard_normal((2 * m.nl, 2)) + 1j * rng.standard_normal((2 * m.nl, 2))
     freqs = np.array([1.0, 0.1])
@@ -67,7 +67,7 @@ def test_post_process_matches_oracle(rd, out_type):
 def test_direct_solve_matches_superlu():
     """PARDISO (complex symmetric) on the dual-volume scaled system against
     SuperLU on the reference's unscaled CSR; pinned by the interior residual."""
-    m = synth.make_c1()
+    m = synth.make_block_halfspace_32()
     a1, b1, c1, s1 = O.mesh_coarsen(m.a, m.b, m.c, m.sigma, 2)
     try:
         u = sfdx.direct_solve(a1, b1, c1, s1, 1.0)

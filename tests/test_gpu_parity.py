@@ -52,7 +52,7 @@ def test_operator_fp64_matches_oracle_csr(ctx, aniso):
 
 
 def test_operator_fp32_matches_fp64(ctx):
-    m = synth.make_c1()
+    m = synth.make_block_halfspace_32()
     x = _rand(m.nl, 2)
     y64 = ctx.apply_operator(m.a, m.b, m.c, m.sigma, 1.0, x)
     y32 = ctx.apply_operator(m.a, m.b, m.c, m.sigma, 1.0, x, precision=sfdx.PREC_MIXED)
@@ -184,7 +184,7 @@ def test_fwd_solve_multigrid_matches_oracle(ctx, use_excmg, mg_levels):
 
 
 def test_halfspace_on_gpu(ctx):
-    m = synth.make_c1()
+    m = synth.make_block_halfspace_32()
     m.sigma[m.nair * m.nx * m.ny:, :3] = 0.01
     opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=200, norm=sfdx.NORM_INTERIOR)
     ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)

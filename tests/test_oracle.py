@@ -429,7 +429,7 @@ def test_arho_interp_exact_on_quadratics_rd1():
 
 @pytest.mark.slow
 def test_halfspace_apparent_resistivity_and_phase():
-    m = synth.make_c1()
+    m = synth.make_block_halfspace_32()
     m.sigma[m.nair * m.nx * m.ny:, :3] = 0.01
     opts = O.Options(norm="interior")
     ex, _ = O.fwd_solver(m.a, m.b, m.c, m.nair, 3, 1e-10, m.sigma, m.freqs, opts)
