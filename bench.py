@@ -145,7 +145,7 @@ def run_c4(ctx, args):
         small = small[:3]
         big.freqs = big.freqs[::4]
     models = [big] + small
-    opts = solve_opts(big, sfdx.PREC_MIXED, 1e-8)
+    opts = solve_opts(big, sfdx.PREC_FP64, 1e-8)
     nrhs = 2 * sum(len(m.freqs) for m in models)
     t0 = time.perf_counter()
     exb, stb = ctx.solve_batch(models, opts, nstreams=args.nstreams)
@@ -162,7 +162,7 @@ def run_c4(ctx, args):
                       "speedup_vs_one_stream": round(t_seq / t_batch, 3),
                       "status": [int(s.status) for s in stb], "fine_iters": [int(s.fine_iters_total) for s in stb],
                       "max_rel_field_diff_batch_vs_one_stream": diff,
-                      "dtype": "complex128 Krylov/operator + complex64 preconditioner"}))
+                      "dtype": "complex128"}))
 
 
 def run_c3(ctx, args):
