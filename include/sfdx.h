@@ -163,6 +163,17 @@ typedef struct {
 int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_opts *o, int nstreams,
                      sfdx_stats *st);
 
+/* Forward solve and station responses without moving the fields: after each
+ * frequency's fine solve the surface E and H (cell centres of node plane nair)
+ * are evaluated on the device and only they are copied to the host, where the
+ * station interpolation and Z / rho_a / phi / tipper of post_process follow
+ * (same results, out_type and rd as sfdx_post_process).  Replaces the
+ * fwd_solver + post_process pair of the driver (fdfe_main.f90:1565-1594,
+ * 549-716) when the fields themselves are not needed. */
+int sfdx_fwd_response(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf, const double *freq,
+                      const sfdx_opts *o, int nrec, const double *recv, int out_type, int rd, double *results,
+                      double *tipper, sfdx_stats *st);
+
 /* Data misfit of the fields ex(2*nl, nf) (host only, FP64):
  *   misfit = 1/2 sum_{f, station, xy|yx} (log10 rho_a - log10 rho_obs)^2 + (phi - phi_obs)^2
  * with phi in radians and rho_a, phi computed as post_process does (rd as

@@ -894,8 +894,10 @@ HostLevel host_level(const sfdx_grid *g, const double *sigma)
     return L;
 }
 
+// ex (optional): fields 2*nl*nf; surf (optional): surface fields of each
+// frequency evaluated on the device, nf*2*5*nx*ny (see post_process_surface)
 int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf, const double *freq,
-                   const sfdx_opts *o, double *ex, sfdx_stats *st)
+                   const sfdx_opts *o, double *ex, sfdx_stats *st, zc *surf = nullptr)
 {
     Timer total;
     sfdx_stats S;
@@ -1053,9 +1055,22 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 u[i][pol] = x;
             }
         }
-        for (int pol = 0; pol < 2; pol++)
-            SFDX_CHECK(hipMemcpyAsync(ex + ((size_t)fi * 2 * NLf + (size_t)pol * NLf) * 2, u[Lv][pol],
-                                      NLf * sizeof(zd), hipMemcpyDeviceToHost, ctx->st));
+        if (ex)
+            for (int pol = 0; pol < 2; pol++)
+                SFDX_CHECK(hipMemcpyAsync(ex + ((size_t)fi * 2 * NLf + (size_t)pol * NLf) * 2, u[Lv][pol],
+                                          NLf * sizeof(zd), hipMemcpyDeviceToHost, ctx->st));
+        if (surf) {
+            // only the surface fields leave the device (10 nx*ny values, not 2 nl)
+            const Dims df = make_dims(fine.nx, fine.ny, fine.nz);
+            const size_t ns = (size_t)fine.nx * fine.ny;
+            zd *sd = pool_vec<zd>(ctx, "surf", 10 * ns);
+            const double om = 2 * 3.141592653589793 * ff * kMu0;
+            for (int pol = 0; pol < 2; pol++)
+                launch_surface_fields(df, fine.nair, D64[Lv].a64, D64[Lv].b64, D64[Lv].c64, om, u[Lv][pol],
+                                      sd + (size_t)pol * 5 * ns, ctx->st);
+            SFDX_CHECK(hipMemcpyAsync(surf + (size_t)fi * 10 * ns, sd, 10 * ns * sizeof(zd), hipMemcpyDeviceToHost,
+                                      ctx->st));
+        }
         SFDX_CHECK(hipStreamSynchronize(ctx->st));
     }
     S.total_seconds = total.s();
@@ -1323,6 +1338,28 @@ int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_
             }
         }
         return worst;
+    });
+}
+
+int sfdx_fwd_response(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf, const double *freq,
+                      const sfdx_opts *o, int nrec, const double *recv, int out_type, int rd, double *results,
+                      double *tipper, sfdx_stats *st)
+{
+    if (!ctx || !g || !sigma || !freq || !recv || !results || nf <= 0 || nrec <= 0) return SFDX_ERR_ARG;
+    sfdx_opts def;
+    if (!o) { sfdx_default_opts(&def); o = &def; }
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        std::vector<zc> surf((size_t)nf * 10 * g->nx * g->ny);
+        const int rc = fwd_solve_impl(ctx, g, sigma, nf, freq, o, nullptr, st, surf.data());
+        if (rc > 0) return rc;
+        const PostGrid pg{g->nx, g->ny, g->nz, g->nair, g->a, g->b, g->c};
+        const int bad = post_process_surface(pg, nf, freq, nrec, recv, surf.data(), out_type, rd, results, tipper);
+        if (bad) {
+            g_err = "station " + std::to_string(bad) + " lies outside the interpolation area";
+            return (int)SFDX_ERR_ARG;
+        }
+        return rc;
     });
 }
 

@@ -413,6 +413,39 @@ static int station_interp(const PostGrid &g, const Surface &S, int pol, int rd, 
     return 0;
 }
 
+// Station responses of one frequency from its surface fields (both polarisations).
+static int respond(const PostGrid &g, double freq, const Surface &S, int nrec, const double *recv, int out_type, int rd,
+                   double *results, double *tipper)
+{
+    std::vector<zc> s1, s2;
+    int st = station_interp(g, S, 0, rd, recv, nrec, s1);
+    if (!st) st = station_interp(g, S, 1, rd, recv, nrec, s2);
+    if (st) return st;
+    const double om = 2 * kPi * freq * kMu0;
+    for (int r = 0; r < nrec; r++) {
+        const zc *A1 = &s1[(size_t)r * 5], *A2 = &s2[(size_t)r * 5];
+        const zc t = A1[2] * A2[3] - A2[2] * A1[3];
+        const zc zxy = (A2[0] * A1[2] - A1[0] * A2[2]) / t;
+        const zc zyx = (A1[1] * A2[3] - A2[1] * A1[3]) / t;
+        double *o = results + (size_t)r * 4;
+        if (out_type == 2) {
+            o[0] = std::abs(zxy * zxy * zc(0, 1)) / om;
+            o[1] = std::abs(zyx * zyx * zc(0, 1)) / om;
+            o[2] = -std::atan(zxy.imag() / zxy.real()) * 180.0 / kPi;
+            o[3] = -std::atan(zyx.imag() / zyx.real()) * 180.0 / kPi;
+        } else {
+            o[0] = zxy.real(); o[1] = zxy.imag(); o[2] = zyx.real(); o[3] = zyx.imag();
+        }
+        if (tipper) {
+            const zc tx = (A2[3] * A1[4] - A1[3] * A2[4]) / t;
+            const zc ty = (A1[2] * A2[4] - A2[2] * A1[4]) / t;
+            double *tp = tipper + (size_t)r * 4;
+            tp[0] = tx.real(); tp[1] = tx.imag(); tp[2] = ty.real(); tp[3] = ty.imag();
+        }
+    }
+    return 0;
+}
+
 int post_process(const PostGrid &g, int nf, const double *freq, int nrec, const double *recv, const zc *ex,
                  int out_type, int rd, double *results, double *tipper)
 {
@@ -423,32 +456,27 @@ int post_process(const PostGrid &g, int nf, const double *freq, int nrec, const 
         Surface S;
         surface_fields(g, freq[k], x1, 0, S);
         surface_fields(g, freq[k], x2, 1, S);
-        std::vector<zc> s1, s2;
-        int st = station_interp(g, S, 0, rd, recv, nrec, s1);
-        if (!st) st = station_interp(g, S, 1, rd, recv, nrec, s2);
+        const int st = respond(g, freq[k], S, nrec, recv, out_type, rd, results + (size_t)k * nrec * 4,
+                               tipper ? tipper + (size_t)k * nrec * 4 : nullptr);
         if (st) return st;
-        const double om = 2 * kPi * freq[k] * kMu0;
-        for (int r = 0; r < nrec; r++) {
-            const zc *A1 = &s1[(size_t)r * 5], *A2 = &s2[(size_t)r * 5];
-            const zc t = A1[2] * A2[3] - A2[2] * A1[3];
-            const zc zxy = (A2[0] * A1[2] - A1[0] * A2[2]) / t;
-            const zc zyx = (A1[1] * A2[3] - A2[1] * A1[3]) / t;
-            double *o = results + ((size_t)k * nrec + r) * 4;
-            if (out_type == 2) {
-                o[0] = std::abs(zxy * zxy * zc(0, 1)) / om;
-                o[1] = std::abs(zyx * zyx * zc(0, 1)) / om;
-                o[2] = -std::atan(zxy.imag() / zxy.real()) * 180.0 / kPi;
-                o[3] = -std::atan(zyx.imag() / zyx.real()) * 180.0 / kPi;
-            } else {
-                o[0] = zxy.real(); o[1] = zxy.imag(); o[2] = zyx.real(); o[3] = zyx.imag();
+    }
+    return 0;
+}
+
+int post_process_surface(const PostGrid &g, int nf, const double *freq, int nrec, const double *recv, const zc *surf,
+                         int out_type, int rd, double *results, double *tipper)
+{
+    const size_t ns = (size_t)g.nx * g.ny;
+    for (int k = 0; k < nf; k++) {
+        Surface S;
+        for (int pol = 0; pol < 2; pol++)
+            for (int q = 0; q < 5; q++) {
+                const zc *src = surf + (((size_t)k * 2 + pol) * 5 + q) * ns;
+                S.f[q][pol].assign(src, src + ns);
             }
-            if (tipper) {
-                const zc tx = (A2[3] * A1[4] - A1[3] * A2[4]) / t;
-                const zc ty = (A1[2] * A2[4] - A2[2] * A1[4]) / t;
-                double *tp = tipper + ((size_t)k * nrec + r) * 4;
-                tp[0] = tx.real(); tp[1] = tx.imag(); tp[2] = ty.real(); tp[3] = ty.imag();
-            }
-        }
+        const int st = respond(g, freq[k], S, nrec, recv, out_type, rd, results + (size_t)k * nrec * 4,
+                               tipper ? tipper + (size_t)k * nrec * 4 : nullptr);
+        if (st) return st;
     }
     return 0;
 }

@@ -47,6 +47,10 @@ void coarse_system(const HostLevel &L, double freq, const std::vector<std::compl
 bool direct_solve(const CoarseSystem &S, std::vector<std::complex<double>> &sol, std::string &err);
 int post_process(const PostGrid &g, int nf, const double *freq, int nrec, const double *recv,
                  const std::complex<double> *ex, int out_type, int rd, double *results, double *tipper);
+// post_process from surface fields already evaluated (on the device):
+// surf[((f*2 + pol)*5 + q)*nx*ny + i + nx*j], q = ex, ey, hx, hy, hz
+int post_process_surface(const PostGrid &g, int nf, const double *freq, int nrec, const double *recv,
+                         const std::complex<double> *surf, int out_type, int rd, double *results, double *tipper);
 // Misfit 1/2 sum over frequencies, stations and the xy/yx modes of
 // (log10 rho_a - log10 rho_obs)^2 + (phi - phi_obs)^2 (phase in radians), obs
 // given as post_process out_type 2 rows (rho_xy, rho_yx, phi_xy, phi_yx in

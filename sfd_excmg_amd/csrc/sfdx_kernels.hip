@@ -568,6 +568,42 @@ __global__ __launch_bounds__(TPB) void k_zero_boundary(Dims d, zd *__restrict__ 
     if (!edge_interior(d, S.o, i, j, S.k)) v[S.base + p] = zd(0);
 }
 
+// Surface fields at the cell centres of node plane nair (post_process,
+// src/fdfe_main.f90:549-640): E averaged from the two bounding edges, H = curl
+// E / (i omega mu) with a one-sided z difference over c(nair+1).  One thread per
+// surface cell; out[q*nx*ny + i + nx*j] for q = ex, ey, hx, hy, hz.
+__global__ __launch_bounds__(TPB) void k_surface_fields(Dims d, int nair, const double *__restrict__ a,
+                                                        const double *__restrict__ b, const double *__restrict__ c,
+                                                        double om, const zd *__restrict__ x, zd *__restrict__ out)
+{
+    const int p = blockIdx.x * TPB + threadIdx.x;
+    const int nx = d.nx, ny = d.ny, m = nair;
+    if (p >= nx * ny) return;
+    const int j = p / nx, i = p - j * nx;
+    const int64_t n1 = d.XN, n2 = d.YN;
+    const int64_t idx = (int64_t)m * nx * (ny + 1) + (int64_t)j * nx + i;
+    const int64_t idx2 = n1 + (int64_t)m * ny * (nx + 1) + (int64_t)j * (nx + 1) + i;
+    const int64_t idx3 = n1 + n2 + (int64_t)m * (nx + 1) * (ny + 1) + (int64_t)j * (nx + 1) + i;
+    const int64_t P = (int64_t)nx * (ny + 1), Q = (int64_t)ny * (nx + 1);
+    const double cz = c[m], ai = a[i], bj = b[j];
+    const zd ex = (x[idx] + x[idx + nx]) * 0.5;
+    const zd ey = (x[idx2] + x[idx2 + 1]) * 0.5;
+    const zd exz = ((x[idx + P] + x[idx + P + nx]) * 0.5 - ex) * (1.0 / cz);
+    const zd ezx = ((x[idx3 + 1] + x[idx3 + nx + 2]) * 0.5 - (x[idx3 + nx + 1] + x[idx3]) * 0.5) * (1.0 / ai);
+    const zd eyz = ((x[idx2 + Q] + x[idx2 + Q + 1]) * 0.5 - ey) * (1.0 / cz);
+    const zd ezy = ((x[idx3 + nx + 1] + x[idx3 + nx + 2]) * 0.5 - (x[idx3] + x[idx3 + 1]) * 0.5) * (1.0 / bj);
+    const zd exy = (x[idx + nx] - x[idx]) * (1.0 / bj);
+    const zd eyx = (x[idx2 + 1] - x[idx2]) * (1.0 / ai);
+    // 1/(i om) = -i/om
+    auto div_kap = [&](zd v) { return zd(v.im / om, -v.re / om); };
+    const int64_t s = p, ns = (int64_t)nx * ny;
+    out[s] = ex;
+    out[ns + s] = ey;
+    out[2 * ns + s] = div_kap(ezy - eyz);
+    out[3 * ns + s] = div_kap(exz - ezx);
+    out[4 * ns + s] = div_kap(exy - eyx);
+}
+
 // generic dots over all edges: mode 0: u^T v (unconjugated); 1: conj(u)^T v
 template <typename R, int MODE>
 __global__ __launch_bounds__(TPB) void k_dot(int64_t n, const cx<R> *__restrict__ u, const cx<R> *__restrict__ v,
@@ -1525,6 +1561,12 @@ void launch_cell_grad(const DevGrid<double> &g, double wmu, const zd *e, const z
     const unsigned gx = (unsigned)((plane + TPB - 1) / TPB);
     hipLaunchKernelGGL(k_cell_grad, dim3(gx, pad_y(gx, (unsigned)g.d.nz), 1), dim3(TPB), 0, st, g.d, g.a, g.b, g.c,
                        g.sig, wmu, e, lam, grad);
+}
+
+void launch_surface_fields(const Dims &d, int nair, const double *a, const double *b, const double *c, double om,
+                           const zd *x, zd *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_surface_fields, grid1((int64_t)d.nx * d.ny), dim3(TPB), 0, st, d, nair, a, b, c, om, x, out);
 }
 
 void launch_zero_boundary(const Dims &d, zd *v, hipStream_t st)

@@ -78,7 +78,7 @@ class Job(ctypes.Structure):
 EXPORTS = ["sfdx_create", "sfdx_destroy", "sfdx_last_error", "sfdx_default_opts", "sfdx_fwd_solve",
            "sfdx_post_process", "sfdx_mt1d_anl", "sfdx_apply_operator", "sfdx_ssor_apply", "sfdx_div_correct",
            "sfdx_extint_v", "sfdx_bench_kernels", "sfdx_direct_solve", "sfdx_solve_batch", "sfdx_misfit",
-           "sfdx_adjoint"]
+           "sfdx_adjoint", "sfdx_fwd_response"]
 
 _lib = None
 
@@ -106,6 +106,8 @@ def load(path: str = LIB_PATH):
     L.sfdx_bench_kernels.argtypes = [P, ctypes.POINTER(Grid), P, D, I, I, P]
     L.sfdx_direct_solve.argtypes = [ctypes.POINTER(Grid), P, D, P]
     L.sfdx_solve_batch.argtypes = [P, I, ctypes.POINTER(Job), ctypes.POINTER(Opts), I, ctypes.POINTER(Stats)]
+    L.sfdx_fwd_response.argtypes = [P, ctypes.POINTER(Grid), P, I, P, ctypes.POINTER(Opts), I, P, I, I, P, P,
+                                    ctypes.POINTER(Stats)]
     L.sfdx_misfit.argtypes = [ctypes.POINTER(Grid), I, P, I, P, I, P, P, ctypes.POINTER(D), P]
     L.sfdx_adjoint.argtypes = [P, ctypes.POINTER(Grid), P, I, P, I, P, I, P, ctypes.POINTER(Opts), ctypes.POINTER(D), P,
                                P, ctypes.POINTER(Stats)]
@@ -214,6 +216,24 @@ class Context:
         o = opts if opts is not None else default_opts()
         _check(self.lib.sfdx_solve_batch(self.h, n, jobs, ctypes.byref(o), int(nstreams), st))
         return [ex.T.copy() for ex in outs], list(st)
+
+    def fwd_response(self, a, b, c, nair, sigma, freqs, recv, opts: Opts | None = None, out_type: int = 1,
+                     rd: int = 2):
+        """Forward solve + station responses with only the surface fields
+        leaving the device (sfdx_fwd_response); returns (results, tipper, stats)
+        shaped like post_process."""
+        G = _GridHolder(a, b, c, nair)
+        sf = sigma_flat(sigma)
+        fr = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
+        rv = np.ascontiguousarray(recv, np.float64)
+        res = np.zeros((len(fr) * len(rv), 4))
+        tip = np.zeros((len(fr) * len(rv), 4))
+        st = Stats()
+        o = opts if opts is not None else default_opts()
+        _check(self.lib.sfdx_fwd_response(self.h, ctypes.byref(G.g), _ptr(sf), len(fr), _ptr(fr), ctypes.byref(o),
+                                          len(rv), _ptr(rv), int(out_type), int(rd), _ptr(res), _ptr(tip),
+                                          ctypes.byref(st)))
+        return res, tip, st
 
     def adjoint(self, a, b, c, nair, sigma, freqs, recv, obs, opts: Opts | None = None, rd: int = 2,
                 return_fields: bool = False):

@@ -315,3 +315,18 @@ def test_autograd_function_returns_adjoint_gradient(ctx):
     phi2, grad, _ = ctx.adjoint(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, obs, opts)
     assert float(phi.detach()) == pytest.approx(phi2, rel=1e-12)
     assert np.allclose(ls.grad.numpy(), grad, rtol=0, atol=1e-12 * np.abs(grad).max())
+
+
+@pytest.mark.parametrize("out_type,rd", [(1, 2), (2, 1)])
+def test_fwd_response_matches_host_post_process(ctx, out_type, rd):
+    """Surface fields evaluated on the device (only they leave it) give the same
+    station responses as the host post-process of the full fields."""
+    m = small_excmg_model(True)
+    m.freqs = np.array([1.0, 4.0])
+    opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR)
+    res, tip, st = ctx.fwd_response(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, opts, out_type=out_type, rd=rd)
+    assert st.status == 0
+    ex, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
+    ref, reft = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, out_type=out_type, rd=rd)
+    assert np.max(np.abs(res - ref) / np.abs(ref)) < 1e-10
+    assert np.max(np.abs(tip - reft)) < 1e-10 * np.abs(reft).max()
