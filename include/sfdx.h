@@ -99,6 +99,9 @@ typedef struct {
     int nodal_smoother; /* nodal part of the hybrid preconditioner: 0 8-colour SSOR on
                            P = G^T M G, 1 (default) two damped Jacobi steps (one pass
                            over P instead of 16 dependent colour sweeps) */
+    int coarse_solver;  /* EXCMG levels 1-2: 0 host direct solve (MKL PARDISO, the
+                           reference's u1u2), 1 Krylov solves on the device from a
+                           zero guess to eps*10^(i-L) */
 } sfdx_opts;
 
 typedef struct {

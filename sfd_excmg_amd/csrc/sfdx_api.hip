@@ -921,7 +921,10 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     for (int i = Lv - 1; i >= 1; i--) lev[i] = coarsen(fine, Lv - i);
     std::vector<LevelDev<double>> D64(Lv + 1);
     std::vector<LevelDev<float>> D32(Lv + 1);
-    const int first_dev = o->use_excmg ? 3 : Lv;  // levels solved by the Krylov method
+    // levels solved by the Krylov method on the device: with coarse_solver = 1
+    // also the two coarsest EXCMG levels (instead of the host direct solves)
+    const bool coarse_dev = o->use_excmg && o->coarse_solver == 1;
+    const int first_dev = o->use_excmg ? (coarse_dev ? 1 : 3) : Lv;
     const int first_up = use_mg ? 1 : first_dev;  // levels resident on the device
     for (int i = first_up; i <= Lv; i++) {
         D64[i] = upload_level<double>(ctx, lev[i], "L" + std::to_string(i));
@@ -989,7 +992,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
             const Dims d = make_dims(lev[i].nx, lev[i].ny, lev[i].nz);
             return pool_vec<zd>(ctx, "u" + std::to_string(i) + "_" + std::to_string(pol), d.NL);
         };
-        if (o->use_excmg) {
+        if (o->use_excmg && !coarse_dev) {
             // levels 1 and 2: direct solves (u1u2, src/fdfe_main.f90:1228-1256)
             for (int i = 1; i <= 2; i++) {
                 const std::vector<zc> edc = background_field(lev[i], ff);
@@ -1012,7 +1015,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 SFDX_CHECK(hipStreamSynchronize(ctx->st));
             }
         }
-        double tol = o->eps * std::pow(10.0, 2 - Lv);  // after u1u2 (:1312, :1323)
+        double tol = o->eps * std::pow(10.0, first_dev - 1 - Lv);  // (:1312, :1323)
         for (int i = first_dev; i <= Lv; i++) {
             tol *= 10;                                   // level tolerance eps*10^(i-L) (:1331)
             if (!o->use_excmg) tol = o->eps;
@@ -1023,7 +1026,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
             const std::string tag = "L" + std::to_string(i);
             for (int pol = 0; pol < 2; pol++) {
                 zd *x = ubuf(i, pol);
-                if (o->use_excmg) extint_device(ctx, lev[i], lev[i - 1], u[i - 2][pol], u[i - 1][pol], x, tag);
+                if (o->use_excmg && i >= 3) extint_device(ctx, lev[i], lev[i - 1], u[i - 2][pol], u[i - 1][pol], x, tag);
                 else SFDX_CHECK(hipMemsetAsync(x, 0, d.NL * sizeof(zd), ctx->st));
                 const int maxit = o->use_excmg ? o->maxit_excmg : o->maxit_plain;
                 SFDX_CHECK(hipStreamSynchronize(ctx->st));
@@ -1206,6 +1209,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->restart_drop = 1e-6;
     o->use_graph = 1;
     o->nodal_smoother = 1;
+    o->coarse_solver = 0;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)

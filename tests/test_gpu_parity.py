@@ -183,6 +183,22 @@ def test_fwd_solve_multigrid_matches_oracle(ctx, use_excmg, mg_levels):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
+@pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
+def test_device_coarse_levels_match_direct_solves(ctx, precision):
+    """EXCMG levels 1-2 solved by the device Krylov method (coarse_solver 1)
+    instead of the host direct solves: same impedances to 1e-6, and the two
+    coarse levels report Krylov iterations."""
+    m = small_excmg_model(True)
+    kw = dict(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR,
+              precision=precision, sigma_floor32=0.0)
+    exd, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(**kw))
+    exk, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(coarse_solver=1, **kw))
+    assert st.status == 0
+    assert min(st.iters[0][0], st.iters[0][1], st.iters[1][0], st.iters[1][1]) > 0
+    for p, q in zip(_impedances(m, exk), _impedances(m, exd)):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+
+
 def test_halfspace_on_gpu(ctx):
     m = synth.make_block_halfspace_32()
     m.sigma[m.nair * m.nx * m.ny:, :3] = 0.01
