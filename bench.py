@@ -300,7 +300,12 @@ def main():
         return
     if args.kernels_only:
         ms64, ms32 = kernel_times(ctx, m, reps=20)
-        print(json.dumps({"spmv_ms": {"fp64": ms64[0], "fp32": ms32[0]}, "copy_ms": {"fp64": ms64[4], "fp32": ms32[4]},
+        print(json.dumps({"spmv_ms": {"fp64": ms64[0], "fp32": ms32[0]},
+                          "spmv_gather_ms": {"fp64": ms64[6], "fp32": ms32[6]},
+                          "spmv_march_ms_by_tile": {v: [ms64[7 + u], ms32[7 + u]] for u, v in
+                                                    enumerate(("32x4pf", "32x4", "32x8pf", "32x8"))},
+                          "spmv_march_32x4pf_kc4_ms": [ms64[11], ms32[11]],
+                          "copy_ms": {"fp64": ms64[4], "fp32": ms32[4]},
                           "copy_bytes": {"fp64": 2 * nl * 16, "fp32": 2 * nl * 8},
                           "spmv_bytes": {"fp64": b64, "fp32": b32}}))
         return

@@ -444,15 +444,15 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             launch_spmv<R>(D.g, wmu, x, r, (R)-1, 0, nullptr, ctx->rb, ctx->st);
             if (f) launch_axpby<R>(d.NL, cx<R>(1), f, cx<R>(1), r, ctx->st);
             apply_prec<R>(ctx, D, tag, wmu, r, z, ko);
-            launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
+            const int64_t nbs = launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
             // The direction p is kept one step behind: the update kernel (mode 6)
             // forms p = z + beta p itself, so here only Ap is advanced.
             if (state == 0) {
-                reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_RHO, ctx->st);  // also beta = 0
+                reduce_final(ctx->rb, nbs, 2, SL_TMP, OP_COCR_RHO, ctx->st);  // also beta = 0
                 launch_copy<R>(d.NL, z, pv, ctx->st);
                 launch_copy<R>(d.NL, Az, Ap, ctx->st);
             } else {
-                reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_BETA, ctx->st);
+                reduce_final(ctx->rb, nbs, 2, SL_TMP, OP_COCR_BETA, ctx->st);
                 launch_vec<R>(7, D.g, ctx->rb, nullptr, nullptr, Ap, Az, nullptr, nullptr, ctx->st);
             }
             launch_vec<R>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
@@ -525,8 +525,8 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
         };
         auto part2 = [&]() {
-            launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
-            reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_BETA, ctx->st);
+            const int64_t nbs = launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
+            reduce_final(ctx->rb, nbs, 2, SL_TMP, OP_COCR_BETA, ctx->st);
             launch_vec<R>(7, D.g, ctx->rb, nullptr, nullptr, Ap, Az, nullptr, nullptr, ctx->st);
         };
         since_best++;
@@ -624,12 +624,12 @@ SolveInfo bicgstab(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu,
         reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_BICG_RHO, ctx->st);
         launch_vec<R>(4, D.g, ctx->rb, pv, r, v, nullptr, nullptr, nullptr, ctx->st);
         apply_prec<R>(ctx, D, tag, wmu, pv, y, ko);
-        launch_spmv<R>(D.g, wmu, y, v, (R)1, 2, rh, ctx->rb, ctx->st);
-        reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_BICG_ALPHA, ctx->st);
+        int64_t nbs = launch_spmv<R>(D.g, wmu, y, v, (R)1, 2, rh, ctx->rb, ctx->st);
+        reduce_final(ctx->rb, nbs, 2, SL_TMP, OP_BICG_ALPHA, ctx->st);
         launch_vec<R>(1, D.g, ctx->rb, s, r, v, nullptr, nullptr, nullptr, ctx->st);
         apply_prec<R>(ctx, D, tag, wmu, s, z, ko);
-        launch_spmv<R>(D.g, wmu, z, t, (R)1, 3, s, ctx->rb, ctx->st);
-        reduce_final(ctx->rb, nbe, 3, SL_TMP, OP_BICG_OMEGA, ctx->st);
+        nbs = launch_spmv<R>(D.g, wmu, z, t, (R)1, 3, s, ctx->rb, ctx->st);
+        reduce_final(ctx->rb, nbs, 3, SL_TMP, OP_BICG_OMEGA, ctx->st);
         launch_vec<R>(2, D.g, ctx->rb, x, y, z, r, s, t, ctx->st);
         reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
         read_slots(ctx);
@@ -697,8 +697,8 @@ SolveInfo gpbicg(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, c
             continue;
         }
         launch_gpbicg<R>(0, D.g, ctx->rb, v, ctx->st);                         // w, p
-        launch_spmv<R>(D.g, wmu, v.p, v.ap, (R)1, 2, v.r0, ctx->rb, ctx->st);  // ap, (r0,ap)
-        reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_BICG_ALPHA, ctx->st);         // alpha = rho/(r0,ap)
+        const int64_t nbs = launch_spmv<R>(D.g, wmu, v.p, v.ap, (R)1, 2, v.r0, ctx->rb, ctx->st);  // ap, (r0,ap)
+        reduce_final(ctx->rb, nbs, 2, SL_TMP, OP_BICG_ALPHA, ctx->st);  // alpha = rho/(r0,ap)
         launch_gpbicg<R>(1, D.g, ctx->rb, v, ctx->st);                         // y, t
         apply_prec<R>(ctx, D, tag, wmu, v.ap, v.tmp, ko);                      // tmp = M ap
         launch_gpbicg<R>(2, D.g, ctx->rb, v, ctx->st);                         // q, tp
@@ -1601,6 +1601,17 @@ int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, d
             cx<R> *tn = pool_vec<cx<R>>(ctx, std::string("bench/tn") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
             cx<R> *ph = pool_vec<cx<R>>(ctx, std::string("bench/ph") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
             ms[5] = timed([&](int q) { launch_hybrid<R>(D.g, wmu, D.P, V(2 * q), V(2 * q + 1), tn, ph, (R)1, ctx->st); });
+            // the one-row-per-thread gather form of the SpMV, and the k-marching
+            // form in its four tile variants (default chunk) and the default
+            // variant over 4-plane chunks
+            ms[6] = timed([&](int q) { launch_spmv_gather<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, 0, nullptr, ctx->rb, ctx->st); });
+            for (int u = 0; u < 4; u++)
+                ms[7 + u] = timed([&](int q) {
+                    launch_spmv_march<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, 0, nullptr, ctx->rb, ctx->st, 0, u);
+                });
+            ms[11] = timed([&](int q) {
+                launch_spmv_march<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, 0, nullptr, ctx->rb, ctx->st, 4, 0);
+            });
         };
         if (precision == SFDX_PREC_FP64) run(0.0); else run(0.0f);
         ms[3] = ms[0] + ms[1] + ms[2];

@@ -22,6 +22,8 @@
 // the partials in a fixed order, so results are run-to-run deterministic.
 #include "sfdx_kernels.h"
 
+#include <stdexcept>
+
 namespace sfdx {
 
 HipError::HipError(hipError_t c, const char *expr, const char *file, int line) : code(c)
@@ -153,10 +155,10 @@ __device__ __forceinline__ bool colour_decode(const Dims &d, int par, int k, int
 }
 
 // ------------------------------------------------------------------ reductions
-template <int K>
+template <int K, int NTHR = TPB>
 __device__ __forceinline__ void block_reduce_store(double (&v)[K], double *partials)
 {
-    __shared__ double sm[K][TPB / 64];
+    __shared__ double sm[K][NTHR / 64];
 #pragma unroll
     for (int q = 0; q < K; q++) {
         double t = v[q];
@@ -176,7 +178,7 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[K], double *parti
 #pragma unroll
         for (int q = 0; q < K; q++) {
             double t = 0;
-            for (int w = 0; w < TPB / 64; w++) t += sm[q][w];
+            for (int w = 0; w < NTHR / 64; w++) t += sm[q][w];
             partials[q * nblk + blk] = t;  // component-major: the final pass reads contiguously
         }
     }
@@ -282,6 +284,284 @@ __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, c
         }
     }
     if (DOT >= 1 && DOT <= 3) block_reduce_store<4>(acc, partials);
+}
+
+// ------------------------------------------------------------------ K1 k-marching stencil
+// The operator of k_spmv, organised so that each value is fetched from memory
+// about once per block instead of once per row that reads it.  A block owns a
+// TX x TY tile of node columns (i,j) and marches up a chunk of node planes; at
+// step k every column of the tile produces its rows Ex(i,j,k), Ey(i,j,k) and
+// Ez(i,j,k).  Those rows read Ex/Ey at planes k and k+1, Ez at k-1 and k, and
+// the cell tensors of the cell planes k-1 and k.  These planes live in LDS
+// rings of two planes over the tile plus a one-column halo: at each step one
+// thread per footprint column loads Ex/Ey(k+1), Ez(k) and sigma(k) (9 vector
+// loads for 3 rows, against 23 per row when every row gathers its 13
+// neighbours and 12 cell values itself) into the ring slots that held the
+// planes no longer needed.  The rows are formed by the first TX*TY threads,
+// one per tile column, so every lane of those wavefronts has a row to form.
+namespace mt {
+template <int TY>
+struct Geo {
+    static constexpr int TX = 32;
+    static constexpr int PX = TX + 2, PY = TY + 2;  // edge footprint: i0-1..i0+TX, j0-1..j0+TY
+    static constexpr int SX = TX + 1, SY = TY + 1;  // cell footprint: i0-1..i0+TX-1, j0-1..j0+TY-1
+    static constexpr int NP = PX * PY, NS = SX * SY;
+    static constexpr int NT = (NP + 63) / 64 * 64;  // threads: footprint rounded to wavefronts
+    // LDS words of type R: Ex, Ey, Ez rings (2 complex planes each), sigma ring
+    // (2 planes x 6 components), widths a, 1/a over i0-1..i0+TX-1 and b, 1/b
+    // over j0-1..j0+TY-1
+    static constexpr int LDS_R = 3 * 2 * 2 * NP + 2 * 6 * NS + 2 * (TX + 1) + 2 * (TY + 1);
+};
+}  // namespace mt
+
+// K - i*wmu*M from the real-coefficient accumulators K and M, times alpha
+template <typename R>
+__device__ __forceinline__ cx<R> km_combine(cx<R> K, cx<R> M, R wmu, R alpha)
+{
+    return cx<R>(alpha * (K.re + wmu * M.im), alpha * (K.im - wmu * M.re));
+}
+
+template <typename R, int DOT>
+__device__ __forceinline__ void memit(cx<R> *__restrict__ y, const cx<R> *__restrict__ w, int64_t e, cx<R> out,
+                                      bool interior, cx<R> xe, double (&acc)[4])
+{
+    if (DOT == 4 && interior) out = w[e] + out;
+    y[e] = out;
+    if (DOT == 1) {
+        acc[0] += (double)xe.re * out.re - (double)xe.im * out.im;
+        acc[1] += (double)xe.re * out.im + (double)xe.im * out.re;
+    } else if ((DOT == 2 || DOT == 3) && interior) {
+        const cx<R> wv = w[e];
+        if (DOT == 2) {
+            acc[0] += (double)wv.re * out.re + (double)wv.im * out.im;
+            acc[1] += (double)wv.re * out.im - (double)wv.im * out.re;
+        } else {
+            acc[0] += (double)out.re * wv.re + (double)out.im * wv.im;
+            acc[1] += (double)out.re * wv.im - (double)out.im * wv.re;
+            acc[2] += (double)out.re * out.re + (double)out.im * out.im;
+        }
+    }
+}
+
+template <typename R, int DOT, int TY, bool PF>
+__global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
+    Dims d, const R *__restrict__ a, const R *__restrict__ b, const R *__restrict__ c, const R *__restrict__ ra,
+    const R *__restrict__ rb, const R *__restrict__ rc, const R *__restrict__ sig, R wmu,
+    const cx<R> *__restrict__ x, cx<R> *__restrict__ y, R alpha, const cx<R> *__restrict__ w,
+    double *__restrict__ partials, int ntx, int kc)
+{
+    using G = mt::Geo<TY>;
+    constexpr int TX = G::TX, PX = G::PX, SX = G::SX, SY = G::SY, NP = G::NP, NS = G::NS;
+    __shared__ __align__(16) R lds[G::LDS_R];
+    cx<R> *const EX = reinterpret_cast<cx<R> *>(lds);  // [2][NP], slot k&1 holds plane k
+    cx<R> *const EY = EX + 2 * NP;
+    cx<R> *const EZ = EY + 2 * NP;
+    R *const SG = lds + 12 * NP;                        // [2][6][NS], components xx yy zz xy xz yz
+    R *const TA = SG + 12 * NS;                         // a, then 1/a
+    R *const TB = TA + 2 * (TX + 1);                    // b, then 1/b
+    const Tile TL = tile_id();
+    const int tj = TL.x / ntx, ti = TL.x - tj * ntx;
+    const int i0 = ti * TX, j0 = tj * TY;
+    const int kb = TL.y * kc, ke = min(kb + kc, d.nz + 1);
+    const int t = threadIdx.x;
+    const int64_t XP = (int64_t)(d.ny + 1) * d.nx, YP = (int64_t)d.ny * (d.nx + 1);
+    const int64_t ZP = (int64_t)(d.ny + 1) * (d.nx + 1), CP = (int64_t)d.nx * d.ny, NC = d.NC;
+    // loader role: footprint column (li, lj) = node column (i0-1+li, j0-1+lj)
+    const int lj = t / PX, li = t - lj * PX;
+    const int gi = i0 - 1 + li, gj = j0 - 1 + lj;
+    const bool foot = t < NP;
+    const bool iin = gi >= 0 && gi <= d.nx, jin = gj >= 0 && gj <= d.ny;
+    const bool xv = foot && iin && jin && gi < d.nx;  // Ex(gi,gj,.) exists
+    const bool yv = foot && iin && jin && gj < d.ny;  // Ey
+    const bool zv = foot && iin && jin;               // Ez
+    const bool sv = foot && li < SX && lj < SY;       // inside the cell footprint
+    const bool cv = sv && gi >= 0 && gi < d.nx && gj >= 0 && gj < d.ny;
+    const int lf = lj * PX + li, ls = lj * SX + li;
+    const int64_t lex = (int64_t)gj * d.nx + gi, ley = d.XN + (int64_t)gj * (d.nx + 1) + gi;
+    const int64_t lez = d.XN + d.YN + (int64_t)gj * (d.nx + 1) + gi, lc = (int64_t)gj * d.nx + gi;
+    // values entering at step k: Ex/Ey(k+1), Ez(k), sigma(k)
+    cx<R> inX, inY, inZ;
+    R iS[6] = {0, 0, 0, 0, 0, 0};
+    auto fetch = [&](int k) {
+        inX = cx<R>(0); inY = cx<R>(0); inZ = cx<R>(0);
+#pragma unroll
+        for (int q = 0; q < 6; q++) iS[q] = 0;
+        if (k < d.nz) {
+            if (xv) inX = x[lex + (int64_t)(k + 1) * XP];
+            if (yv) inY = x[ley + (int64_t)(k + 1) * YP];
+            if (zv) inZ = x[lez + (int64_t)k * ZP];
+            if (cv) {
+                const int64_t q = lc + (int64_t)k * CP;
+#pragma unroll
+                for (int u = 0; u < 6; u++) iS[u] = sig[u * NC + q];
+            }
+        }
+    };
+    // compute role: tile column (i, j)
+    const int cy = t / TX, cxl = t - cy * TX;
+    const int i = i0 + cxl, j = j0 + cy;
+    const bool own = t < TX * TY && i < d.nx && j < d.ny;
+    const int fo = (cy + 1) * PX + cxl + 1, so = (cy + 1) * SX + cxl + 1;
+    const int64_t ex0 = (int64_t)j * d.nx + i, ey0 = d.XN + (int64_t)j * (d.nx + 1) + i;
+    const int64_t ez0 = d.XN + d.YN + (int64_t)j * (d.nx + 1) + i;
+    cx<R> exP, eyP;  // own Ex, Ey at k-1
+    if (kb < ke) {
+        // prologue: Ex/Ey(kb), Ez(kb-1), sigma(kb-1) into their ring slots, width tables
+        const bool lo = kb >= 1;
+        const int s0 = kb & 1, s1 = s0 ^ 1;
+        if (foot) {
+            EX[s0 * NP + lf] = xv ? x[lex + (int64_t)kb * XP] : cx<R>(0);
+            EY[s0 * NP + lf] = yv ? x[ley + (int64_t)kb * YP] : cx<R>(0);
+            EZ[s1 * NP + lf] = (zv && lo) ? x[lez + (int64_t)(kb - 1) * ZP] : cx<R>(0);
+        }
+        if (sv) {
+            const int64_t q = lc + (int64_t)(kb - 1) * CP;
+#pragma unroll
+            for (int u = 0; u < 6; u++) SG[(s1 * 6 + u) * NS + ls] = (cv && lo) ? sig[u * NC + q] : (R)0;
+        }
+        if (t <= TX) {
+            const int ii = i0 - 1 + t;
+            const bool ok = ii >= 0 && ii < d.nx;
+            TA[t] = ok ? a[ii] : (R)0;
+            TA[TX + 1 + t] = ok ? ra[ii] : (R)0;
+        }
+        if (t <= TY) {
+            const int jj = j0 - 1 + t;
+            const bool ok = jj >= 0 && jj < d.ny;
+            TB[t] = ok ? b[jj] : (R)0;
+            TB[TY + 1 + t] = ok ? rb[jj] : (R)0;
+        }
+        if (own && lo) {
+            exP = x[ex0 + (int64_t)(kb - 1) * XP];
+            eyP = x[ey0 + (int64_t)(kb - 1) * YP];
+        }
+        fetch(kb);
+    }
+    const R hE = (R)0.5, q4 = (R)0.25, q8 = (R)0.125;
+    double acc[4] = {0, 0, 0, 0};
+    for (int k = kb; k < ke; k++) {
+        const int sk = k & 1, sn = sk ^ 1;  // ring slots of plane k and of planes k+1 / k-1
+        __syncthreads();                    // the previous step's reads of the slots refilled here are done
+        if (foot) {
+            EX[sn * NP + lf] = inX;
+            EY[sn * NP + lf] = inY;
+            EZ[sk * NP + lf] = inZ;
+        }
+        if (sv) {
+#pragma unroll
+            for (int u = 0; u < 6; u++) SG[(sk * 6 + u) * NS + ls] = iS[u];
+        }
+        // with PF the next step's loads are in flight while this step's rows are formed
+        if (PF && k + 1 < ke) fetch(k + 1);
+        __syncthreads();
+        if (own) {
+            const cx<R> *EXK = EX + sk * NP, *EXN = EX + sn * NP, *EYK = EY + sk * NP, *EYN = EY + sn * NP;
+            const cx<R> *EZP = EZ + sn * NP, *EZK = EZ + sk * NP;
+            const R *SP = SG + sn * 6 * NS, *SK = SG + sk * 6 * NS;  // component u at u*NS
+            const bool zk = k >= 1 && k < d.nz;  // both cell planes k-1, k exist
+            const R cm = zk ? c[k - 1] : (R)0, cp = zk ? c[k] : (R)0;
+            const R ai = TA[cxl + 1], am = TA[cxl], rai = TA[TX + 2 + cxl], ram = TA[TX + 1 + cxl];
+            const R bj = TB[cy + 1], bm = TB[cy], rbj = TB[TY + 2 + cy], rbm = TB[TY + 1 + cy];
+            const cx<R> exK = EXK[fo], exN = EXN[fo], eyK = EYK[fo], eyN = EYN[fo];
+            const cx<R> ezP = EZP[fo], ezK = EZK[fo];
+            auto S2 = [&](int u, int o) { return cm * SP[u * NS + o] + cp * SK[u * NS + o]; };
+            // ---- Ex(i,j,k)
+            {
+                cx<R> out(0);
+                const bool in = j >= 1 && zk;
+                if (in) {
+                    const R rcm = rc[k - 1], rcp = rc[k];
+                    const R hb = hE * ai * (bm + bj), hc = hE * ai * (cm + cp);
+                    const R wkm = hb * rcm, wkp = hb * rcp, wjm = hc * rbm, wjp = hc * rbj;
+                    const R hy = hE * (cm + cp), hz = hE * (bm + bj);
+                    const R kmd = ai * (bm * S2(0, so - SX) + bj * S2(0, so)) * q4;
+                    const R my1 = ai * bm * S2(3, so - SX) * q8, my2 = ai * bj * S2(3, so) * q8;
+                    const R mz1 = ai * cm * (bm * SP[4 * NS + so - SX] + bj * SP[4 * NS + so]) * q8;
+                    const R mz2 = ai * cp * (bm * SK[4 * NS + so - SX] + bj * SK[4 * NS + so]) * q8;
+                    const cx<R> ey00 = EYK[fo - PX], ey10 = EYK[fo - PX + 1], ey11 = EYK[fo + 1];
+                    const cx<R> ez1 = EZP[fo + 1], ez2 = EZK[fo + 1];
+                    const cx<R> K = (wkm + wkp + wjm + wjp) * exK - wkm * exP - wkp * exN - wjm * EXK[fo - PX]
+                                    - wjp * EXK[fo + PX] + hy * (ey00 - ey10 - eyK + ey11)
+                                    + hz * (ezP - ez1 - ezK + ez2);
+                    const cx<R> M = kmd * exK + my1 * (ey00 + ey10) + my2 * (eyK + ey11) + mz1 * (ezP + ez1)
+                                    + mz2 * (ezK + ez2);
+                    out = km_combine(K, M, wmu, alpha);
+                }
+                memit<R, DOT>(y, w, ex0 + (int64_t)k * XP, out, in, exK, acc);
+            }
+            // rows one after another: the scheduler may not hoist the next row's
+            // LDS reads above this point, which bounds the registers live at once
+            __builtin_amdgcn_sched_barrier(0);
+            // ---- Ey(i,j,k)
+            {
+                cx<R> out(0);
+                const bool in = i >= 1 && zk;
+                if (in) {
+                    const R rcm = rc[k - 1], rcp = rc[k];
+                    const R ha = hE * bj * (am + ai), hc = hE * bj * (cm + cp);
+                    const R wkm = ha * rcm, wkp = ha * rcp, wim = hc * ram, wip = hc * rai;
+                    const R hx = hE * (cm + cp), hz = hE * (am + ai);
+                    const R kmd = bj * (am * S2(1, so - 1) + ai * S2(1, so)) * q4;
+                    const R mx1 = bj * am * S2(3, so - 1) * q8, mx2 = bj * ai * S2(3, so) * q8;
+                    const R mz1 = bj * cm * (am * SP[5 * NS + so - 1] + ai * SP[5 * NS + so]) * q8;
+                    const R mz2 = bj * cp * (am * SK[5 * NS + so - 1] + ai * SK[5 * NS + so]) * q8;
+                    const cx<R> x0 = EXK[fo - 1], x2 = EXK[fo + PX - 1], x3 = EXK[fo + PX];
+                    const cx<R> z1 = EZP[fo + PX], z3 = EZK[fo + PX];
+                    const cx<R> K = (wkm + wkp + wim + wip) * eyK - wkm * eyP - wkp * eyN - wim * EYK[fo - 1]
+                                    - wip * EYK[fo + 1] + hx * (x0 - exK - x2 + x3) + hz * (ezP - z1 - ezK + z3);
+                    const cx<R> M = kmd * eyK + mx1 * (x0 + x2) + mx2 * (exK + x3) + mz1 * (ezP + z1)
+                                    + mz2 * (ezK + z3);
+                    out = km_combine(K, M, wmu, alpha);
+                }
+                memit<R, DOT>(y, w, ey0 + (int64_t)k * YP, out, in, eyK, acc);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            // ---- Ez(i,j,k), between node planes k and k+1
+            if (k < d.nz) {
+                cx<R> out(0);
+                const bool in = i >= 1 && j >= 1;
+                if (in) {
+                    const R ck = c[k];
+                    const R ha = hE * ck * (am + ai), hb = hE * ck * (bm + bj);
+                    const R wjm = ha * rbm, wjp = ha * rbj, wim = hb * ram, wip = hb * rai;
+                    const R hx = hE * (bm + bj), hy = hE * (am + ai);
+                    const R *zz = SK + 2 * NS, *xz = SK + 4 * NS, *yz = SK + 5 * NS;
+                    const int o00 = so - SX - 1, o01 = so - 1, o10 = so - SX;
+                    const R kmd = ck * (am * (bm * zz[o00] + bj * zz[o01]) + ai * (bm * zz[o10] + bj * zz[so])) * q4;
+                    const R mx1 = ck * am * (bm * xz[o00] + bj * xz[o01]) * q8;
+                    const R mx2 = ck * ai * (bm * xz[o10] + bj * xz[so]) * q8;
+                    const R my1 = ck * bm * (am * yz[o00] + ai * yz[o10]) * q8;
+                    const R my2 = ck * bj * (am * yz[o01] + ai * yz[so]) * q8;
+                    const cx<R> exm = EXK[fo - 1], exmN = EXN[fo - 1];
+                    const cx<R> eym = EYK[fo - PX], eymN = EYN[fo - PX];
+                    const cx<R> K = (wjm + wjp + wim + wip) * ezK - wjm * EZK[fo - PX] - wjp * EZK[fo + PX]
+                                    - wim * EZK[fo - 1] - wip * EZK[fo + 1] + hx * (exm - exK - exmN + exN)
+                                    + hy * (eym - eyK - eymN + eyN);
+                    const cx<R> M = kmd * ezK + mx1 * (exm + exmN) + mx2 * (exK + exN) + my1 * (eym + eymN)
+                                    + my2 * (eyK + eyN);
+                    out = km_combine(K, M, wmu, alpha);
+                }
+                memit<R, DOT>(y, w, ez0 + (int64_t)k * ZP, out, in, ezK, acc);
+            }
+            // boundary edges of the last column i = nx and the last row j = ny
+            const cx<R> zero(0);
+            if (i == d.nx - 1) {
+                y[ey0 + 1 + (int64_t)k * YP] = zero;
+                if (k < d.nz) y[ez0 + 1 + (int64_t)k * ZP] = zero;
+            }
+            if (j == d.ny - 1) {
+                y[ex0 + d.nx + (int64_t)k * XP] = zero;
+                if (k < d.nz) {
+                    y[ez0 + (d.nx + 1) + (int64_t)k * ZP] = zero;
+                    if (i == d.nx - 1) y[ez0 + (d.nx + 2) + (int64_t)k * ZP] = zero;
+                }
+            }
+            exP = exK;
+            eyP = eyK;
+        }
+        if (!PF && k + 1 < ke) fetch(k + 1);
+    }
+    if (DOT >= 1 && DOT <= 3) block_reduce_store<4, G::NT>(acc, partials);
 }
 
 // w = M_sigma x on interior edges
@@ -1193,9 +1473,65 @@ void reduce_final(RedBuf &rb, int64_t nblk, int K, int dst, int op, hipStream_t 
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, st, rb.partials, nblk, K, rb.slots, dst, op);
 }
 
+// Chunk of node planes per block of the k-marching stencil: at least 4 steps
+// (the first step of a chunk re-loads one plane), and enough chunks that the
+// grid fills the 256 CUs about three blocks deep.
+int spmv_chunk(const Dims &d)
+{
+    const int64_t ntile = (int64_t)((d.nx + 31) / 32) * ((d.ny + 3) / 4);
+    const int np = d.nz + 1;
+    int kc = 16;
+    while (kc > 4 && ntile * ((np + kc - 1) / kc) < 1024) kc /= 2;
+    return kc;
+}
+
+template <typename R, int TY, bool PF>
+static int64_t march_go(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode,
+                        const cx<R> *w, RedBuf &rb, hipStream_t st, int kc)
+{
+    using G = mt::Geo<TY>;
+    const Dims &d = g.d;
+    if (kc <= 0) kc = spmv_chunk(d);
+    const int ntx = (d.nx + G::TX - 1) / G::TX, nty = (d.ny + TY - 1) / TY;
+    const unsigned gx = (unsigned)(ntx * nty), nch = (unsigned)((d.nz + kc) / kc);
+    const dim3 grid(gx, pad_y(gx, nch), 1), block(G::NT, 1, 1);
+    const int64_t nblk = (int64_t)grid.x * grid.y;
+    if (dot_mode >= 1 && dot_mode <= 3 && nblk > rb.max_blocks)
+        throw std::runtime_error("reduction buffer smaller than the stencil grid");
+#define SFDX_MARCH(M)                                                                                      \
+    hipLaunchKernelGGL((k_spmv_march<R, M, TY, PF>), grid, block, 0, st, d, g.a, g.b, g.c, g.ra, g.rb, g.rc, \
+                       g.sig, wmu, x, y, alpha, w, rb.partials, ntx, kc)
+    switch (dot_mode) {
+    case 0: SFDX_MARCH(0); break;
+    case 1: SFDX_MARCH(1); break;
+    case 2: SFDX_MARCH(2); break;
+    case 3: SFDX_MARCH(3); break;
+    case 4: SFDX_MARCH(4); break;
+    default: throw std::runtime_error("launch_spmv_march: bad mode");
+    }
+#undef SFDX_MARCH
+    return nblk;
+}
+
+// variant: 0 = tile 32x4 with prefetch (the default), 1 = 32x4 without,
+// 2 = 32x8 with prefetch, 3 = 32x8 without
 template <typename R>
-void launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode, const cx<R> *w,
-                 RedBuf &rb, hipStream_t st)
+int64_t launch_spmv_march(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode,
+                          const cx<R> *w, RedBuf &rb, hipStream_t st, int kc, int variant)
+{
+    switch (variant) {
+    case 1: return march_go<R, 4, false>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
+    case 2: return march_go<R, 8, true>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
+    case 3: return march_go<R, 8, false>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
+    default: return march_go<R, 4, true>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
+    }
+}
+
+// One row per thread (13-neighbour gather); kept for the split residual and for
+// comparison with the k-marching form.
+template <typename R>
+int64_t launch_spmv_gather(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode,
+                           const cx<R> *w, RedBuf &rb, hipStream_t st)
 {
     SliceLaunch L = edge_launch(g.d);
     switch (dot_mode) {
@@ -1205,6 +1541,14 @@ void launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, 
     case 3: hipLaunchKernelGGL((k_spmv<R, 3>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
     default: hipLaunchKernelGGL((k_spmv<R, 4>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
     }
+    return L.nblocks();
+}
+
+template <typename R>
+int64_t launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode, const cx<R> *w,
+                    RedBuf &rb, hipStream_t st)
+{
+    return launch_spmv_march<R>(g, wmu, x, y, alpha, dot_mode, w, rb, st, 0, 0);
 }
 
 // t = r - A z - M (G psi): residual of z + (i/wmu) G psi without forming A G psi
@@ -1627,8 +1971,12 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
 
 // explicit instantiations
 #define SFDX_INST(R)                                                                                              \
-    template void launch_spmv<R>(const DevGrid<R> &, R, const cx<R> *, cx<R> *, R, int, const cx<R> *, RedBuf &,  \
-                                 hipStream_t);                                                                    \
+    template int64_t launch_spmv<R>(const DevGrid<R> &, R, const cx<R> *, cx<R> *, R, int, const cx<R> *,         \
+                                    RedBuf &, hipStream_t);                                                        \
+    template int64_t launch_spmv_march<R>(const DevGrid<R> &, R, const cx<R> *, cx<R> *, R, int, const cx<R> *,   \
+                                          RedBuf &, hipStream_t, int, int);                                        \
+    template int64_t launch_spmv_gather<R>(const DevGrid<R> &, R, const cx<R> *, cx<R> *, R, int, const cx<R> *,  \
+                                           RedBuf &, hipStream_t);                                                 \
     template void launch_mass<R>(const DevGrid<R> &, const cx<R> *, cx<R> *, hipStream_t);                        \
     template void launch_ssor<R>(const DevGrid<R> &, R, const cx<R> *, cx<R> *, R, hipStream_t);                 \
     template void launch_vec<R>(int, const DevGrid<R> &, RedBuf &, cx<R> *, cx<R> *, cx<R> *, cx<R> *, cx<R> *,   \

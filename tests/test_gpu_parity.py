@@ -31,9 +31,12 @@ def _rand(n, seed):
     return rng.standard_normal(n) + 1j * rng.standard_normal(n)
 
 
+# shapes: one partial tile; a second tile in x and in y holding one column/row;
+# several chunks of node planes per block column
+@pytest.mark.parametrize("shape", [(9, 7, 8), (33, 9, 5), (40, 17, 26)])
 @pytest.mark.parametrize("aniso", [False, True])
-def test_operator_fp64_matches_oracle_csr(ctx, aniso):
-    a, b, c, nair, sig = tiny_model(9, 7, 8, 2, aniso=aniso)
+def test_operator_fp64_matches_oracle_csr(ctx, aniso, shape):
+    a, b, c, nair, sig = tiny_model(*shape, 2, aniso=aniso)
     f = 0.37
     S = O.build_system(a, b, c, sig, f)
     ve = O.row_scale(a, b, c, S.bnd)
