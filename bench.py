@@ -348,14 +348,16 @@ def main():
         "max_rel_impedance_err_mixed_vs_fp64": zerr,
         "fine_solve_hbm_gbs_algorithmic": {"mixed": round(mx["fine_bytes"] / mx["fine"] / 1e9, 1) if mx["fine"] else None,
                                           "fp64": round(f64["fine_bytes"] / f64["fine"] / 1e9, 1) if f64["fine"] else None},
-        "roofline": {"kernel": "k_spmv<double,0> (complex128 stencil)", "bound": "hbm", "achieved": round(ach64, 1),
+        "roofline": {"kernel": "k_spmv_march<double,0,4,true> (complex128 k-marching stencil)", "bound": "hbm",
+                     "achieved": round(ach64, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),
-                     "traffic": None if args.small else profiled_traffic("k_spmv<double,0>"),
+                     "traffic": None if args.small else profiled_traffic("k_spmv_march<double,0>"),
                      "avg_ms": round(ms64[0], 5), "algorithmic_bytes": b64,
-                     "complex64": {"kernel": "k_spmv<float,0>", "achieved": round(ach32, 1),
-                                   "traffic": None if args.small else profiled_traffic("k_spmv<float,0>"),
+                     "complex64": {"kernel": "k_spmv_march<float,0,4,true>", "achieved": round(ach32, 1),
+                                   "traffic": None if args.small else profiled_traffic("k_spmv_march<float,0>"),
                                    "frac": round(ach32 / HBM_PEAK_GBS, 4), "avg_ms": round(ms32[0], 5),
                                    "algorithmic_bytes": b32},
+                     "gather_form_ms": {"fp64": round(ms64[6], 5), "fp32": round(ms32[6], 5)},
                      "ssor_apply_ms": {"fp32": round(ms32[1], 4), "fp64": round(ms64[1], 4)},
                      "hybrid_apply_ms": {"fp32": round(ms32[5], 4), "fp64": round(ms64[5], 4)},
                      "copy_ms": {"fp32": round(ms32[4], 5), "fp64": round(ms64[4], 5),

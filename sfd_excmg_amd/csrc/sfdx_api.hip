@@ -111,7 +111,7 @@ template <typename R>
 struct LevelDev {
     DevGrid<R> g;
     double *a64 = nullptr, *b64 = nullptr, *c64 = nullptr, *sig64 = nullptr;
-    R *P = nullptr;  // 19 * NP divergence-correction operator
+    R *P = nullptr;  // 29 * NP: nodal operator G^T M G (19 slots) and its symmetric form
 };
 
 template <typename R>
@@ -159,7 +159,7 @@ template <typename R>
 void ensure_P(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag)
 {
     if (D.P) return;
-    D.P = pool_vec<R>(ctx, tag + (sizeof(R) == 8 ? "/P64" : "/P32"), (size_t)19 * D.g.d.NP);
+    D.P = pool_vec<R>(ctx, tag + (sizeof(R) == 8 ? "/P64" : "/P32"), (size_t)29 * D.g.d.NP);
     launch_pbuild<R>(D.g.d, D.a64, D.b64, D.c64, D.sig64, D.P, ctx->st);
 }
 

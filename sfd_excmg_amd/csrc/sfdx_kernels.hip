@@ -1053,6 +1053,24 @@ __global__ __launch_bounds__(TPB) void k_pbuild(Dims d, const double *__restrict
     }
 }
 
+// Symmetric form of P for the Jacobi smoother, stored after the 19 slots:
+// slot 19 = 1/diag(P), slots 20..28 = the 9 couplings to the upper neighbours
+// (positive linear offset).  The coupling of node n to a lower neighbour
+// m = n - o is the upper coupling of m, P_nm = P_mn, so one pass reads 10
+// values per node from memory instead of 19 (boundary rows, whose couplings
+// are dropped this way, only ever meet a zero input).
+__constant__ int c_pup[9] = {2, 4, 6, 9, 10, 13, 14, 17, 18};
+
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_psym(int64_t np, R *__restrict__ P)
+{
+    const int64_t n = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (n >= np) return;
+    P[19 * np + n] = (R)1 / P[n];
+#pragma unroll
+    for (int q = 0; q < 9; q++) P[(20 + q) * np + n] = P[(int64_t)c_pup[q] * np + n];
+}
+
 // y = P x (+ acc conj(x)^T y when DOT)
 template <typename R, int DOT>
 __global__ __launch_bounds__(TPB) void k_papply(Dims d, const R *__restrict__ P, const cx<R> *__restrict__ x,
@@ -1180,11 +1198,12 @@ __global__ __launch_bounds__(TPB) void k_pssor(Dims d, const R *__restrict__ P, 
     else phi[n] = phi[n] - s * dinv;
 }
 
-// t = G^T r on interior nodes (0 on the boundary), no reduction
-template <typename R>
+// t = G^T r on interior nodes (0 on the boundary), no reduction; with SCALE
+// the result is multiplied by dinv = 1/diag(P) (the input of k_pjacobi)
+template <typename R, bool SCALE = false>
 __global__ __launch_bounds__(TPB) void k_gradT(Dims d, const R *__restrict__ a, const R *__restrict__ b,
                                                const R *__restrict__ c, const cx<R> *__restrict__ w,
-                                               cx<R> *__restrict__ t)
+                                               cx<R> *__restrict__ t, const R *__restrict__ dinv)
 {
     const Tile TL = tile_id();
     const int k = TL.y;
@@ -1198,6 +1217,7 @@ __global__ __launch_bounds__(TPB) void k_gradT(Dims d, const R *__restrict__ a, 
         out = w[ex_id(d, i - 1, j, k)] * ((R)1 / a[i - 1]) - w[ex_id(d, i, j, k)] * ((R)1 / a[i]);
         out = out + w[ey_id(d, i, j - 1, k)] * ((R)1 / b[j - 1]) - w[ey_id(d, i, j, k)] * ((R)1 / b[j]);
         out = out + w[ez_id(d, i, j, k - 1)] * ((R)1 / c[k - 1]) - w[ez_id(d, i, j, k)] * ((R)1 / c[k]);
+        if (SCALE) out = out * dinv[n];
     }
     t[n] = out;
 }
@@ -1662,12 +1682,15 @@ void launch_node_div(const DevGrid<R> &g, const cx<R> *w, cx<R> *t, RedBuf &rb, 
     hipLaunchKernelGGL((k_node_div<R>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, w, t, rb.partials);
 }
 
+// P holds 29 * NP values: the 19 slots of k_pbuild, then the symmetric form
+// (1/diag, 9 upper couplings) of k_psym
 template <typename R>
 void launch_pbuild(const Dims &d, const double *a, const double *b, const double *c, const double *sig, R *P,
                    hipStream_t st)
 {
     SliceLaunch L = node_launch(d);
     hipLaunchKernelGGL((k_pbuild<R>), L.grid, L.block, 0, st, d, a, b, c, sig, P);
+    hipLaunchKernelGGL((k_psym<R>), dim3((unsigned)((d.NP + TPB - 1) / TPB)), dim3(TPB), 0, st, d.NP, P);
 }
 
 template <typename R>
@@ -1725,11 +1748,13 @@ static void pssor_one(const Dims &d, const R *P, const cx<R> *t, cx<R> *phi, hip
 
 // Two damped Jacobi steps on P from phi = 0, as one symmetric polynomial:
 //   phi = omega D^-1 ((2 - omega) t - omega sum_{m != n} P_nm D^-1_m t_m)
+//       = omega (2 - omega) u_n - omega^2 D^-1_n sum_{m != n} P_nm u_m,  u = D^-1 t,
 // i.e. [2 omega D^-1 - omega^2 D^-1 P D^-1] t: one pass over P with every node
 // in flight, instead of the 16 dependent colour sweeps of the nodal SSOR.
+// Input u = D^-1 t (k_gradT with SCALE), Ps = the symmetric slots 19..28.
 template <typename R>
-__global__ __launch_bounds__(TPB) void k_pjacobi2(Dims d, const R *__restrict__ P, const cx<R> *__restrict__ t,
-                                                  cx<R> *__restrict__ phi, R omega)
+__global__ __launch_bounds__(TPB) void k_pjacobi(Dims d, const R *__restrict__ Ps, const cx<R> *__restrict__ u,
+                                                 cx<R> *__restrict__ phi, R omega)
 {
     const Tile TL = tile_id();
     const int k = TL.y;
@@ -1742,24 +1767,34 @@ __global__ __launch_bounds__(TPB) void k_pjacobi2(Dims d, const R *__restrict__ 
         phi[n] = cx<R>(0);
         return;
     }
-    const int64_t sy = nI, sz = (int64_t)nI * nJ;
+    const int64_t sy = nI, sz = (int64_t)nI * nJ, np = d.NP;
     cx<R> s(0);
 #pragma unroll
-    for (int q = 1; q < 19; q++) {
-        const int64_t m = n + c_p19[q][0] + c_p19[q][1] * sy + c_p19[q][2] * sz;
-        s = s + (P[(int64_t)q * d.NP + n] / P[m]) * t[m];  // t = 0 on boundary nodes
+    for (int q = 0; q < 9; q++) {
+        const int sl = c_pup[q];
+        const int64_t o = c_p19[sl][0] + c_p19[sl][1] * sy + c_p19[sl][2] * sz;
+        const R *Pu = Ps + (1 + q) * np;
+        s = s + Pu[n] * u[n + o] + Pu[n - o] * u[n - o];  // u = 0 on boundary nodes
     }
-    phi[n] = (omega / P[n]) * (((R)2 - omega) * t[n] - omega * s);
+    phi[n] = (omega * ((R)2 - omega)) * u[n] - (omega * omega * Ps[n]) * s;
 }
 
+// phi = N_P (G^T in): nodal 1 the Jacobi polynomial (on D^-1 G^T in), else the
+// 8-colour nodal SSOR; tnode is the nodal work vector
 template <typename R>
-static void nodal_smooth(const Dims &d, const R *P, const cx<R> *t, cx<R> *phi, int nodal, hipStream_t st)
+static void nodal_part(const DevGrid<R> &g, const R *P, const cx<R> *in, cx<R> *tnode, cx<R> *phi, int nodal,
+                       hipStream_t st)
 {
+    const Dims &d = g.d;
+    SliceLaunch Ln = node_launch(d);
     if (nodal == 1) {
-        SliceLaunch L = node_launch(d);
-        hipLaunchKernelGGL((k_pjacobi2<R>), L.grid, L.block, 0, st, d, P, t, phi, (R)0.8);
+        const R *Ps = P + 19 * d.NP;
+        hipLaunchKernelGGL((k_gradT<R, true>), Ln.grid, Ln.block, 0, st, d, g.a, g.b, g.c, in, tnode, Ps);
+        hipLaunchKernelGGL((k_pjacobi<R>), Ln.grid, Ln.block, 0, st, d, Ps, tnode, phi, (R)0.8);
         return;
     }
+    hipLaunchKernelGGL((k_gradT<R, false>), Ln.grid, Ln.block, 0, st, d, g.a, g.b, g.c, in, tnode, (const R *)nullptr);
+    const cx<R> *t = tnode;
     pssor_one<R, 0, 0>(d, P, t, phi, st);
     pssor_one<R, 1, 0>(d, P, t, phi, st);
     pssor_one<R, 2, 0>(d, P, t, phi, st);
@@ -1785,9 +1820,7 @@ void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R
                    cx<R> *phi, R omega, hipStream_t st, int nodal, const cx<R> *dotw, double *partials)
 {
     launch_ssor<R>(g, wmu, in, out, omega, st);
-    SliceLaunch Ln = node_launch(g.d);
-    hipLaunchKernelGGL((k_gradT<R>), Ln.grid, Ln.block, 0, st, g.d, g.a, g.b, g.c, in, tnode);
-    nodal_smooth<R>(g.d, P, tnode, phi, nodal, st);
+    nodal_part<R>(g, P, in, tnode, phi, nodal, st);
     // s = -1/kappa = -1/(i wmu) = i/wmu
     const cx<R> s((R)0, (R)1 / wmu);
     SliceLaunch Le = edge_launch(g.d);
@@ -1804,9 +1837,7 @@ void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R
 void launch_nodal_correction(const DevGrid<double> &g, double wmu, const double *P, const zd *in, zd *out, zd *tnode,
                              zd *phi, hipStream_t st, const zd *dotw, double *partials)
 {
-    SliceLaunch Ln = node_launch(g.d);
-    hipLaunchKernelGGL((k_gradT<double>), Ln.grid, Ln.block, 0, st, g.d, g.a, g.b, g.c, in, tnode);
-    nodal_smooth<double>(g.d, P, tnode, phi, 1, st);
+    nodal_part<double>(g, P, in, tnode, phi, 1, st);
     const zd s(0.0, 1.0 / wmu);
     SliceLaunch Le = edge_launch(g.d);
     if (dotw)
@@ -1826,9 +1857,7 @@ void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, 
     const int64_t nl = g64.d.NL;
     launch_convert<double, float>(nl, in, in32, st);
     launch_ssor<float>(g32, (float)wmu, in32, z32, omega, st);
-    SliceLaunch Ln = node_launch(g32.d);
-    hipLaunchKernelGGL((k_gradT<float>), Ln.grid, Ln.block, 0, st, g32.d, g32.a, g32.b, g32.c, in32, tnode);
-    nodal_smooth<float>(g32.d, P32, tnode, phi, nodal, st);
+    nodal_part<float>(g32, P32, in32, tnode, phi, nodal, st);
     const zd s(0.0, 1.0 / wmu);
     SliceLaunch Le = edge_launch(g64.d);
     if (dotw)
@@ -1862,7 +1891,8 @@ void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, c
         ssor_one<R, 0, 0, 2>(g, wmu, in, out, (R)1, st);
     }
     SliceLaunch Ln = node_launch(g.d);
-    hipLaunchKernelGGL((k_gradT<R>), Ln.grid, Ln.block, 0, st, g.d, g.a, g.b, g.c, in, tnode);
+    hipLaunchKernelGGL((k_gradT<R, false>), Ln.grid, Ln.block, 0, st, g.d, g.a, g.b, g.c, in, tnode,
+                       (const R *)nullptr);
     if (dir == 0) {
         pssor_one<R, 0, 0>(g.d, P, tnode, phi, st);
         pssor_one<R, 1, 0>(g.d, P, tnode, phi, st);
