@@ -80,7 +80,7 @@ def solve_opts(m, precision, eps, verbose=0, precond=None):
     return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0,
                              norm=sfdx.NORM_INTERIOR, precision=precision, verbose=verbose,
                              precond=sfdx.PRECOND_HYBRID if precond is None else precond,
-                             sigma_floor32=0.0, restart_drop=RESTART_DROP, maxit_excmg=4000, use_graph=USE_GRAPH,
+                             sigma_floor32=0.0, restart_drop=RESTART_DROP, maxit_excmg=8000, use_graph=USE_GRAPH,
                              nodal_smoother=nodal)
 
 

@@ -118,6 +118,8 @@ typedef struct {
     int fallback_fp64;              /* solves (any level, frequency, polarisation) that switched from
                                        the complex64 to the complex128 preconditioner after repeated
                                        restarts without progress */
+    int solver_switches;            /* COCR solves that stagnated (restarts without progress) and
+                                       were continued by GPBiCG from the current iterate */
 } sfdx_stats;
 
 /* Context: device selection, stream, workspace. */

@@ -32,7 +32,8 @@ module sfdx_iface
     real(c_double) :: total_seconds, fine_seconds, fine_bytes
     integer(c_int) :: iters(2, 8)
     real(c_double) :: relres(2, 8)
-    integer(c_int) :: dc_calls(2, 8), pcg_iters(2, 8), refine_steps(2), fine_iters_total, fallback_fp64
+    integer(c_int) :: dc_calls(2, 8), pcg_iters(2, 8), refine_steps(2), fine_iters_total, fallback_fp64, &
+                      solver_switches
   end type
 
   interface

@@ -57,7 +57,8 @@ class Stats(ctypes.Structure):
                 ("fine_bytes", ctypes.c_double), ("iters", (ctypes.c_int * 2) * MAX_LEVELS),
                 ("relres", (ctypes.c_double * 2) * MAX_LEVELS), ("dc_calls", (ctypes.c_int * 2) * MAX_LEVELS),
                 ("pcg_iters", (ctypes.c_int * 2) * MAX_LEVELS), ("refine_steps", ctypes.c_int * 2),
-                ("fine_iters_total", ctypes.c_int), ("fallback_fp64", ctypes.c_int)]
+                ("fine_iters_total", ctypes.c_int), ("fallback_fp64", ctypes.c_int),
+                ("solver_switches", ctypes.c_int)]
 
     def to_dict(self, levels=MAX_LEVELS):
         return dict(status=self.status, total_seconds=self.total_seconds, fine_seconds=self.fine_seconds,
@@ -67,7 +68,8 @@ class Stats(ctypes.Structure):
                     dc_calls=[[self.dc_calls[i][p] for p in range(2)] for i in range(levels)],
                     pcg_iters=[[self.pcg_iters[i][p] for p in range(2)] for i in range(levels)],
                     refine_steps=[self.refine_steps[0], self.refine_steps[1]],
-                    fine_iters_total=self.fine_iters_total, fallback_fp64=self.fallback_fp64)
+                    fine_iters_total=self.fine_iters_total, fallback_fp64=self.fallback_fp64,
+                    solver_switches=self.solver_switches)
 
 
 class Job(ctypes.Structure):
