@@ -303,8 +303,8 @@ def main():
         print(json.dumps({"spmv_ms": {"fp64": ms64[0], "fp32": ms32[0]},
                           "spmv_gather_ms": {"fp64": ms64[6], "fp32": ms32[6]},
                           "spmv_march_ms_by_tile": {v: [ms64[7 + u], ms32[7 + u]] for u, v in
-                                                    enumerate(("32x4pf", "32x4", "32x8pf", "32x8"))},
-                          "spmv_march_32x4pf_kc4_ms": [ms64[11], ms32[11]],
+                                                    enumerate(("32x8pf", "32x8", "32x4pf", "32x4"))},
+                          "spmv_march_32x8pf_kc4_ms": [ms64[11], ms32[11]],
                           "copy_ms": {"fp64": ms64[4], "fp32": ms32[4]},
                           "copy_bytes": {"fp64": 2 * nl * 16, "fp32": 2 * nl * 8},
                           "spmv_bytes": {"fp64": b64, "fp32": b32}}))
@@ -348,12 +348,12 @@ def main():
         "max_rel_impedance_err_mixed_vs_fp64": zerr,
         "fine_solve_hbm_gbs_algorithmic": {"mixed": round(mx["fine_bytes"] / mx["fine"] / 1e9, 1) if mx["fine"] else None,
                                           "fp64": round(f64["fine_bytes"] / f64["fine"] / 1e9, 1) if f64["fine"] else None},
-        "roofline": {"kernel": "k_spmv_march<double,0,4,true> (complex128 k-marching stencil)", "bound": "hbm",
+        "roofline": {"kernel": "k_spmv_march<double,0,4,true> (complex128 k-marching stencil, 32x4 tile)", "bound": "hbm",
                      "achieved": round(ach64, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),
                      "traffic": None if args.small else profiled_traffic("k_spmv_march<double,0>"),
                      "avg_ms": round(ms64[0], 5), "algorithmic_bytes": b64,
-                     "complex64": {"kernel": "k_spmv_march<float,0,4,true>", "achieved": round(ach32, 1),
+                     "complex64": {"kernel": "k_spmv_march<float,0,8,true>", "achieved": round(ach32, 1),
                                    "traffic": None if args.small else profiled_traffic("k_spmv_march<float,0>"),
                                    "frac": round(ach32 / HBM_PEAK_GBS, 4), "avg_ms": round(ms32[0], 5),
                                    "algorithmic_bytes": b32},

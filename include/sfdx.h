@@ -233,7 +233,7 @@ int sfdx_extint_v(sfdx_ctx *ctx, const sfdx_grid *g, const double *u0, const dou
  * ms[2] COCR vector update, ms[3] = ms[0]+ms[1]+ms[2], ms[4] plain copy of
  * one edge vector, ms[5] hybrid preconditioner apply, ms[6] SpMV in the
  * one-row-per-thread gather form, ms[7..10] the k-marching SpMV with tiles of
- * 32x4 columns with and without prefetch, then 32x8 with and without,
+ * 32x8 columns with and without prefetch, then 32x4 with and without,
  * ms[11] the first of these over 4-plane chunks.  ms holds 12 doubles.
  * No counterpart in the reference (measurement only). */
 int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, int precision,

@@ -1632,8 +1632,8 @@ int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, d
             cx<R> *ph = pool_vec<cx<R>>(ctx, std::string("bench/ph") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
             ms[5] = timed([&](int q) { launch_hybrid<R>(D.g, wmu, D.P, V(2 * q), V(2 * q + 1), tn, ph, (R)1, ctx->st); });
             // the one-row-per-thread gather form of the SpMV, and the k-marching
-            // form in its four tile variants (default chunk) and the default
-            // variant over 4-plane chunks
+            // form in its four tile variants (default chunk) and the 32x8 tile
+            // over 4-plane chunks
             ms[6] = timed([&](int q) { launch_spmv_gather<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, 0, nullptr, ctx->rb, ctx->st); });
             for (int u = 0; u < 4; u++)
                 ms[7 + u] = timed([&](int q) {

@@ -159,8 +159,8 @@ template <typename R>
 int64_t launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode, const cx<R> *w,
                     RedBuf &rb, hipStream_t st);  // returns the number of partial sums written (dot modes 1-3)
 // k-marching stencil with an explicit chunk of node planes per block (kc <= 0:
-// the default) and tile variant (0: 32x4 with prefetch, 1: 32x4 without, 2:
-// 32x8 with prefetch, 3: 32x8 without)
+// the default) and tile variant (0: 32x8 with prefetch, 1: 32x8 without, 2:
+// 32x4 with prefetch, 3: 32x4 without)
 template <typename R>
 int64_t launch_spmv_march(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode,
                           const cx<R> *w, RedBuf &rb, hipStream_t st, int kc, int variant);
@@ -168,7 +168,7 @@ int64_t launch_spmv_march(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, 
 template <typename R>
 int64_t launch_spmv_gather(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode,
                            const cx<R> *w, RedBuf &rb, hipStream_t st);
-int spmv_chunk(const Dims &d);
+int spmv_chunk(const Dims &d, int ty);
 template <typename R>
 void launch_mass(const DevGrid<R> &g, const cx<R> *x, cx<R> *w, hipStream_t st);
 template <typename R>

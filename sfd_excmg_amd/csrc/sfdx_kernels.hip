@@ -297,8 +297,9 @@ __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, c
 // thread per footprint column loads Ex/Ey(k+1), Ez(k) and sigma(k) (9 vector
 // loads for 3 rows, against 23 per row when every row gathers its 13
 // neighbours and 12 cell values itself) into the ring slots that held the
-// planes no longer needed.  The rows are formed by the first TX*TY threads,
-// one per tile column, so every lane of those wavefronts has a row to form.
+// planes no longer needed.  Each thread forms the rows of one tile column and
+// loads one or two footprint columns (the tile plus its halo), so every
+// wavefront both loads and computes.
 namespace mt {
 template <int TY>
 struct Geo {
@@ -306,7 +307,8 @@ struct Geo {
     static constexpr int PX = TX + 2, PY = TY + 2;  // edge footprint: i0-1..i0+TX, j0-1..j0+TY
     static constexpr int SX = TX + 1, SY = TY + 1;  // cell footprint: i0-1..i0+TX-1, j0-1..j0+TY-1
     static constexpr int NP = PX * PY, NS = SX * SY;
-    static constexpr int NT = (NP + 63) / 64 * 64;  // threads: footprint rounded to wavefronts
+    static constexpr int NT = TX * TY;  // threads: one per tile column; each loads footprint columns t, t + NT
+    static_assert(NP <= 2 * NT, "a thread loads at most two footprint columns");
     // LDS words of type R: Ex, Ey, Ez rings (2 complex planes each), sigma ring
     // (2 planes x 6 components), widths a, 1/a over i0-1..i0+TX-1 and b, 1/b
     // over j0-1..j0+TY-1
@@ -343,6 +345,86 @@ __device__ __forceinline__ void memit(cx<R> *__restrict__ y, const cx<R> *__rest
     }
 }
 
+// one footprint column of the k-marching stencil: where it lives, which of
+// its edges and cell exist, and the values entering at the current step
+template <typename R>
+struct MLoad {
+    int lf, ls;                   // LDS index in the edge / cell footprint
+    bool on, xv, yv, zv, sv, cv;  // in the footprint; Ex, Ey, Ez exist; in the cell footprint; cell exists
+    int64_t ex, ey, ez, cc;       // indices at k = 0
+    cx<R> inX, inY, inZ;          // Ex/Ey(k+1), Ez(k)
+    R iS[6];                      // sigma(k)
+
+    template <int PX, int SX, int SY>
+    __device__ __forceinline__ void init(const Dims &d, int f, int np, int i0, int j0)
+    {
+        on = f < np;
+        const int lj = f / PX, li = f - lj * PX;
+        const int gi = i0 - 1 + li, gj = j0 - 1 + lj;
+        const bool iin = gi >= 0 && gi <= d.nx, jin = gj >= 0 && gj <= d.ny;
+        xv = on && iin && jin && gi < d.nx;
+        yv = on && iin && jin && gj < d.ny;
+        zv = on && iin && jin;
+        sv = on && li < SX && lj < SY;
+        cv = sv && gi >= 0 && gi < d.nx && gj >= 0 && gj < d.ny;
+        lf = lj * PX + li;
+        ls = lj * SX + li;
+        ex = (int64_t)gj * d.nx + gi;
+        ey = d.XN + (int64_t)gj * (d.nx + 1) + gi;
+        ez = d.XN + d.YN + (int64_t)gj * (d.nx + 1) + gi;
+        cc = (int64_t)gj * d.nx + gi;
+    }
+    __device__ __forceinline__ void fetch(const Dims &d, const cx<R> *__restrict__ x, const R *__restrict__ sig,
+                                          int k, int64_t XP, int64_t YP, int64_t ZP, int64_t CP)
+    {
+        inX = cx<R>(0); inY = cx<R>(0); inZ = cx<R>(0);
+#pragma unroll
+        for (int q = 0; q < 6; q++) iS[q] = 0;
+        if (k < d.nz) {
+            if (xv) inX = x[ex + (int64_t)(k + 1) * XP];
+            if (yv) inY = x[ey + (int64_t)(k + 1) * YP];
+            if (zv) inZ = x[ez + (int64_t)k * ZP];
+            if (cv) {
+                const int64_t q = cc + (int64_t)k * CP;
+#pragma unroll
+                for (int u = 0; u < 6; u++) iS[u] = sig[u * d.NC + q];
+            }
+        }
+    }
+    // before the first step kb: Ex/Ey(kb), Ez(kb-1), sigma(kb-1) into their ring slots
+    __device__ __forceinline__ void prime(const Dims &d, const cx<R> *__restrict__ x, const R *__restrict__ sig,
+                                          int kb, int64_t XP, int64_t YP, int64_t ZP, int64_t CP, cx<R> *EX,
+                                          cx<R> *EY, cx<R> *EZ, R *SG, int np, int ns)
+    {
+        const bool lo = kb >= 1;
+        const int s0 = kb & 1, s1 = s0 ^ 1;
+        if (on) {
+            EX[s0 * np + lf] = xv ? x[ex + (int64_t)kb * XP] : cx<R>(0);
+            EY[s0 * np + lf] = yv ? x[ey + (int64_t)kb * YP] : cx<R>(0);
+            EZ[s1 * np + lf] = (zv && lo) ? x[ez + (int64_t)(kb - 1) * ZP] : cx<R>(0);
+        }
+        if (sv) {
+            const int64_t q = cc + (int64_t)(kb - 1) * CP;
+#pragma unroll
+            for (int u = 0; u < 6; u++) SG[(s1 * 6 + u) * ns + ls] = (cv && lo) ? sig[u * d.NC + q] : (R)0;
+        }
+    }
+    // at step k: Ex/Ey(k+1) into slot (k+1)&1, Ez(k) and sigma(k) into slot k&1
+    __device__ __forceinline__ void store(int k, cx<R> *EX, cx<R> *EY, cx<R> *EZ, R *SG, int np, int ns) const
+    {
+        const int sk = k & 1, sn = sk ^ 1;
+        if (on) {
+            EX[sn * np + lf] = inX;
+            EY[sn * np + lf] = inY;
+            EZ[sk * np + lf] = inZ;
+        }
+        if (sv) {
+#pragma unroll
+            for (int u = 0; u < 6; u++) SG[(sk * 6 + u) * ns + ls] = iS[u];
+        }
+    }
+};
+
 template <typename R, int DOT, int TY, bool PF>
 __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
     Dims d, const R *__restrict__ a, const R *__restrict__ b, const R *__restrict__ c, const R *__restrict__ ra,
@@ -366,59 +448,25 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
     const int t = threadIdx.x;
     const int64_t XP = (int64_t)(d.ny + 1) * d.nx, YP = (int64_t)d.ny * (d.nx + 1);
     const int64_t ZP = (int64_t)(d.ny + 1) * (d.nx + 1), CP = (int64_t)d.nx * d.ny, NC = d.NC;
-    // loader role: footprint column (li, lj) = node column (i0-1+li, j0-1+lj)
-    const int lj = t / PX, li = t - lj * PX;
-    const int gi = i0 - 1 + li, gj = j0 - 1 + lj;
-    const bool foot = t < NP;
-    const bool iin = gi >= 0 && gi <= d.nx, jin = gj >= 0 && gj <= d.ny;
-    const bool xv = foot && iin && jin && gi < d.nx;  // Ex(gi,gj,.) exists
-    const bool yv = foot && iin && jin && gj < d.ny;  // Ey
-    const bool zv = foot && iin && jin;               // Ez
-    const bool sv = foot && li < SX && lj < SY;       // inside the cell footprint
-    const bool cv = sv && gi >= 0 && gi < d.nx && gj >= 0 && gj < d.ny;
-    const int lf = lj * PX + li, ls = lj * SX + li;
-    const int64_t lex = (int64_t)gj * d.nx + gi, ley = d.XN + (int64_t)gj * (d.nx + 1) + gi;
-    const int64_t lez = d.XN + d.YN + (int64_t)gj * (d.nx + 1) + gi, lc = (int64_t)gj * d.nx + gi;
-    // values entering at step k: Ex/Ey(k+1), Ez(k), sigma(k)
-    cx<R> inX, inY, inZ;
-    R iS[6] = {0, 0, 0, 0, 0, 0};
-    auto fetch = [&](int k) {
-        inX = cx<R>(0); inY = cx<R>(0); inZ = cx<R>(0);
-#pragma unroll
-        for (int q = 0; q < 6; q++) iS[q] = 0;
-        if (k < d.nz) {
-            if (xv) inX = x[lex + (int64_t)(k + 1) * XP];
-            if (yv) inY = x[ley + (int64_t)(k + 1) * YP];
-            if (zv) inZ = x[lez + (int64_t)k * ZP];
-            if (cv) {
-                const int64_t q = lc + (int64_t)k * CP;
-#pragma unroll
-                for (int u = 0; u < 6; u++) iS[u] = sig[u * NC + q];
-            }
-        }
-    };
+    // loader role: footprint columns t and t + NT
+    constexpr int NT = G::NT;
+    MLoad<R> L0, L1;
+    L0.template init<PX, SX, SY>(d, t, NP, i0, j0);
+    L1.template init<PX, SX, SY>(d, t + NT, NP, i0, j0);
+    const bool two = t + NT < NP;
     // compute role: tile column (i, j)
     const int cy = t / TX, cxl = t - cy * TX;
     const int i = i0 + cxl, j = j0 + cy;
-    const bool own = t < TX * TY && i < d.nx && j < d.ny;
+    const bool own = i < d.nx && j < d.ny;
     const int fo = (cy + 1) * PX + cxl + 1, so = (cy + 1) * SX + cxl + 1;
     const int64_t ex0 = (int64_t)j * d.nx + i, ey0 = d.XN + (int64_t)j * (d.nx + 1) + i;
     const int64_t ez0 = d.XN + d.YN + (int64_t)j * (d.nx + 1) + i;
     cx<R> exP, eyP;  // own Ex, Ey at k-1
     if (kb < ke) {
-        // prologue: Ex/Ey(kb), Ez(kb-1), sigma(kb-1) into their ring slots, width tables
+        // prologue: the first planes into their ring slots, width tables
         const bool lo = kb >= 1;
-        const int s0 = kb & 1, s1 = s0 ^ 1;
-        if (foot) {
-            EX[s0 * NP + lf] = xv ? x[lex + (int64_t)kb * XP] : cx<R>(0);
-            EY[s0 * NP + lf] = yv ? x[ley + (int64_t)kb * YP] : cx<R>(0);
-            EZ[s1 * NP + lf] = (zv && lo) ? x[lez + (int64_t)(kb - 1) * ZP] : cx<R>(0);
-        }
-        if (sv) {
-            const int64_t q = lc + (int64_t)(kb - 1) * CP;
-#pragma unroll
-            for (int u = 0; u < 6; u++) SG[(s1 * 6 + u) * NS + ls] = (cv && lo) ? sig[u * NC + q] : (R)0;
-        }
+        L0.prime(d, x, sig, kb, XP, YP, ZP, CP, EX, EY, EZ, SG, NP, NS);
+        if (two) L1.prime(d, x, sig, kb, XP, YP, ZP, CP, EX, EY, EZ, SG, NP, NS);
         if (t <= TX) {
             const int ii = i0 - 1 + t;
             const bool ok = ii >= 0 && ii < d.nx;
@@ -435,24 +483,21 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
             exP = x[ex0 + (int64_t)(kb - 1) * XP];
             eyP = x[ey0 + (int64_t)(kb - 1) * YP];
         }
-        fetch(kb);
+        L0.fetch(d, x, sig, kb, XP, YP, ZP, CP);
+        if (two) L1.fetch(d, x, sig, kb, XP, YP, ZP, CP);
     }
     const R hE = (R)0.5, q4 = (R)0.25, q8 = (R)0.125;
     double acc[4] = {0, 0, 0, 0};
     for (int k = kb; k < ke; k++) {
         const int sk = k & 1, sn = sk ^ 1;  // ring slots of plane k and of planes k+1 / k-1
         __syncthreads();                    // the previous step's reads of the slots refilled here are done
-        if (foot) {
-            EX[sn * NP + lf] = inX;
-            EY[sn * NP + lf] = inY;
-            EZ[sk * NP + lf] = inZ;
-        }
-        if (sv) {
-#pragma unroll
-            for (int u = 0; u < 6; u++) SG[(sk * 6 + u) * NS + ls] = iS[u];
-        }
+        L0.store(k, EX, EY, EZ, SG, NP, NS);
+        if (two) L1.store(k, EX, EY, EZ, SG, NP, NS);
         // with PF the next step's loads are in flight while this step's rows are formed
-        if (PF && k + 1 < ke) fetch(k + 1);
+        if (PF && k + 1 < ke) {
+            L0.fetch(d, x, sig, k + 1, XP, YP, ZP, CP);
+            if (two) L1.fetch(d, x, sig, k + 1, XP, YP, ZP, CP);
+        }
         __syncthreads();
         if (own) {
             const cx<R> *EXK = EX + sk * NP, *EXN = EX + sn * NP, *EYK = EY + sk * NP, *EYN = EY + sn * NP;
@@ -559,7 +604,10 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
             exP = exK;
             eyP = eyK;
         }
-        if (!PF && k + 1 < ke) fetch(k + 1);
+        if (!PF && k + 1 < ke) {
+            L0.fetch(d, x, sig, k + 1, XP, YP, ZP, CP);
+            if (two) L1.fetch(d, x, sig, k + 1, XP, YP, ZP, CP);
+        }
     }
     if (DOT >= 1 && DOT <= 3) block_reduce_store<4, G::NT>(acc, partials);
 }
@@ -1496,9 +1544,9 @@ void reduce_final(RedBuf &rb, int64_t nblk, int K, int dst, int op, hipStream_t 
 // Chunk of node planes per block of the k-marching stencil: at least 4 steps
 // (the first step of a chunk re-loads one plane), and enough chunks that the
 // grid fills the 256 CUs about three blocks deep.
-int spmv_chunk(const Dims &d)
+int spmv_chunk(const Dims &d, int ty)
 {
-    const int64_t ntile = (int64_t)((d.nx + 31) / 32) * ((d.ny + 3) / 4);
+    const int64_t ntile = (int64_t)((d.nx + 31) / 32) * ((d.ny + ty - 1) / ty);
     const int np = d.nz + 1;
     int kc = 16;
     while (kc > 4 && ntile * ((np + kc - 1) / kc) < 1024) kc /= 2;
@@ -1511,7 +1559,7 @@ static int64_t march_go(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R 
 {
     using G = mt::Geo<TY>;
     const Dims &d = g.d;
-    if (kc <= 0) kc = spmv_chunk(d);
+    if (kc <= 0) kc = spmv_chunk(d, TY);
     const int ntx = (d.nx + G::TX - 1) / G::TX, nty = (d.ny + TY - 1) / TY;
     const unsigned gx = (unsigned)(ntx * nty), nch = (unsigned)((d.nz + kc) / kc);
     const dim3 grid(gx, pad_y(gx, nch), 1), block(G::NT, 1, 1);
@@ -1533,17 +1581,17 @@ static int64_t march_go(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R 
     return nblk;
 }
 
-// variant: 0 = tile 32x4 with prefetch (the default), 1 = 32x4 without,
-// 2 = 32x8 with prefetch, 3 = 32x8 without
+// variant: 0 = tile 32x8 (256 threads) with prefetch, 1 = 32x8 without,
+// 2 = 32x4 (128 threads) with prefetch, 3 = 32x4 without
 template <typename R>
 int64_t launch_spmv_march(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode,
                           const cx<R> *w, RedBuf &rb, hipStream_t st, int kc, int variant)
 {
     switch (variant) {
-    case 1: return march_go<R, 4, false>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
-    case 2: return march_go<R, 8, true>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
-    case 3: return march_go<R, 8, false>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
-    default: return march_go<R, 4, true>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
+    case 1: return march_go<R, 8, false>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
+    case 2: return march_go<R, 4, true>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
+    case 3: return march_go<R, 4, false>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
+    default: return march_go<R, 8, true>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
     }
 }
 
@@ -1568,7 +1616,10 @@ template <typename R>
 int64_t launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode, const cx<R> *w,
                     RedBuf &rb, hipStream_t st)
 {
-    return launch_spmv_march<R>(g, wmu, x, y, alpha, dot_mode, w, rb, st, 0, 0);
+    // tile chosen by the variant sweep of sfdx_bench_kernels on 128x128x96:
+    // 32x4 columns in complex128 (0.070 against 0.074 ms for 32x8), 32x8 in
+    // complex64 (0.045 against 0.048 ms)
+    return launch_spmv_march<R>(g, wmu, x, y, alpha, dot_mode, w, rb, st, 0, sizeof(R) == 8 ? 2 : 0);
 }
 
 // t = r - A z - M (G psi): residual of z + (i/wmu) G psi without forming A G psi
