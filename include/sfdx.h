@@ -102,6 +102,10 @@ typedef struct {
     int coarse_solver;  /* EXCMG levels 1-2: 0 host direct solve (MKL PARDISO, the
                            reference's u1u2), 1 Krylov solves on the device from a
                            zero guess to eps*10^(i-L) */
+    int full_background;  /* 0: the reference's 1-D boundary (diagonal of cell column (1,1),
+                             cross-polar boundary components 0; fdfe_main.f90:938-942,
+                             470-494); 1: full tensor of that column, with the cross-polar
+                             components of the 1-D field on the boundary (Ez stays 0) */
 } sfdx_opts;
 
 typedef struct {

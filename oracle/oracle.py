@@ -64,6 +64,9 @@ def lib():
         L.orc_nnodes.argtypes = [i32, i32, i32]
         L.orc_build_system.restype = i64
         L.orc_build_system.argtypes = [i32, i32, i32, P, P, P, P, dbl, P, P, P, P, P, P, P]
+        L.orc_build_system_full.restype = i64
+        L.orc_build_system_full.argtypes = [i32, i32, i32, P, P, P, P, dbl, P, P, P, P, P, P, P, P]
+        L.orc_mt1d_anl_full.argtypes = [i32, P, P, dbl, P, P]
         L.orc_row_scale.argtypes = [i32, i32, i32, P, P, P, P, P]
         L.orc_zamux.argtypes = [i64, P, P, P, P, P]
         L.orc_ssor_apply.argtypes = [i64, P, P, P, dbl, P, P]
@@ -136,12 +139,27 @@ def mt1d_anl(freq: float, conv: np.ndarray, c: np.ndarray):
     return ex, hx, imp
 
 
-def background_column(sigma: np.ndarray, nx: int, ny: int, nz: int) -> np.ndarray:
+def background_column(sigma: np.ndarray, nx: int, ny: int, nz: int, full: bool = False) -> np.ndarray:
     """cont1(i,1:3) = sigma((i-1)*nx*ny+1, 1:3), cols 4:6 zero
-    (src/fdfe_main.f90:938-942, 1376-1380)."""
+    (src/fdfe_main.f90:938-942, 1376-1380); with full, all 6 components of
+    that column (full-tensor boundary, a deliberate deviation)."""
     conv = np.zeros((nz, 6))
-    conv[:, :3] = np.asarray(sigma)[np.arange(nz) * nx * ny, :3]
+    cols = 6 if full else 3
+    conv[:, :cols] = np.asarray(sigma)[np.arange(nz) * nx * ny, :cols]
     return conv
+
+
+def mt1d_anl_full(freq: float, conv: np.ndarray, c: np.ndarray):
+    """mt1d_anl with the cross-polar components: returns (ex, exc), ex as
+    mt1d_anl, exc = Ey of the xy mode then Ex of the yx mode (2*nz+2)."""
+    conv = np.asarray(conv, dtype=np.float64)
+    nz = conv.shape[0]
+    cf = np.ascontiguousarray(conv.T).reshape(-1)
+    ex = np.zeros(2 * nz + 2, np.complex128)
+    exc = np.zeros(2 * nz + 2, np.complex128)
+    c = _f64(c)
+    lib().orc_mt1d_anl_full(nz, _p(cf), _p(c), float(freq), _p(ex), _p(exc))
+    return ex, exc
 
 
 # ---------------------------------------------------------------- system
@@ -160,12 +178,17 @@ class System:
     e_dc: np.ndarray
 
 
-def build_system(a, b, c, sigma, freq, e_dc=None) -> System:
-    """fd_assemb + dcfast (src/fdfe_main.f90:226-546)."""
+def build_system(a, b, c, sigma, freq, e_dc=None, full=False, e_x=None) -> System:
+    """fd_assemb + dcfast (src/fdfe_main.f90:226-546).  With full, the 1-D
+    background uses the whole tensor of the first column and the cross-polar
+    boundary components e_x are set as well (orc_build_system_full)."""
     a, b, c = _f64(a), _f64(b), _f64(c)
     nx, ny, nz = len(a), len(b), len(c)
     if e_dc is None:
-        e_dc, _, _ = mt1d_anl(freq, background_column(sigma, nx, ny, nz), c)
+        if full:
+            e_dc, e_x = mt1d_anl_full(freq, background_column(sigma, nx, ny, nz, True), c)
+        else:
+            e_dc, _, _ = mt1d_anl(freq, background_column(sigma, nx, ny, nz), c)
     e_dc = _c128(e_dc)
     nl = nedges(nx, ny, nz)
     ia = np.zeros(nl + 1, np.int64)
@@ -175,8 +198,13 @@ def build_system(a, b, c, sigma, freq, e_dc=None) -> System:
     p2 = np.zeros(nl, np.complex128)
     bnd = np.zeros(nl, np.uint8)
     sf = sigma_flat(sigma)
-    nnz = lib().orc_build_system(nx, ny, nz, _p(a), _p(b), _p(c), _p(sf), float(freq), _p(e_dc),
-                                 _p(ia), _p(ja), _p(val), _p(p1), _p(p2), _p(bnd))
+    if full:
+        e_x = _c128(e_x)
+        nnz = lib().orc_build_system_full(nx, ny, nz, _p(a), _p(b), _p(c), _p(sf), float(freq), _p(e_dc),
+                                          _p(e_x), _p(ia), _p(ja), _p(val), _p(p1), _p(p2), _p(bnd))
+    else:
+        nnz = lib().orc_build_system(nx, ny, nz, _p(a), _p(b), _p(c), _p(sf), float(freq), _p(e_dc),
+                                     _p(ia), _p(ja), _p(val), _p(p1), _p(p2), _p(bnd))
     if nnz < 0:
         raise MemoryError("orc_build_system failed")
     ja = ja[:nnz].copy()
@@ -359,6 +387,7 @@ class Options:
     maxit_plain: int = 5000      # src/fdfe_main.f90:993
     maxit_excmg: int = 1000      # src/fdfe_main.f90:1414
     norm: str = "reference"      # or "interior"
+    full_background: bool = False  # full-tensor 1-D boundary with cross-polar components
 
 
 @dataclass
@@ -373,7 +402,7 @@ class LevelReport:
 def solve_level(a, b, c, sigma, freq, tol, opts: Options, x0=None, direct=False):
     """doublemode_nor (src/fdfe_main.f90:892-1063) for one grid: 1-D background,
     assembly, then both polarisations (direct for the coarse levels)."""
-    S = build_system(a, b, c, sigma, freq)
+    S = build_system(a, b, c, sigma, freq, full=opts.full_background)
     nl = len(S.p1)
     if direct:
         u1, u2 = direct_solve(S)

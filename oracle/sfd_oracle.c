@@ -45,6 +45,11 @@
  *                      G^T M x to the PCG tolerance
  *   orc_mt1d_anl       plane wave in a uniform earth; axial anisotropy decouples
  *                      the modes; two layers against a fine 1-D FD solve
+ *   orc_mt1d_anl_full, orc_build_system_full
+ *                      layered model with a rotated (full) tensor: the 3-D
+ *                      solve with the full-tensor boundary reproduces both
+ *                      1-D components inside the grid to discretisation
+ *                      error, the diagonal boundary does not
  *   orc_extint_v       polynomial reproduction of I2 / I4 on nonuniform nodes;
  *                      extrapolation constants from the text
  *   orc_mesh_coarsen   conductance integral preserved
@@ -91,10 +96,39 @@ int64_t orc_nnodes(int nx, int ny, int nz) { return (int64_t)(nx + 1) * (ny + 1)
  * exactly as dcfast does (column order = slot order, zero values dropped).
  * ia must hold nl+1, ja/val 13*nl, p1/p2/bnd nl entries.  Returns nnz.
  * -------------------------------------------------------------------------- */
+static int64_t build_system_core(int nx, int ny, int nz, const double *a, const double *b,
+                                 const double *c, const double *sigma, double freq,
+                                 const zc *e_dc, const zc *e_x, int64_t *ia, int32_t *ja, zc *val,
+                                 zc *p1, zc *p2, uint8_t *bnd);
+
 int64_t orc_build_system(int nx, int ny, int nz, const double *a, const double *b,
                          const double *c, const double *sigma, double freq,
                          const zc *e_dc, int64_t *ia, int32_t *ja, zc *val, zc *p1,
                          zc *p2, uint8_t *bnd)
+{
+    return build_system_core(nx, ny, nz, a, b, c, sigma, freq, e_dc, NULL, ia, ja, val, p1, p2, bnd);
+}
+
+/* Full-tensor boundary (a deliberate deviation from the reference, which
+ * builds the 1-D field from the diagonal of the first column only and sets the
+ * cross-polar boundary components to 0, src/fdfe_main.f90:938-942, 470-494):
+ * as orc_build_system, plus the cross-polar components of the 1-D field on
+ * the boundary.  e_x holds 2*nz+2 values: Ey of the xy mode at the
+ * z-nodes, then Ex of the yx mode; boundary y-edges of p1 and x-edges of p2
+ * at node plane k get e_x[k] / e_x[nz+1+k] (0 on the top plane, where the
+ * modes are normalised to E = (1,0) and (0,1)). */
+int64_t orc_build_system_full(int nx, int ny, int nz, const double *a, const double *b,
+                              const double *c, const double *sigma, double freq,
+                              const zc *e_dc, const zc *e_x, int64_t *ia, int32_t *ja, zc *val,
+                              zc *p1, zc *p2, uint8_t *bnd)
+{
+    return build_system_core(nx, ny, nz, a, b, c, sigma, freq, e_dc, e_x, ia, ja, val, p1, p2, bnd);
+}
+
+static int64_t build_system_core(int nx, int ny, int nz, const double *a, const double *b,
+                                 const double *c, const double *sigma, double freq,
+                                 const zc *e_dc, const zc *e_x, int64_t *ia, int32_t *ja, zc *val,
+                                 zc *p1, zc *p2, uint8_t *bnd)
 {
     const int64_t xnl = n_xedge(nx, ny, nz), ynl = n_yedge(nx, ny, nz);
     const int64_t nl = xnl + ynl + n_zedge(nx, ny, nz);
@@ -234,6 +268,19 @@ int64_t orc_build_system(int nx, int ny, int nz, const double *a, const double *
             p2[xnl + (int64_t)k * (nx + 1) * ny + (int64_t)jj * (nx + 1) - 1] = e_dc[nz + 1 + k];
         }
     for (int64_t e = xnl + ynl - (int64_t)(nx + 1) * ny; e < xnl + ynl; e++) p2[e] = e_dc[2 * nz + 1];
+    if (e_x) {
+        /* cross-polar boundary values: y-edges of p1, x-edges of p2 */
+        for (int k = 1; k <= nz + 1; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx + 1; i++)
+                    if (k == 1 || k == nz + 1 || i == 1 || i == nx + 1)
+                        p1[EY1(i, j, k) - 1] = (k == 1) ? 0 : e_x[k - 1];
+        for (int k = 1; k <= nz + 1; k++)
+            for (int j = 1; j <= ny + 1; j++)
+                for (int i = 1; i <= nx; i++)
+                    if (k == 1 || k == nz + 1 || j == 1 || j == ny + 1)
+                        p2[EX1(i, j, k) - 1] = (k == 1) ? 0 : e_x[nz + k];
+    }
 
     /* boundary elimination and CSR compaction, src/fdfe_main.f90:496-530 */
     int64_t kk = 0;
@@ -916,7 +963,27 @@ static void mat4mul(zc out[4][4], zc A[4][4], zc B[4][4])
     memcpy(out, t, sizeof(t));
 }
 
+static void mt1d_core(int nz, const double *conv, const double *c, double freq, zc *ex, zc *hx, zc *imp,
+                      zc *exc);
+
 void orc_mt1d_anl(int nz, const double *conv, const double *c, double freq, zc *ex, zc *hx, zc *imp)
+{
+    mt1d_core(nz, conv, c, freq, ex, hx, imp, NULL);
+}
+
+/* The same field with the cross-polar components as well (full-tensor
+ * boundary): exc[0..nz] = Ey of the xy mode, exc[nz+1..2nz+1] = Ex of the yx
+ * mode. */
+void orc_mt1d_anl_full(int nz, const double *conv, const double *c, double freq, zc *ex, zc *exc)
+{
+    zc *hx = calloc((size_t)(2 * nz + 2), sizeof(zc));
+    zc imp[4];
+    mt1d_core(nz, conv, c, freq, ex, hx, imp, exc);
+    free(hx);
+}
+
+static void mt1d_core(int nz, const double *conv, const double *c, double freq, zc *ex, zc *hx, zc *imp,
+                      zc *exc)
 {
     const double w = 2 * PI * freq;
     zc (*sprod)[4][4] = calloc((size_t)(nz + 1), sizeof(zc[4][4]));
@@ -975,8 +1042,8 @@ void orc_mt1d_anl(int nz, const double *conv, const double *c, double freq, zc *
                 f[r] = 0;
                 for (int q = 0; q < 4; q++) f[r] += sprod[i - 1][r][q] * mc[q];
             }
-            if (mode == 0) { ex[i - 1] = f[0]; hx[i - 1] = f[3]; }
-            else { ex[i + nz] = f[1]; hx[i + nz] = f[2]; }
+            if (mode == 0) { ex[i - 1] = f[0]; hx[i - 1] = f[3]; if (exc) exc[i - 1] = f[1]; }
+            else { ex[i + nz] = f[1]; hx[i + nz] = f[2]; if (exc) exc[i + nz] = f[0]; }
             if (i == nz + 1)
                 for (int r = 0; r < 4; r++) atmp[mode * 4 + r] = f[r];
         }

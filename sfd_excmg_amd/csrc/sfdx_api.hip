@@ -770,11 +770,11 @@ KOpts kopts_of(const sfdx_opts &o)
 // Right-hand-side norm of a level: ||b_int|| (unscaled, = ||A_ib g / ve||) and
 // ||g||, for the chosen residual convention.
 double rhs_norm(sfdx_ctx *ctx, LevelDev<double> &D, const std::string &tag, const zd *edc, int pol, int norm_mode,
-                double wmu)
+                double wmu, int full)
 {
     const Dims &d = D.g.d;
     zd *g = pool_vec<zd>(ctx, tag + "/bn_g", d.NL), *r = pool_vec<zd>(ctx, tag + "/bn_r", d.NL);
-    launch_dirichlet<double>(d, edc, pol, g, 1, ctx->st);
+    launch_dirichlet<double>(d, edc, pol, g, 1, ctx->st, full);
     launch_spmv<double>(D.g, wmu, g, r, -1.0, 0, nullptr, ctx->rb, ctx->st);
     SliceLaunch L = edge_launch(d);
     launch_vec<double>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
@@ -802,8 +802,8 @@ LevelSolveResult solve_level(sfdx_ctx *ctx, const sfdx_opts &o, LevelDev<double>
 {
     const Dims &d = D64.g.d;
     const double wmu = 2 * 3.141592653589793 * freq * kMu0;
-    launch_dirichlet<double>(d, edc, pol, x, 0, ctx->st);
-    const double bnorm = rhs_norm(ctx, D64, tag, edc, pol, o.norm, wmu);
+    launch_dirichlet<double>(d, edc, pol, x, 0, ctx->st, o.full_background);
+    const double bnorm = rhs_norm(ctx, D64, tag, edc, pol, o.norm, wmu, o.full_background);
     KOpts ko = kopts_of(o);
     ko.mg64 = h64;
     ko.mg32 = h32;
@@ -1021,7 +1021,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         if (o->use_excmg && !coarse_dev) {
             // levels 1 and 2: direct solves (u1u2, src/fdfe_main.f90:1228-1256)
             for (int i = 1; i <= 2; i++) {
-                const std::vector<zc> edc = background_field(lev[i], ff);
+                const std::vector<zc> edc = background_field(lev[i], ff, o->full_background != 0);
                 CoarseSystem CS;
                 coarse_system(lev[i], ff, edc, CS);
                 std::vector<zc> sol;
@@ -1033,7 +1033,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                         int ii, jj, kk;
                         int oo = edge_decode(d, e, ii, jj, kk);
                         if (CS.imap[e] >= 0) full[e] = sol[(size_t)pol * CS.n + CS.imap[e]];
-                        else dirichlet_value(d, edc.data(), pol, oo, ii, jj, kk, full[e]);
+                        else dirichlet_value(d, edc.data(), pol, oo, ii, jj, kk, full[e], o->full_background != 0);
                     }
                     u[i][pol] = ubuf(i, pol);
                     SFDX_CHECK(hipMemcpyAsync(u[i][pol], full.data(), d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
@@ -1045,7 +1045,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         for (int i = first_dev; i <= Lv; i++) {
             tol *= 10;                                   // level tolerance eps*10^(i-L) (:1331)
             if (!o->use_excmg) tol = o->eps;
-            const std::vector<zc> edc_h = background_field(lev[i], ff);
+            const std::vector<zc> edc_h = background_field(lev[i], ff, o->full_background != 0);
             const Dims d = make_dims(lev[i].nx, lev[i].ny, lev[i].nz);
             zd *edc = pool_vec<zd>(ctx, "edc" + std::to_string(i), edc_h.size());
             SFDX_CHECK(hipMemcpyAsync(edc, edc_h.data(), edc_h.size() * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
@@ -1239,6 +1239,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->use_graph = 1;
     o->nodal_smoother = 1;
     o->coarse_solver = 0;
+    o->full_background = 0;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)

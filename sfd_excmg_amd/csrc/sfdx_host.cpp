@@ -39,7 +39,7 @@ static M4 mul4(const M4 &A, const M4 &B)
     return C;
 }
 
-void mt1d_anl(int nz, const double *conv, const double *c, double freq, zc *ex, zc *hx, zc *imp)
+void mt1d_anl(int nz, const double *conv, const double *c, double freq, zc *ex, zc *hx, zc *imp, zc *exc)
 {
     const double w = 2 * kPi * freq;
     const zc iwm(0.0, w * kMu0);
@@ -97,8 +97,8 @@ void mt1d_anl(int nz, const double *conv, const double *c, double freq, zc *ex, 
                 f[r] = 0;
                 for (int q = 0; q < 4; q++) f[r] += prod[n][r][q] * mc[q];
             }
-            if (mode == 0) { ex[n] = f[0]; if (hx) hx[n] = f[3]; }
-            else { ex[nz + 1 + n] = f[1]; if (hx) hx[nz + 1 + n] = f[2]; }
+            if (mode == 0) { ex[n] = f[0]; if (hx) hx[n] = f[3]; if (exc) exc[n] = f[1]; }
+            else { ex[nz + 1 + n] = f[1]; if (hx) hx[nz + 1 + n] = f[2]; if (exc) exc[nz + 1 + n] = f[0]; }
             if (n == nz)
                 for (int r = 0; r < 4; r++) bottom[4 * mode + r] = f[r];
         }
@@ -113,15 +113,19 @@ void mt1d_anl(int nz, const double *conv, const double *c, double freq, zc *ex, 
 }
 
 // Background column of a model: cell (1,1) of every layer, diagonal only
-// (sfd_excmg/src/fdfe_main.f90:938-942).
-std::vector<zc> background_field(const HostLevel &L, double freq)
+// (sfd_excmg/src/fdfe_main.f90:938-942): 2*nz+2 values.  With full, the whole
+// tensor of that column, and the cross-polar components appended (4*nz+4
+// values, the layout dirichlet_value reads with full).
+std::vector<zc> background_field(const HostLevel &L, double freq, bool full)
 {
     const int nz = L.nz;
     std::vector<double> conv((size_t)6 * nz, 0.0);
+    const int ncomp = full ? 6 : 3;
     for (int k = 0; k < nz; k++)
-        for (int q = 0; q < 3; q++) conv[(size_t)q * nz + k] = L.sig[(size_t)q * L.ncell() + (size_t)k * L.nx * L.ny];
-    std::vector<zc> e(2 * nz + 2);
-    mt1d_anl(nz, conv.data(), L.c.data(), freq, e.data(), nullptr, nullptr);
+        for (int q = 0; q < ncomp; q++)
+            conv[(size_t)q * nz + k] = L.sig[(size_t)q * L.ncell() + (size_t)k * L.nx * L.ny];
+    std::vector<zc> e(full ? 4 * nz + 4 : 2 * nz + 2);
+    mt1d_anl(nz, conv.data(), L.c.data(), freq, e.data(), nullptr, nullptr, full ? e.data() + 2 * nz + 2 : nullptr);
     return e;
 }
 
@@ -225,6 +229,7 @@ void coarse_system(const HostLevel &L, double freq, const std::vector<zc> &edc, 
 {
     const Dims d = make_dims(L.nx, L.ny, L.nz);
     const double wmu = 2 * kPi * freq * kMu0;
+    const bool full = edc.size() == (size_t)(4 * L.nz + 4);  // full-tensor boundary data
     S.imap.assign(d.NL, -1);
     S.inner.clear();
     for (int64_t e = 0; e < d.NL; e++) {
@@ -255,7 +260,7 @@ void coarse_system(const HostLevel &L, double freq, const std::vector<zc> &edc, 
                 int oo = edge_decode(d, nb, ii, jj, kk);
                 for (int pol = 0; pol < 2; pol++) {
                     zc g(0);
-                    dirichlet_value(d, edc.data(), pol, oo, ii, jj, kk, g);
+                    dirichlet_value(d, edc.data(), pol, oo, ii, jj, kk, g, full);
                     S.rhs[(size_t)pol * n + r] -= cf * g;
                 }
             }

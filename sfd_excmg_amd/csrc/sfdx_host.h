@@ -39,8 +39,8 @@ struct PostGrid {
 };
 
 void mt1d_anl(int nz, const double *conv, const double *c, double freq, std::complex<double> *ex,
-              std::complex<double> *hx, std::complex<double> *imp);
-std::vector<std::complex<double>> background_field(const HostLevel &L, double freq);
+              std::complex<double> *hx, std::complex<double> *imp, std::complex<double> *exc = nullptr);
+std::vector<std::complex<double>> background_field(const HostLevel &L, double freq, bool full = false);
 HostLevel coarsen(const HostLevel &F, int k);
 void prolong_tables(const HostLevel &fine, int m, ProlongHost &T);
 void coarse_system(const HostLevel &L, double freq, const std::vector<std::complex<double>> &edc, CoarseSystem &S);

@@ -186,7 +186,8 @@ void launch_add_lo(int64_t n, zd *x, const zf *d, hipStream_t st);
 template <typename R>
 void launch_axpby(int64_t n, cx<R> al, const cx<R> *u, cx<R> be, cx<R> *v, hipStream_t st);
 template <typename R>
-void launch_dirichlet(const Dims &d, const cx<R> *edc, int pol, cx<R> *x, int zero_inner, hipStream_t st);
+void launch_dirichlet(const Dims &d, const cx<R> *edc, int pol, cx<R> *x, int zero_inner, hipStream_t st,
+                      int full = 0);  // full: edc holds the full-tensor boundary data (4 nz + 4)
 template <typename R>
 void launch_node_div(const DevGrid<R> &g, const cx<R> *w, cx<R> *t, RedBuf &rb, hipStream_t st, int64_t &nblk);
 template <typename R>

@@ -989,7 +989,7 @@ __global__ __launch_bounds__(TPB) void k_axpby(int64_t n, cx<R> al, const cx<R> 
 // x_e = g_e on boundary edges; interior edges untouched (or zeroed if zero_inner).
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_dirichlet(Dims d, const cx<R> *__restrict__ edc, int pol,
-                                                   cx<R> *__restrict__ x, int zero_inner)
+                                                   cx<R> *__restrict__ x, int zero_inner, int full)
 {
     const Tile TL = tile_id();
     const Slice S = edge_slice(d, TL.y);
@@ -1002,7 +1002,7 @@ __global__ __launch_bounds__(TPB) void k_dirichlet(Dims d, const cx<R> *__restri
         return;
     }
     cx<R> v(0);
-    dirichlet_value(d, edc, pol, S.o, i, j, S.k, v);
+    dirichlet_value(d, edc, pol, S.o, i, j, S.k, v, full != 0);
     x[e] = v;
 }
 
@@ -1719,10 +1719,10 @@ void launch_axpby(int64_t n, cx<R> al, const cx<R> *u, cx<R> be, cx<R> *v, hipSt
 }
 
 template <typename R>
-void launch_dirichlet(const Dims &d, const cx<R> *edc, int pol, cx<R> *x, int zero_inner, hipStream_t st)
+void launch_dirichlet(const Dims &d, const cx<R> *edc, int pol, cx<R> *x, int zero_inner, hipStream_t st, int full)
 {
     SliceLaunch L = edge_launch(d);
-    hipLaunchKernelGGL((k_dirichlet<R>), L.grid, L.block, 0, st, d, edc, pol, x, zero_inner);
+    hipLaunchKernelGGL((k_dirichlet<R>), L.grid, L.block, 0, st, d, edc, pol, x, zero_inner, full);
 }
 
 template <typename R>
@@ -2065,7 +2065,7 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
     template void launch_dot<R>(int, int64_t, const cx<R> *, const cx<R> *, RedBuf &, hipStream_t, int64_t &);    \
     template void launch_copy<R>(int64_t, const cx<R> *, cx<R> *, hipStream_t);                                   \
     template void launch_axpby<R>(int64_t, cx<R>, const cx<R> *, cx<R>, cx<R> *, hipStream_t);                    \
-    template void launch_dirichlet<R>(const Dims &, const cx<R> *, int, cx<R> *, int, hipStream_t);              \
+    template void launch_dirichlet<R>(const Dims &, const cx<R> *, int, cx<R> *, int, hipStream_t, int);         \
     template void launch_node_div<R>(const DevGrid<R> &, const cx<R> *, cx<R> *, RedBuf &, hipStream_t,          \
                                      int64_t &);                                                                  \
     template void launch_pbuild<R>(const Dims &, const double *, const double *, const double *, const double *, \

@@ -236,8 +236,14 @@ SFDX_HD bool node_boundary(const Dims &d, int i, int j, int k)
 // Dirichlet value of boundary edge (o,i,j,k) for polarisation pol (0 = xy,
 // 1 = yx) from the 1-D field e_dc[2*nz+2] (Ex at z-nodes, then Ey), following
 // dcfast (sfd_excmg/src/fdfe_main.f90:470-494).  Returns false for 0.
+// With full (full-tensor boundary, a deliberate deviation: the reference sets
+// the cross-polar components to 0), e_dc holds 4*nz+4 values and the
+// boundary y-edges of xy and x-edges of yx take the cross-polar components
+// e_dc[2nz+2+k] (Ey of the xy mode) and e_dc[3nz+3+k] (Ex of the yx mode),
+// 0 on the top plane where the modes are normalised to E = (1,0), (0,1).
 template <typename C>
-SFDX_HD bool dirichlet_value(const Dims &d, const C *e_dc, int pol, int o, int i, int j, int k, C &v)
+SFDX_HD bool dirichlet_value(const Dims &d, const C *e_dc, int pol, int o, int i, int j, int k, C &v,
+                             bool full = false)
 {
     if (pol == 0 && o == 0) {
         if (k == 0) { v = C(1.0); return true; }
@@ -248,6 +254,17 @@ SFDX_HD bool dirichlet_value(const Dims &d, const C *e_dc, int pol, int o, int i
         if (k == 0) { v = C(1.0); return true; }
         if (k == d.nz) { v = e_dc[2 * d.nz + 1]; return true; }
         if (i == 0 || i == d.nx) { v = e_dc[d.nz + 1 + k]; return true; }
+    }
+    if (full) {
+        const int n1 = d.nz + 1;
+        if (pol == 0 && o == 1 && (k == 0 || k == d.nz || i == 0 || i == d.nx)) {
+            v = (k == 0) ? C(0.0) : e_dc[2 * n1 + k];
+            return true;
+        }
+        if (pol == 1 && o == 0 && (k == 0 || k == d.nz || j == 0 || j == d.ny)) {
+            v = (k == 0) ? C(0.0) : e_dc[3 * n1 + k];
+            return true;
+        }
     }
     return false;
 }

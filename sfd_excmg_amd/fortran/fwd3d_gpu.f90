@@ -14,6 +14,7 @@
 ! (fp64|mixed), SFDX_NORM (reference|interior), SFDX_SOLVER (overrides slv:
 ! 1 BiCGStab, 3 GPBiCG, 5 COCR), SFDX_PRECOND (0 SSOR, 1 hybrid, 2 multigrid),
 ! SFDX_COARSE (0 direct solves of EXCMG levels 1-2, 1 device Krylov solves),
+! SFDX_FULL_BG (1: full-tensor 1-D boundary with cross-polar components),
 ! SFDX_VERBOSE, SFDX_RES_FULL (also write <name>.resx with 17 significant digits).
 module fwd3d
   use iso_c_binding
@@ -88,6 +89,7 @@ contains
     o%verbose = env_int('SFDX_VERBOSE', 0)
     o%precond = env_int('SFDX_PRECOND', int(o%precond))
     o%coarse_solver = env_int('SFDX_COARSE', 0)
+    o%full_background = env_int('SFDX_FULL_BG', 0)
     rc = sfdx_fwd_solve(sfdx_handle, g, sigma, int(nf, c_int), freq, o, ex, st)
     if (rc > 0) then
       write(*,*) 'Error in the GPU forward solve: ', sfdx_error_text()

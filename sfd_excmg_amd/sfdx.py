@@ -49,7 +49,8 @@ class Opts(ctypes.Structure):
                 ("dc_at_start", ctypes.c_int), ("inner_maxit", ctypes.c_int), ("precond", ctypes.c_int),
                 ("sigma_floor32", ctypes.c_double), ("mg_levels", ctypes.c_int), ("mg_coarse_pairs", ctypes.c_int),
                 ("restart_drop", ctypes.c_double), ("use_graph", ctypes.c_int),
-                ("nodal_smoother", ctypes.c_int), ("coarse_solver", ctypes.c_int)]
+                ("nodal_smoother", ctypes.c_int), ("coarse_solver", ctypes.c_int),
+                ("full_background", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):

@@ -202,6 +202,24 @@ def test_device_coarse_levels_match_direct_solves(ctx, precision):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
+def test_full_tensor_boundary_matches_oracle(ctx):
+    """Full-tensor 1-D boundary (cross-polar Dirichlet components, a deliberate
+    deviation from the reference's diagonal-only background): GPU impedances
+    equal the oracle's with the same boundary to 1e-6, and differ from the
+    reference boundary's on a model whose first column is anisotropic."""
+    m = small_excmg_model(True)
+    kw = dict(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR)
+    exf, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(full_background=1, **kw))
+    assert st.status == 0
+    exr, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(**kw))
+    oopt = O.Options(solver=O.SOLVER_COCR, corr_intv=50, norm="interior", full_background=True)
+    exo, _ = O.fwd_solver(m.a, m.b, m.c, m.nair, 3, 1e-11, m.sigma, m.freqs, oopt)
+    zf, zo, zr = _impedances(m, exf), _impedances(m, exo), _impedances(m, exr)
+    for p, q in zip(zf, zo):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+    assert max(np.max(np.abs(p - q) / np.abs(q)) for p, q in zip(zf, zr)) > 1e-4
+
+
 def test_halfspace_on_gpu(ctx):
     m = synth.make_block_halfspace_32()
     m.sigma[m.nair * m.nx * m.ny:, :3] = 0.01

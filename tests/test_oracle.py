@@ -92,6 +92,61 @@ def test_stencil_matches_reference_formulas_on_one_row():
     assert np.isclose(col[ey + 1], -gy - kap * sv1, rtol=1e-14)
 
 
+def _layered_rotated_model(nx=6, ny=6, nair=2):
+    """Horizontally layered earth whose layers carry tensors rotated about z
+    (azimuthal anisotropy: xx, yy, xy nonzero, xz = yz = 0), under air."""
+    ce = np.array([200.0, 300, 500, 800, 1500, 3000])
+    c = synth.with_air(ce, nair)
+    nz = len(c)
+    layers = [(0.01, 0.002, 0.01, 30), (0.05, 0.01, 0.05, -50), (0.003, 0.03, 0.003, 70)]
+    sig = np.zeros((nx * ny * nz, 6))
+    for k in range(nz):
+        if k < nair:
+            cols = np.array([1e-8, 1e-8, 1e-8, 0, 0, 0])
+        else:
+            sx, sy, sz, strike = layers[min((k - nair) // 2, 2)]
+            cols = synth.tensor_to_cols(synth.euler_tensor(sx, sy, sz, np.deg2rad(strike), 0.0, 0.0))
+        sig[k * nx * ny:(k + 1) * nx * ny] = cols
+    return np.full(nx, 1000.0), np.full(ny, 1000.0), c, sig
+
+
+@pytest.mark.parametrize("pol", [0, 1])
+def test_full_tensor_boundary_reproduces_1d_field_in_layered_anisotropic_model(pol):
+    """Full-tensor 1-D boundary (a deliberate deviation from the reference's
+    diagonal-only background, src/fdfe_main.f90:938-942, 470-494): in a layered
+    model with rotated tensors both horizontal components of the 3-D solution
+    follow the 1-D field everywhere inside the grid (to the discretisation
+    error), while the reference boundary, which zeroes the cross-polar
+    component on the faces, leaves an O(1) error in it."""
+    a, b, c, sig = _layered_rotated_model()
+    nx, ny, nz = len(a), len(b), len(c)
+    f = 1.0
+    conv = O.background_column(sig, nx, ny, nz, True)
+    ex, exc = O.mt1d_anl_full(f, conv, c)
+    assert abs(exc[0]) < 1e-12 and abs(exc[nz + 1]) < 1e-12  # top normalisation E = (1,0), (0,1)
+    co = ex[: nz + 1] if pol == 0 else ex[nz + 1:]            # co-polar 1-D component at the z-nodes
+    cr = exc[: nz + 1] if pol == 0 else exc[nz + 1:]          # cross-polar
+    assert np.abs(cr).max() > 0.05 * np.abs(co).max()        # the rotation couples the modes
+    xnl = nx * (ny + 1) * (nz + 1)
+    ynl = (nx + 1) * ny * (nz + 1)
+
+    def errors(full):
+        S = O.build_system(a, b, c, sig, f, full=full)
+        x = spla.spsolve(S.A.tocsc(), S.p1 if pol == 0 else S.p2)
+        exs = x[:xnl].reshape(nz + 1, ny + 1, nx)[1:nz, 1:ny, :]          # interior x-edges
+        eys = x[xnl:xnl + ynl].reshape(nz + 1, ny, nx + 1)[1:nz, :, 1:nx]  # interior y-edges
+        ref_x, ref_y = (co, cr) if pol == 0 else (cr, co)
+        scale = np.abs(co).max()
+        return (np.abs(exs - ref_x[1:nz, None, None]).max() / scale,
+                np.abs(eys - ref_y[1:nz, None, None]).max() / scale)
+
+    ex_full, ey_full = errors(True)
+    ex_ref, ey_ref = errors(False)
+    cross_full, cross_ref = (ey_full, ey_ref) if pol == 0 else (ex_full, ex_ref)
+    assert max(ex_full, ey_full) < 0.02
+    assert cross_ref > 10 * cross_full
+
+
 def test_layered_model_equals_1d_fd_solution_and_converges_to_analytic():
     """In a horizontally layered isotropic model the xy-mode 3-D solution is
     Ex(z) only and equals the 3-point 1-D FD solution on the same z grid; the
