@@ -575,8 +575,9 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
 // reference (src/solvers.f90:630-646).
 template <typename R>
 SolveInfo bicgstab(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *f, cx<R> *x,
-                   double bnorm, double eps, int maxit, const KOpts &ko)
+                   double bnorm, double eps, int maxit, const KOpts &ko_in)
 {
+    KOpts ko = ko_in;  // the complex64 preconditioner may be dropped below
     const Dims &d = D.g.d;
     const R omega = (R)ko.omega;
     const std::string p = tag + (sizeof(R) == 8 ? "/b64" : "/b32");
@@ -594,6 +595,15 @@ SolveInfo bicgstab(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu,
         info.dc_calls++;
     }
     int state = 0;  // 0 fresh, 1 residual refresh keeping p, v and the scalars, 2 iterate
+    // With a complex64 preconditioner the recursive residual drifts and the
+    // iteration can stall: as in cocr, restart from the true residual after
+    // each restart_drop reduction, on stagnation (200 iterations without a new
+    // best) or divergence (100x above the best), accept convergence only on a
+    // recomputed residual, and after 3 unproductive restarts continue with the
+    // complex128 preconditioner.
+    bool fresh = false;
+    double rel0 = 0, best = 1e300, best_at_restart = 1e300;
+    int since_best = 0, idle_restarts = 0;
     while (true) {
         if (state != 2) {
             launch_spmv<R>(D.g, wmu, x, r, (R)-1, 0, nullptr, ctx->rb, ctx->st);
@@ -608,12 +618,42 @@ SolveInfo bicgstab(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu,
             reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
             read_slots(ctx);
             state = 2;
+            fresh = true;
         }
         info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
         if (!(info.relres == info.relres) || std::isinf(info.relres)) { info.status = SFDX_ERR_NAN; break; }
         if (ko.verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  bicgstab it %d relres %.3e\n", info.iters, info.relres);
-        if (info.relres < eps) { info.status = 0; break; }
+        const bool lo32 = ko.lo || ko.mg32;
+        const bool fresh_now = fresh;
+        if (fresh) { rel0 = info.relres; best = info.relres; since_best = 0; fresh = false; }
+        if (info.relres < best) { best = info.relres; since_best = 0; }
+        if (info.relres < eps) {
+            if (lo32 && !fresh_now) { state = 0; continue; }  // confirm on the true residual
+            info.status = 0;
+            break;
+        }
         if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
+        if (lo32) {
+            const bool stalled = since_best > 200, diverging = info.relres > 100 * best;
+            const bool planned = ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0;
+            if (stalled || diverging || planned) {
+                if (!planned) {
+                    idle_restarts = (best < 0.5 * best_at_restart) ? 0 : idle_restarts + 1;
+                    if (idle_restarts >= 3) {
+                        ko.lo = nullptr;
+                        ko.mg32 = nullptr;
+                        ko.restart_drop = 0;
+                        info.fallback = 1;
+                        idle_restarts = 0;
+                    }
+                }
+                best_at_restart = std::min(best_at_restart, best);
+                state = 0;
+                info.restarts++;
+                continue;
+            }
+        }
+        since_best++;
         if (ctx->hslots[SL_BREAK] != 0) { info.status = SFDX_ERR_BREAKDOWN; break; }
         if (ko.corr_intv > 0 && info.iters > 0 && info.iters % ko.corr_intv == 0 &&
             info.dc_calls < info.iters / ko.corr_intv + (ko.dc_at_start ? 1 : 0)) {
@@ -1631,7 +1671,8 @@ int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, d
             ensure_P(ctx, D, "bench");
             cx<R> *tn = pool_vec<cx<R>>(ctx, std::string("bench/tn") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
             cx<R> *ph = pool_vec<cx<R>>(ctx, std::string("bench/ph") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
-            ms[5] = timed([&](int q) { launch_hybrid<R>(D.g, wmu, D.P, V(2 * q), V(2 * q + 1), tn, ph, (R)1, ctx->st); });
+            // (with the nodal Jacobi polynomial, the solver's default)
+            ms[5] = timed([&](int q) { launch_hybrid<R>(D.g, wmu, D.P, V(2 * q), V(2 * q + 1), tn, ph, (R)1, ctx->st, 1); });
             // the one-row-per-thread gather form of the SpMV, and the k-marching
             // form in its four tile variants (default chunk) and the 32x8 tile
             // over 4-plane chunks

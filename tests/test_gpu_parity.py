@@ -171,6 +171,20 @@ def test_fwd_solve_mixed_matches_fp64(ctx, precond):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
+@pytest.mark.parametrize("solver", [sfdx.SOLVER_BICGSTAB, sfdx.SOLVER_GPBICG])
+def test_fwd_solve_mixed_bicg_family_matches_fp64(ctx, solver):
+    """BiCGStab and GPBiCG with the complex64 preconditioner (restarts from the
+    true residual, convergence confirmed on it): same impedances as FP64."""
+    m = small_excmg_model(True)
+    kw = dict(level=3, solver=solver, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR)
+    ex64, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(**kw))
+    exmx, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs,
+                             sfdx.default_opts(precision=sfdx.PREC_MIXED, restart_drop=1e-3, sigma_floor32=0.0, **kw))
+    assert st.status == 0
+    for p, q in zip(_impedances(m, exmx), _impedances(m, ex64)):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+
+
 @pytest.mark.parametrize("use_excmg,mg_levels", [(1, 0), (0, 2), (0, 0)])
 def test_fwd_solve_multigrid_matches_oracle(ctx, use_excmg, mg_levels):
     """Multigrid-preconditioned COCR (split edge/nodal V-cycle) reaches the
