@@ -362,11 +362,8 @@ bool apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
             LevelDev<double> &D64 = *(LevelDev<double> *)&D;
             ensure_P(ctx, D64, tag);
             zd *tn64 = pool_vec<zd>(ctx, tag + "/mp_t64", d.NP), *ph64 = pool_vec<zd>(ctx, tag + "/mp_phi64", d.NP);
-            launch_convert<double, float>(d.NL, (const zd *)in, in32, ctx->st);
-            launch_ssor<float>(F.g, (float)wmu, in32, z32, (float)ko.omega, ctx->st);
-            launch_convert<float, double>(d.NL, z32, (zd *)out, ctx->st);
-            launch_nodal_correction(D64.g, (double)wmu, D64.P, (const zd *)in, (zd *)out, tn64, ph64, ctx->st,
-                                    (const zd *)dotw, ctx->rb.partials);
+            launch_hybrid_edge32(D64.g, F.g, (double)wmu, D64.P, (const zd *)in, (zd *)out, in32, z32, tn64, ph64,
+                                 (float)ko.omega, ctx->st, (const zd *)dotw, ctx->rb.partials);
             return dotw != nullptr;
         }
         if (ko.precond == SFDX_PRECOND_HYBRID) {

@@ -1315,11 +1315,12 @@ __global__ __launch_bounds__(TPB) void k_grad(Dims d, const R *__restrict__ a, c
 }
 
 // Mixed-precision recombination of the hybrid preconditioner:
-// z64_e = z32_e + s * (G phi32)_e, with G applied in FP64 so that later FP64
-// operator applications see the gradient part exactly.
-template <bool DOT = false>
+// z64_e = z32_e + s * (G phi)_e (phi in complex64 or complex128), with G
+// applied in FP64 so that later FP64 operator applications see the gradient
+// part exactly; the complex64 SSOR output z32 is widened here, in the same pass.
+template <typename RP, bool DOT = false>
 __global__ __launch_bounds__(TPB) void k_hyb_combine(Dims d, const double *__restrict__ a, const double *__restrict__ b,
-                                                     const double *__restrict__ c, zd s, const zf *__restrict__ phi,
+                                                     const double *__restrict__ c, zd s, const cx<RP> *__restrict__ phi,
                                                      const zf *__restrict__ z32, zd *__restrict__ z64,
                                                      const zd *__restrict__ w, double *__restrict__ partials)
 {
@@ -1332,7 +1333,8 @@ __global__ __launch_bounds__(TPB) void k_hyb_combine(Dims d, const double *__res
         const int64_t e = S.base + p;
         int64_t n0, n1;
         const double l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
-        const zf f1 = phi[n1], f0 = phi[n0], zz = z32[e];
+        const cx<RP> f1 = phi[n1], f0 = phi[n0];
+        const zf zz = z32[e];
         const zd g(((double)f1.re - (double)f0.re) / l, ((double)f1.im - (double)f0.im) / l);
         const zd zn = fma_c(zd((double)zz.re, (double)zz.im), s, g);
         z64[e] = zn;
@@ -1883,20 +1885,26 @@ void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R
                            (const cx<R> *)nullptr, (double *)nullptr);
 }
 
-// out += s * G N_P(G^T in) with the one-pass Jacobi nodal smoother, complex128
-// (the nodal half of the hybrid preconditioner on its own), optional fused dot
-void launch_nodal_correction(const DevGrid<double> &g, double wmu, const double *P, const zd *in, zd *out, zd *tnode,
-                             zd *phi, hipStream_t st, const zd *dotw, double *partials)
+// Hybrid preconditioner with the edge SSOR in complex64 and the nodal part
+// (one Jacobi pass over P, where the air's large potentials live) in
+// complex128: out = SSOR32(in32) + s G N_P(G^T in), in32 = complex64 copy of
+// in; the complex64 SSOR output is widened inside the recombination pass
+// (optional fused unconjugated dot with dotw).
+void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const double *P64,
+                          const zd *in, zd *out, zf *in32, zf *z32, zd *tnode, zd *phi, float omega,
+                          hipStream_t st, const zd *dotw, double *partials)
 {
-    nodal_part<double>(g, P, in, tnode, phi, 1, st);
+    launch_convert<double, float>(g64.d.NL, in, in32, st);
+    launch_ssor<float>(g32, (float)wmu, in32, z32, omega, st);
+    nodal_part<double>(g64, P64, in, tnode, phi, 1, st);
     const zd s(0.0, 1.0 / wmu);
-    SliceLaunch Le = edge_launch(g.d);
+    SliceLaunch Le = edge_launch(g64.d);
     if (dotw)
-        hipLaunchKernelGGL((k_grad_add<double, true>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out, dotw,
-                           partials);
+        hipLaunchKernelGGL((k_hyb_combine<double, true>), Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi,
+                           z32, out, dotw, partials);
     else
-        hipLaunchKernelGGL((k_grad_add<double, false>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out,
-                           (const zd *)nullptr, (double *)nullptr);
+        hipLaunchKernelGGL((k_hyb_combine<double, false>), Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s,
+                           phi, z32, out, (const zd *)nullptr, (double *)nullptr);
 }
 
 // out64 = SSOR32(in) + s * G SSOR_P32(G^T in) with the FP32 pieces on a
@@ -1912,10 +1920,10 @@ void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, 
     const zd s(0.0, 1.0 / wmu);
     SliceLaunch Le = edge_launch(g64.d);
     if (dotw)
-        hipLaunchKernelGGL(k_hyb_combine<true>, Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, out,
+        hipLaunchKernelGGL((k_hyb_combine<float, true>), Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, out,
                            dotw, partials);
     else
-        hipLaunchKernelGGL(k_hyb_combine<false>, Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, out,
+        hipLaunchKernelGGL((k_hyb_combine<float, false>), Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, out,
                            (const zd *)nullptr, (double *)nullptr);
 }
 
@@ -2036,8 +2044,8 @@ void launch_grad_add(const DevGrid<R> &g, cx<R> s, const cx<R> *phi, cx<R> *z, h
 void launch_hyb_combine(const DevGrid<double> &g64, zd s, const zf *phi, const zf *z32, zd *z64, hipStream_t st)
 {
     SliceLaunch L = edge_launch(g64.d);
-    hipLaunchKernelGGL(k_hyb_combine<false>, L.grid, L.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32, z64,
-                       (const zd *)nullptr, (double *)nullptr);
+    hipLaunchKernelGGL((k_hyb_combine<float, false>), L.grid, L.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32,
+                       z64, (const zd *)nullptr, (double *)nullptr);
 }
 
 template <typename R>
