@@ -348,7 +348,7 @@ def main():
         "max_rel_impedance_err_mixed_vs_fp64": zerr,
         "fine_solve_hbm_gbs_algorithmic": {"mixed": round(mx["fine_bytes"] / mx["fine"] / 1e9, 1) if mx["fine"] else None,
                                           "fp64": round(f64["fine_bytes"] / f64["fine"] / 1e9, 1) if f64["fine"] else None},
-        "roofline": {"kernel": "k_spmv_march<double,0,4,true> (complex128 k-marching stencil, 32x4 tile)", "bound": "hbm",
+        "roofline": {"kernel": "k_spmv_march<double,0,8,true> (complex128 k-marching stencil, 32x8 tile)", "bound": "hbm",
                      "achieved": round(ach64, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),
                      "traffic": None if args.small else profiled_traffic("k_spmv_march<double,0>"),

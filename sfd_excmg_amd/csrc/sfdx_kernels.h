@@ -168,7 +168,7 @@ int64_t launch_spmv_march(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, 
 template <typename R>
 int64_t launch_spmv_gather(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode,
                            const cx<R> *w, RedBuf &rb, hipStream_t st);
-int spmv_chunk(const Dims &d, int ty);
+int spmv_chunk(const Dims &d, int ty, int64_t slots);
 template <typename R>
 void launch_mass(const DevGrid<R> &g, const cx<R> *x, cx<R> *w, hipStream_t st);
 template <typename R>

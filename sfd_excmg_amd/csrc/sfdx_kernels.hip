@@ -23,6 +23,7 @@
 #include "sfdx_kernels.h"
 
 #include <stdexcept>
+#include <cstdint>
 
 namespace sfdx {
 
@@ -1543,16 +1544,28 @@ void reduce_final(RedBuf &rb, int64_t nblk, int K, int dst, int op, hipStream_t 
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, st, rb.partials, nblk, K, rb.slots, dst, op);
 }
 
-// Chunk of node planes per block of the k-marching stencil: at least 4 steps
-// (the first step of a chunk re-loads one plane), and enough chunks that the
-// grid fills the 256 CUs about three blocks deep.
-int spmv_chunk(const Dims &d, int ty)
+// Chunk of node planes per block of the k-marching stencil.  A block re-loads
+// one plane at the start of its chunk, so long chunks are cheaper, but the grid
+// should fill the device in whole rounds: with `slots` blocks resident at once,
+// the time goes as rounds * (kc + 1).  kc is taken in [4, 16] minimising that
+// (the longer chunk on ties); slots <= 0 falls back to ~1000 blocks.
+int spmv_chunk(const Dims &d, int ty, int64_t slots)
 {
     const int64_t ntile = (int64_t)((d.nx + 31) / 32) * ((d.ny + ty - 1) / ty);
     const int np = d.nz + 1;
-    int kc = 16;
-    while (kc > 4 && ntile * ((np + kc - 1) / kc) < 1024) kc /= 2;
-    return kc;
+    if (slots <= 0) {
+        int kc = 16;
+        while (kc > 4 && ntile * ((np + kc - 1) / kc) < 1024) kc /= 2;
+        return kc;
+    }
+    int best = 16;
+    int64_t best_cost = INT64_MAX;
+    for (int kc = 16; kc >= 4; kc--) {
+        const int64_t nblk = ntile * ((np + kc - 1) / kc);
+        const int64_t cost = ((nblk + slots - 1) / slots) * (kc + 1);
+        if (cost < best_cost) { best_cost = cost; best = kc; }
+    }
+    return best;
 }
 
 template <typename R, int TY, bool PF>
@@ -1561,7 +1574,20 @@ static int64_t march_go(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R 
 {
     using G = mt::Geo<TY>;
     const Dims &d = g.d;
-    if (kc <= 0) kc = spmv_chunk(d, TY);
+    if (kc <= 0) {
+        // blocks of this instance resident on the whole device (queried once)
+        static int64_t slots = -1;
+        if (slots < 0) {
+            int per_cu = 0, dev = 0, ncu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)&k_spmv_march<R, 0, TY, PF>, G::NT, 0) !=
+                    hipSuccess ||
+                hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+                per_cu = ncu = 0;
+            slots = (int64_t)per_cu * ncu;
+        }
+        kc = spmv_chunk(d, TY, slots);
+    }
     const int ntx = (d.nx + G::TX - 1) / G::TX, nty = (d.ny + TY - 1) / TY;
     const unsigned gx = (unsigned)(ntx * nty), nch = (unsigned)((d.nz + kc) / kc);
     const dim3 grid(gx, pad_y(gx, nch), 1), block(G::NT, 1, 1);
@@ -1618,10 +1644,10 @@ template <typename R>
 int64_t launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode, const cx<R> *w,
                     RedBuf &rb, hipStream_t st)
 {
-    // tile chosen by the variant sweep of sfdx_bench_kernels on 128x128x96:
-    // 32x4 columns in complex128 (0.070 against 0.074 ms for 32x8), 32x8 in
-    // complex64 (0.045 against 0.048 ms)
-    return launch_spmv_march<R>(g, wmu, x, y, alpha, dot_mode, w, rb, st, 0, sizeof(R) == 8 ? 2 : 0);
+    // tile chosen by the variant sweep of sfdx_bench_kernels on 128x128x96
+    // (chunks filling whole rounds): 32x8 columns with prefetch, 0.058 ms in
+    // complex128 (32x4: 0.059) and 0.035 ms in complex64 (32x4: 0.037)
+    return launch_spmv_march<R>(g, wmu, x, y, alpha, dot_mode, w, rb, st, 0, 0);
 }
 
 // t = r - A z - M (G psi): residual of z + (i/wmu) G psi without forming A G psi
