@@ -331,7 +331,7 @@ template <typename R>
 // With dotw the partial sums of dotw^T out are produced by the last kernel when
 // the preconditioner allows it (hybrid); returns whether it did.
 bool apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *in, cx<R> *out,
-                const KOpts &ko, const cx<R> *dotw = nullptr)
+                const KOpts &ko, const cx<R> *dotw = nullptr, bool in32_ready = false)
 {
     if (ko.precond == SFDX_PRECOND_MG) {
         const Dims &d = D.g.d;
@@ -363,7 +363,7 @@ bool apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
             ensure_P(ctx, D64, tag);
             zd *tn64 = pool_vec<zd>(ctx, tag + "/mp_t64", d.NP), *ph64 = pool_vec<zd>(ctx, tag + "/mp_phi64", d.NP);
             launch_hybrid_edge32(D64.g, F.g, (double)wmu, D64.P, (const zd *)in, (zd *)out, in32, z32, tn64, ph64,
-                                 (float)ko.omega, ctx->st, (const zd *)dotw, ctx->rb.partials);
+                                 (float)ko.omega, ctx->st, (const zd *)dotw, ctx->rb.partials, !in32_ready);
             return dotw != nullptr;
         }
         if (ko.precond == SFDX_PRECOND_HYBRID) {
@@ -403,6 +403,14 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     cx<R> *r = pool_vec<cx<R>>(ctx, p + "r", d.NL), *z = pool_vec<cx<R>>(ctx, p + "z", d.NL);
     cx<R> *Az = pool_vec<cx<R>>(ctx, p + "Az", d.NL), *pv = pool_vec<cx<R>>(ctx, p + "p", d.NL);
     cx<R> *Ap = pool_vec<cx<R>>(ctx, p + "Ap", d.NL), *q = pool_vec<cx<R>>(ctx, p + "q", d.NL);
+    // With the complex64-edge hybrid preconditioner, the update of Ap also
+    // writes the complex64 copy the preconditioner reads (apply_prec's "/mp_in"
+    // buffer), so apply_prec(Ap) skips its conversion pass.  Re-evaluated at
+    // every use: the complex64 preconditioner may be dropped during the solve.
+    auto ap32_of = [&]() -> zf * {
+        if (sizeof(R) != 8 || !ko.lo || ko.precond != SFDX_PRECOND_HYBRID || ko.nodal != 1) return nullptr;
+        return pool_vec<zf>(ctx, tag + "/mp_in", d.NL);
+    };
     SliceLaunch L = edge_launch(d);
     const int64_t nbe = L.nblocks();
     const double sc = 2.0 * sizeof(R), vecb = (double)d.NL * sc;
@@ -449,10 +457,10 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             if (state == 0) {
                 reduce_final(ctx->rb, nbs, 2, SL_TMP, OP_COCR_RHO, ctx->st);  // also beta = 0
                 launch_copy<R>(d.NL, z, pv, ctx->st);
-                launch_copy<R>(d.NL, Az, Ap, ctx->st);
+                launch_cocr_ap<R>(d.NL, ctx->rb, Ap, Az, ap32_of(), true, ctx->st);
             } else {
                 reduce_final(ctx->rb, nbs, 2, SL_TMP, OP_COCR_BETA, ctx->st);
-                launch_vec<R>(7, D.g, ctx->rb, nullptr, nullptr, Ap, Az, nullptr, nullptr, ctx->st);
+                launch_cocr_ap<R>(d.NL, ctx->rb, Ap, Az, ap32_of(), false, ctx->st);
             }
             launch_vec<R>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
             reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
@@ -519,7 +527,9 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
         auto part1 = [&]() {
             int64_t nbd = nbe;
             // q = M Ap, with (Ap, q) fused into the preconditioner's last kernel when possible
-            if (!apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko, Ap)) launch_dot<R>(0, d.NL, Ap, q, ctx->rb, ctx->st, nbd);
+            zf *ap32 = ap32_of();
+            if (!apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko, Ap, ap32 != nullptr))
+                launch_dot<R>(0, d.NL, Ap, q, ctx->rb, ctx->st, nbd);
             reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
             launch_vec<R>(6, D.g, ctx->rb, x, pv, Ap, r, q, z, ctx->st);
             reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
@@ -527,7 +537,7 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
         auto part2 = [&]() {
             const int64_t nbs = launch_spmv<R>(D.g, wmu, z, Az, (R)1, 1, nullptr, ctx->rb, ctx->st);
             reduce_final(ctx->rb, nbs, 2, SL_TMP, OP_COCR_BETA, ctx->st);
-            launch_vec<R>(7, D.g, ctx->rb, nullptr, nullptr, Ap, Az, nullptr, nullptr, ctx->st);
+            launch_cocr_ap<R>(d.NL, ctx->rb, Ap, Az, ap32_of(), false, ctx->st);
         };
         since_best++;
         if (ko.use_graph && ko.corr_intv <= 0 && pending) {

@@ -209,7 +209,11 @@ void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R
                    R omega, hipStream_t st, int nodal = 0, const cx<R> *dotw = nullptr, double *partials = nullptr);
 void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const double *P64,
                           const zd *in, zd *out, zf *in32, zf *z32, zd *tnode, zd *phi, float omega,
-                          hipStream_t st, const zd *dotw = nullptr, double *partials = nullptr);
+                          hipStream_t st, const zd *dotw = nullptr, double *partials = nullptr,
+                          bool convert_in = true);  // convert_in false: in32 already holds in
+// Ap = Az + beta Ap (copy: Ap = Az); ap32, when set, receives a complex64 copy
+template <typename R>
+void launch_cocr_ap(int64_t n, RedBuf &rb, cx<R> *Ap, const cx<R> *Az, zf *ap32, bool copy, hipStream_t st);
 void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *P32,
                          const zd *in, zd *out, zf *in32, zf *z32, zf *tnode, zf *phi, float omega, hipStream_t st,
                          int nodal = 0, const zd *dotw = nullptr, double *partials = nullptr);

@@ -787,6 +787,24 @@ __global__ __launch_bounds__(TPB) void k_vec(Dims d, const R *__restrict__ a, co
     if (MODE == 0 || MODE == 2 || MODE == 5 || MODE == 6) block_reduce_store<4>(acc, partials);
 }
 
+// COCR direction of A: Ap = Az + beta Ap (copy: Ap = Az), with an optional
+// complex64 copy of the result for a complex64 preconditioner (which then needs
+// no conversion pass of its own)
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_cocr_ap(int64_t n, const double *__restrict__ slots, cx<R> *__restrict__ Ap,
+                                                 const cx<R> *__restrict__ Az, zf *__restrict__ ap32, int copy)
+{
+    const int64_t e = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (e >= n) return;
+    cx<R> v = Az[e];
+    if (!copy) {
+        const cx<R> be((R)slots[SL_BETA], (R)slots[SL_BETA + 1]);
+        v = fma_c(v, be, Ap[e]);
+    }
+    Ap[e] = v;
+    if (ap32) ap32[e] = zf((float)v.re, (float)v.im);
+}
+
 // GPBiCG fused vector kernels (modes documented at launch_gpbicg)
 template <typename R, int MODE>
 __global__ __launch_bounds__(TPB) void k_gpbicg(Dims d, const R *__restrict__ a, const R *__restrict__ b,
@@ -1696,6 +1714,13 @@ void launch_ssor(const DevGrid<R> &g, R wmu, const cx<R> *in, cx<R> *out, R omeg
 }
 
 template <typename R>
+void launch_cocr_ap(int64_t n, RedBuf &rb, cx<R> *Ap, const cx<R> *Az, zf *ap32, bool copy, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_cocr_ap<R>), dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, n, rb.slots, Ap, Az,
+                       ap32, copy ? 1 : 0);
+}
+
+template <typename R>
 void launch_vec(int mode, const DevGrid<R> &g, RedBuf &rb, cx<R> *v0, cx<R> *v1, cx<R> *v2, cx<R> *v3, cx<R> *v4,
                 cx<R> *v5, hipStream_t st)
 {
@@ -1918,9 +1943,9 @@ void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R
 // (optional fused unconjugated dot with dotw).
 void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const double *P64,
                           const zd *in, zd *out, zf *in32, zf *z32, zd *tnode, zd *phi, float omega,
-                          hipStream_t st, const zd *dotw, double *partials)
+                          hipStream_t st, const zd *dotw, double *partials, bool convert_in)
 {
-    launch_convert<double, float>(g64.d.NL, in, in32, st);
+    if (convert_in) launch_convert<double, float>(g64.d.NL, in, in32, st);
     launch_ssor<float>(g32, (float)wmu, in32, z32, omega, st);
     nodal_part<double>(g64, P64, in, tnode, phi, 1, st);
     const zd s(0.0, 1.0 / wmu);
@@ -2094,6 +2119,7 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
                                            RedBuf &, hipStream_t);                                                 \
     template void launch_mass<R>(const DevGrid<R> &, const cx<R> *, cx<R> *, hipStream_t);                        \
     template void launch_ssor<R>(const DevGrid<R> &, R, const cx<R> *, cx<R> *, R, hipStream_t);                 \
+    template void launch_cocr_ap<R>(int64_t, RedBuf &, cx<R> *, const cx<R> *, zf *, bool, hipStream_t);          \
     template void launch_vec<R>(int, const DevGrid<R> &, RedBuf &, cx<R> *, cx<R> *, cx<R> *, cx<R> *, cx<R> *,   \
                                 cx<R> *, hipStream_t);                                                            \
     template void launch_dot<R>(int, int64_t, const cx<R> *, const cx<R> *, RedBuf &, hipStream_t, int64_t &);    \
