@@ -696,8 +696,9 @@ SolveInfo bicgstab(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu,
 // restart (the reference resets r and r0 only, src/solvers.f90:223-235).
 template <typename R>
 SolveInfo gpbicg(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *f, cx<R> *x,
-                 double bnorm, double eps, int maxit, const KOpts &ko)
+                 double bnorm, double eps, int maxit, const KOpts &ko_in)
 {
+    KOpts ko = ko_in;  // the complex64 preconditioner may be dropped below
     const Dims &d = D.g.d;
     const std::string pf = tag + (sizeof(R) == 8 ? "/g64" : "/g32");
     GPVecs<R> v;
@@ -715,6 +716,10 @@ SolveInfo gpbicg(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, c
     }
     bool start = true;
     int step = 0;  // iterations since the last (re)start
+    // complex64 preconditioner: the same restart policy as bicgstab and cocr
+    bool fresh = false;
+    double rel0 = 0, best = 1e300, best_at_restart = 1e300;
+    int since_best = 0, idle_restarts = 0;
     while (true) {
         if (start) {
             launch_spmv<R>(D.g, wmu, x, v.r, (R)-1, 0, nullptr, ctx->rb, ctx->st);
@@ -732,13 +737,43 @@ SolveInfo gpbicg(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, c
             step = 0;
             info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
             if (info.relres >= eps) apply_prec<R>(ctx, D, tag, wmu, v.r, v.rp, ko);
+            fresh = true;
         } else {
             info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
         }
         if (!(info.relres == info.relres) || std::isinf(info.relres)) { info.status = SFDX_ERR_NAN; break; }
         if (ko.verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  gpbicg it %d relres %.3e\n", info.iters, info.relres);
-        if (info.relres < eps) { info.status = 0; break; }
+        const bool lo32 = ko.lo || ko.mg32;
+        const bool fresh_now = fresh;
+        if (fresh) { rel0 = info.relres; best = info.relres; since_best = 0; fresh = false; }
+        if (info.relres < best) { best = info.relres; since_best = 0; }
+        if (info.relres < eps) {
+            if (lo32 && !fresh_now) { start = true; continue; }  // confirm on the true residual
+            info.status = 0;
+            break;
+        }
         if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
+        if (lo32) {
+            const bool stalled = since_best > 200, diverging = info.relres > 100 * best;
+            const bool planned = ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0;
+            if (stalled || diverging || planned) {
+                if (!planned) {
+                    idle_restarts = (best < 0.5 * best_at_restart) ? 0 : idle_restarts + 1;
+                    if (idle_restarts >= 3) {
+                        ko.lo = nullptr;
+                        ko.mg32 = nullptr;
+                        ko.restart_drop = 0;
+                        info.fallback = 1;
+                        idle_restarts = 0;
+                    }
+                }
+                best_at_restart = std::min(best_at_restart, best);
+                start = true;
+                info.restarts++;
+                continue;
+            }
+        }
+        since_best++;
         if (ctx->hslots[SL_BREAK] != 0) { info.status = SFDX_ERR_BREAKDOWN; break; }
         if (ko.corr_intv > 0 && info.iters > 0 && step > 0 && info.iters % ko.corr_intv == 0) {
             info.pcg_iters += div_correct<R>(ctx, D, tag, x, ko.corr_tol);
