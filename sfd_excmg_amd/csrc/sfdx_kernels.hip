@@ -1,18 +1,27 @@
 // sfdx_kernels.hip -- gfx950 kernels of the MT forward solve.
 //
-//  K1 k_spmv        matrix-free 13-point edge stencil y = alpha*A_s x with an
-//                   optional fused dot product (replaces zamux on the CSR of
-//                   fd_assemb + dcfast, sfd_excmg/src/div_correc.f90:431-449)
+//  K1 k_spmv_march  matrix-free 13-point edge stencil y = alpha*A_s x, marching
+//                   up node planes with LDS-staged planes, optional fused dot
+//                   products (replaces zamux on the CSR of fd_assemb + dcfast,
+//                   sfd_excmg/src/div_correc.f90:431-449); k_spmv is the
+//                   one-row-per-thread form (split residual, comparison)
 //  K2 k_ssor        one colour sweep of multicolour SSOR (replaces the
 //                   lexicographic ssorfac + lusol, src/div_correc.f90:171-220,
 //                   src/solvers.f90:721-762)
-//  K3 k_vec_*       fused Krylov vector updates with wavefront/block partial
+//  K3 k_vec, k_cocr_ap, k_gpbicg, k_dot
+//                   fused Krylov vector updates with wavefront/block partial
 //                   reductions and a deterministic second-stage reduction
 //  K4 k_prolong     EXCMG block prolongation with separable Lagrange weights
 //                   (src/excmg.f90:357-772)
-//  K5 k_mass, k_node_div, k_pbuild, k_papply, k_grad_update
+//  K5 k_mass, k_node_div, k_pbuild, k_psym, k_papply, k_grad_update
 //                   device divergence correction (src/div_correc.f90:11-147)
 //  K6 k_dirichlet   boundary data from the 1-D field (src/fdfe_main.f90:470-494)
+//  K7 k_gradT, k_pjacobi, k_pssor, k_grad_add, k_hyb_combine
+//                   nodal half of the hybrid preconditioner
+//  K8 k_restrict, k_prolong_add, k_prolong_node_add, k_grad
+//                   multigrid transfers
+//  K9 k_cell_grad, k_surface_fields, k_zero_boundary
+//                   adjoint gradient and device post-process
 //
 // Launch geometry: edge kernels use one grid row (blockIdx.y) per
 // (orientation, k) slice and 256 threads over the contiguous (i,j) plane of that
@@ -694,12 +703,14 @@ __global__ __launch_bounds__(TPB) void k_ssor(Dims d, const R *__restrict__ a, c
 
 // ------------------------------------------------------------------ K3 vectors
 // Generic fused vector update over all edges with an unscaled-residual norm:
-//  mode 0 (COCR): x += alpha p; r -= alpha Ap; z -= alpha q; acc |r/ve|^2
+//  mode 0 (COCR, 9 vectors; kernel micro-benchmark): x += alpha p; r -= alpha Ap;
+//                         z -= alpha q; acc |r/ve|^2
 //  mode 1 (BiCGStab s):   s = r - alpha v
 //  mode 2 (BiCGStab end): x += alpha y + omega z; r = s - omega t; acc |r/ve|^2, conj(rh)^T r
-//  mode 3 (COCR dirs):    p = z + beta p; Ap = Az + beta Ap
 //  mode 4 (BiCGStab p):   p = r + beta (p - omega v)
 //  mode 5 (norm only):    acc |r/ve|^2 (r = v0)
+//  mode 6 (COCR):         p = z + beta p; x += alpha p; r -= alpha Ap; z -= alpha q;
+//                         acc |r/ve|^2 (the direction is kept one step behind)
 template <typename R, int MODE>
 __global__ __launch_bounds__(TPB) void k_vec(Dims d, const R *__restrict__ a, const R *__restrict__ b,
                                              const R *__restrict__ c, const double *__restrict__ slots,
@@ -742,10 +753,6 @@ __global__ __launch_bounds__(TPB) void k_vec(Dims d, const R *__restrict__ a, co
                 double ve = (double)dual_volume(a, b, c, S.o, i, j, S.k);
                 acc[0] = ((double)rr.re * rr.re + (double)rr.im * rr.im) / (ve * ve);
             }
-        } else if (MODE == 3) {
-            const cx<R> be((R)slots[SL_BETA], (R)slots[SL_BETA + 1]);
-            v0[e] = fma_c(v1[e], be, v0[e]);
-            v2[e] = fma_c(v3[e], be, v2[e]);
         } else if (MODE == 4) {
             const cx<R> be((R)slots[SL_BETA], (R)slots[SL_BETA + 1]);
             const cx<R> om((R)slots[SL_OMEGA], (R)slots[SL_OMEGA + 1]);
@@ -778,10 +785,6 @@ __global__ __launch_bounds__(TPB) void k_vec(Dims d, const R *__restrict__ a, co
                 double ve = (double)dual_volume(a, b, c, S.o, i, j, S.k);
                 acc[0] = ((double)rr.re * rr.re + (double)rr.im * rr.im) / (ve * ve);
             }
-        } else if (MODE == 7) {
-            // Ap = Az + beta Ap  (v2 = Ap, v3 = Az)
-            const cx<R> be((R)slots[SL_BETA], (R)slots[SL_BETA + 1]);
-            v2[e] = fma_c(v3[e], be, v2[e]);
         }
     }
     if (MODE == 0 || MODE == 2 || MODE == 5 || MODE == 6) block_reduce_store<4>(acc, partials);
@@ -1730,11 +1733,10 @@ void launch_vec(int mode, const DevGrid<R> &g, RedBuf &rb, cx<R> *v0, cx<R> *v1,
     case 0: SFDX_VEC(0); break;
     case 1: SFDX_VEC(1); break;
     case 2: SFDX_VEC(2); break;
-    case 3: SFDX_VEC(3); break;
     case 4: SFDX_VEC(4); break;
     case 5: SFDX_VEC(5); break;
     case 6: SFDX_VEC(6); break;
-    default: SFDX_VEC(7); break;
+    default: throw std::runtime_error("launch_vec: bad mode");
     }
 #undef SFDX_VEC
 }
