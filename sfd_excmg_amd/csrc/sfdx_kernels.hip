@@ -1596,17 +1596,17 @@ static int64_t march_go(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R 
     using G = mt::Geo<TY>;
     const Dims &d = g.d;
     if (kc <= 0) {
-        // blocks of this instance resident on the whole device (queried once)
-        static int64_t slots = -1;
-        if (slots < 0) {
+        // blocks of this instance resident on the whole device, queried once
+        // (a magic static: safe when several host threads launch, as the batch does)
+        static const int64_t slots = []() -> int64_t {
             int per_cu = 0, dev = 0, ncu = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)&k_spmv_march<R, 0, TY, PF>, G::NT, 0) !=
                     hipSuccess ||
                 hipGetDevice(&dev) != hipSuccess ||
                 hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-                per_cu = ncu = 0;
-            slots = (int64_t)per_cu * ncu;
-        }
+                return 0;
+            return (int64_t)per_cu * ncu;
+        }();
         kc = spmv_chunk(d, TY, slots);
     }
     const int ntx = (d.nx + G::TX - 1) / G::TX, nty = (d.ny + TY - 1) / TY;
