@@ -507,6 +507,14 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
                         info.fallback = 1;
                         idle_restarts = 0;
                         graph.reset();  // the captured iteration used the complex64 preconditioner
+                    } else if (ko.precond == SFDX_PRECOND_MG) {
+                        // the V-cycle can diverge when the coarse grids no longer
+                        // resolve the skin depth (measured at 25 and 63 Hz on the
+                        // 128x128x96 model): continue with the hybrid preconditioner
+                        ko.precond = SFDX_PRECOND_HYBRID;
+                        info.fallback = 1;
+                        idle_restarts = 0;
+                        graph.reset();
                     } else {
                         info.status = SFDX_WARN_NOCONV;
                         info.stalled = 1;
@@ -818,6 +826,7 @@ SolveInfo krylov(sfdx_ctx *ctx, int solver, LevelDev<R> &D, const std::string &t
             k2.mg32 = nullptr;
             k2.restart_drop = 0;
             k2.dc_at_start = 0;
+            if (k2.precond == SFDX_PRECOND_MG) k2.precond = SFDX_PRECOND_HYBRID;
             SolveInfo J = gpbicg<R>(ctx, D, tag, wmu, f, x, bnorm, eps, maxit - I.iters, k2);
             J.iters += I.iters;
             J.restarts += I.restarts;
