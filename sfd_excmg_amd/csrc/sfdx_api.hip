@@ -112,6 +112,7 @@ struct LevelDev {
     DevGrid<R> g;
     double *a64 = nullptr, *b64 = nullptr, *c64 = nullptr, *sig64 = nullptr;
     R *P = nullptr;  // 29 * NP: nodal operator G^T M G (19 slots) and its symmetric form
+    float *Ps32 = nullptr;  // 10 * NP: the symmetric form in FP32 (complex64-edge hybrid)
 };
 
 template <typename R>
@@ -352,7 +353,6 @@ bool apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
     }
     if (sizeof(R) == 8 && ko.lo) {
         LevelDev<float> &F = *ko.lo;
-        ensure_P(ctx, F, tag);
         const Dims &d = F.g.d;
         zf *in32 = pool_vec<zf>(ctx, tag + "/mp_in", d.NL), *z32 = pool_vec<zf>(ctx, tag + "/mp_z", d.NL);
         zf *tn = pool_vec<zf>(ctx, tag + "/mp_t", d.NP), *ph = pool_vec<zf>(ctx, tag + "/mp_phi", d.NP);
@@ -361,12 +361,17 @@ bool apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
             // the air's large potentials live) in complex128
             LevelDev<double> &D64 = *(LevelDev<double> *)&D;
             ensure_P(ctx, D64, tag);
+            if (!D64.Ps32) {
+                D64.Ps32 = pool_vec<float>(ctx, tag + "/Ps32", (size_t)10 * d.NP);
+                launch_psym32(d.NP, D64.P, D64.Ps32, ctx->st);
+            }
             zd *tn64 = pool_vec<zd>(ctx, tag + "/mp_t64", d.NP), *ph64 = pool_vec<zd>(ctx, tag + "/mp_phi64", d.NP);
-            launch_hybrid_edge32(D64.g, F.g, (double)wmu, D64.P, (const zd *)in, (zd *)out, in32, z32, tn64, ph64,
+            launch_hybrid_edge32(D64.g, F.g, (double)wmu, D64.Ps32, (const zd *)in, (zd *)out, in32, z32, tn64, ph64,
                                  (float)ko.omega, ctx->st, (const zd *)dotw, ctx->rb.partials, !in32_ready);
             return dotw != nullptr;
         }
         if (ko.precond == SFDX_PRECOND_HYBRID) {
+            ensure_P(ctx, F, tag);
             launch_hybrid_mixed(*(DevGrid<double> *)&D.g, F.g, (double)wmu, F.P, (const zd *)in, (zd *)out, in32, z32,
                                 tn, ph, (float)ko.omega, ctx->st, ko.nodal, (const zd *)dotw, ctx->rb.partials);
             return dotw != nullptr;

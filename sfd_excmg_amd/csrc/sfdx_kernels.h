@@ -207,7 +207,8 @@ void launch_prolong(const Dims &df, const Dims &dc, int m, const ProlongDev &T, 
 template <typename R>
 void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode, cx<R> *phi,
                    R omega, hipStream_t st, int nodal = 0, const cx<R> *dotw = nullptr, double *partials = nullptr);
-void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const double *P64,
+void launch_psym32(int64_t np, const double *P, float *Ps32, hipStream_t st);
+void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *Ps32,
                           const zd *in, zd *out, zf *in32, zf *z32, zd *tnode, zd *phi, float omega,
                           hipStream_t st, const zd *dotw = nullptr, double *partials = nullptr,
                           bool convert_in = true);  // convert_in false: in32 already holds in

@@ -1270,10 +1270,10 @@ __global__ __launch_bounds__(TPB) void k_pssor(Dims d, const R *__restrict__ P, 
 
 // t = G^T r on interior nodes (0 on the boundary), no reduction; with SCALE
 // the result is multiplied by dinv = 1/diag(P) (the input of k_pjacobi)
-template <typename R, bool SCALE = false>
+template <typename R, bool SCALE = false, typename RP = R>
 __global__ __launch_bounds__(TPB) void k_gradT(Dims d, const R *__restrict__ a, const R *__restrict__ b,
                                                const R *__restrict__ c, const cx<R> *__restrict__ w,
-                                               cx<R> *__restrict__ t, const R *__restrict__ dinv)
+                                               cx<R> *__restrict__ t, const RP *__restrict__ dinv)
 {
     const Tile TL = tile_id();
     const int k = TL.y;
@@ -1287,7 +1287,7 @@ __global__ __launch_bounds__(TPB) void k_gradT(Dims d, const R *__restrict__ a, 
         out = w[ex_id(d, i - 1, j, k)] * ((R)1 / a[i - 1]) - w[ex_id(d, i, j, k)] * ((R)1 / a[i]);
         out = out + w[ey_id(d, i, j - 1, k)] * ((R)1 / b[j - 1]) - w[ey_id(d, i, j, k)] * ((R)1 / b[j]);
         out = out + w[ez_id(d, i, j, k - 1)] * ((R)1 / c[k - 1]) - w[ez_id(d, i, j, k)] * ((R)1 / c[k]);
-        if (SCALE) out = out * dinv[n];
+        if (SCALE) out = out * (R)dinv[n];
     }
     t[n] = out;
 }
@@ -1852,14 +1852,30 @@ static void pssor_one(const Dims &d, const R *P, const cx<R> *t, cx<R> *phi, hip
     hipLaunchKernelGGL((k_pssor<R, COL, DIR>), gr, dim3(TPB), 0, st, d, P, t, phi);
 }
 
+// The symmetric slots (1/diag and the 9 upper couplings) rounded to FP32, for a
+// complex128 nodal smoother that reads the operator in half the bytes (the
+// operator is data: rounding it keeps the smoother a fixed linear operator)
+__global__ __launch_bounds__(TPB) void k_psym32(int64_t np, const double *__restrict__ P, float *__restrict__ Ps32)
+{
+    const int64_t n = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (n >= np) return;
+#pragma unroll
+    for (int q = 0; q < 10; q++) Ps32[q * np + n] = (float)P[(19 + q) * np + n];
+}
+
+void launch_psym32(int64_t np, const double *P, float *Ps32, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_psym32, dim3((unsigned)((np + TPB - 1) / TPB)), dim3(TPB), 0, st, np, P, Ps32);
+}
+
 // Two damped Jacobi steps on P from phi = 0, as one symmetric polynomial:
 //   phi = omega D^-1 ((2 - omega) t - omega sum_{m != n} P_nm D^-1_m t_m)
 //       = omega (2 - omega) u_n - omega^2 D^-1_n sum_{m != n} P_nm u_m,  u = D^-1 t,
 // i.e. [2 omega D^-1 - omega^2 D^-1 P D^-1] t: one pass over P with every node
 // in flight, instead of the 16 dependent colour sweeps of the nodal SSOR.
 // Input u = D^-1 t (k_gradT with SCALE), Ps = the symmetric slots 19..28.
-template <typename R>
-__global__ __launch_bounds__(TPB) void k_pjacobi(Dims d, const R *__restrict__ Ps, const cx<R> *__restrict__ u,
+template <typename R, typename RP = R>
+__global__ __launch_bounds__(TPB) void k_pjacobi(Dims d, const RP *__restrict__ Ps, const cx<R> *__restrict__ u,
                                                  cx<R> *__restrict__ phi, R omega)
 {
     const Tile TL = tile_id();
@@ -1879,10 +1895,10 @@ __global__ __launch_bounds__(TPB) void k_pjacobi(Dims d, const R *__restrict__ P
     for (int q = 0; q < 9; q++) {
         const int sl = c_pup[q];
         const int64_t o = c_p19[sl][0] + c_p19[sl][1] * sy + c_p19[sl][2] * sz;
-        const R *Pu = Ps + (1 + q) * np;
-        s = s + Pu[n] * u[n + o] + Pu[n - o] * u[n - o];  // u = 0 on boundary nodes
+        const RP *Pu = Ps + (1 + q) * np;
+        s = s + (R)Pu[n] * u[n + o] + (R)Pu[n - o] * u[n - o];  // u = 0 on boundary nodes
     }
-    phi[n] = (omega * ((R)2 - omega)) * u[n] - (omega * omega * Ps[n]) * s;
+    phi[n] = (omega * ((R)2 - omega)) * u[n] - (omega * omega * (R)Ps[n]) * s;
 }
 
 // phi = N_P (G^T in): nodal 1 the Jacobi polynomial (on D^-1 G^T in), else the
@@ -1941,15 +1957,20 @@ void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R
 // Hybrid preconditioner with the edge SSOR in complex64 and the nodal part
 // (one Jacobi pass over P, where the air's large potentials live) in
 // complex128: out = SSOR32(in32) + s G N_P(G^T in), in32 = complex64 copy of
-// in; the complex64 SSOR output is widened inside the recombination pass
-// (optional fused unconjugated dot with dotw).
-void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const double *P64,
+// in; P's symmetric slots are read in FP32 (Ps32, k_psym32); the complex64
+// SSOR output is widened inside the recombination pass (optional fused
+// unconjugated dot with dotw).
+void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *Ps32,
                           const zd *in, zd *out, zf *in32, zf *z32, zd *tnode, zd *phi, float omega,
                           hipStream_t st, const zd *dotw, double *partials, bool convert_in)
 {
     if (convert_in) launch_convert<double, float>(g64.d.NL, in, in32, st);
     launch_ssor<float>(g32, (float)wmu, in32, z32, omega, st);
-    nodal_part<double>(g64, P64, in, tnode, phi, 1, st);
+    // nodal part in complex128 on the FP32 symmetric operator
+    SliceLaunch Ln = node_launch(g64.d);
+    hipLaunchKernelGGL((k_gradT<double, true, float>), Ln.grid, Ln.block, 0, st, g64.d, g64.a, g64.b, g64.c, in, tnode,
+                       Ps32);
+    hipLaunchKernelGGL((k_pjacobi<double, float>), Ln.grid, Ln.block, 0, st, g64.d, Ps32, tnode, phi, 0.8);
     const zd s(0.0, 1.0 / wmu);
     SliceLaunch Le = edge_launch(g64.d);
     if (dotw)
