@@ -105,6 +105,35 @@ def test_multicolour_ssor_matches_dense(ctx, omega):
     assert np.abs(y[order] - yo).max() <= 1e-11 * np.abs(yo).max()
 
 
+# Larger and odd shapes than the dense check (a colour row of nx = 300 spans
+# two 256-thread blocks; odd nx, ny, nz put both parities at the row ends);
+# M^-1 applied by two sparse triangular solves.
+@pytest.mark.parametrize("shape,omega", [((300, 5, 4), 1.0), ((23, 19, 18), 1.0), ((20, 17, 21), 1.3)])
+def test_multicolour_ssor_matches_triangular_solves(ctx, shape, omega):
+    import scipy.sparse.linalg as spla
+    nx, ny, nz = shape
+    a, b, c, nair, sig = tiny_model(nx, ny, nz, 2, aniso=True)
+    f = 0.7
+    S = O.build_system(a, b, c, sig, f)
+    ve = O.row_scale(a, b, c, S.bnd)
+    order, cols = _colour_order(nx, ny, nz, S.bnd)
+    Ap = (sp.diags(ve) @ S.A).tocsr()[order][:, order].tocsr()
+    cp = cols[order]
+    d = Ap.diagonal()
+    C = Ap.tocoo()
+    low, up = cp[C.row] > cp[C.col], cp[C.row] < cp[C.col]
+    n = len(order)
+    Lw = sp.csr_matrix((C.data[low], (C.row[low], C.col[low])), shape=(n, n)) + sp.diags(d / omega)
+    Uw = sp.csr_matrix((C.data[up], (C.row[up], C.col[up])), shape=(n, n)) + sp.diags(d / omega)
+    x = _rand(len(S.p1), 11)
+    x[S.bnd] = 0
+    y = ctx.ssor_apply(a, b, c, sig, f, x, omega=omega)
+    assert np.all(y[S.bnd] == 0)
+    t = spla.spsolve_triangular(Lw.tocsr(), x[order], lower=True)
+    yo = spla.spsolve_triangular(Uw.tocsr(), d * t, lower=False) * ((2 - omega) / omega)
+    assert np.abs(y[order] - yo).max() <= 1e-11 * np.abs(yo).max()
+
+
 def test_extint_v_matches_oracle(ctx):
     rng = np.random.default_rng(5)
     nx, ny, nz, nair = 16, 24, 16, 4
