@@ -38,7 +38,10 @@
  *                      or swapped xz/yz moves the result by 0.24 / 0.11 relative)
  *   orc_ssor_apply     dense (D+L) D^-1 (D+U) factorisation
  *   orc_bicgstab       iterates equal SciPy's BiCGStab; dense solution reached
- *   orc_gpbicg         dense solution reached -- iterate-level parity unpinned
+ *   orc_gpbicg         dense solution reached; first step equals SciPy's BiCGStab
+ *                      step (eta = 0); finite termination within the number of
+ *                      interior unknowns (a dropped or sign-flipped term of u, z,
+ *                      y, w, eta or zeta leaves >= 1e-4 of the residual)
  *   orc_cocr           dense solution reached (new solver: the reference's COCR
  *                      cannot run, so there is nothing to be at parity with)
  *   orc_dc_matrix      isotropic P, T equal the reference formulas x 8; DC drives

@@ -302,6 +302,32 @@ def test_bicgstab_iterates_match_scipy(k):
     assert abs(rn(x) - rn(xs)) <= 1e-3 * rn(xs)
 
 
+@pytest.mark.parametrize("dims", [(3, 2, 2), (3, 3, 2)])
+def test_gpbicg_first_step_is_bicgstab_and_terminates(dims):
+    """GPBiCG (src/solvers.f90:137-315) pinned at iterate level by two
+    properties of the method (Zhang, SIAM J. Sci. Comput. 18 (1997) 537-551):
+    its first step has eta = 0 and zeta = (At, t)/(At, At), i.e. it is the first
+    BiCGStab step, checked against SciPy's bicgstab; and its residual is a
+    product of the BiCG polynomial, so in exact arithmetic it vanishes within
+    as many steps as there are interior unknowns (the boundary rows are the
+    identity).  Dropping the beta*u term, flipping the sign of eta, transposing
+    an inner product in zeta, or dropping a term of y, z or w leaves a relative
+    interior residual of 1e-4 or more after those steps."""
+    a, b, c, nair, sig = tiny_model(*dims, 1, aniso=True)
+    S = O.build_system(a, b, c, sig, 1.0)
+    inner = ~S.bnd
+    x0 = np.zeros_like(S.p1)
+    M = spla.LinearOperator(S.A.shape, matvec=lambda v: O.ssor_apply(S, v), dtype=complex)
+    xs, _ = spla.bicgstab(S.A, S.p1, x0=x0, M=M, rtol=1e-30, atol=0, maxiter=1)
+    x1, st = O.krylov(S, S.p1, x0, a, b, c, sig, O.SOLVER_GPBICG, 1e-300, 1, 0, 1e-3)
+    assert st.iters == 1
+    assert np.linalg.norm(x1 - xs) <= 1e-12 * np.linalg.norm(xs)
+    n = int(inner.sum())
+    x, st = O.krylov(S, S.p1, x0, a, b, c, sig, O.SOLVER_GPBICG, 1e-300, n + 1, 0, 1e-3)
+    red = np.linalg.norm((S.p1 - S.A @ x)[inner]) / np.linalg.norm(S.p1[inner])
+    assert red <= 1e-8, red
+
+
 # ------------------------------------------------------------------ divergence correction
 def reference_iso_PT(a, b, c, sig):
     """Isotropic P (7-point) and T (6 per node), retyped from
