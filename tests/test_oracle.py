@@ -508,6 +508,38 @@ def test_arho_interp_exact_on_quadratics_rd1():
         assert np.allclose(out[:, q], exp * (q + 1), rtol=1e-12)
 
 
+def test_arho_interp_rd2_samples_the_transposed_window():
+    """rd=2 (src/fdfe_main.f90:763-768, Q7) reads E and H at cell (jrec+m,
+    irec+n) with the x-weights of irec and the y-weights of jrec; hz is read at
+    (irec+m, jrec+n).  On cells of one width h the Lagrange weights reproduce
+    quadratics, and the window (jrec.., irec..) is the station's own window
+    shifted by d = (jrec - irec) h in x and -d in y, so for a biquadratic field
+    E, H come out at (x + d, y - d) and hz at (x, y).  Transposing the weights
+    instead of the cells, or hz as well, misses these points."""
+    rng = np.random.default_rng(4)
+    nx, ny, h = 14, 11, 100.0
+    a, b = np.full(nx, h), np.full(ny, h)
+    xc, yc = synth.cell_centres(a), synth.cell_centres(b)
+    F = lambda x, y: 1 + 0.002 * x - 0.003 * y + 1e-6 * x * x + 2e-6 * x * y - 1e-6 * y * y + 1e-9 * x * x * y * y
+    grid = F(xc[:, None], yc[None, :])
+    fields = tuple(np.stack([grid * (q + 1), grid * (q + 2)], -1).astype(complex) for q in range(5))
+    recv, shift = [], []
+    while len(recv) < 12:
+        x, y = rng.uniform(xc[1], xc[-2]), rng.uniform(yc[1], yc[-2])
+        irec = int(np.searchsorted(xc, x)) - 1     # xc[irec] <= x < xc[irec + 1]
+        jrec = int(np.searchsorted(yc, y)) - 1
+        # the transposed window must lie inside the grid as well
+        if 1 <= jrec <= nx - 2 and 1 <= irec <= ny - 2:
+            recv.append([x, y, 0]); shift.append((jrec - irec) * h)
+    recv, d = np.array(recv), np.array(shift)
+    assert np.any(d != 0)
+    out = O.arho_interp2(1, 2, a, b, fields, recv)
+    moved = F(recv[:, 0] + d, recv[:, 1] - d)
+    for q in range(4):
+        assert np.allclose(out[:, q], moved * (q + 1), rtol=1e-11)
+    assert np.allclose(out[:, 4], F(recv[:, 0], recv[:, 1]) * 5, rtol=1e-11)
+
+
 @pytest.mark.slow
 def test_halfspace_apparent_resistivity_and_phase():
     m = synth.make_block_halfspace_32()
