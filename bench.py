@@ -15,7 +15,7 @@ path, averaged over the timed steps:
     hybrid preconditioner in complex64, its nodal part in complex128; restarts
     from the true residual after every 1e-6 drop and on stagnation, and a
     final check of the true residual.
-The roofline object is for the stencil SpMV (k_spmv), timed with HIP events on
+The roofline object is for the stencil SpMV (k_spmv_march), timed with HIP events on
 the library's stream.  The CPU baseline is the FP64 oracle (CSR, lexicographic
 SSOR, COCR) on one core, timed on a bounded number of fine-level iterations and
 extrapolated to the FP64 GPU iteration count.
@@ -135,7 +135,10 @@ def kernel_times(ctx, m, reps=50):
 def run_c4(ctx, args):
     """Batched workload: the 128x128x96 anisotropic model at 16 frequencies (1e-3..1e3 Hz)
     x 2 polarisations plus 8 isotropic models of uneven sizes at 1 Hz, solved
-    (a) as one batch over concurrent streams and (b) one item at a time."""
+    (a) as one batch over concurrent streams and (b) one item at a time.  Mixed
+    precision (complex64 preconditioner inside the FP64 refinement loop) unless
+    --batch-fp64; the impedances of the batch are compared with those of a
+    complex128 batch."""
     from sfd_excmg_amd import sfdx, synth
 
     big = synth.make_aniso_blocks_128() if not args.small else synth.make_aniso_blocks_128(nx=64, nz_earth=40, nair=8, nstations=20)
@@ -145,7 +148,8 @@ def run_c4(ctx, args):
         small = small[:3]
         big.freqs = big.freqs[::4]
     models = [big] + small
-    opts = solve_opts(big, sfdx.PREC_FP64, 1e-8)
+    prec = sfdx.PREC_FP64 if args.batch_fp64 else sfdx.PREC_MIXED
+    opts = solve_opts(big, prec, 1e-8)
     nrhs = 2 * sum(len(m.freqs) for m in models)
     t0 = time.perf_counter()
     exb, stb = ctx.solve_batch(models, opts, nstreams=args.nstreams)
@@ -155,6 +159,12 @@ def run_c4(ctx, args):
     exs, sts = ctx.solve_batch(models, opts, nstreams=1)
     t_seq = time.perf_counter() - t0
     diff = max(float(np.max(np.abs(p - q)) / np.max(np.abs(q))) for p, q in zip(exb, exs))
+    zerr = None
+    if not args.batch_fp64:
+        # impedances of the mixed batch against a complex128 batch of the same items
+        ex64, _ = ctx.solve_batch(models, solve_opts(big, sfdx.PREC_FP64, 1e-8), nstreams=args.nstreams)
+        zerr = max(float(np.max(np.abs(impedances(m, p) - impedances(m, q)) / np.abs(impedances(m, q))))
+                   for m, p, q in zip(models, exb, ex64))
     print(json.dumps({"metric": "batched forward solves (16 frequencies x 128x128x96 + 8 uneven models): right-hand sides per second", "value": round(nrhs / t_batch, 3),
                       "unit": "RHS/s", "n_gpus": 1, "higher_is_better": True, "nstreams": args.nstreams,
                       "rhs": nrhs, "models": [[m.nx, m.ny, m.nz] for m in models],
@@ -162,7 +172,9 @@ def run_c4(ctx, args):
                       "speedup_vs_one_stream": round(t_seq / t_batch, 3),
                       "status": [int(s.status) for s in stb], "fine_iters": [int(s.fine_iters_total) for s in stb],
                       "max_rel_field_diff_batch_vs_one_stream": diff,
-                      "dtype": "complex128"}))
+                      "max_rel_impedance_err_vs_complex128_batch": zerr,
+                      "fallback_fp64": [int(s.fallback_fp64) for s in stb],
+                      "dtype": "complex128" if args.batch_fp64 else "complex128 Krylov/operator + complex64 preconditioner"}))
 
 
 def run_c3(ctx, args):
@@ -203,8 +215,8 @@ def run_c3(ctx, args):
 def run_c5(ctx, args):
     """Adjoint workload: misfit gradient with respect to log sigma over all cells of the
     96x96x64 isotropic random-field model, 6 frequencies, 20x20 stations, data
-    from the seed-6 model; central differences (step 1e-3 in log sigma) on
-    seeded random earth cells under the station area."""
+    from the seed-6 model; central differences (step --fd-step in log sigma) on
+    --fd-cells seeded random earth cells under the station area."""
     from sfd_excmg_amd import sfdx, synth
 
     kw = dict(nx=48, ny=48, nz_earth=24, nstations=8) if args.small else {}
@@ -227,7 +239,7 @@ def run_c5(ctx, args):
         i, j = (int(v) for v in rng.integers(npad + 2, nx - npad - 2, 2))
         k = int(rng.integers(m.nair, m.nair + 10))
         cells.append((k * ny + j) * nx + i)
-    h = 1e-3
+    h = args.fd_step
     errs, fds = [], []
     t0 = time.perf_counter()
     for cell in cells:
@@ -267,7 +279,9 @@ def main():
     ap.add_argument("--nstreams", type=int, default=8)
     ap.add_argument("--adjoint", "--c5", dest="c5", action="store_true", help="adjoint gradient on the 96x96x64 random-field model with a "
                     "central-difference check on --fd-cells random cells")
-    ap.add_argument("--fd-cells", type=int, default=4)
+    ap.add_argument("--fd-cells", type=int, default=20)
+    ap.add_argument("--fd-step", type=float, default=1e-4, help="central-difference step in log sigma")
+    ap.add_argument("--batch-fp64", action="store_true", help="run the batched workload in complex128")
     ap.add_argument("--topography", "--c3", dest="c3", action="store_true", help="128x128x96 anisotropic model with topography: mixed vs FP64, "
                     "per-impedance errors")
     ap.add_argument("--verbose", type=int, default=0)

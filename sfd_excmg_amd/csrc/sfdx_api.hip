@@ -439,10 +439,13 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     StreamGraph graph;
     // Safeguards: restart from the true residual on a breakdown (non-finite
     // scalars; the update of x is skipped on the device), on stagnation (no new
-    // best residual for 200 iterations: the recurrence residual has reached its
-    // attainable accuracy or the recurrences drift) and, with the complex64
-    // preconditioner (restart_drop > 0), on divergence (residual 100x above the
-    // best since the last restart).  After 3 such restarts in a row that did not
+    // best residual for 200 iterations, 400 with the complex64 preconditioner:
+    // the recurrence residual has reached its attainable accuracy or the
+    // recurrences drift; with complex64 the residual wanders longer between new
+    // bests at high frequencies, and a 200-iteration window restarted it up to
+    // 12 times per solve on the 128x128x96 model above 1 Hz) and, with the
+    // complex64 preconditioner (restart_drop > 0), on divergence (residual 100x
+    // above the best since the last restart).  After 3 such restarts in a row that did not
     // halve the best residual, the complex64 preconditioner is replaced by the
     // complex128 one (a fixed linear operator); without it the solve stops with
     // SFDX_WARN_NOCONV and the last iterate.
@@ -480,7 +483,7 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
         if (fresh) { rel0 = info.relres; fresh = false; best = info.relres; since_best = 0; }
         if (info.relres < best) { best = info.relres; since_best = 0; }
         const bool diverging = guard && info.relres > 100 * best;
-        const bool stalled = since_best > 200;
+        const bool stalled = since_best > (guard ? 400 : 200);
         const bool restart = (ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0) || diverging || stalled;
         if (bad || info.relres < eps || info.iters >= maxit || cur[SL_BREAK] != 0 || restart) {
             if (pending) {  // decide on the newest iteration instead
