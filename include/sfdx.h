@@ -106,6 +106,12 @@ typedef struct {
                              cross-polar boundary components 0; fdfe_main.f90:938-942,
                              470-494); 1: full tensor of that column, with the cross-polar
                              components of the 1-D field on the boundary (Ez stays 0) */
+    int spmv_tile;      /* k-marching SpMV tile of the Krylov levels: 0 (default) 32x8 node
+                           columns with prefetch, 1 32x8 without, 2 32x4 with, 3 32x4 without;
+                           -1 autotune: time the four once per grid shape and precision on
+                           the device and keep the fastest (cached in the context).  The
+                           tile sets the grouping of the fused reductions, so a fixed tile
+                           keeps repeated runs bit-identical */
 } sfdx_opts;
 
 typedef struct {
@@ -124,6 +130,7 @@ typedef struct {
                                        restarts without progress */
     int solver_switches;            /* COCR solves that stagnated (restarts without progress) and
                                        were continued by GPBiCG from the current iterate */
+    int spmv_tile;                  /* SpMV tile used on the fine level (complex128 operator) */
 } sfdx_stats;
 
 /* Context: device selection, stream, workspace. */

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <map>
 #include <memory>
@@ -67,6 +68,7 @@ struct sfdx_ctx {
     double *hlag = nullptr;    // pinned, 2 x SL_COUNT: lagged convergence test
     hipEvent_t lag_ev[2] = {nullptr, nullptr};
     std::vector<sfdx_ctx *> workers;  // per-stream contexts of sfdx_solve_batch
+    std::map<std::array<int, 4>, int> tiles;  // autotuned SpMV tile per (nx, ny, nz, sizeof R)
 };
 
 namespace {
@@ -84,6 +86,32 @@ void ensure_partials(sfdx_ctx *ctx, int64_t nblk)
         SFDX_CHECK(hipMalloc(&ctx->rb.partials, (size_t)nblk * 8 * sizeof(double)));
         ctx->rb.max_blocks = nblk;
     }
+}
+
+// SpMV tile of a level from the option spmv_tile: fixed (0..3), or (-1) timed
+// once per grid shape and precision by tune_spmv and cached in the context.
+// Needs the reduction buffer sized for the level (ensure_partials).
+template <typename R>
+void choose_tile(sfdx_ctx *ctx, DevGrid<R> &g, const std::string &tag, int opt, int verbose)
+{
+    if (opt >= 0) {
+        g.spmv_variant = opt;
+        return;
+    }
+    const Dims &d = g.d;
+    const std::array<int, 4> key{d.nx, d.ny, d.nz, (int)sizeof(R)};
+    auto it = ctx->tiles.find(key);
+    if (it == ctx->tiles.end()) {
+        cx<R> *x = pool_vec<cx<R>>(ctx, tag + "/tune_x", d.NL), *y = pool_vec<cx<R>>(ctx, tag + "/tune_y", d.NL);
+        SFDX_CHECK(hipMemsetAsync(x, 0, d.NL * sizeof(cx<R>), ctx->st));
+        float ms[4];
+        const int v = tune_spmv<R>(g, (R)1, x, y, ctx->rb, ctx->st, 5, ms);
+        if (verbose)
+            fprintf(stderr, "  spmv tile %dx%dx%d %s: %.4f %.4f %.4f %.4f ms -> variant %d\n", d.nx, d.ny, d.nz,
+                    sizeof(R) == 8 ? "c128" : "c64", ms[0], ms[1], ms[2], ms[3], v);
+        it = ctx->tiles.emplace(key, v).first;
+    }
+    g.spmv_variant = it->second;
 }
 
 void read_slots(sfdx_ctx *ctx)
@@ -1040,6 +1068,10 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         g_err = "solver not supported on the device path";
         return SFDX_ERR_UNSUPPORTED;
     }
+    if (o->spmv_tile < -1 || o->spmv_tile > 3) {
+        g_err = "spmv_tile must be -1 (autotune) or 0..3";
+        return SFDX_ERR_ARG;
+    }
     // level hierarchy: lev[i] for i = 1..Lv (Lv = fine)
     std::vector<HostLevel> lev(Lv + 1);
     lev[Lv] = fine;
@@ -1107,6 +1139,11 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         SFDX_CHECK(hipStreamSynchronize(ctx->st));
     }
     ensure_partials(ctx, edge_launch(make_dims(fine.nx, fine.ny, fine.nz)).nblocks() + 64);
+    for (int i = first_up; i <= Lv; i++) {
+        choose_tile(ctx, D64[i].g, "L" + std::to_string(i) + "/d", o->spmv_tile, o->verbose);
+        if (o->precision != SFDX_PREC_FP64) choose_tile(ctx, D32[i].g, "L" + std::to_string(i) + "/f", o->spmv_tile, o->verbose);
+    }
+    S.spmv_tile = D64[Lv].g.spmv_variant;
     const int64_t NLf = make_dims(fine.nx, fine.ny, fine.nz).NL;
     int worst = 0;
     for (int fi = 0; fi < nf; fi++) {
@@ -1220,6 +1257,8 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     LevelDev<float> D32;
     if (o->precision == SFDX_PREC_MIXED) D32 = upload_level<float>(ctx, fine, "ADJ", o->sigma_floor32);
     ensure_partials(ctx, edge_launch(d).nblocks() + 64);
+    choose_tile(ctx, D64.g, "ADJ/d", o->spmv_tile, o->verbose);
+    if (o->precision == SFDX_PREC_MIXED) choose_tile(ctx, D32.g, "ADJ/f", o->spmv_tile, o->verbose);
     double *gdev = pool_vec<double>(ctx, "adj/g", d.NC);
     SFDX_CHECK(hipMemsetAsync(gdev, 0, d.NC * sizeof(double), ctx->st));
     zd *ed = pool_vec<zd>(ctx, "adj/e", 2 * d.NL), *wd = pool_vec<zd>(ctx, "adj/w", 2 * d.NL);
@@ -1250,6 +1289,7 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
         S.fine_iters_total += sf.fine_iters_total;
         S.fallback_fp64 += sf.fallback_fp64;
         S.solver_switches += sf.solver_switches;
+        S.spmv_tile = sf.spmv_tile;
         if (ex_out) std::memcpy(ex_out + (size_t)f * 2 * d.NL * 2, exf.data(), (size_t)2 * d.NL * sizeof(zc));
         double phf = 0;
         const int bad = misfit_fields(pg, 1, freq + f, nrec, recv, rd, obs + (size_t)f * nrec * 4, exf.data(), phf,
@@ -1339,6 +1379,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->nodal_smoother = 1;
     o->coarse_solver = 0;
     o->full_background = 0;
+    o->spmv_tile = 0;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)
@@ -1461,6 +1502,7 @@ int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_
                 S.fine_iters_total += T.fine_iters_total;
                 S.fallback_fp64 += T.fallback_fp64;
                 S.solver_switches += T.solver_switches;
+                S.spmv_tile = T.spmv_tile;
                 if (T.status < 0) S.status = T.status;
                 if (items[k].f == jobs[items[k].j].nf - 1) {
                     std::memcpy(S.iters, T.iters, sizeof(S.iters));

@@ -60,6 +60,7 @@ struct DevGrid {
     R *a = nullptr, *b = nullptr, *c = nullptr;     // widths (device)
     R *ra = nullptr, *rb = nullptr, *rc = nullptr;  // reciprocal widths (device)
     R *sig = nullptr;                               // 6*NC (device)
+    int spmv_variant = 0;                           // k-marching SpMV tile (launch_spmv_march)
 };
 
 // Reduction scratch: partial sums per block, final slots.

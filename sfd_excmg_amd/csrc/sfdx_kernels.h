@@ -158,6 +158,11 @@ void reduce_final(RedBuf &rb, int64_t nblk, int K, int dst, int op, hipStream_t 
 template <typename R>
 int64_t launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode, const cx<R> *w,
                     RedBuf &rb, hipStream_t st);  // returns the number of partial sums written (dot modes 1-3)
+// tile autotuner: median of `reps` timed launches per variant, returns the
+// fastest (host-synchronising; not under stream capture)
+template <typename R>
+int tune_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, RedBuf &rb, hipStream_t st, int reps,
+              float *ms_out);
 // k-marching stencil with an explicit chunk of node planes per block (kc <= 0:
 // the default) and tile variant (0: 32x8 with prefetch, 1: 32x8 without, 2:
 // 32x4 with prefetch, 3: 32x4 without)

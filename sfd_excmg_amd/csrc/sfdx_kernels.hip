@@ -1665,10 +1665,46 @@ template <typename R>
 int64_t launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode, const cx<R> *w,
                     RedBuf &rb, hipStream_t st)
 {
-    // tile chosen by the variant sweep of sfdx_bench_kernels on 128x128x96
-    // (chunks filling whole rounds): 32x8 columns with prefetch, 0.058 ms in
-    // complex128 (32x4: 0.059) and 0.035 ms in complex64 (32x4: 0.037)
-    return launch_spmv_march<R>(g, wmu, x, y, alpha, dot_mode, w, rb, st, 0, 0);
+    // the level's tile: 0 unless set by tune_spmv or the caller's option; the
+    // variant sweep of sfdx_bench_kernels on 128x128x96 found 32x8 columns with
+    // prefetch fastest (0.058 ms in complex128, 32x4: 0.059; 0.035 ms in
+    // complex64, 32x4: 0.037)
+    return launch_spmv_march<R>(g, wmu, x, y, alpha, dot_mode, w, rb, st, 0, g.spmv_variant);
+}
+
+// Tile autotuner: time the four k-marching variants on this grid (as the
+// Krylov solvers launch them, with the fused dot, chunk from the occupancy
+// rule) and return the fastest.  One untimed launch per variant, then the
+// median of `reps` timed ones (HIP events on st).  Host-synchronising: call
+// it at set-up, never inside a stream capture.  x must hold NL finite values,
+// y is overwritten.
+template <typename R>
+int tune_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, RedBuf &rb, hipStream_t st, int reps,
+              float *ms_out)
+{
+    hipEvent_t e0, e1;
+    SFDX_CHECK(hipEventCreate(&e0));
+    SFDX_CHECK(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 0;
+    std::vector<float> t(reps > 0 ? reps : 1);
+    for (int v = 0; v < 4; v++) {
+        launch_spmv_march<R>(g, wmu, x, y, (R)1, 1, nullptr, rb, st, 0, v);
+        for (auto &ms : t) {
+            SFDX_CHECK(hipEventRecord(e0, st));
+            launch_spmv_march<R>(g, wmu, x, y, (R)1, 1, nullptr, rb, st, 0, v);
+            SFDX_CHECK(hipEventRecord(e1, st));
+            SFDX_CHECK(hipEventSynchronize(e1));
+            SFDX_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        }
+        std::sort(t.begin(), t.end());
+        const float med = t[t.size() / 2];
+        if (ms_out) ms_out[v] = med;
+        if (v == 0 || med < best_ms) { best = v; best_ms = med; }
+    }
+    SFDX_CHECK(hipEventDestroy(e0));
+    SFDX_CHECK(hipEventDestroy(e1));
+    return best;
 }
 
 // t = r - A z - M (G psi): residual of z + (i/wmu) G psi without forming A G psi
@@ -2134,6 +2170,7 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
 
 // explicit instantiations
 #define SFDX_INST(R)                                                                                              \
+    template int tune_spmv<R>(const DevGrid<R> &, R, const cx<R> *, cx<R> *, RedBuf &, hipStream_t, int, float *);  \
     template int64_t launch_spmv<R>(const DevGrid<R> &, R, const cx<R> *, cx<R> *, R, int, const cx<R> *,         \
                                     RedBuf &, hipStream_t);                                                        \
     template int64_t launch_spmv_march<R>(const DevGrid<R> &, R, const cx<R> *, cx<R> *, R, int, const cx<R> *,   \

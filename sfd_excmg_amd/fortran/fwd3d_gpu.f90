@@ -90,6 +90,7 @@ contains
     o%precond = env_int('SFDX_PRECOND', int(o%precond))
     o%coarse_solver = env_int('SFDX_COARSE', 0)
     o%full_background = env_int('SFDX_FULL_BG', 0)
+    o%spmv_tile = env_int('SFDX_SPMV_TILE', 0)
     rc = sfdx_fwd_solve(sfdx_handle, g, sigma, int(nf, c_int), freq, o, ex, st)
     if (rc > 0) then
       write(*,*) 'Error in the GPU forward solve: ', sfdx_error_text()

@@ -24,7 +24,7 @@ module sfdx_iface
     real(c_double) :: sigma_floor32
     integer(c_int) :: mg_levels, mg_coarse_pairs
     real(c_double) :: restart_drop
-    integer(c_int) :: use_graph, nodal_smoother, coarse_solver, full_background
+    integer(c_int) :: use_graph, nodal_smoother, coarse_solver, full_background, spmv_tile
   end type
 
   type, bind(C) :: sfdx_stats_t
@@ -33,7 +33,7 @@ module sfdx_iface
     integer(c_int) :: iters(2, 8)
     real(c_double) :: relres(2, 8)
     integer(c_int) :: dc_calls(2, 8), pcg_iters(2, 8), refine_steps(2), fine_iters_total, fallback_fp64, &
-                      solver_switches
+                      solver_switches, spmv_tile
   end type
 
   interface

@@ -50,7 +50,7 @@ class Opts(ctypes.Structure):
                 ("sigma_floor32", ctypes.c_double), ("mg_levels", ctypes.c_int), ("mg_coarse_pairs", ctypes.c_int),
                 ("restart_drop", ctypes.c_double), ("use_graph", ctypes.c_int),
                 ("nodal_smoother", ctypes.c_int), ("coarse_solver", ctypes.c_int),
-                ("full_background", ctypes.c_int)]
+                ("full_background", ctypes.c_int), ("spmv_tile", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -59,7 +59,7 @@ class Stats(ctypes.Structure):
                 ("relres", (ctypes.c_double * 2) * MAX_LEVELS), ("dc_calls", (ctypes.c_int * 2) * MAX_LEVELS),
                 ("pcg_iters", (ctypes.c_int * 2) * MAX_LEVELS), ("refine_steps", ctypes.c_int * 2),
                 ("fine_iters_total", ctypes.c_int), ("fallback_fp64", ctypes.c_int),
-                ("solver_switches", ctypes.c_int)]
+                ("solver_switches", ctypes.c_int), ("spmv_tile", ctypes.c_int)]
 
     def to_dict(self, levels=MAX_LEVELS):
         return dict(status=self.status, total_seconds=self.total_seconds, fine_seconds=self.fine_seconds,
@@ -70,7 +70,7 @@ class Stats(ctypes.Structure):
                     pcg_iters=[[self.pcg_iters[i][p] for p in range(2)] for i in range(levels)],
                     refine_steps=[self.refine_steps[0], self.refine_steps[1]],
                     fine_iters_total=self.fine_iters_total, fallback_fp64=self.fallback_fp64,
-                    solver_switches=self.solver_switches)
+                    solver_switches=self.solver_switches, spmv_tile=self.spmv_tile)
 
 
 class Job(ctypes.Structure):
