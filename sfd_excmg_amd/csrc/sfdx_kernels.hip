@@ -1910,31 +1910,71 @@ void launch_psym32(int64_t np, const double *P, float *Ps32, hipStream_t st)
 // i.e. [2 omega D^-1 - omega^2 D^-1 P D^-1] t: one pass over P with every node
 // in flight, instead of the 16 dependent colour sweeps of the nodal SSOR.
 // Input u = D^-1 t (k_gradT with SCALE), Ps = the symmetric slots 19..28.
+// One block per 32x8 tile of nodes of one plane; u of planes k-1, k, k+1 over
+// the tile and a one-node halo is staged in LDS, so each node issues 4 vector
+// loads of u (instead of 19 gathers) plus its own 10 and its 9 lower
+// neighbours' coupling values.  The 9 upper couplings, in the order of the
+// symmetric slots (P slots 2, 4, 6, 9, 10, 13, 14, 17, 18): offsets
+// (1,0,0) (0,1,0) (0,0,1) (-1,1,0) (1,1,0) (-1,0,1) (1,0,1) (0,-1,1) (0,1,1).
+namespace pj {
+constexpr int TX = 32, TY = 8, PX = TX + 2, PY = TY + 2, NPL = PX * PY;
+constexpr int DX[9] = {1, 0, 0, -1, 1, -1, 1, 0, 0};
+constexpr int DY[9] = {0, 1, 0, 1, 1, 0, 0, -1, 1};
+constexpr int DZ[9] = {0, 0, 1, 0, 0, 1, 1, 1, 1};
+}  // namespace pj
+
 template <typename R, typename RP = R>
-__global__ __launch_bounds__(TPB) void k_pjacobi(Dims d, const RP *__restrict__ Ps, const cx<R> *__restrict__ u,
-                                                 cx<R> *__restrict__ phi, R omega)
+__global__ __launch_bounds__(pj::TX * pj::TY) void k_pjacobi(Dims d, const RP *__restrict__ Ps,
+                                                           const cx<R> *__restrict__ u, cx<R> *__restrict__ phi,
+                                                           R omega, int ntx)
 {
+    using namespace pj;
+    __shared__ R ure[3 * NPL], uim[3 * NPL];  // planes k-1, k, k+1
     const Tile TL = tile_id();
     const int k = TL.y;
-    const int p = (k <= d.nz) ? TL.x * TPB + threadIdx.x : kNoTile;
+    if (k > d.nz) return;  // padding tile (uniform over the block)
+    const int tj = TL.x / ntx, ti = TL.x - tj * ntx;
+    const int i0 = ti * TX, j0 = tj * TY;
     const int nI = d.nx + 1, nJ = d.ny + 1;
-    if (p >= nI * nJ) return;
-    const int j = p / nI, i = p - j * nI;
-    const int64_t n = (int64_t)k * nI * nJ + p;
+    const int64_t sy = nI, sz = (int64_t)nI * nJ, np = d.NP;
+    for (int f = threadIdx.x; f < 3 * NPL; f += TX * TY) {
+        const int pl = f / NPL, r = f - pl * NPL, lj = r / PX, li = r - lj * PX;
+        const int gi = i0 - 1 + li, gj = j0 - 1 + lj, gk = k - 1 + pl;
+        cx<R> v(0);
+        if (gi >= 0 && gi < nI && gj >= 0 && gj < nJ && gk >= 0 && gk <= d.nz) v = u[gk * sz + (int64_t)gj * sy + gi];
+        ure[f] = v.re;
+        uim[f] = v.im;
+    }
+    __syncthreads();
+    const int li = threadIdx.x % TX, lj = threadIdx.x / TX;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= nI || j >= nJ) return;
+    const int64_t n = k * sz + (int64_t)j * sy + i;
     if (node_boundary(d, i, j, k)) {
         phi[n] = cx<R>(0);
         return;
     }
-    const int64_t sy = nI, sz = (int64_t)nI * nJ, np = d.NP;
+    const int c = NPL + (lj + 1) * PX + li + 1;  // own node in the k plane of LDS
     cx<R> s(0);
 #pragma unroll
     for (int q = 0; q < 9; q++) {
-        const int sl = c_pup[q];
-        const int64_t o = c_p19[sl][0] + c_p19[sl][1] * sy + c_p19[sl][2] * sz;
+        const int64_t o = DX[q] + DY[q] * sy + DZ[q] * sz;
+        const int lo = DX[q] + DY[q] * PX + DZ[q] * NPL;
         const RP *Pu = Ps + (1 + q) * np;
-        s = s + (R)Pu[n] * u[n + o] + (R)Pu[n - o] * u[n - o];  // u = 0 on boundary nodes
+        const cx<R> up(ure[c + lo], uim[c + lo]), dn(ure[c - lo], uim[c - lo]);
+        s = s + (R)Pu[n] * up + (R)Pu[n - o] * dn;  // u = 0 on boundary nodes
     }
-    phi[n] = (omega * ((R)2 - omega)) * u[n] - (omega * omega * (R)Ps[n]) * s;
+    const cx<R> un(ure[c], uim[c]);
+    phi[n] = (omega * ((R)2 - omega)) * un - (omega * omega * (R)Ps[n]) * s;
+}
+
+template <typename R, typename RP>
+static void launch_pjacobi(const Dims &d, const RP *Ps, const cx<R> *u, cx<R> *phi, R omega, hipStream_t st)
+{
+    const int ntx = (d.nx + 1 + pj::TX - 1) / pj::TX, nty = (d.ny + 1 + pj::TY - 1) / pj::TY;
+    const unsigned gx = (unsigned)(ntx * nty);
+    hipLaunchKernelGGL((k_pjacobi<R, RP>), dim3(gx, pad_y(gx, (unsigned)(d.nz + 1))), dim3(pj::TX * pj::TY), 0, st, d, Ps,
+                       u, phi, omega, ntx);
 }
 
 // phi = N_P (G^T in): nodal 1 the Jacobi polynomial (on D^-1 G^T in), else the
@@ -1948,7 +1988,7 @@ static void nodal_part(const DevGrid<R> &g, const R *P, const cx<R> *in, cx<R> *
     if (nodal == 1) {
         const R *Ps = P + 19 * d.NP;
         hipLaunchKernelGGL((k_gradT<R, true>), Ln.grid, Ln.block, 0, st, d, g.a, g.b, g.c, in, tnode, Ps);
-        hipLaunchKernelGGL((k_pjacobi<R>), Ln.grid, Ln.block, 0, st, d, Ps, tnode, phi, (R)0.8);
+        launch_pjacobi<R, R>(d, Ps, tnode, phi, (R)0.8, st);
         return;
     }
     hipLaunchKernelGGL((k_gradT<R, false>), Ln.grid, Ln.block, 0, st, d, g.a, g.b, g.c, in, tnode, (const R *)nullptr);
@@ -2006,7 +2046,7 @@ void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32,
     SliceLaunch Ln = node_launch(g64.d);
     hipLaunchKernelGGL((k_gradT<double, true, float>), Ln.grid, Ln.block, 0, st, g64.d, g64.a, g64.b, g64.c, in, tnode,
                        Ps32);
-    hipLaunchKernelGGL((k_pjacobi<double, float>), Ln.grid, Ln.block, 0, st, g64.d, Ps32, tnode, phi, 0.8);
+    launch_pjacobi<double, float>(g64.d, Ps32, tnode, phi, 0.8, st);
     const zd s(0.0, 1.0 / wmu);
     SliceLaunch Le = edge_launch(g64.d);
     if (dotw)
