@@ -232,6 +232,16 @@ int sfdx_apply_operator(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, 
  * div_correc.f90:171-220, solvers.f90:721-762); x must vanish on boundary edges. */
 int sfdx_ssor_apply(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, double omega,
                     int precision, const double *x, double *y);
+/* y = M_h^-1 x, the hybrid (auxiliary-space) preconditioner of the device
+ * solvers: SSOR_e(x) + (i/(omega mu)) G N_P(G^T x), with N_P the two-step Jacobi
+ * polynomial (nodal 1) or the 8-colour SSOR (nodal 0) on P = G^T M_sigma G.
+ * No counterpart in the reference (whose preconditioner is the SSOR alone,
+ * div_correc.f90:171-220); exposed so that callers and tests can apply the
+ * exact operator the solvers use.  precision SFDX_PREC_MIXED: the complex64
+ * edge SSOR with the complex128 nodal part (nodal must be 1).  x must vanish
+ * on boundary edges. */
+int sfdx_hybrid_apply(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, double omega,
+                      int nodal, int precision, const double *x, double *y);
 /* In-place divergence correction of an edge field (replaces div_correc,
  * div_correc.f90:11-147).  pcg_iters may be NULL. */
 int sfdx_div_correct(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double *x, double corr_tol,

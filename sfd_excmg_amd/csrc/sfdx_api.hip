@@ -1672,6 +1672,38 @@ int sfdx_ssor_apply(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, doub
     });
 }
 
+int sfdx_hybrid_apply(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, double omega, int nodal,
+                      int precision, const double *x, double *y)
+{
+    if (!ctx || !g || !sigma || !x || !y || (nodal != 0 && nodal != 1)) return SFDX_ERR_ARG;
+    if (precision == SFDX_PREC_MIXED && nodal != 1) return SFDX_ERR_ARG;
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        HostLevel L = host_level(g, sigma);
+        const Dims d = make_dims(L.nx, L.ny, L.nz);
+        const double wmu = 2 * 3.141592653589793 * freq * kMu0;
+        LevelDev<double> D = upload_level<double>(ctx, L, "op");
+        ensure_P(ctx, D, "op");
+        zd *xd = pool_vec<zd>(ctx, "op/x", d.NL), *yd = pool_vec<zd>(ctx, "op/y", d.NL);
+        zd *tn = pool_vec<zd>(ctx, "op/tn", d.NP), *ph = pool_vec<zd>(ctx, "op/ph", d.NP);
+        SFDX_CHECK(hipMemcpyAsync(xd, x, d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+        if (precision == SFDX_PREC_FP64) {
+            launch_hybrid<double>(D.g, wmu, D.P, xd, yd, tn, ph, omega, ctx->st, nodal);
+        } else {
+            // the mixed path's preconditioner: edge SSOR in complex64, nodal part
+            // in complex128 on the FP32 symmetric nodal operator
+            LevelDev<float> F = upload_level<float>(ctx, L, "op");
+            float *ps32 = pool_vec<float>(ctx, "op/Ps32", (size_t)10 * d.NP);
+            launch_psym32(d.NP, D.P, ps32, ctx->st);
+            zf *in32 = pool_vec<zf>(ctx, "op/in32", d.NL), *z32 = pool_vec<zf>(ctx, "op/z32", d.NL);
+            launch_hybrid_edge32(D.g, F.g, wmu, ps32, xd, yd, in32, z32, tn, ph, (float)omega, ctx->st);
+        }
+        SFDX_CHECK(hipMemcpyAsync(y, yd, d.NL * sizeof(zd), hipMemcpyDeviceToHost, ctx->st));
+        SFDX_CHECK(hipStreamSynchronize(ctx->st));
+        return SFDX_OK;
+    });
+}
+
 int sfdx_div_correct(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double *x, double corr_tol, int *pcg_iters)
 {
     if (!ctx || !g || !sigma || !x) return SFDX_ERR_ARG;

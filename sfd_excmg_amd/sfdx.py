@@ -79,7 +79,7 @@ class Job(ctypes.Structure):
 
 
 EXPORTS = ["sfdx_create", "sfdx_destroy", "sfdx_last_error", "sfdx_default_opts", "sfdx_fwd_solve",
-           "sfdx_post_process", "sfdx_mt1d_anl", "sfdx_apply_operator", "sfdx_ssor_apply", "sfdx_div_correct",
+           "sfdx_post_process", "sfdx_mt1d_anl", "sfdx_apply_operator", "sfdx_ssor_apply", "sfdx_hybrid_apply", "sfdx_div_correct",
            "sfdx_extint_v", "sfdx_bench_kernels", "sfdx_direct_solve", "sfdx_solve_batch", "sfdx_misfit",
            "sfdx_adjoint", "sfdx_fwd_response"]
 
@@ -104,6 +104,7 @@ def load(path: str = LIB_PATH):
     L.sfdx_mt1d_anl.argtypes = [I, P, P, D, P, P, P]
     L.sfdx_apply_operator.argtypes = [P, ctypes.POINTER(Grid), P, D, I, P, P]
     L.sfdx_ssor_apply.argtypes = [P, ctypes.POINTER(Grid), P, D, D, I, P, P]
+    L.sfdx_hybrid_apply.argtypes = [P, ctypes.POINTER(Grid), P, D, D, I, I, P, P]
     L.sfdx_div_correct.argtypes = [P, ctypes.POINTER(Grid), P, P, D, ctypes.POINTER(I)]
     L.sfdx_extint_v.argtypes = [P, ctypes.POINTER(Grid), P, P, P]
     L.sfdx_bench_kernels.argtypes = [P, ctypes.POINTER(Grid), P, D, I, I, P]
@@ -275,6 +276,15 @@ class Context:
         y = np.zeros_like(x)
         _check(self.lib.sfdx_ssor_apply(self.h, ctypes.byref(G.g), _ptr(sigma_flat(sigma)), float(freq),
                                         float(omega), int(precision), _ptr(x), _ptr(y)))
+        return y
+
+    def hybrid_apply(self, a, b, c, sigma, freq, x, omega=1.0, nodal=1, precision=PREC_FP64):
+        """The solvers' hybrid preconditioner applied to x (boundary entries 0)."""
+        G = _GridHolder(a, b, c)
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.zeros_like(x)
+        _check(self.lib.sfdx_hybrid_apply(self.h, ctypes.byref(G.g), _ptr(sigma_flat(sigma)), float(freq),
+                                          float(omega), int(nodal), int(precision), _ptr(x), _ptr(y)))
         return y
 
     def div_correct(self, a, b, c, sigma, x, corr_tol=1e-3):

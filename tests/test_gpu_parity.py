@@ -134,6 +134,72 @@ def test_multicolour_ssor_matches_triangular_solves(ctx, shape, omega):
     assert np.abs(y[order] - yo).max() <= 1e-11 * np.abs(yo).max()
 
 
+def _ssor_colour_ref(nx, ny, nz, As, bnd, x, omega):
+    """M^-1 x of the multicolour SSOR of As (colour order, two sparse triangular solves)."""
+    import scipy.sparse.linalg as spla
+    order, cols = _colour_order(nx, ny, nz, bnd)
+    Ap = As.tocsr()[order][:, order].tocsr()
+    cp = cols[order]
+    d = Ap.diagonal()
+    C = Ap.tocoo()
+    low, up = cp[C.row] > cp[C.col], cp[C.row] < cp[C.col]
+    n = len(order)
+    Lw = sp.csr_matrix((C.data[low], (C.row[low], C.col[low])), shape=(n, n)) + sp.diags(d / omega)
+    Uw = sp.csr_matrix((C.data[up], (C.row[up], C.col[up])), shape=(n, n)) + sp.diags(d / omega)
+    t = spla.spsolve_triangular(Lw.tocsr(), x[order], lower=True)
+    y = np.zeros_like(x)
+    y[order] = spla.spsolve_triangular(Uw.tocsr(), d * t, lower=False) * ((2 - omega) / omega)
+    return y
+
+
+@pytest.mark.parametrize("shape", [(5, 4, 5), (20, 17, 9)])
+@pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
+def test_hybrid_preconditioner_matches_cpu_construction(ctx, shape, precision):
+    """The solvers' preconditioner z = SSOR_e(r) + (i/wmu) G phi, phi the
+    two-step Jacobi polynomial w(2-w) u - w^2 D^-1 (P - D) u with u = D^-1 G^T r
+    and w = 0.8 on the interior nodes, against a CPU construction: the edge SSOR
+    by triangular solves in colour order, G from the oracle's gradient, and P
+    from G^T A_s G = -i wmu P (A_s the oracle's ve-scaled operator; the test
+    checks that this P is real, i.e. that the curl part of A_s annihilates G).
+    FP64 to 1e-10; the complex64 edge SSOR of the mixed path to 1e-5."""
+    nx, ny, nz = shape
+    a, b, c, nair, sig = tiny_model(nx, ny, nz, 2, aniso=True)
+    f = 0.9
+    wmu = 2 * np.pi * f * O.MU0
+    S = O.build_system(a, b, c, sig, f)
+    ve = O.row_scale(a, b, c, S.bnd)
+    As = (sp.diags(ve) @ S.A).tocsr()
+    nn = O.nnodes(nx, ny, nz)
+    rows, cols, vals = [], [], []
+    for m in range(nn):
+        e = np.zeros(nn)
+        e[m] = 1
+        g = O.grad(a, b, c, e)
+        nz_ = np.flatnonzero(g)
+        rows.append(nz_); cols.append(np.full(len(nz_), m)); vals.append(g[nz_])
+    Gm = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(len(S.p1), nn))
+    idx = np.arange(nn)
+    ii, jj, kk = idx % (nx + 1), (idx // (nx + 1)) % (ny + 1), idx // ((nx + 1) * (ny + 1))
+    inner = (ii > 0) & (ii < nx) & (jj > 0) & (jj < ny) & (kk > 0) & (kk < nz)
+    Gi = Gm[:, inner]
+    Pc = (Gi.T @ As @ Gi).toarray() / (-1j * wmu)
+    assert np.abs(Pc.imag).max() <= 1e-9 * np.abs(Pc.real).max()
+    P = Pc.real
+    r = _rand(len(S.p1), 12)
+    r[S.bnd] = 0
+    t = (Gm.T @ r)[inner]
+    D = np.diag(P)
+    u = t / D
+    phi_i = 0.8 * 1.2 * u - 0.64 * (P @ u - D * u) / D
+    phi = np.zeros(nn, complex)
+    phi[inner] = phi_i
+    z = _ssor_colour_ref(nx, ny, nz, As, S.bnd, r, 1.0) + (1j / wmu) * (Gm @ phi)
+    y = ctx.hybrid_apply(a, b, c, sig, f, r, omega=1.0, nodal=1, precision=precision)
+    assert np.all(y[S.bnd] == 0)
+    tol = 1e-10 if precision == sfdx.PREC_FP64 else 1e-5
+    assert np.abs(y - z).max() <= tol * np.abs(z).max()
+
+
 def test_extint_v_matches_oracle(ctx):
     rng = np.random.default_rng(5)
     nx, ny, nz, nair = 16, 24, 16, 4
