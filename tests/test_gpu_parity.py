@@ -153,15 +153,17 @@ def _ssor_colour_ref(nx, ny, nz, As, bnd, x, omega):
 
 
 @pytest.mark.parametrize("shape", [(5, 4, 5), (20, 17, 9)])
-@pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
-def test_hybrid_preconditioner_matches_cpu_construction(ctx, shape, precision):
-    """The solvers' preconditioner z = SSOR_e(r) + (i/wmu) G phi, phi the
-    two-step Jacobi polynomial w(2-w) u - w^2 D^-1 (P - D) u with u = D^-1 G^T r
-    and w = 0.8 on the interior nodes, against a CPU construction: the edge SSOR
-    by triangular solves in colour order, G from the oracle's gradient, and P
-    from G^T A_s G = -i wmu P (A_s the oracle's ve-scaled operator; the test
-    checks that this P is real, i.e. that the curl part of A_s annihilates G).
-    FP64 to 1e-10; the complex64 edge SSOR of the mixed path to 1e-5."""
+@pytest.mark.parametrize("precision,nodal", [(sfdx.PREC_FP64, 1), (sfdx.PREC_MIXED, 1), (sfdx.PREC_FP64, 0)])
+def test_hybrid_preconditioner_matches_cpu_construction(ctx, shape, precision, nodal):
+    """The solvers' preconditioner z = SSOR_e(r) + (i/wmu) G phi on the interior
+    nodes, against a CPU construction: the edge SSOR by triangular solves in
+    colour order, G from the oracle's gradient, and P from G^T A_s G = -i wmu P
+    (A_s the oracle's ve-scaled operator; the test checks that this P is real,
+    i.e. that the curl part of A_s annihilates G).  phi: nodal 1, the two-step
+    Jacobi polynomial w(2-w) u - w^2 D^-1 (P - D) u, u = D^-1 G^T r, w = 0.8;
+    nodal 0, symmetric Gauss-Seidel on P in the node colours
+    (i&1) + 2(j&1) + 4(k&1): (D + U)^-1 D (D + L)^-1 G^T r.  FP64 to 1e-10;
+    the complex64 edge SSOR of the mixed path to 1e-5."""
     nx, ny, nz = shape
     a, b, c, nair, sig = tiny_model(nx, ny, nz, 2, aniso=True)
     f = 0.9
@@ -189,12 +191,23 @@ def test_hybrid_preconditioner_matches_cpu_construction(ctx, shape, precision):
     r[S.bnd] = 0
     t = (Gm.T @ r)[inner]
     D = np.diag(P)
-    u = t / D
-    phi_i = 0.8 * 1.2 * u - 0.64 * (P @ u - D * u) / D
+    if nodal == 1:
+        u = t / D
+        phi_i = 0.8 * 1.2 * u - 0.64 * (P @ u - D * u) / D
+    else:
+        import scipy.linalg as sla
+        col = ((ii & 1) + 2 * (jj & 1) + 4 * (kk & 1))[inner]
+        o = np.argsort(col, kind="stable")
+        Po, co = P[np.ix_(o, o)], col[o]
+        lower = np.where(co[:, None] > co[None, :], Po, 0) + np.diag(np.diag(Po))
+        upper = np.where(co[:, None] < co[None, :], Po, 0) + np.diag(np.diag(Po))
+        yv = sla.solve_triangular(lower, t[o], lower=True)
+        phi_i = np.zeros_like(t)
+        phi_i[o] = sla.solve_triangular(upper, np.diag(Po) * yv, lower=False)
     phi = np.zeros(nn, complex)
     phi[inner] = phi_i
     z = _ssor_colour_ref(nx, ny, nz, As, S.bnd, r, 1.0) + (1j / wmu) * (Gm @ phi)
-    y = ctx.hybrid_apply(a, b, c, sig, f, r, omega=1.0, nodal=1, precision=precision)
+    y = ctx.hybrid_apply(a, b, c, sig, f, r, omega=1.0, nodal=nodal, precision=precision)
     assert np.all(y[S.bnd] == 0)
     tol = 1e-10 if precision == sfdx.PREC_FP64 else 1e-5
     assert np.abs(y - z).max() <= tol * np.abs(z).max()
