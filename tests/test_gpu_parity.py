@@ -1,7 +1,8 @@
 """GPU parity of the HIP path against the FP64 oracle, through the C ABI.
 
 Tolerances: FP64 kernels 1e-12 relative (stencil/SSOR/prolongation are the
-same arithmetic in a different order); complex64 kernels 1e-5 relative;
+same arithmetic in a different order); the complex64 stencil 1e-6 relative,
+the complex64 SSOR inside the hybrid preconditioner 1e-5;
 impedances of full solves 1e-6 relative at an interior residual of 1e-10.
 """
 import os
@@ -54,12 +55,19 @@ def test_operator_fp64_matches_oracle_csr(ctx, aniso, shape):
     assert np.abs(y2 - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
-def test_operator_fp32_matches_fp64(ctx):
-    m = synth.make_block_halfspace_32()
+@pytest.mark.parametrize("model", ["halfspace32", "aniso64"])
+@pytest.mark.parametrize("f", [1.0, 0.1])
+def test_operator_fp32_matches_fp64(ctx, model, f):
+    """complex64 stencil against complex128 to 1e-6 of the largest entry
+    (measured 1.3-1.5e-7 on these models)."""
+    if model == "halfspace32":
+        m = synth.make_block_halfspace_32()
+    else:
+        m = synth.make_aniso_blocks_128(nx=64, nz_earth=40, nair=8, nstations=20)
     x = _rand(m.nl, 2)
-    y64 = ctx.apply_operator(m.a, m.b, m.c, m.sigma, 1.0, x)
-    y32 = ctx.apply_operator(m.a, m.b, m.c, m.sigma, 1.0, x, precision=sfdx.PREC_MIXED)
-    assert np.abs(y32 - y64).max() <= 1e-5 * np.abs(y64).max()
+    y64 = ctx.apply_operator(m.a, m.b, m.c, m.sigma, f, x)
+    y32 = ctx.apply_operator(m.a, m.b, m.c, m.sigma, f, x, precision=sfdx.PREC_MIXED)
+    assert np.abs(y32 - y64).max() <= 1e-6 * np.abs(y64).max()
 
 
 def _colour_order(nx, ny, nz, bnd):
