@@ -360,7 +360,7 @@ template <typename R>
 // With dotw the partial sums of dotw^T out are produced by the last kernel when
 // the preconditioner allows it (hybrid); returns whether it did.
 bool apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, const cx<R> *in, cx<R> *out,
-                const KOpts &ko, const cx<R> *dotw = nullptr, bool in32_ready = false)
+                const KOpts &ko, const cx<R> *dotw = nullptr, bool in32_ready = false, bool store = true)
 {
     if (ko.precond == SFDX_PRECOND_MG) {
         const Dims &d = D.g.d;
@@ -395,7 +395,8 @@ bool apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
             }
             zd *tn64 = pool_vec<zd>(ctx, tag + "/mp_t64", d.NP), *ph64 = pool_vec<zd>(ctx, tag + "/mp_phi64", d.NP);
             launch_hybrid_edge32(D64.g, F.g, (double)wmu, D64.Ps32, (const zd *)in, (zd *)out, in32, z32, tn64, ph64,
-                                 (float)ko.omega, ctx->st, (const zd *)dotw, ctx->rb.partials, !in32_ready);
+                                 (float)ko.omega, ctx->st, (const zd *)dotw, ctx->rb.partials, !in32_ready,
+                                 store || !dotw);
             return dotw != nullptr;
         }
         if (ko.precond == SFDX_PRECOND_HYBRID) {
@@ -572,6 +573,17 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             int64_t nbd = nbe;
             // q = M Ap, with (Ap, q) fused into the preconditioner's last kernel when possible
             zf *ap32 = ap32_of();
+            if (ap32) {
+                // complex64-edge hybrid: q is not stored; the update forms it again
+                // from the preconditioner's complex64 SSOR output and nodal correction
+                apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko, Ap, true, false);
+                reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
+                launch_cocr_upd_hyb(*(DevGrid<double> *)&D.g, ctx->rb, (zd *)x, (zd *)pv, (const zd *)Ap, (zd *)r,
+                                    (zd *)z, pool_vec<zf>(ctx, tag + "/mp_z", d.NL),
+                                    pool_vec<zd>(ctx, tag + "/mp_phi64", d.NP), (double)wmu, ctx->st);
+                reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+                return;
+            }
             if (!apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko, Ap, ap32 != nullptr))
                 launch_dot<R>(0, d.NL, Ap, q, ctx->rb, ctx->st, nbd);
             reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);

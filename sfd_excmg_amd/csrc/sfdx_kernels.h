@@ -216,7 +216,11 @@ void launch_psym32(int64_t np, const double *P, float *Ps32, hipStream_t st);
 void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *Ps32,
                           const zd *in, zd *out, zf *in32, zf *z32, zd *tnode, zd *phi, float omega,
                           hipStream_t st, const zd *dotw = nullptr, double *partials = nullptr,
-                          bool convert_in = true);  // convert_in false: in32 already holds in
+                          bool convert_in = true,  // convert_in false: in32 already holds in
+                          bool store = true);      // store false: only the dot (dotw required)
+// COCR update (k_vec mode 6) with q formed from z32 and phi of launch_hybrid_edge32
+void launch_cocr_upd_hyb(const DevGrid<double> &g, RedBuf &rb, zd *x, zd *pv, const zd *Ap, zd *r, zd *z,
+                         const zf *z32, const zd *phi, double wmu, hipStream_t st);
 // Ap = Az + beta Ap (copy: Ap = Az); ap32, when set, receives a complex64 copy
 template <typename R>
 void launch_cocr_ap(int64_t n, RedBuf &rb, cx<R> *Ap, const cx<R> *Az, zf *ap32, bool copy, hipStream_t st);

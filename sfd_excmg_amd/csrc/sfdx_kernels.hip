@@ -1340,7 +1340,24 @@ __global__ __launch_bounds__(TPB) void k_grad(Dims d, const R *__restrict__ a, c
 // z64_e = z32_e + s * (G phi)_e (phi in complex64 or complex128), with G
 // applied in FP64 so that later FP64 operator applications see the gradient
 // part exactly; the complex64 SSOR output z32 is widened here, in the same pass.
-template <typename RP, bool DOT = false>
+// The hybrid preconditioner's output on edge e of slice S: widened complex64
+// edge SSOR value plus s times the gradient of the nodal correction.  Shared by
+// k_hyb_combine and k_cocr_upd_hyb, so both form the same value.
+template <typename RP>
+__device__ __forceinline__ zd hyb_value(const Dims &d, const double *__restrict__ a, const double *__restrict__ b,
+                                        const double *__restrict__ c, zd s, const cx<RP> *__restrict__ phi,
+                                        const zf *__restrict__ z32, const Slice &S, int i, int j, int64_t e)
+{
+    int64_t n0, n1;
+    const double l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
+    const cx<RP> f1 = phi[n1], f0 = phi[n0];
+    const zf zz = z32[e];
+    const zd g(((double)f1.re - (double)f0.re) / l, ((double)f1.im - (double)f0.im) / l);
+    return fma_c(zd((double)zz.re, (double)zz.im), s, g);
+}
+
+// z64 = widen(z32) + s G phi (STORE), with the unconjugated dot w^T z64 (DOT)
+template <typename RP, bool DOT, bool STORE = true>
 __global__ __launch_bounds__(TPB) void k_hyb_combine(Dims d, const double *__restrict__ a, const double *__restrict__ b,
                                                      const double *__restrict__ c, zd s, const cx<RP> *__restrict__ phi,
                                                      const zf *__restrict__ z32, zd *__restrict__ z64,
@@ -1353,13 +1370,8 @@ __global__ __launch_bounds__(TPB) void k_hyb_combine(Dims d, const double *__res
     if (p < S.nI * S.nJ) {
         const int j = p / S.nI, i = p - j * S.nI;
         const int64_t e = S.base + p;
-        int64_t n0, n1;
-        const double l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
-        const cx<RP> f1 = phi[n1], f0 = phi[n0];
-        const zf zz = z32[e];
-        const zd g(((double)f1.re - (double)f0.re) / l, ((double)f1.im - (double)f0.im) / l);
-        const zd zn = fma_c(zd((double)zz.re, (double)zz.im), s, g);
-        z64[e] = zn;
+        const zd zn = hyb_value<RP>(d, a, b, c, s, phi, z32, S, i, j, e);
+        if (STORE) z64[e] = zn;
         if (DOT) {
             const zd wv = w[e];
             acc[0] = wv.re * zn.re - wv.im * zn.im;
@@ -1367,6 +1379,52 @@ __global__ __launch_bounds__(TPB) void k_hyb_combine(Dims d, const double *__res
         }
     }
     if (DOT) block_reduce_store<2>(acc, partials);
+}
+
+// COCR update of k_vec mode 6 with q = M Ap formed from the complex64-edge
+// hybrid preconditioner's pieces (z32, phi) instead of read from a stored
+// vector: p = z + beta p; x += alpha p; r -= alpha Ap; z -= alpha q; |r/ve|^2.
+__global__ __launch_bounds__(TPB) void k_cocr_upd_hyb(Dims d, const double *__restrict__ a,
+                                                      const double *__restrict__ b, const double *__restrict__ c,
+                                                      const double *__restrict__ slots, zd *x, zd *pv,
+                                                      const zd *__restrict__ Ap, zd *r, zd *z, const zf *__restrict__ z32,
+                                                      const zd *__restrict__ phi, zd s, double *__restrict__ partials)
+{
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
+    double acc[4] = {0, 0, 0, 0};
+    if (p < S.nI * S.nJ) {
+        const int j = p / S.nI, i = p - j * S.nI;
+        const int64_t e = S.base + p;
+        const bool inner = edge_interior(d, S.o, i, j, S.k);
+        const bool brk = slots[SL_BREAK] != 0;
+        zd rr = r[e];
+        if (!brk) {
+            const zd al(slots[SL_ALPHA], slots[SL_ALPHA + 1]);
+            const zd be(slots[SL_BETA], slots[SL_BETA + 1]);
+            const zd zv = z[e];
+            const zd pn = fma_c(zv, be, pv[e]);
+            pv[e] = pn;
+            x[e] = fma_c(x[e], al, pn);
+            rr = rr - al * Ap[e];
+            r[e] = rr;
+            z[e] = zv - al * hyb_value<double>(d, a, b, c, s, phi, z32, S, i, j, e);
+        }
+        if (inner) {
+            double ve = dual_volume(a, b, c, S.o, i, j, S.k);
+            acc[0] = (rr.re * rr.re + rr.im * rr.im) / (ve * ve);
+        }
+    }
+    block_reduce_store<4>(acc, partials);
+}
+
+void launch_cocr_upd_hyb(const DevGrid<double> &g, RedBuf &rb, zd *x, zd *pv, const zd *Ap, zd *r, zd *z,
+                         const zf *z32, const zd *phi, double wmu, hipStream_t st)
+{
+    SliceLaunch L = edge_launch(g.d);
+    hipLaunchKernelGGL(k_cocr_upd_hyb, L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, rb.slots, x, pv, Ap, r, z, z32, phi,
+                       zd(0.0, 1.0 / wmu), rb.partials);
 }
 
 // ------------------------------------------------------------------ K8 multigrid transfers
@@ -2038,7 +2096,7 @@ void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R
 // unconjugated dot with dotw).
 void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *Ps32,
                           const zd *in, zd *out, zf *in32, zf *z32, zd *tnode, zd *phi, float omega,
-                          hipStream_t st, const zd *dotw, double *partials, bool convert_in)
+                          hipStream_t st, const zd *dotw, double *partials, bool convert_in, bool store)
 {
     if (convert_in) launch_convert<double, float>(g64.d.NL, in, in32, st);
     launch_ssor<float>(g32, (float)wmu, in32, z32, omega, st);
@@ -2049,7 +2107,12 @@ void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32,
     launch_pjacobi<double, float>(g64.d, Ps32, tnode, phi, 0.8, st);
     const zd s(0.0, 1.0 / wmu);
     SliceLaunch Le = edge_launch(g64.d);
-    if (dotw)
+    if (!store) {
+        // only the dot: the caller forms the output itself from z32 and phi
+        if (!dotw) throw std::runtime_error("launch_hybrid_edge32: nothing to do without store or dot");
+        hipLaunchKernelGGL((k_hyb_combine<double, true, false>), Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s,
+                           phi, z32, out, dotw, partials);
+    } else if (dotw)
         hipLaunchKernelGGL((k_hyb_combine<double, true>), Le.grid, Le.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi,
                            z32, out, dotw, partials);
     else
