@@ -414,7 +414,8 @@ bool apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
         ensure_P(ctx, D, tag);
         const std::string p = tag + (sizeof(R) == 8 ? "/hy64" : "/hy32");
         cx<R> *tn = pool_vec<cx<R>>(ctx, p + "t", D.g.d.NP), *ph = pool_vec<cx<R>>(ctx, p + "phi", D.g.d.NP);
-        launch_hybrid<R>(D.g, wmu, D.P, in, out, tn, ph, (R)ko.omega, ctx->st, ko.nodal, dotw, ctx->rb.partials);
+        launch_hybrid<R>(D.g, wmu, D.P, in, out, tn, ph, (R)ko.omega, ctx->st, ko.nodal, dotw, ctx->rb.partials,
+                         store || !dotw);
         return dotw != nullptr;
     }
     launch_ssor<R>(D.g, wmu, in, out, (R)ko.omega, ctx->st);
@@ -581,6 +582,17 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
                 launch_cocr_upd_hyb(*(DevGrid<double> *)&D.g, ctx->rb, (zd *)x, (zd *)pv, (const zd *)Ap, (zd *)r,
                                     (zd *)z, pool_vec<zf>(ctx, tag + "/mp_z", d.NL),
                                     pool_vec<zd>(ctx, tag + "/mp_phi64", d.NP), (double)wmu, ctx->st);
+                reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+                return;
+            }
+            if (ko.precond == SFDX_PRECOND_HYBRID && !ko.lo && !ko.mg32) {
+                // hybrid preconditioner: q holds the edge SSOR output only; the
+                // update adds s G phi itself (q = M Ap is not stored)
+                apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko, Ap, false, false);
+                reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
+                launch_cocr_upd_grad<R>(D.g, ctx->rb, x, pv, Ap, r, z, q,
+                                        pool_vec<cx<R>>(ctx, tag + (sizeof(R) == 8 ? "/hy64phi" : "/hy32phi"), d.NP),
+                                        wmu, ctx->st);
                 reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
                 return;
             }

@@ -211,7 +211,12 @@ void launch_prolong(const Dims &df, const Dims &dc, int m, const ProlongDev &T, 
                     hipStream_t st);
 template <typename R>
 void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode, cx<R> *phi,
-                   R omega, hipStream_t st, int nodal = 0, const cx<R> *dotw = nullptr, double *partials = nullptr);
+                   R omega, hipStream_t st, int nodal = 0, const cx<R> *dotw = nullptr, double *partials = nullptr,
+                   bool store = true);  // store false: out = edge SSOR output, only the dot of the sum (dotw required)
+// COCR update (k_vec mode 6) with q = zs + s G phi formed from launch_hybrid's pieces (store false)
+template <typename R>
+void launch_cocr_upd_grad(const DevGrid<R> &g, RedBuf &rb, cx<R> *x, cx<R> *pv, const cx<R> *Ap, cx<R> *r, cx<R> *z,
+                          const cx<R> *zs, const cx<R> *phi, R wmu, hipStream_t st);
 void launch_psym32(int64_t np, const double *P, float *Ps32, hipStream_t st);
 void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *Ps32,
                           const zd *in, zd *out, zf *in32, zf *z32, zd *tnode, zd *phi, float omega,

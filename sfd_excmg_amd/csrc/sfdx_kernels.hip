@@ -1293,7 +1293,21 @@ __global__ __launch_bounds__(TPB) void k_gradT(Dims d, const R *__restrict__ a, 
 }
 
 // z_e += s * (G phi)_e; with DOT also the partial sums of w^T z (unconjugated)
-template <typename R, bool DOT = false>
+// z_e + s (G phi)_e: the hybrid preconditioner's output on edge e, z the edge
+// SSOR output.  Shared by k_grad_add and k_cocr_upd_grad.
+template <typename R>
+__device__ __forceinline__ cx<R> grad_add_value(const Dims &d, const R *__restrict__ a, const R *__restrict__ b,
+                                                const R *__restrict__ c, cx<R> s, const cx<R> *__restrict__ phi,
+                                                cx<R> z, const Slice &S, int i, int j)
+{
+    int64_t n0, n1;
+    const R l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
+    const cx<R> g = (phi[n1] - phi[n0]) * ((R)1 / l);
+    return fma_c(z, s, g);
+}
+
+// z += s G phi (STORE), with the unconjugated dot w^T (z + s G phi) (DOT)
+template <typename R, bool DOT, bool STORE = true>
 __global__ __launch_bounds__(TPB) void k_grad_add(Dims d, const R *__restrict__ a, const R *__restrict__ b,
                                                   const R *__restrict__ c, cx<R> s, const cx<R> *__restrict__ phi,
                                                   cx<R> *__restrict__ z, const cx<R> *__restrict__ w,
@@ -1306,11 +1320,8 @@ __global__ __launch_bounds__(TPB) void k_grad_add(Dims d, const R *__restrict__ 
     if (p < S.nI * S.nJ) {
         const int j = p / S.nI, i = p - j * S.nI;
         const int64_t e = S.base + p;
-        int64_t n0, n1;
-        const R l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
-        const cx<R> g = (phi[n1] - phi[n0]) * ((R)1 / l);
-        const cx<R> zn = fma_c(z[e], s, g);
-        z[e] = zn;
+        const cx<R> zn = grad_add_value<R>(d, a, b, c, s, phi, z[e], S, i, j);
+        if (STORE) z[e] = zn;
         if (DOT) {
             const cx<R> wv = w[e];
             acc[0] = (double)wv.re * zn.re - (double)wv.im * zn.im;
@@ -1318,6 +1329,54 @@ __global__ __launch_bounds__(TPB) void k_grad_add(Dims d, const R *__restrict__ 
         }
     }
     if (DOT) block_reduce_store<2>(acc, partials);
+}
+
+// COCR update of k_vec mode 6 with q = zs + s G phi formed here from the edge
+// SSOR output zs and the nodal correction phi of launch_hybrid (not stored).
+template <typename R>
+__global__ __launch_bounds__(TPB) void k_cocr_upd_grad(Dims d, const R *__restrict__ a, const R *__restrict__ b,
+                                                       const R *__restrict__ c, const double *__restrict__ slots,
+                                                       cx<R> *x, cx<R> *pv, const cx<R> *__restrict__ Ap, cx<R> *r,
+                                                       cx<R> *z, const cx<R> *__restrict__ zs,
+                                                       const cx<R> *__restrict__ phi, cx<R> s,
+                                                       double *__restrict__ partials)
+{
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
+    double acc[4] = {0, 0, 0, 0};
+    if (p < S.nI * S.nJ) {
+        const int j = p / S.nI, i = p - j * S.nI;
+        const int64_t e = S.base + p;
+        const bool inner = edge_interior(d, S.o, i, j, S.k);
+        const bool brk = slots[SL_BREAK] != 0;
+        cx<R> rr = r[e];
+        if (!brk) {
+            const cx<R> al((R)slots[SL_ALPHA], (R)slots[SL_ALPHA + 1]);
+            const cx<R> be((R)slots[SL_BETA], (R)slots[SL_BETA + 1]);
+            const cx<R> zv = z[e];
+            const cx<R> pn = fma_c(zv, be, pv[e]);
+            pv[e] = pn;
+            x[e] = fma_c(x[e], al, pn);
+            rr = rr - al * Ap[e];
+            r[e] = rr;
+            z[e] = zv - al * grad_add_value<R>(d, a, b, c, s, phi, zs[e], S, i, j);
+        }
+        if (inner) {
+            double ve = (double)dual_volume(a, b, c, S.o, i, j, S.k);
+            acc[0] = ((double)rr.re * rr.re + (double)rr.im * rr.im) / (ve * ve);
+        }
+    }
+    block_reduce_store<4>(acc, partials);
+}
+
+template <typename R>
+void launch_cocr_upd_grad(const DevGrid<R> &g, RedBuf &rb, cx<R> *x, cx<R> *pv, const cx<R> *Ap, cx<R> *r, cx<R> *z,
+                          const cx<R> *zs, const cx<R> *phi, R wmu, hipStream_t st)
+{
+    SliceLaunch L = edge_launch(g.d);
+    hipLaunchKernelGGL((k_cocr_upd_grad<R>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, rb.slots, x, pv, Ap, r, z, zs,
+                       phi, cx<R>((R)0, (R)1 / wmu), rb.partials);
 }
 
 // g_e = (G phi)_e on every edge (0 on boundary edges when phi is 0 on boundary nodes)
@@ -2073,14 +2132,19 @@ static void nodal_part(const DevGrid<R> &g, const R *P, const cx<R> *in, cx<R> *
 // (nodal 0: 8-colour SSOR on P; 1: two-step Jacobi polynomial)
 template <typename R>
 void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode,
-                   cx<R> *phi, R omega, hipStream_t st, int nodal, const cx<R> *dotw, double *partials)
+                   cx<R> *phi, R omega, hipStream_t st, int nodal, const cx<R> *dotw, double *partials, bool store)
 {
     launch_ssor<R>(g, wmu, in, out, omega, st);
     nodal_part<R>(g, P, in, tnode, phi, nodal, st);
     // s = -1/kappa = -1/(i wmu) = i/wmu
     const cx<R> s((R)0, (R)1 / wmu);
     SliceLaunch Le = edge_launch(g.d);
-    if (dotw)
+    if (!store) {
+        // only the dot: out keeps the edge SSOR output, the caller adds s G phi itself
+        if (!dotw) throw std::runtime_error("launch_hybrid: nothing to do without store or dot");
+        hipLaunchKernelGGL((k_grad_add<R, true, false>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out, dotw,
+                           partials);
+    } else if (dotw)
         hipLaunchKernelGGL((k_grad_add<R, true>), Le.grid, Le.block, 0, st, g.d, g.a, g.b, g.c, s, phi, out, dotw,
                            partials);
     else
@@ -2302,7 +2366,9 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
                                     hipStream_t);                                                                 \
     template void launch_extrap<R>(int64_t, R, const cx<R> *, R, cx<R> *, hipStream_t);                        \
     template void launch_hybrid<R>(const DevGrid<R> &, R, const R *, const cx<R> *, cx<R> *, cx<R> *, cx<R> *, R, \
-                                   hipStream_t, int, const cx<R> *, double *);                                   \
+                                   hipStream_t, int, const cx<R> *, double *, bool);                             \
+    template void launch_cocr_upd_grad<R>(const DevGrid<R> &, RedBuf &, cx<R> *, cx<R> *, const cx<R> *, cx<R> *,  \
+                                          cx<R> *, const cx<R> *, const cx<R> *, R, hipStream_t);                 \
     template void launch_hybrid_gs<R>(const DevGrid<R> &, R, const R *, const cx<R> *, cx<R> *, cx<R> *, cx<R> *,  \
                                       int, hipStream_t, bool);                                                    \
     template void launch_resid_split<R>(const DevGrid<R> &, R, const cx<R> *, const cx<R> *, const cx<R> *,       \
