@@ -1107,9 +1107,15 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     const bool coarse_dev = o->use_excmg && o->coarse_solver == 1;
     const int first_dev = o->use_excmg ? (coarse_dev ? 1 : 3) : Lv;
     const int first_up = use_mg ? 1 : first_dev;  // levels resident on the device
+    // The complex64 preconditioner serves the fine level only.  The coarser
+    // Krylov levels (8x and 64x fewer unknowns) take the complex128 one: with
+    // complex64 the 64x64x48 level of the 128x128x96 model needed 1.7x the
+    // iterations.  A complex64 V-cycle from the fine level still visits every
+    // grid, so with multigrid all levels get their complex64 data.
+    const bool lo_fine = o->precision != SFDX_PREC_FP64;
     for (int i = first_up; i <= Lv; i++) {
         D64[i] = upload_level<double>(ctx, lev[i], "L" + std::to_string(i));
-        if (o->precision != SFDX_PREC_FP64)
+        if (lo_fine && (i == Lv || use_mg))
             D32[i] = upload_level<float>(ctx, lev[i], "L" + std::to_string(i), o->sigma_floor32);
     }
     // Multigrid hierarchy mg[1..Lmg] (mg[Lmg] = fine); the EXCMG level i is
@@ -1165,7 +1171,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     ensure_partials(ctx, edge_launch(make_dims(fine.nx, fine.ny, fine.nz)).nblocks() + 64);
     for (int i = first_up; i <= Lv; i++) {
         choose_tile(ctx, D64[i].g, "L" + std::to_string(i) + "/d", o->spmv_tile, o->verbose);
-        if (o->precision != SFDX_PREC_FP64) choose_tile(ctx, D32[i].g, "L" + std::to_string(i) + "/f", o->spmv_tile, o->verbose);
+        if (lo_fine && i == Lv) choose_tile(ctx, D32[i].g, "L" + std::to_string(i) + "/f", o->spmv_tile, o->verbose);
     }
     S.spmv_tile = D64[Lv].g.spmv_variant;
     const int64_t NLf = make_dims(fine.nx, fine.ny, fine.nz).NL;
@@ -1217,9 +1223,9 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 const int maxit = o->use_excmg ? o->maxit_excmg : o->maxit_plain;
                 SFDX_CHECK(hipStreamSynchronize(ctx->st));
                 Timer tl;
-                LevelSolveResult R = solve_level(ctx, *o, D64[i], o->precision != SFDX_PREC_FP64 ? &D32[i] : nullptr,
+                LevelSolveResult R = solve_level(ctx, *o, D64[i], (lo_fine && i == Lv) ? &D32[i] : nullptr,
                                                  tag, ff, edc, pol, x, tol, maxit, use_mg ? &H64 : nullptr,
-                                                 (use_mg && o->precision != SFDX_PREC_FP64) ? &H32 : nullptr, i + off);
+                                                 (use_mg && lo_fine && i == Lv) ? &H32 : nullptr, i + off);
                 SFDX_CHECK(hipStreamSynchronize(ctx->st));
                 const double secs = tl.s();
                 if (i == Lv) {
