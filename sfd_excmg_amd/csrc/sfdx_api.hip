@@ -446,6 +446,12 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
         if (sizeof(R) != 8 || !ko.lo || ko.precond != SFDX_PRECOND_HYBRID || ko.nodal != 1) return nullptr;
         return pool_vec<zf>(ctx, tag + "/mp_in", d.NL);
     };
+    // The preconditioner pieces the update forms q = M Ap from (apply_prec's
+    // buffers of the same tags): complex64 SSOR output and complex128 nodal
+    // correction of the complex64-edge hybrid, nodal correction of the hybrid
+    zf *const mp_z = sizeof(R) == 8 && ko.lo ? pool_vec<zf>(ctx, tag + "/mp_z", d.NL) : nullptr;
+    zd *const mp_phi = sizeof(R) == 8 && ko.lo ? pool_vec<zd>(ctx, tag + "/mp_phi64", d.NP) : nullptr;
+    cx<R> *const hy_phi = pool_vec<cx<R>>(ctx, tag + (sizeof(R) == 8 ? "/hy64phi" : "/hy32phi"), d.NP);
     SliceLaunch L = edge_launch(d);
     const int64_t nbe = L.nblocks();
     const double sc = 2.0 * sizeof(R), vecb = (double)d.NL * sc;
@@ -580,8 +586,7 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
                 apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko, Ap, true, false);
                 reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
                 launch_cocr_upd_hyb(*(DevGrid<double> *)&D.g, ctx->rb, (zd *)x, (zd *)pv, (const zd *)Ap, (zd *)r,
-                                    (zd *)z, pool_vec<zf>(ctx, tag + "/mp_z", d.NL),
-                                    pool_vec<zd>(ctx, tag + "/mp_phi64", d.NP), (double)wmu, ctx->st);
+                                    (zd *)z, mp_z, mp_phi, (double)wmu, ctx->st);
                 reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
                 return;
             }
@@ -590,9 +595,7 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
                 // update adds s G phi itself (q = M Ap is not stored)
                 apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko, Ap, false, false);
                 reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
-                launch_cocr_upd_grad<R>(D.g, ctx->rb, x, pv, Ap, r, z, q,
-                                        pool_vec<cx<R>>(ctx, tag + (sizeof(R) == 8 ? "/hy64phi" : "/hy32phi"), d.NP),
-                                        wmu, ctx->st);
+                launch_cocr_upd_grad<R>(D.g, ctx->rb, x, pv, Ap, r, z, q, hy_phi, wmu, ctx->st);
                 reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
                 return;
             }
