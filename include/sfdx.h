@@ -42,7 +42,10 @@ extern "C" {
 
 #define SFDX_PREC_FP64 0
 #define SFDX_PREC_MIXED 1       /* FP64 Krylov and operator, complex64 preconditioner */
-#define SFDX_PREC_MIXED_IR 2    /* complex64 inner Krylov solves inside FP64 refinement */
+#define SFDX_PREC_MIXED_IR 2    /* complex64 inner Krylov solves inside FP64 refinement.
+                                   Experimental: with resistive air (sigma ~1e-8 S/m) the
+                                   complex64 product K(G phi) cancels to noise, the inner
+                                   solves diverge and the solve returns SFDX_WARN_NOCONV */
 
 #define SFDX_PRECOND_SSOR 0
 #define SFDX_PRECOND_HYBRID 1  /* SSOR + G (G^T A G)^-1 G^T, G^T A G = -i w mu P */

@@ -357,6 +357,17 @@ def test_spmv_tiles_and_autotuner_give_the_same_solve(ctx):
         ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(spmv_tile=4, **kw))
 
 
+def test_complex64_krylov_option_reports_nonconvergence_with_air(ctx):
+    """PREC_MIXED_IR (complex64 inner Krylov solves in FP64 refinement) cannot
+    converge with 1e-8 S/m air: the solve must say so (SFDX_WARN_NOCONV)
+    rather than return its fields as converged."""
+    m = small_excmg_model(False)
+    kw = dict(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR)
+    ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs,
+                           sfdx.default_opts(precision=sfdx.PREC_MIXED_IR, **kw))
+    assert st.status == -1      # SFDX_WARN_NOCONV
+
+
 def test_full_tensor_boundary_matches_oracle(ctx):
     """Full-tensor 1-D boundary (cross-polar Dirichlet components, a deliberate
     deviation from the reference's diagonal-only background): GPU impedances
