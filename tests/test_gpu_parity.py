@@ -386,6 +386,23 @@ def test_full_tensor_boundary_matches_oracle(ctx):
     assert max(np.max(np.abs(p - q) / np.abs(q)) for p, q in zip(zf, zr)) > 1e-4
 
 
+@pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
+def test_c1_workload_matches_oracle_fixture(ctx, precision):
+    """The reference's CPU case at full size (32x32x32 block half-space, 3 EXCMG
+    levels, 1 Hz, 10x10 stations): GPU impedances against the oracle's
+    (tests/golden/c1_oracle_impedances.npz, interior residual 1e-11) to 1e-6."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "c1_oracle_impedances.npz"))
+    m = synth.make_block_halfspace_32()
+    assert float(np.sum(m.sigma)) == float(g["sigma_sum"]) and np.array_equal(np.asarray(m.recv, float), g["recv"])
+    opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR,
+                             precision=precision)
+    ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
+    assert st.status == 0
+    zxy, zyx = _impedances(m, ex)
+    for p, q in ((zxy, g["zxy"]), (zyx, g["zyx"])):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+
+
 def test_halfspace_on_gpu(ctx):
     m = synth.make_block_halfspace_32()
     m.sigma[m.nair * m.nx * m.ny:, :3] = 0.01

@@ -541,6 +541,22 @@ def test_arho_interp_rd2_samples_the_transposed_window():
 
 
 @pytest.mark.slow
+def test_c1_fixture_is_the_oracle_solution():
+    """tests/golden/c1_oracle_impedances.npz (written by
+    tests/golden/make_c1_fixture.py) is what the oracle computes now: a change
+    of the oracle or of the model generator shows up here (about 10 s)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_oracle_impedances.npz"))
+    m = synth.make_block_halfspace_32()
+    assert float(np.sum(m.sigma)) == float(g["sigma_sum"])
+    ex, st = O.fwd_solver(m.a, m.b, m.c, m.nair, 3, 1e-11, m.sigma, m.freqs,
+                          O.Options(solver=O.SOLVER_COCR, corr_intv=50, norm="interior"))
+    res, _ = O.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, out_type=1, rd=2)
+    assert np.allclose(res[:, 0] + 1j * res[:, 1], g["zxy"], rtol=1e-10, atol=0)
+    assert np.allclose(res[:, 2] + 1j * res[:, 3], g["zyx"], rtol=1e-10, atol=0)
+
+
+@pytest.mark.slow
 def test_halfspace_apparent_resistivity_and_phase():
     m = synth.make_block_halfspace_32()
     m.sigma[m.nair * m.nx * m.ny:, :3] = 0.01
