@@ -403,6 +403,25 @@ def test_c1_workload_matches_oracle_fixture(ctx, precision):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
+@pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
+def test_aniso64_workload_matches_oracle_fixture(ctx, precision):
+    """The benchmark model's generator at half resolution (64x64x48 cells, 16
+    seeded blocks of rotated full tensors, 0.1 Hz, 4 EXCMG levels, 20x20
+    stations): GPU impedances against the oracle's
+    (tests/golden/aniso64_oracle_impedances.npz, interior residual 1e-10) to
+    1e-6."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "aniso64_oracle_impedances.npz"))
+    m = synth.make_aniso_blocks_128(nx=64, nz_earth=40, nair=8, nstations=20)
+    assert float(np.sum(m.sigma)) == float(g["sigma_sum"]) and np.array_equal(np.asarray(m.recv, float), g["recv"])
+    opts = sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0,
+                             norm=sfdx.NORM_INTERIOR, precision=precision, maxit_excmg=8000)
+    ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
+    assert st.status == 0
+    zxy, zyx = _impedances(m, ex)
+    for p, q in ((zxy, g["zxy"]), (zyx, g["zyx"])):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+
+
 def test_halfspace_on_gpu(ctx):
     m = synth.make_block_halfspace_32()
     m.sigma[m.nair * m.nx * m.ny:, :3] = 0.01
