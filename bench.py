@@ -11,10 +11,11 @@ path, averaged over the timed steps:
   * FP64: complex128 COCR with the hybrid preconditioner (multicolour edge SSOR
     plus the nodal auxiliary-space correction by a two-step Jacobi polynomial,
     in complex128);
-  * mixed: complex128 COCR, operator and residual; the edge SSOR of the
-    hybrid preconditioner in complex64, its nodal part in complex128; restarts
-    from the true residual after every 1e-6 drop and on stagnation, and a
-    final check of the true residual.
+  * mixed: complex128 COCR, operator and residual; on the fine level the edge
+    SSOR of the hybrid preconditioner in complex64, its nodal part in
+    complex128 (the coarser Krylov level uses the complex128 preconditioner);
+    restarts from the true residual after every 1e-6 drop and on stagnation,
+    and a final check of the true residual.
 The roofline object is for the stencil SpMV (k_spmv_march), timed with HIP events on
 the library's stream.  The CPU baseline is the FP64 oracle (CSR, lexicographic
 SSOR, COCR) on one core, timed on a bounded number of fine-level iterations and
