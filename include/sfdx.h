@@ -115,6 +115,12 @@ typedef struct {
                            the device and keep the fastest (cached in the context).  The
                            tile sets the grouping of the fused reductions, so a fixed tile
                            keeps repeated runs bit-identical */
+    double surface_sigma; /* sfdx_fwd_response and sfdx_adjoint: 0 (default) take the station
+                             fields on the flat datum, node plane nair (fdfe_main.f90:597-640);
+                             > 0: topography carved into sigma -- the surface of each cell column
+                             is the top node plane of its first cell, from the top, whose sigma_xx
+                             exceeds this value in S/m (as sfdx_surface_planes; the reference's
+                             topography branch, fdfe_main.f90:604-614, 656-666) */
 } sfdx_opts;
 
 typedef struct {
@@ -123,7 +129,8 @@ typedef struct {
     double fine_seconds;            /* fine-level Krylov time, all frequencies and both polarisations */
     double fine_bytes;              /* algorithmic HBM bytes of the fine-level kernels */
     int iters[SFDX_MAX_LEVELS][2];  /* last frequency: Krylov iterations per level and polarisation */
-    double relres[SFDX_MAX_LEVELS][2];
+    double relres[SFDX_MAX_LEVELS][2];  /* final relative residual of each Krylov solve, recomputed from
+                                           the returned iterate (b - A x), not the recurrence residual */
     int dc_calls[SFDX_MAX_LEVELS][2];
     int pcg_iters[SFDX_MAX_LEVELS][2];
     int refine_steps[2];            /* fine level: restarts (SFDX_PREC_MIXED) or refinement steps */
@@ -154,6 +161,35 @@ int sfdx_fwd_solve(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
  * (arho_interp2 called with rd=2), rd = 1 samples at the station's own cell. */
 int sfdx_post_process(const sfdx_grid *g, int nf, const double *freq, int nrec, const double *recv,
                       const double *ex, int out_type, int rd, double *results, double *tipper);
+
+/* Replaces the topography branch of post_process (sfd_excmg/src/fdfe_main.f90:
+ * 549-716 with topo present, lines 604-614 and 656-666): as sfdx_post_process,
+ * but the surface of cell column (i,j) is node plane surf[i + nx*j] (0-based,
+ * counted from the top of the air, 0 <= surf < nz); surf = NULL is the flat
+ * datum nair.  The fields there are averaged to the cell centre and the z
+ * difference spans the cell below the surface plane.  The reference selects
+ * the first cell k1 whose centre lies below the ground and uses node plane k1
+ * (the bottom of that cell) with the flat-datum spacing c(nair+1); here the
+ * top of that cell, k1-1, and its own thickness are used, so that a flat
+ * ground reproduces the flat-datum result. */
+int sfdx_post_process_topo(const sfdx_grid *g, const int *surf, int nf, const double *freq, int nrec,
+                           const double *recv, const double *ex, int out_type, int rd, double *results, double *tipper);
+
+/* Surface node plane of every cell column of a model with topography carved
+ * into sigma (cells above the ground set to air): surf[i + nx*j] = top node
+ * plane of the column's first cell, from the top, whose sigma_xx exceeds
+ * air_max (S/m).  Returns SFDX_ERR_ARG if a column is air throughout.  Takes
+ * the place of the topography elevations the reference reads
+ * (file_opt.f90:97-128) and compares with cell centres (fdfe_main.f90:606-611). */
+int sfdx_surface_planes(const sfdx_grid *g, const double *sigma, double air_max, int *surf);
+
+/* Replaces mesh_coarsen (sfd_excmg/src/fdfe_main.f90:18-84): widths summed over
+ * 2^k fine cells, sigma the volume-weighted average over each 2^k x 2^k x 2^k
+ * block (with the fine-cell index taken inside the block; the reference's
+ * index runs one layer and one row too far).  Outputs a_c(nx/2^k),
+ * b_c(ny/2^k), c_c(nz/2^k), sigma_c(ncell/8^k, 6) in the sigma layout above. */
+int sfdx_mesh_coarsen(const sfdx_grid *g, const double *sigma, int k, double *a_c, double *b_c, double *c_c,
+                      double *sigma_c);
 
 /* Replaces mt1d_anl (sfd_excmg/src/model_selec.f90:266-401): conv(nz,6)
  * column-major; ex, hx: 2*nz+2 complex; imp: 4 complex. */

@@ -20,7 +20,8 @@ Function map (reference ``sfd_excmg/src`` file:line):
   solve_level     fdfe_main.f90:892-1063     (doublemode_nor)
   excmg           fdfe_main.f90:1228-1563    (u1u2 + doublemode_acc)
   fwd_solver      fdfe_main.f90:1565-1594
-  post_process    fdfe_main.f90:549-790      (+ arho_interp2)
+  post_process    fdfe_main.f90:549-790      (+ arho_interp2; topography branch :604-614, 656-666)
+  surface_planes  fdfe_main.f90:604-614      (first cell below the ground, from carved sigma)
 """
 from __future__ import annotations
 
@@ -512,17 +513,48 @@ def arho_interp2(mode, rd, a, b, fields, recv):
     return out
 
 
-def surface_fields(a, b, c, nair, freq, x1, x2):
-    """E and H at the surface cell centres (src/fdfe_main.f90:597-683)."""
+def surface_planes(a, b, c, sigma, air_max):
+    """Surface node plane (0-based, from the top of the air) of every cell
+    column, (nx, ny): the top of the column's first cell, from the top, whose
+    sigma_xx exceeds air_max -- with cells above the ground carved to air, the
+    first cell whose centre lies below the ground, as the topography branch of
+    post_process selects it (src/fdfe_main.f90:604-614)."""
+    nx, ny, nz = len(a), len(b), len(c)
+    sxx = np.asarray(sigma, np.float64)[:, 0].reshape(nz, ny, nx)
+    earth = sxx > air_max
+    if not earth.any(axis=0).all():
+        raise ValueError("a cell column is air throughout")
+    return np.argmax(earth, axis=0).T.copy()
+
+
+def topo_surface_planes(c, nair, topo):
+    """Surface plane of each column from ground elevations topo(nx, ny) (m above
+    the datum): one less than the first cell k1 (1-based) whose centre depth
+    ccz(k1) >= -topo, the test of src/fdfe_main.f90:606-611."""
+    c = np.asarray(c, np.float64)
+    ccz = -c[:nair].sum() + np.cumsum(c) - c / 2
+    topo = np.asarray(topo, np.float64)
+    below = ccz[None, None, :] >= -topo[:, :, None]
+    return np.argmax(below, axis=2)
+
+
+def surface_fields(a, b, c, nair, freq, x1, x2, surf=None):
+    """E and H at the surface cell centres (src/fdfe_main.f90:597-683).  surf
+    (nx, ny) gives each column's surface node plane (the topography branch,
+    :604-614, 656-666); None is the flat datum nair.  The z difference spans
+    the cell below the surface plane: c(nair+1) on the flat datum, as the
+    reference."""
     nx, ny, nz = len(a), len(b), len(c)
     n1 = nx * (ny + 1) * (nz + 1)
     n2 = ny * (nz + 1) * (nx + 1)
-    m = nair
+    m = np.full((nx, ny), nair, np.int64) if surf is None else np.asarray(surf, np.int64)
+    assert m.shape == (nx, ny) and m.min() >= 0 and m.max() < nz
     I = np.arange(1, nx + 1)[:, None]
     J = np.arange(1, ny + 1)[None, :]
     idx = m * nx * (ny + 1) + (J - 1) * nx + I - 1           # 0-based versions
     idx2 = n1 + m * ny * (nx + 1) + (J - 1) * (nx + 1) + I - 1
     idx3 = n1 + n2 + m * (nx + 1) * (ny + 1) + (J - 1) * (nx + 1) + I - 1
+    cz = c[m]
     kap = 1j * 2.0 * np.pi * freq * MU0
     A = a[:, None]
     B = b[None, :]
@@ -534,9 +566,9 @@ def surface_fields(a, b, c, nair, freq, x1, x2):
     for md, x in enumerate((x1, x2)):
         ex[:, :, md] = (x[idx] + x[idx + nx]) / 2
         ey[:, :, md] = (x[idx2] + x[idx2 + 1]) / 2
-        exz = ((x[idx + nx * (ny + 1)] + x[idx + nx * (ny + 1) + nx]) / 2 - ex[:, :, md]) / c[nair]
+        exz = ((x[idx + nx * (ny + 1)] + x[idx + nx * (ny + 1) + nx]) / 2 - ex[:, :, md]) / cz
         ezx = ((x[idx3 + 1] + x[idx3 + nx + 2]) / 2 - (x[idx3 + nx + 1] + x[idx3]) / 2) / A
-        eyz = ((x[idx2 + ny * (nx + 1)] + x[idx2 + ny * (nx + 1) + 1]) / 2 - ey[:, :, md]) / c[nair]
+        eyz = ((x[idx2 + ny * (nx + 1)] + x[idx2 + ny * (nx + 1) + 1]) / 2 - ey[:, :, md]) / cz
         ezy = (-(x[idx3] + x[idx3 + 1]) / 2 + (x[idx3 + nx + 1] + x[idx3 + nx + 2]) / 2) / B
         exy = (x[idx + nx] - x[idx]) / B
         eyx = (x[idx2 + 1] - x[idx2]) / A
@@ -546,10 +578,11 @@ def surface_fields(a, b, c, nair, freq, x1, x2):
     return ex, ey, hx, hy, hz
 
 
-def post_process(a, b, c, nair, freqs, recv, ex_all, out_type=1, rd=2):
+def post_process(a, b, c, nair, freqs, recv, ex_all, out_type=1, rd=2, surf=None):
     """post_process, src/fdfe_main.f90:549-716.  Returns results (nrec*nf, 4):
     out_type 1 -> (Re Zxy, Im Zxy, Re Zyx, Im Zyx); 2 -> (rho_xy, rho_yx,
-    phi_xy, phi_yx); and the tipper array (nrec*nf, 4)."""
+    phi_xy, phi_yx); and the tipper array (nrec*nf, 4).  surf: per-column
+    surface node planes (topography), see surface_fields."""
     a, b, c = _f64(a), _f64(b), _f64(c)
     recv = np.asarray(recv, np.float64)
     nl = nedges(len(a), len(b), len(c))
@@ -558,7 +591,7 @@ def post_process(a, b, c, nair, freqs, recv, ex_all, out_type=1, rd=2):
     res = np.zeros((nrec * len(freqs), 4))
     tip = np.zeros((nrec * len(freqs), 4))
     for k, ff in enumerate(freqs):
-        flds = surface_fields(a, b, c, nair, ff, ex_all[:nl, k], ex_all[nl:, k])
+        flds = surface_fields(a, b, c, nair, ff, ex_all[:nl, k], ex_all[nl:, k], surf)
         ar1 = arho_interp2(1, rd, a, b, flds, recv)
         ar2 = arho_interp2(2, rd, a, b, flds, recv)
         tmp = ar1[:, 2] * ar2[:, 3] - ar2[:, 2] * ar1[:, 3]

@@ -527,17 +527,25 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
                 pending = false;
                 continue;
             }
+            // The recurrence residual drifts from b - A x (in every precision; with a
+            // complex64 preconditioner by much more): convergence is accepted, and
+            // the final residual reported, only on a residual recomputed from x.
             if (!bad && info.relres < eps) {
-                // with a complex64 preconditioner the recurrence residual can drift
-                // from the true one: accept only a residual recomputed from x
-                if (!fresh_now && (ko.lo || ko.mg32)) {
+                if (!fresh_now) {
                     state = 0;
                     continue;
                 }
                 info.status = 0;
                 break;
             }
-            if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
+            if (info.iters >= maxit) {
+                if (!fresh_now && !bad) {
+                    state = 0;
+                    continue;
+                }
+                info.status = SFDX_WARN_NOCONV;
+                break;
+            }
             const bool planned = !bad && cur[SL_BREAK] == 0 && !stalled && !diverging;
             if (!planned) {
                 // no progress since the last restart: count it
@@ -705,12 +713,18 @@ SolveInfo bicgstab(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu,
         const bool fresh_now = fresh;
         if (fresh) { rel0 = info.relres; best = info.relres; since_best = 0; fresh = false; }
         if (info.relres < best) { best = info.relres; since_best = 0; }
+        // accept convergence (and report the final residual) only on the true
+        // residual b - A x, recomputed from the iterate
         if (info.relres < eps) {
-            if (lo32 && !fresh_now) { state = 0; continue; }  // confirm on the true residual
+            if (!fresh_now) { state = 0; continue; }
             info.status = 0;
             break;
         }
-        if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
+        if (info.iters >= maxit) {
+            if (!fresh_now) { state = 0; continue; }
+            info.status = SFDX_WARN_NOCONV;
+            break;
+        }
         if (lo32) {
             const bool stalled = since_best > 200, diverging = info.relres > 100 * best;
             const bool planned = ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0;
@@ -818,12 +832,18 @@ SolveInfo gpbicg(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, c
         const bool fresh_now = fresh;
         if (fresh) { rel0 = info.relres; best = info.relres; since_best = 0; fresh = false; }
         if (info.relres < best) { best = info.relres; since_best = 0; }
+        // accept convergence (and report the final residual) only on the true
+        // residual b - A x, recomputed from the iterate
         if (info.relres < eps) {
-            if (lo32 && !fresh_now) { start = true; continue; }  // confirm on the true residual
+            if (!fresh_now) { start = true; continue; }
             info.status = 0;
             break;
         }
-        if (info.iters >= maxit) { info.status = SFDX_WARN_NOCONV; break; }
+        if (info.iters >= maxit) {
+            if (!fresh_now) { start = true; continue; }
+            info.status = SFDX_WARN_NOCONV;
+            break;
+        }
         if (lo32) {
             const bool stalled = since_best > 200, diverging = info.relres > 100 * best;
             const bool planned = ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0;
@@ -1158,11 +1178,15 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 }
             }
         }
+        // the host weight tables stay alive until the stream has drained
+        std::vector<std::vector<double>> wtabs;
+        wtabs.reserve((size_t)3 * Lmg);
         for (int j = 2; j <= Lmg; j++) {
             const std::vector<double> *wf[3] = {&mgh[j].a, &mgh[j].b, &mgh[j].c};
             const std::vector<double> *wc[3] = {&mgh[j - 1].a, &mgh[j - 1].b, &mgh[j - 1].c};
             for (int q = 0; q < 3; q++) {
-                const std::vector<double> w = mg_weights_1d(*wf[q], *wc[q]);
+                wtabs.push_back(mg_weights_1d(*wf[q], *wc[q]));
+                const std::vector<double> &w = wtabs.back();
                 double *dv = pool_vec<double>(ctx, "mgwl" + std::to_string(j) + "_" + std::to_string(q), w.size());
                 SFDX_CHECK(hipMemcpyAsync(dv, w.data(), w.size() * 8, hipMemcpyHostToDevice, ctx->st));
                 H64.wl[j][q] = dv;
@@ -1178,6 +1202,22 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     }
     S.spmv_tile = D64[Lv].g.spmv_variant;
     const int64_t NLf = make_dims(fine.nx, fine.ny, fine.nz).NL;
+    // surface node plane per column for the device post-process (topography
+    // carved into sigma), or the flat datum
+    int *planes_dev = nullptr;
+    std::vector<int> planes;
+    if (surf && o->surface_sigma > 0) {
+        surface_planes(fine, o->surface_sigma, planes);
+        for (int v : planes)
+            if (v >= fine.nz) {
+                g_err = "a cell column holds no cell above surface_sigma (no ground surface)";
+                return SFDX_ERR_ARG;
+            }
+        planes_dev = pool_vec<int>(ctx, "surf_planes", planes.size());
+        SFDX_CHECK(hipMemcpyAsync(planes_dev, planes.data(), planes.size() * sizeof(int), hipMemcpyHostToDevice,
+                                  ctx->st));
+        SFDX_CHECK(hipStreamSynchronize(ctx->st));
+    }
     int worst = 0;
     for (int fi = 0; fi < nf; fi++) {
         const double ff = freq[fi];
@@ -1196,16 +1236,19 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 std::vector<zc> sol;
                 if (!direct_solve(CS, sol, g_err)) return SFDX_ERR_DIRECT;
                 const Dims d = make_dims(lev[i].nx, lev[i].ny, lev[i].nz);
+                // both host copies stay alive until the stream has drained
+                std::vector<zc> full[2];
                 for (int pol = 0; pol < 2; pol++) {
-                    std::vector<zc> full(d.NL, 0.0);
+                    full[pol].assign(d.NL, 0.0);
                     for (int64_t e = 0; e < d.NL; e++) {
                         int ii, jj, kk;
                         int oo = edge_decode(d, e, ii, jj, kk);
-                        if (CS.imap[e] >= 0) full[e] = sol[(size_t)pol * CS.n + CS.imap[e]];
-                        else dirichlet_value(d, edc.data(), pol, oo, ii, jj, kk, full[e], o->full_background != 0);
+                        if (CS.imap[e] >= 0) full[pol][e] = sol[(size_t)pol * CS.n + CS.imap[e]];
+                        else dirichlet_value(d, edc.data(), pol, oo, ii, jj, kk, full[pol][e], o->full_background != 0);
                     }
                     u[i][pol] = ubuf(i, pol);
-                    SFDX_CHECK(hipMemcpyAsync(u[i][pol], full.data(), d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+                    SFDX_CHECK(hipMemcpyAsync(u[i][pol], full[pol].data(), d.NL * sizeof(zd), hipMemcpyHostToDevice,
+                                              ctx->st));
                 }
                 SFDX_CHECK(hipStreamSynchronize(ctx->st));
             }
@@ -1265,8 +1308,8 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
             zd *sd = pool_vec<zd>(ctx, "surf", 10 * ns);
             const double om = 2 * 3.141592653589793 * ff * kMu0;
             for (int pol = 0; pol < 2; pol++)
-                launch_surface_fields(df, fine.nair, D64[Lv].a64, D64[Lv].b64, D64[Lv].c64, om, u[Lv][pol],
-                                      sd + (size_t)pol * 5 * ns, ctx->st);
+                launch_surface_fields(df, fine.nair, planes_dev, D64[Lv].a64, D64[Lv].b64, D64[Lv].c64, om,
+                                      u[Lv][pol], sd + (size_t)pol * 5 * ns, ctx->st);
             SFDX_CHECK(hipMemcpyAsync(surf + (size_t)fi * 10 * ns, sd, 10 * ns * sizeof(zd), hipMemcpyDeviceToHost,
                                       ctx->st));
         }
@@ -1285,7 +1328,17 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
 {
     const HostLevel fine = host_level(g, sigma);
     const Dims d = make_dims(fine.nx, fine.ny, fine.nz);
-    const PostGrid pg{g->nx, g->ny, g->nz, g->nair, g->a, g->b, g->c};
+    PostGrid pg{g->nx, g->ny, g->nz, g->nair, g->a, g->b, g->c};
+    std::vector<int> planes;
+    if (o->surface_sigma > 0) {
+        surface_planes(fine, o->surface_sigma, planes);
+        for (int v : planes)
+            if (v >= fine.nz) {
+                g_err = "a cell column holds no cell above surface_sigma (no ground surface)";
+                return SFDX_ERR_ARG;
+            }
+        pg.surf = planes.data();
+    }
     LevelDev<double> D64 = upload_level<double>(ctx, fine, "ADJ");
     LevelDev<float> D32;
     if (o->precision == SFDX_PREC_MIXED) D32 = upload_level<float>(ctx, fine, "ADJ", o->sigma_floor32);
@@ -1413,6 +1466,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->coarse_solver = 0;
     o->full_background = 0;
     o->spmv_tile = 0;
+    o->surface_sigma = 0;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)
@@ -1603,11 +1657,61 @@ int sfdx_adjoint(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
 int sfdx_post_process(const sfdx_grid *g, int nf, const double *freq, int nrec, const double *recv, const double *ex,
                       int out_type, int rd, double *results, double *tipper)
 {
+    return sfdx_post_process_topo(g, nullptr, nf, freq, nrec, recv, ex, out_type, rd, results, tipper);
+}
+
+int sfdx_post_process_topo(const sfdx_grid *g, const int *surf, int nf, const double *freq, int nrec,
+                           const double *recv, const double *ex, int out_type, int rd, double *results, double *tipper)
+{
     if (!g || !freq || !recv || !ex || !results) return SFDX_ERR_ARG;
-    PostGrid pg{g->nx, g->ny, g->nz, g->nair, g->a, g->b, g->c};
+    if (surf)
+        for (size_t q = 0; q < (size_t)g->nx * g->ny; q++)
+            if (surf[q] < 0 || surf[q] >= g->nz) {
+                g_err = "surface plane of column " + std::to_string(q) + " outside 0..nz-1";
+                return SFDX_ERR_ARG;
+            }
+    PostGrid pg{g->nx, g->ny, g->nz, g->nair, g->a, g->b, g->c, surf};
     int s = post_process(pg, nf, freq, nrec, recv, (const zc *)ex, out_type, rd, results, tipper);
     if (s) { g_err = "station " + std::to_string(s) + " lies outside the interpolation area"; return SFDX_ERR_ARG; }
     return SFDX_OK;
+}
+
+int sfdx_surface_planes(const sfdx_grid *g, const double *sigma, double air_max, int *surf)
+{
+    if (!g || !g->a || !g->b || !g->c || !sigma || !surf || g->nx <= 0 || g->ny <= 0 || g->nz <= 0)
+        return SFDX_ERR_ARG;
+    return guarded([&]() {
+        const HostLevel L = host_level(g, sigma);
+        std::vector<int> planes;
+        surface_planes(L, air_max, planes);
+        std::copy(planes.begin(), planes.end(), surf);
+        for (int v : planes)
+            if (v >= g->nz) {
+                g_err = "a cell column holds no cell above air_max (no ground surface)";
+                return (int)SFDX_ERR_ARG;
+            }
+        return (int)SFDX_OK;
+    });
+}
+
+int sfdx_mesh_coarsen(const sfdx_grid *g, const double *sigma, int k, double *a_c, double *b_c, double *c_c,
+                      double *sigma_c)
+{
+    if (!g || !g->a || !g->b || !g->c || !sigma || !a_c || !b_c || !c_c || !sigma_c || k < 0 || k > 16)
+        return SFDX_ERR_ARG;
+    const int f = 1 << k;
+    if (g->nx % f || g->ny % f || g->nz % f) {
+        g_err = "grid dimensions must be divisible by 2^k";
+        return SFDX_ERR_MESH;
+    }
+    return guarded([&]() {
+        const HostLevel C = coarsen(host_level(g, sigma), k);
+        std::copy(C.a.begin(), C.a.end(), a_c);
+        std::copy(C.b.begin(), C.b.end(), b_c);
+        std::copy(C.c.begin(), C.c.end(), c_c);
+        std::copy(C.sig.begin(), C.sig.end(), sigma_c);
+        return (int)SFDX_OK;
+    });
 }
 
 int sfdx_direct_solve(const sfdx_grid *g, const double *sigma, double freq, double *u)

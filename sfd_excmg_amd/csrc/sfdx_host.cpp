@@ -11,6 +11,7 @@
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
+#include <thread>
 
 namespace sfdx {
 
@@ -162,6 +163,23 @@ HostLevel coarsen(const HostLevel &F, int k)
     return C;
 }
 
+// ------------------------------------------------------------------ topography
+// The reference's topography branch of post_process (src/fdfe_main.f90:604-614,
+// 656-666) takes, per column, the first cell k1 whose centre lies below the
+// ground.  With cells above the ground carved to air that is the first
+// non-air cell, and its top node plane is the ground surface.
+void surface_planes(const HostLevel &L, double air_max, std::vector<int> &surf)
+{
+    const size_t nxy = (size_t)L.nx * L.ny;
+    surf.assign(nxy, L.nz);
+    for (size_t s = 0; s < nxy; s++)
+        for (int k = 0; k < L.nz; k++)
+            if (L.sig[(size_t)k * nxy + s] > air_max) {
+                surf[s] = k;
+                break;
+            }
+}
+
 // ------------------------------------------------------------------ EXCMG tables
 // Lagrange basis of degree n on points xs (src/excmg.f90:754-772).
 static double lagrange(int n, const double *xs, int id, double x)
@@ -274,15 +292,35 @@ void coarse_system(const HostLevel &L, double freq, const std::vector<zc> &edc, 
 // MKL PARDISO bound at run time (libmkl_rt); mtype 6 = complex symmetric.
 typedef void (*pardiso_fn)(void *, const int *, const int *, const int *, const int *, const int *, const void *,
                            const int *, const int *, int *, const int *, int *, const int *, void *, void *, int *);
+typedef int (*mkl_int_fn)(int);
+typedef void (*mkl_void_fn)(int);
 
-static pardiso_fn load_pardiso(std::string &err)
+// PARDISO's parallel factorisation sums its updates in an order that depends on
+// the number of threads (measured: the 78,936-unknown coarse level of the
+// 128x128x96 model moves by 2e-7 relative between 1, 3 and 8 threads), and
+// MKL's dynamic threading may pick fewer threads than requested under load.
+// The coarse solutions seed the EXCMG initial guess, so they are pinned: the
+// dynamic adjustment is switched off and every solve runs on a fixed number
+// of threads set thread-locally (SFDX_PARDISO_THREADS, default min(16, cores)),
+// which makes repeated solves on one machine bit-identical whatever the
+// caller's MKL_NUM_THREADS or load.
+struct MklFns {
+    pardiso_fn pardiso = nullptr;
+    mkl_int_fn set_num_threads_local = nullptr;
+    int threads = 1;
+};
+
+static const MklFns *load_mkl(std::string &err)
 {
     static std::mutex mu;  // batched solves call this from several threads
     std::lock_guard<std::mutex> lock(mu);
-    static pardiso_fn fn = nullptr;
+    static MklFns fns;
     static std::string cached_err;
     static bool tried = false;
-    if (tried) { err = cached_err; return fn; }
+    if (tried) {
+        err = cached_err;
+        return fns.pardiso ? &fns : nullptr;
+    }
     tried = true;
     std::vector<std::string> names;
     if (const char *env = std::getenv("SFDX_MKL_RT")) names.push_back(env);
@@ -292,18 +330,44 @@ static pardiso_fn load_pardiso(std::string &err)
     for (auto &nm : names) {
         void *h = dlopen(nm.c_str(), RTLD_NOW | RTLD_GLOBAL);
         if (!h) continue;
-        fn = (pardiso_fn)dlsym(h, "pardiso");
-        if (fn) return fn;
+        fns.pardiso = (pardiso_fn)dlsym(h, "pardiso");
+        if (!fns.pardiso) continue;
+        // the mixed-case names are MKL's C entry points (the lower- and upper-case
+        // ones take their arguments by reference)
+        fns.set_num_threads_local = (mkl_int_fn)dlsym(h, "MKL_Set_Num_Threads_Local");
+        if (mkl_void_fn set_dynamic = (mkl_void_fn)dlsym(h, "MKL_Set_Dynamic")) set_dynamic(0);
+        int nt = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+        if (const char *env = std::getenv("SFDX_PARDISO_THREADS")) nt = std::max(1, std::atoi(env));
+        fns.threads = nt;
+        return &fns;
     }
     cached_err = "MKL PARDISO (libmkl_rt) not found; set SFDX_MKL_RT";
     err = cached_err;
     return nullptr;
 }
 
+int direct_solve_threads()
+{
+    std::string err;
+    const MklFns *m = load_mkl(err);
+    return m ? m->threads : 0;
+}
+
 bool direct_solve(const CoarseSystem &S, std::vector<zc> &sol, std::string &err)
 {
-    pardiso_fn pardiso = load_pardiso(err);
-    if (!pardiso) return false;
+    const MklFns *mkl = load_mkl(err);
+    if (!mkl) return false;
+    pardiso_fn pardiso = mkl->pardiso;
+    // fixed thread count for this thread's MKL calls (restored on return)
+    const int prev_threads = mkl->set_num_threads_local ? mkl->set_num_threads_local(mkl->threads) : 0;
+    struct Restore {
+        const MklFns *m;
+        int prev;
+        ~Restore()
+        {
+            if (m->set_num_threads_local) m->set_num_threads_local(prev);
+        }
+    } restore{mkl, prev_threads};
     void *pt[64];
     std::memset(pt, 0, sizeof(pt));
     int iparm[64];
@@ -339,6 +403,13 @@ bool direct_solve(const CoarseSystem &S, std::vector<zc> &sol, std::string &err)
 // Surface E and H at cell centres (src/fdfe_main.f90:597-683) and biquadratic
 // interpolation to stations (arho_interp2, src/fdfe_main.f90:718-790), then
 // Z = E H^-1 from the two polarisations (src/fdfe_main.f90:687-705).
+// The surface of column (i,j) is node plane m = g.plane(i,j): nair on the flat
+// datum; with topography the top of the column's first earth cell.  The
+// one-sided z difference spans the cell below the surface, c[m] (the
+// reference's c(nair+1) on the flat datum; it keeps c(nair+1) under
+// topography too, and takes the plane below the first earth cell's top,
+// k1 instead of k1-1, which on a flat surface would be one plane under the
+// datum it uses without topography).
 struct Surface {
     int nx, ny;
     std::vector<zc> f[5][2];  // ex, ey, hx, hy, hz for each polarisation, index i + nx*j
@@ -346,12 +417,13 @@ struct Surface {
 
 static void surface_fields(const PostGrid &g, double freq, const zc *x, int pol, Surface &S)
 {
-    const int nx = g.nx, ny = g.ny, nz = g.nz, m = g.nair;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int64_t n1 = (int64_t)nx * (ny + 1) * (nz + 1), n2 = (int64_t)ny * (nz + 1) * (nx + 1);
     const zc kap(0.0, 2 * kPi * freq * kMu0);
     for (int q = 0; q < 5; q++) S.f[q][pol].assign((size_t)nx * ny, 0.0);
     for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++) {
+            const int m = g.plane(i, j);
             const int64_t idx = (int64_t)m * nx * (ny + 1) + (int64_t)j * nx + i;
             const int64_t idx2 = n1 + (int64_t)m * ny * (nx + 1) + (int64_t)j * (nx + 1) + i;
             const int64_t idx3 = n1 + n2 + (int64_t)m * (nx + 1) * (ny + 1) + (int64_t)j * (nx + 1) + i;
@@ -527,13 +599,14 @@ static void station_interp_T(const PostGrid &g, int rd, const double *recv, int 
 
 static void surface_fields_T(const PostGrid &g, double freq, const std::vector<zc> (&gs)[5], zc *w)
 {
-    const int nx = g.nx, ny = g.ny, nz = g.nz, m = g.nair;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int64_t n1 = (int64_t)nx * (ny + 1) * (nz + 1), n2 = (int64_t)ny * (nz + 1) * (nx + 1);
     const zc kap(0.0, 2 * kPi * freq * kMu0);
     const int64_t P = (int64_t)nx * (ny + 1), Q = (int64_t)ny * (nx + 1);
     for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++) {
             const size_t s = (size_t)i + (size_t)nx * j;
+            const int m = g.plane(i, j);
             const int64_t idx = (int64_t)m * nx * (ny + 1) + (int64_t)j * nx + i;
             const int64_t idx2 = n1 + (int64_t)m * ny * (nx + 1) + (int64_t)j * (nx + 1) + i;
             const int64_t idx3 = n1 + n2 + (int64_t)m * (nx + 1) * (ny + 1) + (int64_t)j * (nx + 1) + i;

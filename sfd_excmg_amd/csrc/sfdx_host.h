@@ -36,15 +36,25 @@ struct CoarseSystem {
 struct PostGrid {
     int nx, ny, nz, nair;
     const double *a, *b, *c;
+    // surface node plane of each cell column (index i + nx*j, 0-based plane
+    // counted from the top of the air); null = the flat datum, plane nair
+    const int *surf = nullptr;
+    int plane(int i, int j) const { return surf ? surf[(size_t)i + (size_t)nx * j] : nair; }
 };
 
 void mt1d_anl(int nz, const double *conv, const double *c, double freq, std::complex<double> *ex,
               std::complex<double> *hx, std::complex<double> *imp, std::complex<double> *exc = nullptr);
 std::vector<std::complex<double>> background_field(const HostLevel &L, double freq, bool full = false);
 HostLevel coarsen(const HostLevel &F, int k);
+// Surface node plane of every cell column of a model whose topography is carved
+// into sigma: the top node plane of the first cell, counted from the top, whose
+// sigma_xx exceeds air_max (S/m); nz if the column is all air.  surf[i + nx*j].
+void surface_planes(const HostLevel &L, double air_max, std::vector<int> &surf);
 void prolong_tables(const HostLevel &fine, int m, ProlongHost &T);
 void coarse_system(const HostLevel &L, double freq, const std::vector<std::complex<double>> &edc, CoarseSystem &S);
 bool direct_solve(const CoarseSystem &S, std::vector<std::complex<double>> &sol, std::string &err);
+// threads every PARDISO call runs on (0 if MKL is unavailable)
+int direct_solve_threads();
 int post_process(const PostGrid &g, int nf, const double *freq, int nrec, const double *recv,
                  const std::complex<double> *ex, int out_type, int rd, double *results, double *tipper);
 // post_process from surface fields already evaluated (on the device):

@@ -271,9 +271,10 @@ void launch_gpbicg(int mode, const DevGrid<R> &g, RedBuf &rb, const GPVecs<R> &v
 // adjoint gradient accumulation over cells (see k_cell_grad) and boundary zeroing
 void launch_cell_grad(const DevGrid<double> &g, double wmu, const zd *e, const zd *lam, double *grad, hipStream_t st);
 void launch_zero_boundary(const Dims &d, zd *v, hipStream_t st);
-// surface E and H at the cell centres of node plane nair, 5 arrays of nx*ny
-void launch_surface_fields(const Dims &d, int nair, const double *a, const double *b, const double *c, double om,
-                           const zd *x, zd *out, hipStream_t st);
+// surface E and H at the cell centres of the surface node plane (nair, or surf[i + nx*j]
+// per column when surf is not null), 5 arrays of nx*ny
+void launch_surface_fields(const Dims &d, int nair, const int *surf, const double *a, const double *b,
+                           const double *c, double om, const zd *x, zd *out, hipStream_t st);
 
 template <typename R>
 void launch_extrap(int64_t n, R c1, const cx<R> *u1, R c2, cx<R> *w, hipStream_t st);

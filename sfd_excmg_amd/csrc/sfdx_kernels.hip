@@ -918,18 +918,22 @@ __global__ __launch_bounds__(TPB) void k_zero_boundary(Dims d, zd *__restrict__ 
     if (!edge_interior(d, S.o, i, j, S.k)) v[S.base + p] = zd(0);
 }
 
-// Surface fields at the cell centres of node plane nair (post_process,
+// Surface fields at the cell centres of the surface node plane (post_process,
 // src/fdfe_main.f90:549-640): E averaged from the two bounding edges, H = curl
-// E / (i omega mu) with a one-sided z difference over c(nair+1).  One thread per
-// surface cell; out[q*nx*ny + i + nx*j] for q = ex, ey, hx, hy, hz.
-__global__ __launch_bounds__(TPB) void k_surface_fields(Dims d, int nair, const double *__restrict__ a,
-                                                        const double *__restrict__ b, const double *__restrict__ c,
-                                                        double om, const zd *__restrict__ x, zd *__restrict__ out)
+// E / (i omega mu) with a one-sided z difference over the cell below the
+// surface.  The surface is node plane nair (the flat datum) or, with
+// topography, surf[i + nx*j] per column (0 <= surf < nz, checked by the host).
+// One thread per surface cell; out[q*nx*ny + i + nx*j] for q = ex, ey, hx, hy, hz.
+__global__ __launch_bounds__(TPB) void k_surface_fields(Dims d, int nair, const int *__restrict__ surf,
+                                                        const double *__restrict__ a, const double *__restrict__ b,
+                                                        const double *__restrict__ c, double om,
+                                                        const zd *__restrict__ x, zd *__restrict__ out)
 {
     const int p = blockIdx.x * TPB + threadIdx.x;
-    const int nx = d.nx, ny = d.ny, m = nair;
+    const int nx = d.nx, ny = d.ny;
     if (p >= nx * ny) return;
     const int j = p / nx, i = p - j * nx;
+    const int m = surf ? surf[p] : nair;
     const int64_t n1 = d.XN, n2 = d.YN;
     const int64_t idx = (int64_t)m * nx * (ny + 1) + (int64_t)j * nx + i;
     const int64_t idx2 = n1 + (int64_t)m * ny * (nx + 1) + (int64_t)j * (nx + 1) + i;
@@ -2273,10 +2277,11 @@ void launch_cell_grad(const DevGrid<double> &g, double wmu, const zd *e, const z
                        g.sig, wmu, e, lam, grad);
 }
 
-void launch_surface_fields(const Dims &d, int nair, const double *a, const double *b, const double *c, double om,
-                           const zd *x, zd *out, hipStream_t st)
+void launch_surface_fields(const Dims &d, int nair, const int *surf, const double *a, const double *b,
+                           const double *c, double om, const zd *x, zd *out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_surface_fields, grid1((int64_t)d.nx * d.ny), dim3(TPB), 0, st, d, nair, a, b, c, om, x, out);
+    hipLaunchKernelGGL(k_surface_fields, grid1((int64_t)d.nx * d.ny), dim3(TPB), 0, st, d, nair, surf, a, b, c, om, x,
+                       out);
 }
 
 void launch_zero_boundary(const Dims &d, zd *v, hipStream_t st)

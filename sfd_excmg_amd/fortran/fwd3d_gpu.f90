@@ -4,9 +4,10 @@
 !   fwd_solver(modelname,a,b,c,nair,level,slv,eps,sigma,freq,ex)
 !       same signature as sfd_excmg/src/fdfe_main.f90:1565; allocates
 !       ex(2*nl,nf) (xy then yx per frequency) and fills it on the device.
-!   post_process(modelname,a,b,c,nair,abr,freq,recv,u,out_type)
-!       same signature as sfd_excmg/src/fdfe_main.f90:549 (without topo);
-!       writes <modelname>.res.
+!   post_process(modelname,a,b,c,nair,abr,freq,recv,u,out_type[,topo])
+!       same signature as sfd_excmg/src/fdfe_main.f90:549; with topo (the
+!       elevations read_topo returns) the surface of each column follows the
+!       ground; writes <modelname>.res.
 !
 ! Run-time switches (the reference's compile-time parameters in
 ! global_para.f90:5-19): SFDX_FWD_ACC (0 = single level, the reference
@@ -102,13 +103,15 @@ contains
         st%fine_seconds, ' s, fine iterations', st%fine_iters_total
   end subroutine
 
-  subroutine post_process(modelname, a, b, c, nair, abr, freq, recv, u, out_type)
+  subroutine post_process(modelname, a, b, c, nair, abr, freq, recv, u, out_type, topo)
     character(len=*) :: modelname
     real(8), intent(in), target :: a(:), b(:), c(:), freq(:), recv(:,:)
     integer, intent(in) :: nair, abr(:), out_type
     complex(8), allocatable, intent(in) :: u(:,:)
+    real(8), allocatable, optional :: topo(:,:)
     real(8), allocatable, target :: a_(:), b_(:), c_(:)
     real(8), allocatable :: rt(:,:), res(:,:), tip(:,:), arho(:,:), tipper(:,:)
+    integer(c_int), allocatable :: surf(:,:)
     type(sfdx_grid_t) :: g
     integer :: nrec, nf, rc, k, u2
 
@@ -119,7 +122,20 @@ contains
     g%a = c_loc(a_); g%b = c_loc(b_); g%c = c_loc(c_)
     allocate(rt(3, nrec), res(4, nrec * nf), tip(4, nrec * nf))
     rt = transpose(recv)
-    rc = sfdx_post_process(g, int(nf, c_int), freq, int(nrec, c_int), rt, u, int(out_type, c_int), 2_c_int, res, tip)
+    if (present(topo)) then
+      if (allocated(topo)) then
+        ! topography branch (fdfe_main.f90:604-614, 656-666)
+        allocate(surf(size(a), size(b)))
+        call surface_planes_of(c, nair, topo, surf)
+        rc = sfdx_post_process_topo(g, surf, int(nf, c_int), freq, int(nrec, c_int), rt, u, int(out_type, c_int), &
+                                    2_c_int, res, tip)
+      else
+        rc = sfdx_post_process(g, int(nf, c_int), freq, int(nrec, c_int), rt, u, int(out_type, c_int), 2_c_int, &
+                               res, tip)
+      end if
+    else
+      rc = sfdx_post_process(g, int(nf, c_int), freq, int(nrec, c_int), rt, u, int(out_type, c_int), 2_c_int, res, tip)
+    end if
     if (rc /= 0) then
       write(*,*) 'Error in post_process: ', sfdx_error_text()
       stop 1

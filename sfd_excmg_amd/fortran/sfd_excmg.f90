@@ -4,7 +4,7 @@ program sfd_excmg
   use file_operation
   use fwd3d
   implicit none
-  real(8), allocatable :: a(:), b(:), c(:), sigma(:,:), freq(:), recv(:,:)
+  real(8), allocatable :: a(:), b(:), c(:), sigma(:,:), freq(:), recv(:,:), topo(:,:)
   complex(8), allocatable :: ex(:,:)
   integer :: out_type, nair, level, slv, abu(6), n
   real(8) :: eps
@@ -12,7 +12,7 @@ program sfd_excmg
 
   n = command_argument_count()
   call get_command_argument(1, name)
-  call read_input(name, a, b, c, nair, level, eps, slv, sigma, freq, recv, out_type, abu)
+  call read_input(name, a, b, c, nair, level, eps, slv, sigma, freq, recv, out_type, abu, topo)
   if (n > 1) then
     call get_command_argument(2, str)
     read(str, *) level
@@ -20,5 +20,5 @@ program sfd_excmg
     read(str, *) eps
   end if
   call fwd_solver(name, a, b, c, nair, level, slv, eps, sigma, freq, ex)
-  call post_process(name, a, b, c, nair, abu, freq, recv, ex, out_type)
+  call post_process(name, a, b, c, nair, abu, freq, recv, ex, out_type, topo)
 end program

@@ -25,6 +25,7 @@ module sfdx_iface
     integer(c_int) :: mg_levels, mg_coarse_pairs
     real(c_double) :: restart_drop
     integer(c_int) :: use_graph, nodal_smoother, coarse_solver, full_background, spmv_tile
+    real(c_double) :: surface_sigma
   end type
 
   type, bind(C) :: sfdx_stats_t
@@ -68,6 +69,16 @@ module sfdx_iface
         bind(C, name="sfdx_post_process")
       import :: c_int, c_double, c_double_complex, sfdx_grid_t
       type(sfdx_grid_t), intent(in) :: g
+      integer(c_int), value :: nf, nrec, out_type, rd
+      real(c_double), intent(in) :: freq(*), recv(*)
+      complex(c_double_complex), intent(in) :: ex(*)
+      real(c_double) :: results(*), tipper(*)
+    end function
+    integer(c_int) function sfdx_post_process_topo(g, surf, nf, freq, nrec, recv, ex, out_type, rd, results, &
+        tipper) bind(C, name="sfdx_post_process_topo")
+      import :: c_int, c_double, c_double_complex, sfdx_grid_t
+      type(sfdx_grid_t), intent(in) :: g
+      integer(c_int), intent(in) :: surf(*)
       integer(c_int), value :: nf, nrec, out_type, rd
       real(c_double), intent(in) :: freq(*), recv(*)
       complex(c_double_complex), intent(in) :: ex(*)

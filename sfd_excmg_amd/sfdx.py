@@ -50,7 +50,7 @@ class Opts(ctypes.Structure):
                 ("sigma_floor32", ctypes.c_double), ("mg_levels", ctypes.c_int), ("mg_coarse_pairs", ctypes.c_int),
                 ("restart_drop", ctypes.c_double), ("use_graph", ctypes.c_int),
                 ("nodal_smoother", ctypes.c_int), ("coarse_solver", ctypes.c_int),
-                ("full_background", ctypes.c_int), ("spmv_tile", ctypes.c_int)]
+                ("full_background", ctypes.c_int), ("spmv_tile", ctypes.c_int), ("surface_sigma", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -81,7 +81,8 @@ class Job(ctypes.Structure):
 EXPORTS = ["sfdx_create", "sfdx_destroy", "sfdx_last_error", "sfdx_default_opts", "sfdx_fwd_solve",
            "sfdx_post_process", "sfdx_mt1d_anl", "sfdx_apply_operator", "sfdx_ssor_apply", "sfdx_hybrid_apply", "sfdx_div_correct",
            "sfdx_extint_v", "sfdx_bench_kernels", "sfdx_direct_solve", "sfdx_solve_batch", "sfdx_misfit",
-           "sfdx_adjoint", "sfdx_fwd_response"]
+           "sfdx_adjoint", "sfdx_fwd_response", "sfdx_post_process_topo", "sfdx_surface_planes",
+           "sfdx_mesh_coarsen"]
 
 _lib = None
 
@@ -101,6 +102,9 @@ def load(path: str = LIB_PATH):
     L.sfdx_default_opts.argtypes = [ctypes.POINTER(Opts)]
     L.sfdx_fwd_solve.argtypes = [P, ctypes.POINTER(Grid), P, I, P, ctypes.POINTER(Opts), P, ctypes.POINTER(Stats)]
     L.sfdx_post_process.argtypes = [ctypes.POINTER(Grid), I, P, I, P, P, I, I, P, P]
+    L.sfdx_post_process_topo.argtypes = [ctypes.POINTER(Grid), P, I, P, I, P, P, I, I, P, P]
+    L.sfdx_surface_planes.argtypes = [ctypes.POINTER(Grid), P, D, P]
+    L.sfdx_mesh_coarsen.argtypes = [ctypes.POINTER(Grid), P, I, P, P, P, P]
     L.sfdx_mt1d_anl.argtypes = [I, P, P, D, P, P, P]
     L.sfdx_apply_operator.argtypes = [P, ctypes.POINTER(Grid), P, D, I, P, P]
     L.sfdx_ssor_apply.argtypes = [P, ctypes.POINTER(Grid), P, D, D, I, P, P]
@@ -344,9 +348,11 @@ def fwd_solver(modelname, a, b, c, nair, level, slv, eps, sigma, freq, ctx: Cont
             ctx.close()
 
 
-def post_process(a, b, c, nair, freqs, recv, ex, out_type=1, rd=2):
+def post_process(a, b, c, nair, freqs, recv, ex, out_type=1, rd=2, surf=None):
     """Mirror of fwd3d::post_process (fdfe_main.f90:549-716): returns
-    results(nrec*nf, 4) and tipper(nrec*nf, 4)."""
+    results(nrec*nf, 4) and tipper(nrec*nf, 4).  surf (optional, (nx, ny)
+    integer node planes, see surface_planes) is the topography branch
+    (fdfe_main.f90:604-614, 656-666)."""
     G = _GridHolder(a, b, c, nair)
     fr = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
     recv = np.ascontiguousarray(recv, np.float64)
@@ -354,9 +360,35 @@ def post_process(a, b, c, nair, freqs, recv, ex, out_type=1, rd=2):
     nrec = recv.shape[0]
     res = np.zeros((nrec * len(fr), 4))
     tip = np.zeros((nrec * len(fr), 4))
-    _check(load().sfdx_post_process(ctypes.byref(G.g), len(fr), _ptr(fr), nrec, _ptr(recv), _ptr(exf),
-                                     int(out_type), int(rd), _ptr(res), _ptr(tip)))
+    if surf is None:
+        _check(load().sfdx_post_process(ctypes.byref(G.g), len(fr), _ptr(fr), nrec, _ptr(recv), _ptr(exf),
+                                         int(out_type), int(rd), _ptr(res), _ptr(tip)))
+    else:
+        sp = np.ascontiguousarray(np.asarray(surf, np.int32).T).reshape(-1)   # index i + nx*j
+        _check(load().sfdx_post_process_topo(ctypes.byref(G.g), _ptr(sp), len(fr), _ptr(fr), nrec, _ptr(recv),
+                                              _ptr(exf), int(out_type), int(rd), _ptr(res), _ptr(tip)))
     return res, tip
+
+
+def surface_planes(a, b, c, sigma, air_max):
+    """Surface node plane of every cell column, (nx, ny) int32, of a model with
+    topography carved into sigma (sfdx_surface_planes)."""
+    G = _GridHolder(a, b, c)
+    out = np.zeros(len(G.a) * len(G.b), np.int32)
+    _check(load().sfdx_surface_planes(ctypes.byref(G.g), _ptr(sigma_flat(sigma)), float(air_max), _ptr(out)))
+    return out.reshape(len(G.b), len(G.a)).T.copy()
+
+
+def mesh_coarsen(a, b, c, sigma, k):
+    """Mirror of mesh_coarsen (fdfe_main.f90:18-84): widths and sigma(ncell, 6)
+    of the grid coarsened by 2^k (sfdx_mesh_coarsen)."""
+    G = _GridHolder(a, b, c)
+    f = 1 << int(k)
+    ac, bc, cc = np.zeros(len(G.a) // f), np.zeros(len(G.b) // f), np.zeros(len(G.c) // f)
+    sc = np.zeros(6 * len(ac) * len(bc) * len(cc))
+    _check(load().sfdx_mesh_coarsen(ctypes.byref(G.g), _ptr(sigma_flat(sigma)), int(k), _ptr(ac), _ptr(bc), _ptr(cc),
+                                    _ptr(sc)))
+    return ac, bc, cc, sc.reshape(6, -1).T.copy()
 
 
 def direct_solve(a, b, c, sigma, freq):

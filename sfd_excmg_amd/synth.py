@@ -197,23 +197,38 @@ def topography_depth(rng, x, y, amplitude=500.0):
     return h.max() - h
 
 
+def carve_topography(m: Model, elevation, sigma_air: float = SIGMA_AIR_DEFAULT):
+    """Apply ground elevations elevation(nx, ny) (m above the flat datum) to
+    m.sigma in place, as the driver's corrected .topo reader does: every cell
+    whose centre lies above the ground becomes air (sigma_air diagonal,
+    off-diagonals 0); every air cell whose centre lies below it takes the tensor
+    of the column's first earth cell (reference: file_opt.f90:97-128)."""
+    elevation = np.asarray(elevation, np.float64)
+    nx, ny = m.nx, m.ny
+    zc = np.cumsum(m.c) - m.c / 2 - m.c[: m.nair].sum()    # cell-centre depth below the datum
+    sig = m.sigma.reshape(m.nz, ny, nx, 6)
+    earth = sig[m.nair].copy()                              # (ny, nx, 6)
+    ground = -elevation.T                                   # (ny, nx) depth of the ground
+    for k in range(m.nz):
+        air = zc[k] < ground
+        sig[k][air, :3] = sigma_air
+        sig[k][air, 3:] = 0
+        if k < m.nair:
+            raise_ = ~air
+            sig[k][raise_] = earth[raise_]
+    m.meta["topo"] = elevation.copy()
+    return m
+
+
 def make_aniso_topography_128(seed: int = 3, sigma_air: float = SIGMA_AIR_DEFAULT, **kw) -> Model:
     """The 128x128x96 anisotropic block model plus undulating topography: earth cells whose centre lies above the
-    ground surface become air."""
+    ground surface become air.  meta["topo"] holds the ground elevation (nx, ny) above the flat datum (<= 0 here:
+    the highest ground point lies on the datum)."""
     m = make_aniso_blocks_128(sigma_air=sigma_air, **kw)
     rng = np.random.default_rng(seed)
     xc, yc = cell_centres(m.a), cell_centres(m.b)
     depth = topography_depth(rng, xc, yc)
-    zc = np.cumsum(m.c) - m.c / 2 - m.c[: m.nair].sum()    # depth below the datum
-    nx, ny = m.nx, m.ny
-    for k in range(m.nair, m.nz):
-        air = zc[k] < depth
-        if not air.any():
-            continue
-        sl = m.sigma[k * nx * ny:(k + 1) * nx * ny]
-        mask = air.T.reshape(-1)                              # index j*nx + i
-        sl[mask, :3] = sigma_air
-        sl[mask, 3:] = 0
+    carve_topography(m, -depth, sigma_air)
     m.name = "aniso_topography_128"
     m.meta["topography_max_depth"] = float(depth.max())
     return m
@@ -346,6 +361,30 @@ def write_dat(m: Model, path_prefix: str):
             print(f"{r[0]:.10e} {r[1]:.10e} {r[2]:.10e}", file=f)
 
 
+def write_topo(m: Model, path_prefix: str):
+    """<name>.topo for the driver's corrected reader: "1 n" then the n*n ground
+    elevations of m.meta["topo"], i fastest (square grids)."""
+    if m.nx != m.ny:
+        raise ValueError("the .topo window is square")
+    el = np.asarray(m.meta["topo"], np.float64)
+    with open(path_prefix + ".topo", "w") as f:
+        print(f"1 {m.nx}", file=f)
+        for j in range(m.ny):
+            for i in range(m.nx):
+                print(f"{el[i, j]:.10e}", file=f)
+
+
+def read_topo(path_prefix: str, nx: int, ny: int):
+    """Ground elevations (nx, ny) from <name>.topo, 0 outside its window."""
+    with open(path_prefix + ".topo") as f:
+        toks = f.read().split()
+    n1, n2 = int(toks[0]), int(toks[1])
+    vals = np.array([float(t) for t in toks[2:2 + (n2 - n1 + 1) ** 2]]).reshape(n2 - n1 + 1, n2 - n1 + 1)
+    el = np.zeros((nx, ny))
+    el[n1 - 1:n2, n1 - 1:n2] = vals.T                      # vals[j, i]
+    return el
+
+
 def read_fwd_dat(path_prefix: str, sigma_air: float = SIGMA_AIR_DEFAULT) -> Model:
     """Parse <prefix>.fwd and <prefix>.dat like read_input (file_opt.f90:8-95)."""
     with open(path_prefix + ".fwd") as f:
@@ -380,4 +419,7 @@ def read_fwd_dat(path_prefix: str, sigma_air: float = SIGMA_AIR_DEFAULT) -> Mode
     nrec = int(next(jt))
     recv = np.array([float(next(jt)) for _ in range(3 * nrec)]).reshape(nrec, 3)
     name = path_prefix.rstrip("/").split("/")[-1]
-    return Model(name, a, b, c, nair, sig, freqs, recv, level=level, eps=eps, solver=solver, out_type=out_type)
+    m = Model(name, a, b, c, nair, sig, freqs, recv, level=level, eps=eps, solver=solver, out_type=out_type)
+    if topo:
+        carve_topography(m, read_topo(path_prefix, nx, ny), sigma_air)
+    return m

@@ -110,3 +110,79 @@ def test_misfit_field_gradient_matches_directional_difference(rd):
     fd = (fp - fm) / (2 * h)
     an = float(np.real(np.sum(w * dirn)))
     assert abs(fd - an) <= 1e-6 * abs(an)
+
+
+def test_post_process_topography_matches_oracle():
+    """sfdx_post_process_topo / sfdx_surface_planes (the topography branch,
+    fdfe_main.f90:604-614, 656-666) against the oracle on the carved
+    topography generator at 32x32x24: planes from the carved sigma, then
+    impedances and tippers on random fields, both station samplings."""
+    m = synth.make_aniso_topography_128(nx=32, nz_earth=20, nair=4, nstations=8)
+    p1 = sfdx.surface_planes(m.a, m.b, m.c, m.sigma, 1e-6)
+    p2 = O.surface_planes(m.a, m.b, m.c, m.sigma, 1e-6)
+    assert np.array_equal(p1, p2) and p1.max() > m.nair
+    rng = np.random.default_rng(8)
+    ex = rng.standard_normal((2 * m.nl, 2)) + 1j * rng.standard_normal((2 * m.nl, 2))
+    freqs = np.array([0.1, 2.0])
+    for rd in (1, 2):
+        r1, t1 = sfdx.post_process(m.a, m.b, m.c, m.nair, freqs, m.recv, ex, out_type=1, rd=rd, surf=p1)
+        r2, t2 = O.post_process(m.a, m.b, m.c, m.nair, freqs, m.recv, ex, out_type=1, rd=rd, surf=p2)
+        assert np.allclose(r1, r2, rtol=1e-11, atol=1e-12 * abs(r2).max())
+        assert np.allclose(t1, t2, rtol=1e-11, atol=1e-12 * abs(t2).max())
+        # the flat datum differs on these fields
+        r0, _ = O.post_process(m.a, m.b, m.c, m.nair, freqs, m.recv, ex, out_type=1, rd=rd)
+        assert not np.allclose(r0, r2, rtol=1e-3)
+
+
+def test_post_process_topography_rejects_bad_planes():
+    m = synth.make_iso_block_model("bp", 16, 16, 12, 4, seed=1, nblocks=1, freq=1.0, level=3, nstations=4)
+    ex = np.zeros((2 * m.nl, 1), complex)
+    surf = np.full((m.nx, m.ny), m.nz, np.int32)
+    with pytest.raises(sfdx.SfdxError):
+        sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, surf=surf)
+    air = m.sigma.copy()
+    air[:, :3] = 1e-8
+    air[:, 3:] = 0
+    with pytest.raises(sfdx.SfdxError):
+        sfdx.surface_planes(m.a, m.b, m.c, air, 1e-6)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_mesh_coarsen_matches_oracle(k):
+    """sfdx_mesh_coarsen (the product's coarsening, mesh_coarsen at
+    fdfe_main.f90:18-84 with the index fixed) against the oracle on a
+    full-tensor model with non-uniform widths."""
+    m = synth.make_aniso_blocks_128(nx=32, nz_earth=20, nair=4, nstations=8)
+    a1, b1, c1, s1 = sfdx.mesh_coarsen(m.a, m.b, m.c, m.sigma, k)
+    a2, b2, c2, s2 = O.mesh_coarsen(m.a, m.b, m.c, m.sigma, k)
+    assert np.allclose(a1, a2, rtol=1e-15) and np.allclose(b1, b2, rtol=1e-15) and np.allclose(c1, c2, rtol=1e-15)
+    assert s1.shape == s2.shape
+    assert np.allclose(s1, s2, rtol=1e-13, atol=1e-15 * np.abs(s2).max())
+
+
+def test_direct_solve_is_independent_of_mkl_threads():
+    """The coarse direct solve runs on a fixed number of MKL threads, so the
+    caller's MKL_NUM_THREADS cannot change its result bit pattern (the
+    factorisation's summation order depends on the thread count)."""
+    import subprocess
+    import sys
+    import tempfile
+    script = (
+        "import sys, numpy as np\n"
+        f"sys.path[:0] = [{ROOT!r}, {os.path.join(ROOT, 'oracle')!r}]\n"
+        "import oracle as O\n"
+        "from sfd_excmg_amd import sfdx, synth\n"
+        "m = synth.make_aniso_blocks_128(nx=64, nz_earth=40, nair=8, nstations=8)\n"
+        "a, b, c, s = O.mesh_coarsen(m.a, m.b, m.c, m.sigma, 2)\n"
+        "np.save(sys.argv[1], sfdx.direct_solve(a, b, c, s, 0.1))\n")
+    outs = []
+    with tempfile.TemporaryDirectory() as td:
+        for nt in ("1", "3"):
+            path = os.path.join(td, f"u{nt}.npy")
+            env = dict(os.environ, MKL_NUM_THREADS=nt, OMP_NUM_THREADS=nt)
+            r = subprocess.run([sys.executable, "-c", script, path], env=env, capture_output=True, text=True)
+            if r.returncode != 0 and "PARDISO" in r.stderr:
+                pytest.skip("MKL PARDISO unavailable")
+            assert r.returncode == 0, r.stderr
+            outs.append(np.load(path))
+    assert np.array_equal(outs[0].view(np.uint64), outs[1].view(np.uint64))

@@ -649,3 +649,132 @@ def test_misfit_sensitivity_matches_brute_force_difference():
         sm_[cell] *= np.exp(-h)
         fd = (phi_of(sp_) - phi_of(sm_)) / (2 * h)
         assert sens[q] == pytest.approx(fd, rel=2e-5)
+
+
+# ------------------------------------------------------------------ topography and station algebra
+def _linear_in_z_fields(a, b, c, nair, alpha, beta):
+    """Edge fields of two polarisations that are linear in depth: Ex = alpha +
+    beta z on every x-edge of node plane k (pol 0), Ey likewise (pol 1), all
+    other components 0.  z is the depth of the node plane below the datum."""
+    nx, ny, nz = len(a), len(b), len(c)
+    zn = np.concatenate([[0.0], np.cumsum(c)]) - c[:nair].sum()
+    nl = O.nedges(nx, ny, nz)
+    n1 = nx * (ny + 1) * (nz + 1)
+    n2 = (nx + 1) * ny * (nz + 1)
+    x1 = np.zeros(nl, complex)
+    x2 = np.zeros(nl, complex)
+    x1[:n1] = np.repeat(alpha + beta * zn, nx * (ny + 1))
+    x2[n1:n1 + n2] = np.repeat(alpha + beta * zn, (nx + 1) * ny)
+    return x1, x2, zn
+
+
+def test_topography_surface_samples_its_own_plane_and_cell():
+    """The topography branch (src/fdfe_main.f90:604-614, 656-666) with a
+    per-column surface plane m: on fields linear in depth, E at the surface is
+    exactly alpha + beta z_m and the one-sided z difference over the cell below
+    the surface gives dE/dz = beta exactly, so H = beta/(i w mu).  Sampling the
+    plane below (the reference's k1) or the flat-datum spacing c(nair+1) on a
+    grid whose layer thicknesses vary fails one of the two."""
+    rng = np.random.default_rng(21)
+    nx, ny, nair = 7, 6, 3
+    a, b = rng.uniform(80, 120, nx), rng.uniform(80, 120, ny)
+    c = rng.uniform(20, 200, 12)
+    alpha, beta, f = 0.7 - 0.2j, 1e-3 + 4e-4j, 0.3
+    x1, x2, zn = _linear_in_z_fields(a, b, c, nair, alpha, beta)
+    surf = rng.integers(1, 9, size=(nx, ny))
+    ex, ey, hx, hy, hz = O.surface_fields(a, b, c, nair, f, x1, x2, surf)
+    kap = 1j * 2 * np.pi * f * MU0
+    assert np.allclose(ex[:, :, 0], alpha + beta * zn[surf], rtol=1e-14)
+    assert np.allclose(ey[:, :, 1], alpha + beta * zn[surf], rtol=1e-14)
+    assert np.allclose(hy[:, :, 0], beta / kap, rtol=1e-12)
+    assert np.allclose(hx[:, :, 1], -beta / kap, rtol=1e-12)
+    assert np.all(hz == 0) and np.all(ey[:, :, 0] == 0)
+    # the flat datum is the constant plane nair
+    flat = O.surface_fields(a, b, c, nair, f, x1, x2)
+    same = O.surface_fields(a, b, c, nair, f, x1, x2, np.full((nx, ny), nair))
+    for u, v in zip(flat, same):
+        assert np.array_equal(u, v)
+    # planted: the plane below the surface cell's top, or the datum spacing
+    wrong = O.surface_fields(a, b, c, nair, f, x1, x2, surf + 1)
+    assert not np.allclose(wrong[0][:, :, 0], alpha + beta * zn[surf], rtol=1e-6)
+    assert abs(c[nair] - c[surf]).max() > 1.0   # a datum-spacing slip would change hy
+
+
+def test_surface_planes_from_sigma_match_elevation_test():
+    """Planes read off the carved conductivity equal the reference's test of
+    cell centres against the ground elevation (src/fdfe_main.f90:606-611),
+    on the topography generator at 64x64x48 (ground up to ~950 m below the
+    datum, so up to several carved layers per column)."""
+    m = synth.make_aniso_topography_128(nx=64, nz_earth=40, nair=8, nstations=20)
+    p1 = O.surface_planes(m.a, m.b, m.c, m.sigma, 1e-6)
+    p2 = O.topo_surface_planes(m.c, m.nair, m.meta["topo"])
+    assert np.array_equal(p1, p2)
+    assert p1.min() == m.nair and p1.max() > m.nair + 3
+    # the flat model's surface is the datum everywhere
+    flat = synth.make_aniso_blocks_128(nx=64, nz_earth=40, nair=8, nstations=20)
+    assert np.all(O.surface_planes(flat.a, flat.b, flat.c, flat.sigma, 1e-6) == flat.nair)
+
+
+def test_impedance_and_tipper_solve_the_two_polarisation_system():
+    """Z and T of post_process (src/fdfe_main.f90:686-705) are defined by
+    E = Z H and Hz = T (Hx, Hy) over the two polarisations.  On random fields
+    the oracle's closed forms must equal numpy's solve of those 2x2 systems at
+    every station; a transposed index or a sign slip in zz or tx/ty fails."""
+    rng = np.random.default_rng(23)
+    m = synth.make_iso_block_model("zt", 16, 16, 12, 4, seed=3, nblocks=2, freq=1.0, level=3, nstations=5)
+    ex_all = rng.standard_normal((2 * m.nl, 1)) + 1j * rng.standard_normal((2 * m.nl, 1))
+    res, tip = O.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex_all, out_type=1, rd=1)
+    nl = m.nl
+    flds = O.surface_fields(m.a, m.b, m.c, m.nair, m.freqs[0], ex_all[:nl, 0], ex_all[nl:, 0])
+    ar1 = O.arho_interp2(1, 1, m.a, m.b, flds, m.recv)
+    ar2 = O.arho_interp2(2, 1, m.a, m.b, flds, m.recv)
+    for r in range(len(m.recv)):
+        H = np.array([[ar1[r, 2], ar1[r, 3]], [ar2[r, 2], ar2[r, 3]]])      # rows: pol, cols: hx, hy
+        zrow_x = np.linalg.solve(H, np.array([ar1[r, 0], ar2[r, 0]]))       # Ex = Zxx hx + Zxy hy
+        zrow_y = np.linalg.solve(H, np.array([ar1[r, 1], ar2[r, 1]]))       # Ey = Zyx hx + Zyy hy
+        t = np.linalg.solve(H, np.array([ar1[r, 4], ar2[r, 4]]))            # hz = Tx hx + Ty hy
+        assert np.isclose(res[r, 0] + 1j * res[r, 1], zrow_x[1], rtol=1e-10)
+        assert np.isclose(res[r, 2] + 1j * res[r, 3], zrow_y[0], rtol=1e-10)
+        assert np.isclose(tip[r, 0] + 1j * tip[r, 1], t[0], rtol=1e-10)
+        assert np.isclose(tip[r, 2] + 1j * tip[r, 3], t[1], rtol=1e-10)
+
+
+def test_layered_model_has_no_tipper():
+    """A horizontally layered model has no vertical magnetic field: with the
+    3-point 1-D FD field of the same z grid as Dirichlet data the discrete
+    solution is 1-D (see the layered-model test above), Hz vanishes at every
+    surface cell and T = 0 at every station, while Z stays finite.  A tipper
+    formula reading hx or hy where hz belongs would return O(1)."""
+    nx, ny, f = 12, 12, 1.0
+    rng = np.random.default_rng(6)
+    a, b = rng.uniform(600, 1400, nx), rng.uniform(600, 1400, ny)
+    nair = 3
+    ce = np.array([200.0, 300, 500, 800, 1500, 3000])
+    c = synth.with_air(ce, nair)
+    nz = len(c)
+    layer_sig = np.concatenate([np.full(nair, 1e-8), [0.01, 0.03, 0.1, 0.05, 0.003, 0.003]])
+    sig = np.zeros((nx * ny * nz, 6))
+    for k in range(nz):
+        sig[k * nx * ny:(k + 1) * nx * ny, :3] = layer_sig[k]
+    edc, _, _ = O.mt1d_anl(f, O.background_column(sig, nx, ny, nz), c)
+    kap = 1j * 2 * np.pi * f * MU0
+    n = nz + 1
+    M = np.zeros((n, n), complex)
+    rhs = np.zeros(n, complex)
+    M[0, 0] = 1; rhs[0] = edc[0]; M[-1, -1] = 1; rhs[-1] = edc[nz]
+    for k in range(1, nz):
+        cm, cp = c[k - 1], c[k]
+        M[k, k - 1] = -2 / (cm * (cm + cp)); M[k, k + 1] = -2 / (cp * (cm + cp))
+        se = (layer_sig[k - 1] * cm + layer_sig[k] * cp) / (cm + cp)
+        M[k, k] = -(M[k, k - 1] + M[k, k + 1]) - kap * se
+    e1 = np.linalg.solve(M, rhs)
+    S = O.build_system(a, b, c, sig, f, e_dc=np.concatenate([e1, e1]))
+    lu = spla.splu(S.A.tocsc())
+    ex = np.concatenate([lu.solve(S.p1), lu.solve(S.p2)])[:, None]
+    xc, yc = synth.cell_centres(a), synth.cell_centres(b)
+    recv = np.array([[x, y, 0.0] for x in np.linspace(xc[2], xc[-3], 4) for y in np.linspace(yc[2], yc[-3], 4)])
+    res, tip = O.post_process(a, b, c, nair, [f], recv, ex, out_type=1, rd=1)
+    flds = O.surface_fields(a, b, c, nair, f, ex[:S.A.shape[0], 0], ex[S.A.shape[0]:, 0])
+    assert np.abs(flds[4]).max() < 1e-6 * np.abs(flds[3]).max()
+    assert np.abs(res).max() > 1e-3
+    assert np.abs(tip).max() < 1e-6
