@@ -420,6 +420,16 @@ def test_aniso64_workload_matches_oracle_fixture(ctx, precision):
     zxy, zyx = _impedances(m, ex)
     for p, q in ((zxy, g["zxy"]), (zyx, g["zyx"])):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+    # the library's own post-process (host FP64) of the same fields
+    rl, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, out_type=1, rd=2)
+    for p, q in ((rl[:, 0] + 1j * rl[:, 1], g["zxy"]), (rl[:, 2] + 1j * rl[:, 3], g["zyx"])):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+    # the device post-process (sfdx_fwd_response: surface fields on the GPU,
+    # stations on the host) against the oracle's post-process of oracle fields
+    rr, _, st2 = ctx.fwd_response(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, opts, out_type=1, rd=2)
+    assert st2.status == 0
+    for p, q in ((rr[:, 0] + 1j * rr[:, 1], g["zxy"]), (rr[:, 2] + 1j * rr[:, 3], g["zyx"])):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
 def test_halfspace_on_gpu(ctx):
