@@ -115,6 +115,12 @@ typedef struct {
                            the device and keep the fastest (cached in the context).  The
                            tile sets the grouping of the fused reductions, so a fixed tile
                            keeps repeated runs bit-identical */
+    int ssor_cross_mass;  /* edge SSOR inside the hybrid preconditioner: 0 (default) K plus the
+                             lumped (diagonal) mass -- the cross-orientation mass couplings of
+                             anisotropic cells are left out, so each colour sweep reads one
+                             conductivity component instead of three; 1: the full operator.
+                             The plain SSOR preconditioner (the reference's) and the multigrid
+                             smoothers always use the full operator */
     double surface_sigma; /* sfdx_fwd_response and sfdx_adjoint: 0 (default) take the station
                              fields on the flat datum, node plane nair (fdfe_main.f90:597-640);
                              > 0: topography carved into sigma -- the surface of each cell column
@@ -272,7 +278,8 @@ int sfdx_apply_operator(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, 
 int sfdx_ssor_apply(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, double omega,
                     int precision, const double *x, double *y);
 /* y = M_h^-1 x, the hybrid (auxiliary-space) preconditioner of the device
- * solvers: SSOR_e(x) + (i/(omega mu)) G N_P(G^T x), with N_P the two-step Jacobi
+ * solvers with their default edge SSOR (lumped mass, ssor_cross_mass = 0):
+ * SSOR_e(x) + (i/(omega mu)) G N_P(G^T x), with N_P the two-step Jacobi
  * polynomial (nodal 1) or the 8-colour SSOR (nodal 0) on P = G^T M_sigma G.
  * No counterpart in the reference (whose preconditioner is the SSOR alone,
  * div_correc.f90:171-220); exposed so that callers and tests can apply the

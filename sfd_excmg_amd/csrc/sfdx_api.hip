@@ -1200,6 +1200,14 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         choose_tile(ctx, D64[i].g, "L" + std::to_string(i) + "/d", o->spmv_tile, o->verbose);
         if (lo_fine && i == Lv) choose_tile(ctx, D32[i].g, "L" + std::to_string(i) + "/f", o->spmv_tile, o->verbose);
     }
+    // edge SSOR of the hybrid preconditioner: lumped mass only unless asked
+    // for the cross-orientation terms (the plain SSOR and the multigrid
+    // smoothers keep the full operator)
+    const bool xmass = o->precond != SFDX_PRECOND_HYBRID || o->ssor_cross_mass != 0;
+    for (int i = first_up; i <= Lv; i++) {
+        D64[i].g.ssor_xmass = xmass;
+        D32[i].g.ssor_xmass = xmass;
+    }
     S.spmv_tile = D64[Lv].g.spmv_variant;
     const int64_t NLf = make_dims(fine.nx, fine.ny, fine.nz).NL;
     // surface node plane per column for the device post-process (topography
@@ -1345,6 +1353,8 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     ensure_partials(ctx, edge_launch(d).nblocks() + 64);
     choose_tile(ctx, D64.g, "ADJ/d", o->spmv_tile, o->verbose);
     if (o->precision == SFDX_PREC_MIXED) choose_tile(ctx, D32.g, "ADJ/f", o->spmv_tile, o->verbose);
+    const bool xmass = o->precond != SFDX_PRECOND_HYBRID || o->ssor_cross_mass != 0;
+    D64.g.ssor_xmass = D32.g.ssor_xmass = xmass;
     double *gdev = pool_vec<double>(ctx, "adj/g", d.NC);
     SFDX_CHECK(hipMemsetAsync(gdev, 0, d.NC * sizeof(double), ctx->st));
     zd *ed = pool_vec<zd>(ctx, "adj/e", 2 * d.NL), *wd = pool_vec<zd>(ctx, "adj/w", 2 * d.NL);
@@ -1467,6 +1477,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->full_background = 0;
     o->spmv_tile = 0;
     o->surface_sigma = 0;
+    o->ssor_cross_mass = 0;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)
@@ -1820,6 +1831,7 @@ int sfdx_hybrid_apply(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, do
         const Dims d = make_dims(L.nx, L.ny, L.nz);
         const double wmu = 2 * 3.141592653589793 * freq * kMu0;
         LevelDev<double> D = upload_level<double>(ctx, L, "op");
+        D.g.ssor_xmass = false;  // the solvers' default edge SSOR (lumped mass)
         ensure_P(ctx, D, "op");
         zd *xd = pool_vec<zd>(ctx, "op/x", d.NL), *yd = pool_vec<zd>(ctx, "op/y", d.NL);
         zd *tn = pool_vec<zd>(ctx, "op/tn", d.NP), *ph = pool_vec<zd>(ctx, "op/ph", d.NP);
@@ -1830,6 +1842,7 @@ int sfdx_hybrid_apply(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, do
             // the mixed path's preconditioner: edge SSOR in complex64, nodal part
             // in complex128 on the FP32 symmetric nodal operator
             LevelDev<float> F = upload_level<float>(ctx, L, "op");
+            F.g.ssor_xmass = false;
             float *ps32 = pool_vec<float>(ctx, "op/Ps32", (size_t)10 * d.NP);
             launch_psym32(d.NP, D.P, ps32, ctx->st);
             zf *in32 = pool_vec<zf>(ctx, "op/in32", d.NL), *z32 = pool_vec<zf>(ctx, "op/z32", d.NL);

@@ -61,6 +61,7 @@ struct DevGrid {
     R *ra = nullptr, *rb = nullptr, *rc = nullptr;  // reciprocal widths (device)
     R *sig = nullptr;                               // 6*NC (device)
     int spmv_variant = 0;                           // k-marching SpMV tile (launch_spmv_march)
+    bool ssor_xmass = true;                         // launch_ssor: with the cross-orientation mass terms
 };
 
 // Reduction scratch: partial sums per block, final slots.

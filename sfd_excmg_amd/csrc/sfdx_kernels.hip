@@ -650,7 +650,7 @@ __global__ __launch_bounds__(TPB) void k_mass(Dims d, const R *__restrict__ a, c
 // ------------------------------------------------------------------ K2 SSOR
 // Forward (DIR=0): out_e = omega*(in_e - sum_{lower colours} A_en out_n)/D_e
 // Backward (DIR=1): out_e = omega*(out_e*D_e*(2-omega)/omega - sum_{upper} A_en out_n)/D_e
-template <typename R, int O, int PAR, int DIR>
+template <typename R, int O, int PAR, int DIR, bool XM>
 __global__ __launch_bounds__(TPB) void k_ssor(Dims d, const R *__restrict__ a, const R *__restrict__ b,
                                               const R *__restrict__ c, const R *__restrict__ ra,
                                               const R *__restrict__ rb, const R *__restrict__ rc,
@@ -668,7 +668,7 @@ __global__ __launch_bounds__(TPB) void k_ssor(Dims d, const R *__restrict__ a, c
         return;
     }
     Row<R> r;
-    row_coeffs<R, true>(d, a, b, c, sig, O, i, j, k, r, ra, rb, rc);
+    row_coeffs<R, true, XM>(d, a, b, c, sig, O, i, j, k, r, ra, rb, rc);
     // group orientations: O=0 -> (1,2); O=1 -> (0,2); O=2 -> (0,1)
     constexpr int OA = (O == 0) ? 1 : 0;
     constexpr int OB = (O == 2) ? 1 : 2;
@@ -682,11 +682,13 @@ __global__ __launch_bounds__(TPB) void k_ssor(Dims d, const R *__restrict__ a, c
     }
     if (GA) {
 #pragma unroll
-        for (int q = 4; q < 8; q++) s = fma_c(s, coef_of(r.kg[q], r.km[q], wmu), out[r.nb[q]]);
+        for (int q = 4; q < 8; q++)
+            s = XM ? fma_c(s, coef_of(r.kg[q], r.km[q], wmu), out[r.nb[q]]) : fma_c(s, cx<R>(r.kg[q], (R)0), out[r.nb[q]]);
     }
     if (GB) {
 #pragma unroll
-        for (int q = 8; q < 12; q++) s = fma_c(s, coef_of(r.kg[q], r.km[q], wmu), out[r.nb[q]]);
+        for (int q = 8; q < 12; q++)
+            s = XM ? fma_c(s, coef_of(r.kg[q], r.km[q], wmu), out[r.nb[q]]) : fma_c(s, cx<R>(r.kg[q], (R)0), out[r.nb[q]]);
     }
     const cx<R> D = coef_of(r.kgd, r.kmd, wmu);
     const R rden = (R)1 / (D.re * D.re + D.im * D.im);
@@ -1850,8 +1852,12 @@ template <typename R, int O, int PAR, int DIR>
 static void ssor_one(const DevGrid<R> &g, R wmu, const cx<R> *in, cx<R> *out, R omega, hipStream_t st)
 {
     SliceLaunch L = colour_launch(g.d, 2 * O + PAR);
-    hipLaunchKernelGGL((k_ssor<R, O, PAR, DIR>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, in,
-                       out, omega);
+    if (g.ssor_xmass)
+        hipLaunchKernelGGL((k_ssor<R, O, PAR, DIR, true>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc,
+                           g.sig, wmu, in, out, omega);
+    else
+        hipLaunchKernelGGL((k_ssor<R, O, PAR, DIR, false>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb,
+                           g.rc, g.sig, wmu, in, out, omega);
 }
 
 template <typename R>

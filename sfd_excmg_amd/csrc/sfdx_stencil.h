@@ -109,8 +109,11 @@ struct Row {
 // sigma accessor: sig[comp*NC + cell]
 // With RCP the reciprocal widths ra, rb, rc (1/a, 1/b, 1/c) replace the four
 // divisions of a row by multiplications (device kernels); without, the widths
-// are divided (host assembly).
-template <typename R, bool RCP = false>
+// are divided (host assembly).  With XM false the cross-orientation mass
+// couplings of anisotropic cells (the sigma_xy, sigma_xz, sigma_yz terms) are
+// left out (km[0..11] = 0) and only the row's own diagonal conductivity is
+// read: the lumped-mass approximation some preconditioners use.
+template <typename R, bool RCP = false, bool XM = true>
 SFDX_HD void row_coeffs(const Dims &d, const R *__restrict__ a, const R *__restrict__ b,
                         const R *__restrict__ c, const R *__restrict__ sig, int o, int i, int j, int k,
                         Row<R> &r, const R *__restrict__ ra = nullptr, const R *__restrict__ rb = nullptr,
@@ -136,10 +139,10 @@ SFDX_HD void row_coeffs(const Dims &d, const R *__restrict__ a, const R *__restr
         const R *sxx = sig, *sxy = sig + 3 * NC, *sxz = sig + 4 * NC;
         r.kmd = ai * (sxx[C00] * bm * cm + sxx[C10] * bp * cm + sxx[C01] * bm * cp + sxx[C11] * bp * cp) / 4;
         for (int s = 0; s < 4; s++) r.km[s] = 0;
-        const R my1 = ai * (sxy[C00] * bm * cm + sxy[C01] * bm * cp) / 8;
-        const R my2 = ai * (sxy[C10] * bp * cm + sxy[C11] * bp * cp) / 8;
-        const R mz1 = ai * (sxz[C00] * bm * cm + sxz[C10] * bp * cm) / 8;
-        const R mz2 = ai * (sxz[C01] * bm * cp + sxz[C11] * bp * cp) / 8;
+        const R my1 = XM ? ai * (sxy[C00] * bm * cm + sxy[C01] * bm * cp) / 8 : (R)0;
+        const R my2 = XM ? ai * (sxy[C10] * bp * cm + sxy[C11] * bp * cp) / 8 : (R)0;
+        const R mz1 = XM ? ai * (sxz[C00] * bm * cm + sxz[C10] * bp * cm) / 8 : (R)0;
+        const R mz2 = XM ? ai * (sxz[C01] * bm * cp + sxz[C11] * bp * cp) / 8 : (R)0;
         r.nb[4] = ey_id(d, i, j - 1, k);     r.kg[4] = hy;  r.km[4] = my1;
         r.nb[5] = ey_id(d, i + 1, j - 1, k); r.kg[5] = -hy; r.km[5] = my1;
         r.nb[6] = ey_id(d, i, j, k);         r.kg[6] = -hy; r.km[6] = my2;
@@ -164,10 +167,10 @@ SFDX_HD void row_coeffs(const Dims &d, const R *__restrict__ a, const R *__restr
         const R *syy = sig + NC, *sxy = sig + 3 * NC, *syz = sig + 5 * NC;
         r.kmd = bj * (syy[D00] * am * cm + syy[D10] * ap * cm + syy[D01] * am * cp + syy[D11] * ap * cp) / 4;
         for (int s = 0; s < 4; s++) r.km[s] = 0;
-        const R mx1 = bj * (sxy[D00] * am * cm + sxy[D01] * am * cp) / 8;
-        const R mx2 = bj * (sxy[D10] * ap * cm + sxy[D11] * ap * cp) / 8;
-        const R mz1 = bj * (syz[D00] * am * cm + syz[D10] * ap * cm) / 8;
-        const R mz2 = bj * (syz[D01] * am * cp + syz[D11] * ap * cp) / 8;
+        const R mx1 = XM ? bj * (sxy[D00] * am * cm + sxy[D01] * am * cp) / 8 : (R)0;
+        const R mx2 = XM ? bj * (sxy[D10] * ap * cm + sxy[D11] * ap * cp) / 8 : (R)0;
+        const R mz1 = XM ? bj * (syz[D00] * am * cm + syz[D10] * ap * cm) / 8 : (R)0;
+        const R mz2 = XM ? bj * (syz[D01] * am * cp + syz[D11] * ap * cp) / 8 : (R)0;
         r.nb[4] = ex_id(d, i - 1, j, k);     r.kg[4] = hx;  r.km[4] = mx1;
         r.nb[5] = ex_id(d, i, j, k);         r.kg[5] = -hx; r.km[5] = mx2;
         r.nb[6] = ex_id(d, i - 1, j + 1, k); r.kg[6] = -hx; r.km[6] = mx1;
@@ -192,10 +195,10 @@ SFDX_HD void row_coeffs(const Dims &d, const R *__restrict__ a, const R *__restr
         const R *szz = sig + 2 * NC, *sxz = sig + 4 * NC, *syz = sig + 5 * NC;
         r.kmd = ck * (szz[F00] * bm * am + szz[F01] * bp * am + szz[F10] * bm * ap + szz[F11] * bp * ap) / 4;
         for (int s = 0; s < 4; s++) r.km[s] = 0;
-        const R mx1 = ck * (sxz[F00] * bm * am + sxz[F01] * bp * am) / 8;
-        const R mx2 = ck * (sxz[F10] * bm * ap + sxz[F11] * bp * ap) / 8;
-        const R my1 = ck * (syz[F00] * bm * am + syz[F10] * bm * ap) / 8;
-        const R my2 = ck * (syz[F01] * bp * am + syz[F11] * bp * ap) / 8;
+        const R mx1 = XM ? ck * (sxz[F00] * bm * am + sxz[F01] * bp * am) / 8 : (R)0;
+        const R mx2 = XM ? ck * (sxz[F10] * bm * ap + sxz[F11] * bp * ap) / 8 : (R)0;
+        const R my1 = XM ? ck * (syz[F00] * bm * am + syz[F10] * bm * ap) / 8 : (R)0;
+        const R my2 = XM ? ck * (syz[F01] * bp * am + syz[F11] * bp * ap) / 8 : (R)0;
         r.nb[4] = ex_id(d, i - 1, j, k);     r.kg[4] = hx;  r.km[4] = mx1;
         r.nb[5] = ex_id(d, i, j, k);         r.kg[5] = -hx; r.km[5] = mx2;
         r.nb[6] = ex_id(d, i - 1, j, k + 1); r.kg[6] = -hx; r.km[6] = mx1;

@@ -24,7 +24,7 @@ module sfdx_iface
     real(c_double) :: sigma_floor32
     integer(c_int) :: mg_levels, mg_coarse_pairs
     real(c_double) :: restart_drop
-    integer(c_int) :: use_graph, nodal_smoother, coarse_solver, full_background, spmv_tile
+    integer(c_int) :: use_graph, nodal_smoother, coarse_solver, full_background, spmv_tile, ssor_cross_mass
     real(c_double) :: surface_sigma
   end type
 

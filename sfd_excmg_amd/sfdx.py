@@ -50,7 +50,8 @@ class Opts(ctypes.Structure):
                 ("sigma_floor32", ctypes.c_double), ("mg_levels", ctypes.c_int), ("mg_coarse_pairs", ctypes.c_int),
                 ("restart_drop", ctypes.c_double), ("use_graph", ctypes.c_int),
                 ("nodal_smoother", ctypes.c_int), ("coarse_solver", ctypes.c_int),
-                ("full_background", ctypes.c_int), ("spmv_tile", ctypes.c_int), ("surface_sigma", ctypes.c_double)]
+                ("full_background", ctypes.c_int), ("spmv_tile", ctypes.c_int), ("ssor_cross_mass", ctypes.c_int),
+                ("surface_sigma", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):

@@ -15,7 +15,7 @@ size the FP64 oracle reaches, through the C ABI.
   device post-process (sfdx_fwd_response) and the library's host post-process.
 * C4 (batch): the 8 isotropic models of uneven sizes plus 2 of the 16 sounding
   frequencies of the 128x128x96 model through sfdx_solve_batch; every item
-  against a single solve, the two smallest models against the oracle
+  against a single solve, the 48^3 and 64x64x48 models against the oracle
   (tests/golden/batch_oracle_impedances.npz).
 * C5 (adjoint): the random-field generator at 48x48x32 against the oracle's
   central differences on 3 cells (tests/golden/c5_48_oracle_gradient.npz);
@@ -159,7 +159,7 @@ def test_c4_batch_of_uneven_models_and_sounding_frequencies(ctx):
     """The 8 uneven isotropic models (1 Hz) and the 128x128x96 model at the
     first and last of the 16 sounding frequencies, as one mixed-precision
     batch over 8 streams: every item agrees with its single solve, and the
-    48^3 and 56^3 models with the oracle."""
+    48^3 and 64x64x48 models with the oracle."""
     g = np.load(os.path.join(GOLDEN, "batch_oracle_impedances.npz"))
     small = synth.make_uneven_iso_models()
     big = synth.make_aniso_blocks_128()
