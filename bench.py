@@ -7,15 +7,14 @@ fine-level relative residual of 1e-8 (interior rows).
 
 A step is one full forward solve (all levels, both polarisations).  `value`
 is the fine-grid solve time (both polarisations) of the faster converged GPU
-path, averaged over the timed steps:
-  * FP64: complex128 COCR with the hybrid preconditioner (multicolour edge SSOR
-    plus the nodal auxiliary-space correction by a two-step Jacobi polynomial,
-    in complex128);
-  * mixed: complex128 COCR, operator and residual; on the fine level the edge
-    SSOR of the hybrid preconditioner in complex64, its nodal part in
-    complex128 (the coarser Krylov level uses the complex128 preconditioner);
-    restarts from the true residual after every 1e-6 drop and on stagnation,
-    and a final check of the true residual.
+path, averaged over the timed steps.  Both paths run COCR preconditioned by a
+geometric multigrid V-cycle over the 3 finest grids (hybrid Gauss-Seidel
+smoothing: lumped-mass multicolour edge SSOR sweeps plus the nodal correction
+by a two-step Jacobi polynomial, residuals in split edge + nodal form):
+  * FP64: everything in complex128;
+  * mixed: complex128 COCR, operator and residual; the V-cycle in complex64
+    (the complex128 true residual is recomputed after every 1e-6 drop and on
+    stagnation, and once more before convergence is accepted).
 The roofline object is for the stencil SpMV (k_spmv_march), timed with HIP events on
 the library's stream.  The CPU baseline is the FP64 oracle (CSR, lexicographic
 SSOR, COCR) on one core, timed on a bounded number of fine-level iterations and
@@ -65,22 +64,32 @@ def spmv_bytes(nl, ncell, sc, sr, nsig=6):
 USE_GRAPH = 1
 
 
+# Preconditioner of the headline solves: a multigrid V-cycle over 3 grids
+# (128x128x96, 64x64x48, 32x32x24) with one smoothing pair on the coarsest,
+# hybrid Gauss-Seidel smoothing (lumped-mass edge SSOR sweeps plus the
+# Jacobi-polynomial nodal correction).  Measured at this workload, fine level,
+# FP64 / mixed: 1.167 / 0.836 s (about 420 iterations per polarisation),
+# against 1.36 / 1.23 s with the one-level hybrid preconditioner (about 1240);
+# 4 grids 1.216 / 0.878 s, 2 grids with 2 pairs 1.280 / 0.853 s.
+MG_LEVELS, MG_COARSE_PAIRS = 3, 1
+
+
 def solve_opts(m, precision, eps, verbose=0, precond=None):
     from sfd_excmg_amd import sfdx
 
-    # The hybrid preconditioner corrects the gradient (divergence) components in
-    # every iteration, so the periodic divergence correction of the reference
-    # (every 50 iterations in this workload) is not applied: it breaks the COCR
+    # Both preconditioners correct the gradient (divergence) components in every
+    # iteration, so the periodic divergence correction of the reference (every
+    # 50 iterations in this workload) is not applied: it breaks the COCR
     # recurrences (measured: divergence when the directions are kept, 2.3x more
     # iterations with restarts).
-    # Nodal part of the hybrid preconditioner: the one-pass Jacobi polynomial
-    # (1.76 s against 2.20 s with the nodal SSOR on this workload in complex128).
-    # In the mixed path it runs in complex128 while the edge SSOR runs in
-    # complex64 (fewer iterations than an all-complex64 preconditioner).
+    # Nodal part: the one-pass Jacobi polynomial (the 8-colour nodal SSOR costs
+    # more per iteration and, inside the V-cycle, takes more iterations).
+    # In the mixed path the whole V-cycle runs in complex64 inside complex128 COCR.
     nodal = 1
     return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0,
                              norm=sfdx.NORM_INTERIOR, precision=precision, verbose=verbose,
-                             precond=sfdx.PRECOND_HYBRID if precond is None else precond,
+                             precond=sfdx.PRECOND_MG if precond is None else precond,
+                             mg_levels=MG_LEVELS, mg_coarse_pairs=MG_COARSE_PAIRS,
                              sigma_floor32=0.0, restart_drop=RESTART_DROP, maxit_excmg=8000, use_graph=USE_GRAPH,
                              nodal_smoother=nodal)
 
@@ -198,7 +207,9 @@ def run_c4(ctx, args):
         big.freqs = big.freqs[::4]
     models = [big] + small
     prec = sfdx.PREC_FP64 if args.batch_fp64 else sfdx.PREC_MIXED
-    opts = solve_opts(big, prec, 1e-8)
+    # the V-cycle's coarse grids stop resolving the skin depth at the upper
+    # sounding frequencies, so the batch keeps the one-level hybrid preconditioner
+    opts = solve_opts(big, prec, 1e-8, precond=sfdx.PRECOND_HYBRID)
     nrhs = 2 * sum(len(m.freqs) for m in models)
     t0 = time.perf_counter()
     exb, stb = ctx.solve_batch(models, opts, nstreams=args.nstreams)
@@ -211,7 +222,8 @@ def run_c4(ctx, args):
     zerr = None
     if not args.batch_fp64:
         # impedances of the mixed batch against a complex128 batch of the same items
-        ex64, _ = ctx.solve_batch(models, solve_opts(big, sfdx.PREC_FP64, 1e-8), nstreams=args.nstreams)
+        ex64, _ = ctx.solve_batch(models, solve_opts(big, sfdx.PREC_FP64, 1e-8, precond=sfdx.PRECOND_HYBRID),
+                                  nstreams=args.nstreams)
         zerr = max(float(np.max(np.abs(impedances(m, p) - impedances(m, q)) / np.abs(impedances(m, q))))
                    for m, p, q in zip(models, exb, ex64))
     print(json.dumps({"metric": "batched forward solves (16 frequencies x 128x128x96 + 8 uneven models): right-hand sides per second", "value": round(nrhs / t_batch, 3),
@@ -406,7 +418,9 @@ def main():
         "data": "synthetic (seeded anisotropic block model)",
         "config": {"workload": ("mt3d_fwd_128x128x96_aniso_0.1Hz_2pol_excmg4_cocr" if not args.small
                                 else "mt3d_fwd_64x64x48_aniso_small"), "cells": [m.nx, m.ny, m.nz], "edges": nl,
-                   "levels": m.level, "residual": "interior rows, relative 1e-8", "preconditioner": "hybrid"},
+                   "levels": m.level, "residual": "interior rows, relative 1e-8",
+                   "preconditioner": f"multigrid V-cycle, {MG_LEVELS} grids, {MG_COARSE_PAIRS} coarsest smoothing "
+                                     "pair(s), hybrid Gauss-Seidel smoothing"},
         "fp64_gpu": {"fine_s": round(f64["fine"], 4), "wall_s": round(f64["wall"], 3), "fine_iters": f64["fine_iters"],
                      "status": f64["status"]},
         "mixed_gpu": {"fine_s": round(mx["fine"], 4), "wall_s": round(mx["wall"], 3), "fine_iters": mx["fine_iters"],

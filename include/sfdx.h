@@ -91,17 +91,20 @@ typedef struct {
                              below this floor (S/m) are raised to it, bounding the
                              amplification of near-null air modes; 0 = off */
     int mg_levels;      /* multigrid: grids in the hierarchy counted from the fine
-                           grid (0 = as many as the dimensions allow); never fewer
-                           than the EXCMG levels */
+                           grid (0 = as many as the dimensions allow, at least 2).
+                           Fewer grids than EXCMG levels end the V-cycle above the
+                           coarsest EXCMG level; Krylov levels below its coarsest
+                           grid then use the hybrid preconditioner */
     int mg_coarse_pairs; /* multigrid: symmetric smoothing pairs on the coarsest grid */
     double restart_drop; /* SFDX_PREC_MIXED: restart the FP64 Krylov recurrences from
                             the true residual whenever the residual has dropped by
                             this factor since the last restart (0 = never) */
     int use_graph;      /* 1: replay the steady-state COCR iteration as a captured HIP
                            graph (only without periodic divergence correction) */
-    int nodal_smoother; /* nodal part of the hybrid preconditioner: 0 8-colour SSOR on
-                           P = G^T M G, 1 (default) two damped Jacobi steps (one pass
-                           over P instead of 16 dependent colour sweeps) */
+    int nodal_smoother; /* nodal part of the hybrid preconditioner and of the multigrid
+                           smoother: 0 8-colour SSOR / Gauss-Seidel on P = G^T M G, 1
+                           (default) the two-step damped Jacobi polynomial (one pass over
+                           P instead of 8 dependent colour sweeps per smoothing step) */
     int coarse_solver;  /* EXCMG levels 1-2: 0 host direct solve (MKL PARDISO, the
                            reference's u1u2), 1 Krylov solves on the device from a
                            zero guess to eps*10^(i-L) */

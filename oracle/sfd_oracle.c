@@ -56,10 +56,13 @@
  *   orc_extint_v       polynomial reproduction of I2 / I4 on nonuniform nodes;
  *                      extrapolation constants from the text
  *   orc_mesh_coarsen   conductance integral preserved
- *   post_process       rd = 1 exact on quadratic fields; half-space gives 100
- *                      ohm-m and 45 degrees.  rd = 2 replicates the reference's
- *                      transposed station indexing: parity unpinned (no
- *                      reference fixture exists to confirm it)
+ *   post_process       (oracle.py) rd = 1 exact on quadratic fields; a
+ *                      half-space gives 100 ohm-m and 45 degrees; rd = 2 (the
+ *                      reference's transposed station indexing) pinned in
+ *                      closed form on biquadratic fields; Z and T equal numpy's
+ *                      solve of E = Z H and Hz = T (Hx, Hy); a layered model has
+ *                      T = 0; the topography branch samples each column's own
+ *                      plane with its own cell thickness on fields linear in z
  */
 #include <complex.h>
 #include <math.h>

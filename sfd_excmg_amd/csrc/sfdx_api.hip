@@ -260,6 +260,7 @@ struct MGHier {
     std::vector<LevelDev<R> *> lev;           // index = level (1-based), lev[0] unused
     std::vector<std::array<double *, 3>> wl;  // wl[l]: node weights of level l vs l-1
     int coarse_pairs = 4;
+    int nodal = 0;  // nodal part of the smoother: 0 8-colour Gauss-Seidel on P, 1 Jacobi polynomial
 };
 
 std::vector<double> mg_weights_1d(const std::vector<double> &wf, const std::vector<double> &wc)
@@ -287,10 +288,10 @@ void mg_vcycle(sfdx_ctx *ctx, MGHier<R> &H, int l, R wmu, const cx<R> *r, cx<R> 
     ensure_P(ctx, D, tg);
     hipStream_t st = ctx->st;
     // pre-smoothing from zero: z = GS_e r, psi = GS_n G^T r
-    launch_hybrid_gs<R>(D.g, wmu, D.P, r, z, tn, psi, 0, st, false);
+    launch_hybrid_gs<R>(D.g, wmu, D.P, r, z, tn, psi, 0, st, false, H.nodal);
     auto smooth_add = [&](int dir) {
         launch_resid_split<R>(D.g, wmu, z, psi, r, gt, t, st);
-        launch_hybrid_gs<R>(D.g, wmu, D.P, t, w, tn, ph, dir, st, false);
+        launch_hybrid_gs<R>(D.g, wmu, D.P, t, w, tn, ph, dir, st, false, H.nodal);
         launch_axpby<R>(d.NL, cx<R>(1), w, cx<R>(1), z, st);
         launch_axpby<R>(d.NP, cx<R>(1), ph, cx<R>(1), psi, st);
     };
@@ -1142,7 +1143,10 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
             D32[i] = upload_level<float>(ctx, lev[i], "L" + std::to_string(i), o->sigma_floor32);
     }
     // Multigrid hierarchy mg[1..Lmg] (mg[Lmg] = fine); the EXCMG level i is
-    // mg[i + off].  Grids coarser than the EXCMG levels are uploaded here.
+    // mg[i + off].  Grids coarser than the EXCMG levels are uploaded here; with
+    // fewer grids than EXCMG levels (off < 0) the V-cycle stops above the
+    // coarsest EXCMG level, and EXCMG levels below its coarsest grid are
+    // solved with the hybrid preconditioner.
     int Lmg = Lv;
     if (use_mg) {
         int m = 1;
@@ -1150,12 +1154,12 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                std::min(std::min(fine.nx, fine.ny), fine.nz) >> m >= 2)
             m++;
         Lmg = o->mg_levels > 0 ? std::min(o->mg_levels, m) : m;
-        Lmg = std::max(Lmg, Lv);
+        Lmg = std::max(Lmg, 2);
     }
     const int off = Lmg - Lv;
     std::vector<HostLevel> mgh(Lmg + 1);
-    std::vector<LevelDev<double>> X64(off + 1);
-    std::vector<LevelDev<float>> X32(off + 1);
+    std::vector<LevelDev<double>> X64(std::max(off, 0) + 1);
+    std::vector<LevelDev<float>> X32(std::max(off, 0) + 1);
     MGHier<double> H64;
     MGHier<float> H32;
     if (use_mg) {
@@ -1163,6 +1167,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         H32.lev.assign(Lmg + 1, nullptr);
         H64.wl.assign(Lmg + 1, {nullptr, nullptr, nullptr});
         H64.coarse_pairs = H32.coarse_pairs = o->mg_coarse_pairs > 0 ? o->mg_coarse_pairs : 4;
+        H64.nodal = H32.nodal = o->nodal_smoother;
         for (int j = 1; j <= Lmg; j++) {
             if (j > off) {
                 mgh[j] = lev[j - off];
@@ -1200,13 +1205,17 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         choose_tile(ctx, D64[i].g, "L" + std::to_string(i) + "/d", o->spmv_tile, o->verbose);
         if (lo_fine && i == Lv) choose_tile(ctx, D32[i].g, "L" + std::to_string(i) + "/f", o->spmv_tile, o->verbose);
     }
-    // edge SSOR of the hybrid preconditioner: lumped mass only unless asked
-    // for the cross-orientation terms (the plain SSOR and the multigrid
-    // smoothers keep the full operator)
-    const bool xmass = o->precond != SFDX_PRECOND_HYBRID || o->ssor_cross_mass != 0;
+    // edge SSOR of the hybrid preconditioner and of the multigrid smoothers:
+    // lumped mass only unless asked for the cross-orientation terms (the plain
+    // SSOR preconditioner, the reference's, keeps the full operator)
+    const bool xmass = o->precond == SFDX_PRECOND_SSOR || o->ssor_cross_mass != 0;
     for (int i = first_up; i <= Lv; i++) {
         D64[i].g.ssor_xmass = xmass;
         D32[i].g.ssor_xmass = xmass;
+    }
+    for (int j = 1; j <= off; j++) {
+        X64[j].g.ssor_xmass = xmass;
+        X32[j].g.ssor_xmass = xmass;
     }
     S.spmv_tile = D64[Lv].g.spmv_variant;
     const int64_t NLf = make_dims(fine.nx, fine.ny, fine.nz).NL;
@@ -1277,9 +1286,13 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 const int maxit = o->use_excmg ? o->maxit_excmg : o->maxit_plain;
                 SFDX_CHECK(hipStreamSynchronize(ctx->st));
                 Timer tl;
-                LevelSolveResult R = solve_level(ctx, *o, D64[i], (lo_fine && i == Lv) ? &D32[i] : nullptr,
-                                                 tag, ff, edc, pol, x, tol, maxit, use_mg ? &H64 : nullptr,
-                                                 (use_mg && lo_fine && i == Lv) ? &H32 : nullptr, i + off);
+                // levels below the multigrid hierarchy's coarsest grid: hybrid
+                const bool mg_here = use_mg && i + off >= 1;
+                sfdx_opts oi = *o;
+                if (use_mg && !mg_here) oi.precond = SFDX_PRECOND_HYBRID;
+                LevelSolveResult R = solve_level(ctx, oi, D64[i], (lo_fine && i == Lv) ? &D32[i] : nullptr,
+                                                 tag, ff, edc, pol, x, tol, maxit, mg_here ? &H64 : nullptr,
+                                                 (mg_here && lo_fine && i == Lv) ? &H32 : nullptr, i + off);
                 SFDX_CHECK(hipStreamSynchronize(ctx->st));
                 const double secs = tl.s();
                 if (i == Lv) {
@@ -1353,7 +1366,7 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     ensure_partials(ctx, edge_launch(d).nblocks() + 64);
     choose_tile(ctx, D64.g, "ADJ/d", o->spmv_tile, o->verbose);
     if (o->precision == SFDX_PREC_MIXED) choose_tile(ctx, D32.g, "ADJ/f", o->spmv_tile, o->verbose);
-    const bool xmass = o->precond != SFDX_PRECOND_HYBRID || o->ssor_cross_mass != 0;
+    const bool xmass = o->precond == SFDX_PRECOND_SSOR || o->ssor_cross_mass != 0;
     D64.g.ssor_xmass = D32.g.ssor_xmass = xmass;
     double *gdev = pool_vec<double>(ctx, "adj/g", d.NC);
     SFDX_CHECK(hipMemsetAsync(gdev, 0, d.NC * sizeof(double), ctx->st));

@@ -236,7 +236,8 @@ void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, 
 // add_grad the nodal part is folded in (out += (i/wmu) G phi).
 template <typename R>
 void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode, cx<R> *phi,
-                      int dir, hipStream_t st, bool add_grad = true);
+                      int dir, hipStream_t st, bool add_grad = true,
+                      int nodal = 0);  // nodal 0: 8-colour GS on P in the sweep's direction; 1: Jacobi polynomial
 // t = r - A z - M (G psi): residual of z + (i/wmu) G psi, using A G = -i wmu M G
 // so that the (possibly huge) gradient part never meets the curl-curl term.
 template <typename R>

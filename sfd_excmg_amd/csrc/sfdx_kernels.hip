@@ -2219,7 +2219,7 @@ void launch_hybrid_mixed(const DevGrid<double> &g64, const DevGrid<float> &g32, 
 // sweeps, so that the backward step is the transpose of the forward one.
 template <typename R>
 void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode, cx<R> *phi,
-                      int dir, hipStream_t st, bool add_grad)
+                      int dir, hipStream_t st, bool add_grad, int nodal)
 {
     if (dir == 0) {
         ssor_one<R, 0, 0, 0>(g, wmu, in, out, (R)1, st);
@@ -2237,6 +2237,13 @@ void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, c
         ssor_one<R, 0, 0, 2>(g, wmu, in, out, (R)1, st);
     }
     SliceLaunch Ln = node_launch(g.d);
+    if (nodal == 1) {
+        // the two-step Jacobi polynomial on the symmetric form of P: one pass, the
+        // same symmetric operator before and after the coarse correction
+        const R *Ps = P + 19 * g.d.NP;
+        hipLaunchKernelGGL((k_gradT<R, true>), Ln.grid, Ln.block, 0, st, g.d, g.a, g.b, g.c, in, tnode, Ps);
+        launch_pjacobi<R, R>(g.d, Ps, tnode, phi, (R)0.8, st);
+    } else {
     hipLaunchKernelGGL((k_gradT<R, false>), Ln.grid, Ln.block, 0, st, g.d, g.a, g.b, g.c, in, tnode,
                        (const R *)nullptr);
     if (dir == 0) {
@@ -2257,6 +2264,7 @@ void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, c
         pssor_one<R, 2, 2>(g.d, P, tnode, phi, st);
         pssor_one<R, 1, 2>(g.d, P, tnode, phi, st);
         pssor_one<R, 0, 2>(g.d, P, tnode, phi, st);
+    }
     }
     if (!add_grad) return;  // caller keeps the nodal part phi (split form)
     const cx<R> s((R)0, (R)1 / wmu);
@@ -2381,7 +2389,7 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
     template void launch_cocr_upd_grad<R>(const DevGrid<R> &, RedBuf &, cx<R> *, cx<R> *, const cx<R> *, cx<R> *,  \
                                           cx<R> *, const cx<R> *, const cx<R> *, R, hipStream_t);                 \
     template void launch_hybrid_gs<R>(const DevGrid<R> &, R, const R *, const cx<R> *, cx<R> *, cx<R> *, cx<R> *,  \
-                                      int, hipStream_t, bool);                                                    \
+                                      int, hipStream_t, bool, int);                                                    \
     template void launch_resid_split<R>(const DevGrid<R> &, R, const cx<R> *, const cx<R> *, const cx<R> *,       \
                                         cx<R> *, cx<R> *, hipStream_t);                                           \
     template void launch_prolong_node_add<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *,    \

@@ -27,7 +27,7 @@ from sfd_excmg_amd import synth  # noqa: E402
 
 FREQS = np.array([0.1, 10.0])
 STEP = 1e-3
-OPTS = O.Options(solver=O.SOLVER_COCR, corr_intv=50, norm="interior")
+OPTS = O.Options(solver=O.SOLVER_COCR, corr_intv=50, norm="interior", maxit_excmg=5000)
 
 
 def model(seed=5):
@@ -51,7 +51,10 @@ def cells(m):
 
 
 def solve(m, sigma):
-    ex, _ = O.fwd_solver(m.a, m.b, m.c, m.nair, m.level, 1e-11, sigma, m.freqs, OPTS)
+    ex, rep = O.fwd_solver(m.a, m.b, m.c, m.nair, m.level, 1e-11, sigma, m.freqs, OPTS)
+    for fr in rep:
+        for r in fr:
+            assert all(s.status == 0 for s in r.stats), r
     return ex
 
 

@@ -477,6 +477,42 @@ def test_fortran_driver_matches_library(ctx, tmp_path):
     assert np.max(np.abs(got - ref) / np.abs(ref)) < 1e-6
 
 
+def test_fortran_driver_reads_topography(ctx, tmp_path):
+    """The driver with a .topo file (the corrected reader of file_opt.f90:97-128):
+    from the uncarved model and the ground elevations it carves sigma itself,
+    takes each column's ground surface in post_process, and writes what the
+    library gives on the same carved model with the surface planes read off
+    sigma."""
+    import subprocess
+    drv = os.path.join(ROOT, "sfd_excmg_amd", "bin", "sfd_excmg_gpu")
+    assert os.path.exists(drv), "Fortran driver not built"
+    carved = synth.make_aniso_topography_128(nx=32, nz_earth=20, nair=4, nstations=8)
+    m0 = synth.make_aniso_blocks_128(nx=32, nz_earth=20, nair=4, nstations=8)   # the same model before carving
+    m0.meta["topo"] = carved.meta["topo"]
+    m0.out_type = 1
+    prefix = str(tmp_path / "topo")
+    synth.write_fwd(m0, prefix, topo_indic=1)
+    synth.write_dat(m0, prefix)
+    synth.write_topo(m0, prefix)
+    m = synth.read_fwd_dat(prefix, sigma_air=1e-8)          # reads and carves like the driver
+    assert np.allclose(m.sigma, carved.sigma, rtol=1e-9, atol=0)   # the text format keeps 11 digits
+    env = dict(os.environ, SFDX_FWD_ACC="1", SFDX_SOLVER="5", SFDX_NORM="interior", SFDX_RES_FULL="1",
+               SFDX_SIGMA_AIR="1e-8", SFDX_CORR_INTV="0")
+    print("running", drv, prefix, flush=True)
+    out = subprocess.run([drv, prefix, "3", "1e-10"], cwd=str(tmp_path), env=env, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    got = np.loadtxt(prefix + ".resx")
+    opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR)
+    ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
+    assert st.status == 0
+    surf = sfdx.surface_planes(m.a, m.b, m.c, m.sigma, 1e-6)
+    assert surf.max() > m.nair
+    ref, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, out_type=1, rd=2, surf=surf)
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) < 1e-6 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
 def test_graph_replay_matches_eager(ctx, precision):
     """The steady-state COCR iteration replayed as a captured HIP graph gives

@@ -43,11 +43,14 @@ def ctx():
     c.close()
 
 
-def headline_opts(m, precision, eps):
-    """The benchmark's solver setup: COCR, hybrid preconditioner, no periodic
-    divergence correction, interior residual."""
+def headline_opts(m, precision, eps, precond=sfdx.PRECOND_MG):
+    """The benchmark's solver setup: COCR preconditioned by the V-cycle over the
+    3 finest grids with one coarsest smoothing pair, no periodic divergence
+    correction, interior residual (precond=PRECOND_HYBRID: the one-level
+    hybrid preconditioner instead)."""
     return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0, norm=sfdx.NORM_INTERIOR,
-                             precision=precision, sigma_floor32=0.0, restart_drop=1e-6, maxit_excmg=8000)
+                             precision=precision, sigma_floor32=0.0, restart_drop=1e-6, maxit_excmg=8000,
+                             precond=precond, mg_levels=3, mg_coarse_pairs=1)
 
 
 def zrel(p, q):
@@ -165,13 +168,15 @@ def test_c4_batch_of_uneven_models_and_sounding_frequencies(ctx):
     big = synth.make_aniso_blocks_128()
     big.freqs = synth.sounding_frequencies_16()[[0, 15]]
     models = [big] + small
-    opts = headline_opts(big, sfdx.PREC_MIXED, 1e-8)
+    hyb = sfdx.PRECOND_HYBRID   # the batch's preconditioner (up to 1 kHz)
+    opts = headline_opts(big, sfdx.PREC_MIXED, 1e-8, hyb)
     print("C4 batch: 9 models, 10 frequency items", flush=True)
     exb, stb = ctx.solve_batch(models, opts, nstreams=8)
     for k, (m, ex, st) in enumerate(zip(models, exb, stb)):
         print(f"  model {k} {m.nx}x{m.ny}x{m.nz}: status {st.status} fine iterations {st.fine_iters_total}", flush=True)
         assert st.status == 0
-        ex1, st1 = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
+        # the batch takes each model's own level count; so does the single solve
+        ex1, st1 = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, headline_opts(m, sfdx.PREC_MIXED, 1e-8, hyb))
         assert st1.status == 0
         rb, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex)
         r1, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex1)
@@ -181,7 +186,7 @@ def test_c4_batch_of_uneven_models_and_sounding_frequencies(ctx):
         m, ex = small[int(q)], exb[1 + int(q)]
         assert float(np.sum(m.sigma)) == float(g[f"sigma_sum_{q}"])
         # fixture at 1e-10: re-solve this model to the same depth for the comparison
-        o = headline_opts(m, sfdx.PREC_MIXED, 1e-10)
+        o = headline_opts(m, sfdx.PREC_MIXED, 1e-10, hyb)
         ex10, st10 = ctx.solve_batch([m], o, nstreams=1)
         rb, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex10[0])
         zxy, zyx = impedances(rb)
