@@ -276,6 +276,20 @@ int sfdx_direct_solve(const sfdx_grid *g, const double *sigma, double freq, doub
  * replaces zamux (div_correc.f90:431-449) on the dcfast CSR. */
 int sfdx_apply_operator(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, int precision,
                         const double *x, double *y);
+/* Allocation-free form of sfdx_apply_operator (complex128): y = A_s x on
+ * caller-owned DEVICE memory.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops.  g->a, g->b, g->c (widths, nx/ny/nz doubles), sigma
+ * (sigma[comp*ncell + ie], 6*ncell doubles), x and y (nl complex doubles)
+ * are device pointers on the context's GPU; the call only enqueues work on
+ * `stream` (a hipStream_t; NULL = the context's stream) and returns without
+ * waiting.  It calls no allocator: the reciprocal widths live in a fixed
+ * library scratch array of 3 x 4096 doubles per GPU (so nx, ny, nz <= 4096,
+ * else SFDX_ERR_ARG), whose address is fetched by sfdx_create.  Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (other streams,
+ * other contexts or host threads) race and must be serialised by the caller. */
+int sfdx_apply_operator_noalloc(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, const double *x,
+                                double *y, void *stream);
 /* y = M^-1 x, multicolour SSOR of A_s (replaces ssorfac + lusol,
  * div_correc.f90:171-220, solvers.f90:721-762); x must vanish on boundary edges. */
 int sfdx_ssor_apply(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, double omega,

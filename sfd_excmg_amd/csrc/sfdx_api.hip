@@ -17,6 +17,7 @@
 #include <atomic>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <thread>
 
 #include "../../include/sfdx.h"
@@ -27,6 +28,17 @@ using namespace sfdx;
 typedef std::complex<double> zc;
 
 static thread_local std::string g_err;
+
+// Scratch of the allocation-free entry point sfdx_apply_operator_noalloc: the
+// reciprocal cell widths of the three axes, kNoallocMaxDim doubles each.  One
+// copy per GPU lives until the process ends.  On the host this name is only a
+// stand-in for the symbol: its device address is looked up once per device
+// (noalloc_scratch_init, from sfdx_create) and only that address is used.
+static constexpr int kNoallocMaxDim = 4096;
+static __device__ __align__(256) double scratch[3 * kNoallocMaxDim];
+static constexpr int kMaxDevices = 64;
+static double *g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
 
 namespace {
 
@@ -1439,6 +1451,19 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     return worst;
 }
 
+// Device address of `scratch` on `device` (the current device), looked up once.
+double *noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        SFDX_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = (double *)p;
+    }
+    return g_scratch_dev[device];
+}
+
 template <typename F>
 int guarded(F &&fn)
 {
@@ -1506,6 +1531,7 @@ int sfdx_create(int device, sfdx_ctx **out)
         SFDX_CHECK(hipHostMalloc(&c->hslots, SL_COUNT * sizeof(double), hipHostMallocDefault));
         SFDX_CHECK(hipHostMalloc(&c->hlag, 2 * SL_COUNT * sizeof(double), hipHostMallocDefault));
         for (int q = 0; q < 2; q++) SFDX_CHECK(hipEventCreateWithFlags(&c->lag_ev[q], hipEventDisableTiming));
+        noalloc_scratch_init(device);
         *out = c;
         return SFDX_OK;
     });
@@ -1799,6 +1825,39 @@ int sfdx_apply_operator(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, 
         }
         SFDX_CHECK(hipStreamSynchronize(ctx->st));
         return SFDX_OK;
+    });
+}
+
+int sfdx_apply_operator_noalloc(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, const double *x,
+                                double *y, void *stream)
+{
+    if (!ctx || !g || !g->a || !g->b || !g->c || !sigma || !x || !y || g->nx <= 0 || g->ny <= 0 || g->nz <= 0)
+        return SFDX_ERR_ARG;
+    if (g->nx > kNoallocMaxDim || g->ny > kNoallocMaxDim || g->nz > kNoallocMaxDim) {
+        g_err = "sfdx_apply_operator_noalloc: nx, ny and nz must not exceed " + std::to_string(kNoallocMaxDim);
+        return SFDX_ERR_ARG;
+    }
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        double *sc = g_scratch_dev[ctx->device];
+        if (!sc) throw std::runtime_error("scratch address not initialised on this device");
+        const hipStream_t st = stream ? (hipStream_t)stream : ctx->st;
+        DevGrid<double> D;
+        D.d = make_dims(g->nx, g->ny, g->nz);
+        D.a = const_cast<double *>(g->a);
+        D.b = const_cast<double *>(g->b);
+        D.c = const_cast<double *>(g->c);
+        D.ra = sc;
+        D.rb = sc + kNoallocMaxDim;
+        D.rc = sc + 2 * kNoallocMaxDim;
+        D.sig = const_cast<double *>(sigma);
+        launch_reciprocal(g->nx, g->a, D.ra, st);
+        launch_reciprocal(g->ny, g->b, D.rb, st);
+        launch_reciprocal(g->nz, g->c, D.rc, st);
+        const double wmu = 2 * 3.141592653589793 * freq * kMu0;
+        launch_spmv<double>(D, wmu, (const zd *)x, (zd *)y, 1.0, 0, nullptr, ctx->rb, st);
+        SFDX_CHECK(hipGetLastError());
+        return (int)SFDX_OK;
     });
 }
 

@@ -279,5 +279,7 @@ void launch_surface_fields(const Dims &d, int nair, const int *surf, const doubl
 
 template <typename R>
 void launch_extrap(int64_t n, R c1, const cx<R> *u1, R c2, cx<R> *w, hipStream_t st);
+// out[i] = 1 / w[i], i < n
+void launch_reciprocal(int n, const double *w, double *out, hipStream_t st);
 
 }  // namespace sfdx

@@ -2000,6 +2000,19 @@ void launch_prolong(const Dims &df, const Dims &dc, int m, const ProlongDev &T, 
     hipLaunchKernelGGL((k_prolong<R>), L.grid, L.block, 0, st, df, dc, m, t, uc, uf);
 }
 
+// out[i] = 1 / w[i] (reciprocal cell widths of one axis, FP64, correctly rounded
+// as the host's 1.0 / w)
+__global__ __launch_bounds__(TPB) void k_reciprocal(int n, const double *__restrict__ w, double *__restrict__ out)
+{
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i < n) out[i] = 1.0 / w[i];
+}
+
+void launch_reciprocal(int n, const double *w, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reciprocal, grid1(n), dim3(TPB), 0, st, n, w, out);
+}
+
 template <typename R>
 void launch_extrap(int64_t n, R c1, const cx<R> *u1, R c2, cx<R> *w, hipStream_t st)
 {

@@ -83,7 +83,7 @@ EXPORTS = ["sfdx_create", "sfdx_destroy", "sfdx_last_error", "sfdx_default_opts"
            "sfdx_post_process", "sfdx_mt1d_anl", "sfdx_apply_operator", "sfdx_ssor_apply", "sfdx_hybrid_apply", "sfdx_div_correct",
            "sfdx_extint_v", "sfdx_bench_kernels", "sfdx_direct_solve", "sfdx_solve_batch", "sfdx_misfit",
            "sfdx_adjoint", "sfdx_fwd_response", "sfdx_post_process_topo", "sfdx_surface_planes",
-           "sfdx_mesh_coarsen"]
+           "sfdx_mesh_coarsen", "sfdx_apply_operator_noalloc"]
 
 _lib = None
 
@@ -108,6 +108,7 @@ def load(path: str = LIB_PATH):
     L.sfdx_mesh_coarsen.argtypes = [ctypes.POINTER(Grid), P, I, P, P, P, P]
     L.sfdx_mt1d_anl.argtypes = [I, P, P, D, P, P, P]
     L.sfdx_apply_operator.argtypes = [P, ctypes.POINTER(Grid), P, D, I, P, P]
+    L.sfdx_apply_operator_noalloc.argtypes = [P, ctypes.POINTER(Grid), P, D, P, P, P]
     L.sfdx_ssor_apply.argtypes = [P, ctypes.POINTER(Grid), P, D, D, I, P, P]
     L.sfdx_hybrid_apply.argtypes = [P, ctypes.POINTER(Grid), P, D, D, I, I, P, P]
     L.sfdx_div_correct.argtypes = [P, ctypes.POINTER(Grid), P, P, D, ctypes.POINTER(I)]
