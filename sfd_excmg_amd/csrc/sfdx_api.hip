@@ -377,20 +377,24 @@ bool apply_prec(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, co
 {
     if (ko.precond == SFDX_PRECOND_MG) {
         const Dims &d = D.g.d;
+        // store false (with dotw): out keeps the V-cycle's edge part (or, with the
+        // complex64 V-cycle, nothing) and only the dot of z + (i/wmu) G psi is formed
+        const bool st_out = store || !dotw;
         if (sizeof(R) == 8 && ko.mg32) {
             // FP64 Krylov, complex64 V-cycle; the recombination z + (i/wmu) G psi is done in FP64
             zf *in32 = pool_vec<zf>(ctx, tag + "/mg_in", d.NL), *z32 = pool_vec<zf>(ctx, tag + "/mg_z", d.NL);
             zf *p32 = pool_vec<zf>(ctx, tag + "/mg_p", d.NP);
-            launch_convert<double, float>(d.NL, (const zd *)in, in32, ctx->st);
+            if (!in32_ready) launch_convert<double, float>(d.NL, (const zd *)in, in32, ctx->st);
             mg_vcycle<float>(ctx, *ko.mg32, ko.mg_level, (float)wmu, in32, z32, p32);
-            launch_hyb_combine(*(const DevGrid<double> *)&D.g, zd(0.0, 1.0 / (double)wmu), p32, z32, (zd *)out, ctx->st);
+            launch_hyb_combine(*(const DevGrid<double> *)&D.g, zd(0.0, 1.0 / (double)wmu), p32, z32, (zd *)out, ctx->st,
+                               (const zd *)dotw, ctx->rb.partials, st_out);
         } else {
             MGHier<R> &H = *(MGHier<R> *)(sizeof(R) == 8 ? (void *)ko.mg64 : (void *)ko.mg32);
             cx<R> *pn = pool_vec<cx<R>>(ctx, tag + "/mg_pn", d.NP);
             mg_vcycle<R>(ctx, H, ko.mg_level, wmu, in, out, pn);
-            launch_grad_add<R>(D.g, cx<R>((R)0, (R)1 / wmu), pn, out, ctx->st);
+            launch_grad_add<R>(D.g, cx<R>((R)0, (R)1 / wmu), pn, out, ctx->st, dotw, ctx->rb.partials, st_out);
         }
-        return false;
+        return dotw != nullptr;
     }
     if (sizeof(R) == 8 && ko.lo) {
         LevelDev<float> &F = *ko.lo;
@@ -456,8 +460,10 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     // buffer), so apply_prec(Ap) skips its conversion pass.  Re-evaluated at
     // every use: the complex64 preconditioner may be dropped during the solve.
     auto ap32_of = [&]() -> zf * {
-        if (sizeof(R) != 8 || !ko.lo || ko.precond != SFDX_PRECOND_HYBRID || ko.nodal != 1) return nullptr;
-        return pool_vec<zf>(ctx, tag + "/mp_in", d.NL);
+        if (sizeof(R) != 8) return nullptr;
+        if (ko.lo && ko.precond == SFDX_PRECOND_HYBRID && ko.nodal == 1) return pool_vec<zf>(ctx, tag + "/mp_in", d.NL);
+        if (ko.mg32 && ko.precond == SFDX_PRECOND_MG) return pool_vec<zf>(ctx, tag + "/mg_in", d.NL);  // the V-cycle's input
+        return nullptr;
     };
     // The preconditioner pieces the update forms q = M Ap from (apply_prec's
     // buffers of the same tags): complex64 SSOR output and complex128 nodal
@@ -465,6 +471,12 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     zf *const mp_z = sizeof(R) == 8 && ko.lo ? pool_vec<zf>(ctx, tag + "/mp_z", d.NL) : nullptr;
     zd *const mp_phi = sizeof(R) == 8 && ko.lo ? pool_vec<zd>(ctx, tag + "/mp_phi64", d.NP) : nullptr;
     cx<R> *const hy_phi = pool_vec<cx<R>>(ctx, tag + (sizeof(R) == 8 ? "/hy64phi" : "/hy32phi"), d.NP);
+    // the V-cycle's pieces (apply_prec's buffers of the same tags): its complex64
+    // edge and nodal parts, or its nodal part in precision R
+    const bool mg = ko.precond == SFDX_PRECOND_MG;
+    zf *const mg_z32 = sizeof(R) == 8 && mg && ko.mg32 ? pool_vec<zf>(ctx, tag + "/mg_z", d.NL) : nullptr;
+    zf *const mg_p32 = sizeof(R) == 8 && mg && ko.mg32 ? pool_vec<zf>(ctx, tag + "/mg_p", d.NP) : nullptr;
+    cx<R> *const mg_pn = mg ? pool_vec<cx<R>>(ctx, tag + "/mg_pn", d.NP) : nullptr;
     SliceLaunch L = edge_launch(d);
     const int64_t nbe = L.nblocks();
     const double sc = 2.0 * sizeof(R), vecb = (double)d.NL * sc;
@@ -601,6 +613,24 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
             int64_t nbd = nbe;
             // q = M Ap, with (Ap, q) fused into the preconditioner's last kernel when possible
             zf *ap32 = ap32_of();
+            if (sizeof(R) == 8 && ko.precond == SFDX_PRECOND_MG && ko.mg32) {
+                // complex64 V-cycle: only the dot (Ap, q) here; the update forms q
+                // again from the V-cycle's complex64 edge and nodal parts
+                apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko, Ap, ap32 != nullptr, false);
+                reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
+                launch_cocr_upd_hyb(*(DevGrid<double> *)&D.g, ctx->rb, (zd *)x, (zd *)pv, (const zd *)Ap, (zd *)r,
+                                    (zd *)z, mg_z32, mg_p32, (double)wmu, ctx->st);
+                reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+                return;
+            }
+            if (ko.precond == SFDX_PRECOND_MG && !ko.mg32) {
+                // V-cycle in precision R: q holds its edge part, the update adds s G psi
+                apply_prec<R>(ctx, D, tag, wmu, Ap, q, ko, Ap, false, false);
+                reduce_final(ctx->rb, nbd, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
+                launch_cocr_upd_grad<R>(D.g, ctx->rb, x, pv, Ap, r, z, q, mg_pn, wmu, ctx->st);
+                reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+                return;
+            }
             if (ap32) {
                 // complex64-edge hybrid: q is not stored; the update forms it again
                 // from the preconditioner's complex64 SSOR output and nodal correction

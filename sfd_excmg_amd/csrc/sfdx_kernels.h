@@ -226,6 +226,9 @@ void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32,
 // COCR update (k_vec mode 6) with q formed from z32 and phi of launch_hybrid_edge32
 void launch_cocr_upd_hyb(const DevGrid<double> &g, RedBuf &rb, zd *x, zd *pv, const zd *Ap, zd *r, zd *z,
                          const zf *z32, const zd *phi, double wmu, hipStream_t st);
+// the same with the nodal part in complex64 (the complex64 V-cycle's)
+void launch_cocr_upd_hyb(const DevGrid<double> &g, RedBuf &rb, zd *x, zd *pv, const zd *Ap, zd *r, zd *z,
+                         const zf *z32, const zf *phi, double wmu, hipStream_t st);
 // Ap = Az + beta Ap (copy: Ap = Az); ap32, when set, receives a complex64 copy
 template <typename R>
 void launch_cocr_ap(int64_t n, RedBuf &rb, cx<R> *Ap, const cx<R> *Az, zf *ap32, bool copy, hipStream_t st);
@@ -246,9 +249,14 @@ void launch_resid_split(const DevGrid<R> &g, R wmu, const cx<R> *z, const cx<R> 
 template <typename R>
 void launch_prolong_node_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *pc, cx<R> *pf,
                              hipStream_t st);
+// z += s G phi; with dotw the partial sums of dotw^T (z + s G phi), and with
+// store false z is left unchanged (only the dot)
 template <typename R>
-void launch_grad_add(const DevGrid<R> &g, cx<R> s, const cx<R> *phi, cx<R> *z, hipStream_t st);
-void launch_hyb_combine(const DevGrid<double> &g64, zd s, const zf *phi, const zf *z32, zd *z64, hipStream_t st);
+void launch_grad_add(const DevGrid<R> &g, cx<R> s, const cx<R> *phi, cx<R> *z, hipStream_t st,
+                     const cx<R> *dotw = nullptr, double *partials = nullptr, bool store = true);
+// z64 = widen(z32) + s G phi, phi in complex64; dotw / store as launch_grad_add
+void launch_hyb_combine(const DevGrid<double> &g64, zd s, const zf *phi, const zf *z32, zd *z64, hipStream_t st,
+                        const zd *dotw = nullptr, double *partials = nullptr, bool store = true);
 template <typename R>
 void launch_restrict(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *rf, cx<R> *rc,
                      hipStream_t st);

@@ -1447,13 +1447,15 @@ __global__ __launch_bounds__(TPB) void k_hyb_combine(Dims d, const double *__res
 }
 
 // COCR update of k_vec mode 6 with q = M Ap formed from the complex64-edge
-// hybrid preconditioner's pieces (z32, phi) instead of read from a stored
+// preconditioner's pieces (z32, phi: the hybrid preconditioner's, or the
+// complex64 V-cycle's with phi in complex64) instead of read from a stored
 // vector: p = z + beta p; x += alpha p; r -= alpha Ap; z -= alpha q; |r/ve|^2.
+template <typename RP>
 __global__ __launch_bounds__(TPB) void k_cocr_upd_hyb(Dims d, const double *__restrict__ a,
                                                       const double *__restrict__ b, const double *__restrict__ c,
                                                       const double *__restrict__ slots, zd *x, zd *pv,
                                                       const zd *__restrict__ Ap, zd *r, zd *z, const zf *__restrict__ z32,
-                                                      const zd *__restrict__ phi, zd s, double *__restrict__ partials)
+                                                      const cx<RP> *__restrict__ phi, zd s, double *__restrict__ partials)
 {
     const Tile TL = tile_id();
     const Slice S = edge_slice(d, TL.y);
@@ -1474,7 +1476,7 @@ __global__ __launch_bounds__(TPB) void k_cocr_upd_hyb(Dims d, const double *__re
             x[e] = fma_c(x[e], al, pn);
             rr = rr - al * Ap[e];
             r[e] = rr;
-            z[e] = zv - al * hyb_value<double>(d, a, b, c, s, phi, z32, S, i, j, e);
+            z[e] = zv - al * hyb_value<RP>(d, a, b, c, s, phi, z32, S, i, j, e);
         }
         if (inner) {
             double ve = dual_volume(a, b, c, S.o, i, j, S.k);
@@ -1488,8 +1490,16 @@ void launch_cocr_upd_hyb(const DevGrid<double> &g, RedBuf &rb, zd *x, zd *pv, co
                          const zf *z32, const zd *phi, double wmu, hipStream_t st)
 {
     SliceLaunch L = edge_launch(g.d);
-    hipLaunchKernelGGL(k_cocr_upd_hyb, L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, rb.slots, x, pv, Ap, r, z, z32, phi,
-                       zd(0.0, 1.0 / wmu), rb.partials);
+    hipLaunchKernelGGL(k_cocr_upd_hyb<double>, L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, rb.slots, x, pv, Ap, r, z,
+                       z32, phi, zd(0.0, 1.0 / wmu), rb.partials);
+}
+
+void launch_cocr_upd_hyb(const DevGrid<double> &g, RedBuf &rb, zd *x, zd *pv, const zd *Ap, zd *r, zd *z,
+                         const zf *z32, const zf *phi, double wmu, hipStream_t st)
+{
+    SliceLaunch L = edge_launch(g.d);
+    hipLaunchKernelGGL(k_cocr_upd_hyb<float>, L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, rb.slots, x, pv, Ap, r, z,
+                       z32, phi, zd(0.0, 1.0 / wmu), rb.partials);
 }
 
 // ------------------------------------------------------------------ K8 multigrid transfers
@@ -2343,18 +2353,33 @@ void launch_prolong_node_add(const Dims &df, const Dims &dc, const double *const
 }
 
 template <typename R>
-void launch_grad_add(const DevGrid<R> &g, cx<R> s, const cx<R> *phi, cx<R> *z, hipStream_t st)
+void launch_grad_add(const DevGrid<R> &g, cx<R> s, const cx<R> *phi, cx<R> *z, hipStream_t st, const cx<R> *dotw,
+                     double *partials, bool store)
 {
     SliceLaunch L = edge_launch(g.d);
-    hipLaunchKernelGGL((k_grad_add<R, false>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, s, phi, z,
-                       (const cx<R> *)nullptr, (double *)nullptr);
+    if (dotw && !store)
+        hipLaunchKernelGGL((k_grad_add<R, true, false>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, s, phi, z, dotw,
+                           partials);
+    else if (dotw)
+        hipLaunchKernelGGL((k_grad_add<R, true>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, s, phi, z, dotw, partials);
+    else
+        hipLaunchKernelGGL((k_grad_add<R, false>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, s, phi, z,
+                           (const cx<R> *)nullptr, (double *)nullptr);
 }
 
-void launch_hyb_combine(const DevGrid<double> &g64, zd s, const zf *phi, const zf *z32, zd *z64, hipStream_t st)
+void launch_hyb_combine(const DevGrid<double> &g64, zd s, const zf *phi, const zf *z32, zd *z64, hipStream_t st,
+                        const zd *dotw, double *partials, bool store)
 {
     SliceLaunch L = edge_launch(g64.d);
-    hipLaunchKernelGGL((k_hyb_combine<float, false>), L.grid, L.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi, z32,
-                       z64, (const zd *)nullptr, (double *)nullptr);
+    if (dotw && !store)
+        hipLaunchKernelGGL((k_hyb_combine<float, true, false>), L.grid, L.block, 0, st, g64.d, g64.a, g64.b, g64.c, s,
+                           phi, z32, z64, dotw, partials);
+    else if (dotw)
+        hipLaunchKernelGGL((k_hyb_combine<float, true>), L.grid, L.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi,
+                           z32, z64, dotw, partials);
+    else
+        hipLaunchKernelGGL((k_hyb_combine<float, false>), L.grid, L.block, 0, st, g64.d, g64.a, g64.b, g64.c, s, phi,
+                           z32, z64, (const zd *)nullptr, (double *)nullptr);
 }
 
 template <typename R>
@@ -2407,7 +2432,8 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
                                         cx<R> *, cx<R> *, hipStream_t);                                           \
     template void launch_prolong_node_add<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *,    \
                                              cx<R> *, hipStream_t);                                               \
-    template void launch_grad_add<R>(const DevGrid<R> &, cx<R>, const cx<R> *, cx<R> *, hipStream_t);            \
+    template void launch_grad_add<R>(const DevGrid<R> &, cx<R>, const cx<R> *, cx<R> *, hipStream_t, const cx<R> *, \
+                                     double *, bool);            \
     template void launch_gpbicg<R>(int, const DevGrid<R> &, RedBuf &, const GPVecs<R> &, hipStream_t);           \
     template void launch_restrict<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *, cx<R> *,   \
                                      hipStream_t);                                                                \
