@@ -290,6 +290,16 @@ int sfdx_apply_operator(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, 
  * other contexts or host threads) race and must be serialised by the caller. */
 int sfdx_apply_operator_noalloc(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, const double *x,
                                 double *y, void *stream);
+/* t = r - A_s (z + (i/(omega mu)) G psi) on interior edges, 0 on boundary
+ * edges: the residual of a correction held in the split edge + nodal form of
+ * the multigrid V-cycle preconditioner, with G the node-to-edge gradient.  It
+ * is formed as r - K z + i omega mu M (z + (i/(omega mu)) G psi) (K the curl
+ * part, M the mass part of A_s; K G = 0), so the curl part never acts on
+ * G psi.  r, z, t: nl complex; psi: (nx+1)(ny+1)(nz+1) complex node values.
+ * precision SFDX_PREC_FP64 complex128, else complex64.  No counterpart in the
+ * reference (whose solvers have no multigrid); exposed for parity tests. */
+int sfdx_split_residual(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, int precision,
+                        const double *r, const double *z, const double *psi, double *t);
 /* y = M^-1 x, multicolour SSOR of A_s (replaces ssorfac + lusol,
  * div_correc.f90:171-220, solvers.f90:721-762); x must vanish on boundary edges. */
 int sfdx_ssor_apply(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, double omega,

@@ -295,14 +295,13 @@ void mg_vcycle(sfdx_ctx *ctx, MGHier<R> &H, int l, R wmu, const cx<R> *r, cx<R> 
     const Dims &d = D.g.d;
     const std::string tg = "mg" + std::to_string(l) + (sizeof(R) == 8 ? "d" : "f");
     cx<R> *t = pool_vec<cx<R>>(ctx, tg + "t", d.NL), *w = pool_vec<cx<R>>(ctx, tg + "w", d.NL);
-    cx<R> *gt = pool_vec<cx<R>>(ctx, tg + "g", d.NL);
     cx<R> *tn = pool_vec<cx<R>>(ctx, tg + "tn", d.NP), *ph = pool_vec<cx<R>>(ctx, tg + "ph", d.NP);
     ensure_P(ctx, D, tg);
     hipStream_t st = ctx->st;
     // pre-smoothing from zero: z = GS_e r, psi = GS_n G^T r
     launch_hybrid_gs<R>(D.g, wmu, D.P, r, z, tn, psi, 0, st, false, H.nodal);
     auto smooth_add = [&](int dir) {
-        launch_resid_split<R>(D.g, wmu, z, psi, r, gt, t, st);
+        launch_resid_split<R>(D.g, wmu, z, psi, r, t, st);
         launch_hybrid_gs<R>(D.g, wmu, D.P, t, w, tn, ph, dir, st, false, H.nodal);
         launch_axpby<R>(d.NL, cx<R>(1), w, cx<R>(1), z, st);
         launch_axpby<R>(d.NP, cx<R>(1), ph, cx<R>(1), psi, st);
@@ -318,7 +317,7 @@ void mg_vcycle(sfdx_ctx *ctx, MGHier<R> &H, int l, R wmu, const cx<R> *r, cx<R> 
     const std::string tc = "mg" + std::to_string(l - 1) + (sizeof(R) == 8 ? "d" : "f");
     cx<R> *rc = pool_vec<cx<R>>(ctx, tc + "r", C.g.d.NL), *zc = pool_vec<cx<R>>(ctx, tc + "z", C.g.d.NL);
     cx<R> *pc = pool_vec<cx<R>>(ctx, tc + "p", C.g.d.NP);
-    launch_resid_split<R>(D.g, wmu, z, psi, r, gt, t, st);
+    launch_resid_split<R>(D.g, wmu, z, psi, r, t, st);
     launch_restrict<R>(d, C.g.d, H.wl[l].data(), t, rc, st);
     mg_vcycle<R>(ctx, H, l - 1, wmu, rc, zc, pc);
     launch_prolong_add<R>(d, C.g.d, H.wl[l].data(), zc, z, st);
@@ -1854,6 +1853,42 @@ int sfdx_apply_operator(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, 
             for (int64_t e = 0; e < d.NL; e++) { y[2 * e] = yf[e].re; y[2 * e + 1] = yf[e].im; }
         }
         SFDX_CHECK(hipStreamSynchronize(ctx->st));
+        return SFDX_OK;
+    });
+}
+
+int sfdx_split_residual(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, int precision,
+                        const double *r, const double *z, const double *psi, double *t)
+{
+    if (!ctx || !g || !sigma || !r || !z || !psi || !t) return SFDX_ERR_ARG;
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        HostLevel L = host_level(g, sigma);
+        const Dims d = make_dims(L.nx, L.ny, L.nz);
+        const double wmu = 2 * 3.141592653589793 * freq * kMu0;
+        auto run = [&](auto tag) {
+            using R = decltype(tag);
+            using C = cx<R>;
+            LevelDev<R> D = upload_level<R>(ctx, L, "sr");
+            std::vector<C> hr(d.NL), hz(d.NL), hp(d.NP), ht(d.NL);
+            for (int64_t e = 0; e < d.NL; e++) {
+                hr[e] = C((R)r[2 * e], (R)r[2 * e + 1]);
+                hz[e] = C((R)z[2 * e], (R)z[2 * e + 1]);
+            }
+            for (int64_t n = 0; n < d.NP; n++) hp[n] = C((R)psi[2 * n], (R)psi[2 * n + 1]);
+            const std::string tg = sizeof(R) == 8 ? "sr/d" : "sr/f";
+            C *rd = pool_vec<C>(ctx, tg + "r", d.NL), *zd_ = pool_vec<C>(ctx, tg + "z", d.NL);
+            C *pd = pool_vec<C>(ctx, tg + "p", d.NP), *td = pool_vec<C>(ctx, tg + "t", d.NL);
+            SFDX_CHECK(hipMemcpyAsync(rd, hr.data(), d.NL * sizeof(C), hipMemcpyHostToDevice, ctx->st));
+            SFDX_CHECK(hipMemcpyAsync(zd_, hz.data(), d.NL * sizeof(C), hipMemcpyHostToDevice, ctx->st));
+            SFDX_CHECK(hipMemcpyAsync(pd, hp.data(), d.NP * sizeof(C), hipMemcpyHostToDevice, ctx->st));
+            launch_resid_split<R>(D.g, (R)wmu, zd_, pd, rd, td, ctx->st);
+            SFDX_CHECK(hipMemcpyAsync(ht.data(), td, d.NL * sizeof(C), hipMemcpyDeviceToHost, ctx->st));
+            SFDX_CHECK(hipStreamSynchronize(ctx->st));
+            for (int64_t e = 0; e < d.NL; e++) { t[2 * e] = ht[e].re; t[2 * e + 1] = ht[e].im; }
+        };
+        if (precision == SFDX_PREC_FP64) run(0.0);
+        else run(0.0f);
         return SFDX_OK;
     });
 }

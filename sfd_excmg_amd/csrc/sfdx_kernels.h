@@ -241,11 +241,12 @@ template <typename R>
 void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R> *out, cx<R> *tnode, cx<R> *phi,
                       int dir, hipStream_t st, bool add_grad = true,
                       int nodal = 0);  // nodal 0: 8-colour GS on P in the sweep's direction; 1: Jacobi polynomial
-// t = r - A z - M (G psi): residual of z + (i/wmu) G psi, using A G = -i wmu M G
-// so that the (possibly huge) gradient part never meets the curl-curl term.
+// t = r - A (z + (i/wmu) G psi) on interior edges (0 on boundary edges), formed
+// as r - K z + i wmu M (z + (i/wmu) G psi) so that the (possibly huge)
+// gradient part never meets the curl-curl term (K G = 0).
 template <typename R>
-void launch_resid_split(const DevGrid<R> &g, R wmu, const cx<R> *z, const cx<R> *psi, const cx<R> *r, cx<R> *gtmp,
-                        cx<R> *t, hipStream_t st);
+void launch_resid_split(const DevGrid<R> &g, R wmu, const cx<R> *z, const cx<R> *psi, const cx<R> *r, cx<R> *t,
+                        hipStream_t st);
 template <typename R>
 void launch_prolong_node_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *pc, cx<R> *pf,
                              hipStream_t st);

@@ -3,8 +3,9 @@
 //  K1 k_spmv_march  matrix-free 13-point edge stencil y = alpha*A_s x, marching
 //                   up node planes with LDS-staged planes, optional fused dot
 //                   products (replaces zamux on the CSR of fd_assemb + dcfast,
-//                   sfd_excmg/src/div_correc.f90:431-449); k_spmv is the
-//                   one-row-per-thread form (split residual, comparison)
+//                   sfd_excmg/src/div_correc.f90:431-449), and with a node
+//                   potential ring the split residual of the multigrid
+//                   V-cycle; k_spmv is the one-row-per-thread form (comparison)
 //  K2 k_ssor        one colour sweep of multicolour SSOR (replaces the
 //                   lexicographic ssorfac + lusol, src/div_correc.f90:171-220,
 //                   src/solvers.f90:721-762)
@@ -18,7 +19,7 @@
 //  K6 k_dirichlet   boundary data from the 1-D field (src/fdfe_main.f90:470-494)
 //  K7 k_gradT, k_pjacobi, k_pssor, k_grad_add, k_hyb_combine
 //                   nodal half of the hybrid preconditioner
-//  K8 k_restrict, k_prolong_add, k_prolong_node_add, k_grad
+//  K8 k_restrict, k_prolong_add, k_prolong_node_add
 //                   multigrid transfers
 //  K9 k_cell_grad, k_surface_fields, k_zero_boundary
 //                   adjoint gradient and device post-process
@@ -253,8 +254,7 @@ __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, c
                                               const R *__restrict__ rb, const R *__restrict__ rc,
                                               const R *__restrict__ sig, R wmu,
                                               const cx<R> *__restrict__ x, cx<R> *__restrict__ y, R alpha,
-                                              const cx<R> *__restrict__ w, double *__restrict__ partials,
-                                              const cx<R> *__restrict__ g2)
+                                              const cx<R> *__restrict__ w, double *__restrict__ partials)
 {
     const Tile TL = tile_id();
     const Slice S = edge_slice(d, TL.y);
@@ -269,13 +269,6 @@ __global__ __launch_bounds__(TPB) void k_spmv(Dims d, const R *__restrict__ a, c
             row_coeffs<R, true>(d, a, b, c, sig, S.o, i, j, S.k, r, ra, rb, rc);
             out = alpha * row_apply(r, wmu, x, x[e]);
             if (DOT == 4) out = w[e] + out;
-            if (DOT == 5) {
-                // split residual w - A x - M g2 (g2 = G psi, A G psi = -i wmu M G psi)
-                cx<R> mg = r.kmd * g2[e];
-#pragma unroll
-                for (int q = 4; q < 12; q++) mg = mg + r.km[q] * g2[r.nb[q]];
-                out = w[e] + out - mg;
-            }
         }
         y[e] = out;
         if (DOT == 1) {  // x^T y
@@ -337,7 +330,7 @@ template <typename R, int DOT>
 __device__ __forceinline__ void memit(cx<R> *__restrict__ y, const cx<R> *__restrict__ w, int64_t e, cx<R> out,
                                       bool interior, cx<R> xe, double (&acc)[4])
 {
-    if (DOT == 4 && interior) out = w[e] + out;
+    if ((DOT == 4 || DOT == 5) && interior) out = w[e] + out;
     y[e] = out;
     if (DOT == 1) {
         acc[0] += (double)xe.re * out.re - (double)xe.im * out.im;
@@ -357,13 +350,18 @@ __device__ __forceinline__ void memit(cx<R> *__restrict__ y, const cx<R> *__rest
 
 // one footprint column of the k-marching stencil: where it lives, which of
 // its edges and cell exist, and the values entering at the current step
-template <typename R>
+// With PS the column also carries the node potential psi of the split
+// residual: psi(k+1) enters at step k, scaled by i/wmu, into a ring of three
+// node planes (plane p in slot p mod 3); the node of the column has the index
+// of its Ez edge less XN + YN.
+template <typename R, bool PS = false>
 struct MLoad {
     int lf, ls;                   // LDS index in the edge / cell footprint
     bool on, xv, yv, zv, sv, cv;  // in the footprint; Ex, Ey, Ez exist; in the cell footprint; cell exists
     int64_t ex, ey, ez, cc;       // indices at k = 0
     cx<R> inX, inY, inZ;          // Ex/Ey(k+1), Ez(k)
     R iS[6];                      // sigma(k)
+    cx<R> inP;                    // (i/wmu) psi(k+1)
 
     template <int PX, int SX, int SY>
     __device__ __forceinline__ void init(const Dims &d, int f, int np, int i0, int j0)
@@ -384,16 +382,27 @@ struct MLoad {
         ez = d.XN + d.YN + (int64_t)gj * (d.nx + 1) + gi;
         cc = (int64_t)gj * d.nx + gi;
     }
+    // i/wmu times psi at the column's node of plane k (0 outside the grid)
+    __device__ __forceinline__ cx<R> node_psi(const Dims &d, const cx<R> *__restrict__ psi, int k, int64_t ZP,
+                                              R rw) const
+    {
+        if (!zv) return cx<R>(0);
+        const cx<R> v = psi[ez - d.XN - d.YN + (int64_t)k * ZP];
+        return cx<R>(-v.im * rw, v.re * rw);
+    }
     __device__ __forceinline__ void fetch(const Dims &d, const cx<R> *__restrict__ x, const R *__restrict__ sig,
-                                          int k, int64_t XP, int64_t YP, int64_t ZP, int64_t CP)
+                                          int k, int64_t XP, int64_t YP, int64_t ZP, int64_t CP,
+                                          const cx<R> *__restrict__ psi = nullptr, R rw = 0)
     {
         inX = cx<R>(0); inY = cx<R>(0); inZ = cx<R>(0);
+        if (PS) inP = cx<R>(0);
 #pragma unroll
         for (int q = 0; q < 6; q++) iS[q] = 0;
         if (k < d.nz) {
             if (xv) inX = x[ex + (int64_t)(k + 1) * XP];
             if (yv) inY = x[ey + (int64_t)(k + 1) * YP];
             if (zv) inZ = x[ez + (int64_t)k * ZP];
+            if (PS) inP = node_psi(d, psi, k + 1, ZP, rw);
             if (cv) {
                 const int64_t q = cc + (int64_t)k * CP;
 #pragma unroll
@@ -402,9 +411,11 @@ struct MLoad {
         }
     }
     // before the first step kb: Ex/Ey(kb), Ez(kb-1), sigma(kb-1) into their ring slots
+    // (with PS also psi(kb) and psi(kb-1) into slots kb mod 3 and (kb+2) mod 3 of PN)
     __device__ __forceinline__ void prime(const Dims &d, const cx<R> *__restrict__ x, const R *__restrict__ sig,
                                           int kb, int64_t XP, int64_t YP, int64_t ZP, int64_t CP, cx<R> *EX,
-                                          cx<R> *EY, cx<R> *EZ, R *SG, int np, int ns)
+                                          cx<R> *EY, cx<R> *EZ, R *SG, int np, int ns,
+                                          const cx<R> *__restrict__ psi = nullptr, R rw = 0, cx<R> *PN = nullptr)
     {
         const bool lo = kb >= 1;
         const int s0 = kb & 1, s1 = s0 ^ 1;
@@ -412,6 +423,10 @@ struct MLoad {
             EX[s0 * np + lf] = xv ? x[ex + (int64_t)kb * XP] : cx<R>(0);
             EY[s0 * np + lf] = yv ? x[ey + (int64_t)kb * YP] : cx<R>(0);
             EZ[s1 * np + lf] = (zv && lo) ? x[ez + (int64_t)(kb - 1) * ZP] : cx<R>(0);
+            if (PS) {
+                PN[(kb % 3) * np + lf] = node_psi(d, psi, kb, ZP, rw);
+                PN[((kb + 2) % 3) * np + lf] = lo ? node_psi(d, psi, kb - 1, ZP, rw) : cx<R>(0);
+            }
         }
         if (sv) {
             const int64_t q = cc + (int64_t)(kb - 1) * CP;
@@ -420,13 +435,16 @@ struct MLoad {
         }
     }
     // at step k: Ex/Ey(k+1) into slot (k+1)&1, Ez(k) and sigma(k) into slot k&1
-    __device__ __forceinline__ void store(int k, cx<R> *EX, cx<R> *EY, cx<R> *EZ, R *SG, int np, int ns) const
+    // (with PS also psi(k+1) into slot pn = (k+1) mod 3 of PN)
+    __device__ __forceinline__ void store(int k, cx<R> *EX, cx<R> *EY, cx<R> *EZ, R *SG, int np, int ns,
+                                          cx<R> *PN = nullptr, int pn = 0) const
     {
         const int sk = k & 1, sn = sk ^ 1;
         if (on) {
             EX[sn * np + lf] = inX;
             EY[sn * np + lf] = inY;
             EZ[sk * np + lf] = inZ;
+            if (PS) PN[pn * np + lf] = inP;
         }
         if (sv) {
 #pragma unroll
@@ -440,17 +458,20 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
     Dims d, const R *__restrict__ a, const R *__restrict__ b, const R *__restrict__ c, const R *__restrict__ ra,
     const R *__restrict__ rb, const R *__restrict__ rc, const R *__restrict__ sig, R wmu,
     const cx<R> *__restrict__ x, cx<R> *__restrict__ y, R alpha, const cx<R> *__restrict__ w,
-    double *__restrict__ partials, int ntx, int kc)
+    double *__restrict__ partials, int ntx, int kc, const cx<R> *__restrict__ psi)
 {
     using G = mt::Geo<TY>;
     constexpr int TX = G::TX, PX = G::PX, SX = G::SX, SY = G::SY, NP = G::NP, NS = G::NS;
-    __shared__ __align__(16) R lds[G::LDS_R];
+    constexpr bool PS = DOT == 5;
+    __shared__ __align__(16) R lds[G::LDS_R + (PS ? 6 * NP : 0)];
     cx<R> *const EX = reinterpret_cast<cx<R> *>(lds);  // [2][NP], slot k&1 holds plane k
     cx<R> *const EY = EX + 2 * NP;
     cx<R> *const EZ = EY + 2 * NP;
     R *const SG = lds + 12 * NP;                        // [2][6][NS], components xx yy zz xy xz yz
     R *const TA = SG + 12 * NS;                         // a, then 1/a
     R *const TB = TA + 2 * (TX + 1);                    // b, then 1/b
+    cx<R> *const PN = reinterpret_cast<cx<R> *>(lds + G::LDS_R);  // PS: [3][NP] (i/wmu) psi rings
+    const R rw = (R)1 / wmu;
     const Tile TL = tile_id();
     const int tj = TL.x / ntx, ti = TL.x - tj * ntx;
     const int i0 = ti * TX, j0 = tj * TY;
@@ -460,7 +481,7 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
     const int64_t ZP = (int64_t)(d.ny + 1) * (d.nx + 1), CP = (int64_t)d.nx * d.ny, NC = d.NC;
     // loader role: footprint columns t and t + NT
     constexpr int NT = G::NT;
-    MLoad<R> L0, L1;
+    MLoad<R, PS> L0, L1;
     L0.template init<PX, SX, SY>(d, t, NP, i0, j0);
     L1.template init<PX, SX, SY>(d, t + NT, NP, i0, j0);
     const bool two = t + NT < NP;
@@ -475,8 +496,8 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
     if (kb < ke) {
         // prologue: the first planes into their ring slots, width tables
         const bool lo = kb >= 1;
-        L0.prime(d, x, sig, kb, XP, YP, ZP, CP, EX, EY, EZ, SG, NP, NS);
-        if (two) L1.prime(d, x, sig, kb, XP, YP, ZP, CP, EX, EY, EZ, SG, NP, NS);
+        L0.prime(d, x, sig, kb, XP, YP, ZP, CP, EX, EY, EZ, SG, NP, NS, psi, rw, PN);
+        if (two) L1.prime(d, x, sig, kb, XP, YP, ZP, CP, EX, EY, EZ, SG, NP, NS, psi, rw, PN);
         if (t <= TX) {
             const int ii = i0 - 1 + t;
             const bool ok = ii >= 0 && ii < d.nx;
@@ -493,26 +514,30 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
             exP = x[ex0 + (int64_t)(kb - 1) * XP];
             eyP = x[ey0 + (int64_t)(kb - 1) * YP];
         }
-        L0.fetch(d, x, sig, kb, XP, YP, ZP, CP);
-        if (two) L1.fetch(d, x, sig, kb, XP, YP, ZP, CP);
+        L0.fetch(d, x, sig, kb, XP, YP, ZP, CP, psi, rw);
+        if (two) L1.fetch(d, x, sig, kb, XP, YP, ZP, CP, psi, rw);
     }
     const R hE = (R)0.5, q4 = (R)0.25, q8 = (R)0.125;
     double acc[4] = {0, 0, 0, 0};
     for (int k = kb; k < ke; k++) {
         const int sk = k & 1, sn = sk ^ 1;  // ring slots of plane k and of planes k+1 / k-1
         __syncthreads();                    // the previous step's reads of the slots refilled here are done
-        L0.store(k, EX, EY, EZ, SG, NP, NS);
-        if (two) L1.store(k, EX, EY, EZ, SG, NP, NS);
+        const int pk = k % 3, pm = pk == 0 ? 2 : pk - 1, pp = pk == 2 ? 0 : pk + 1;  // psi slots of k, k-1, k+1
+        L0.store(k, EX, EY, EZ, SG, NP, NS, PN, pp);
+        if (two) L1.store(k, EX, EY, EZ, SG, NP, NS, PN, pp);
         // with PF the next step's loads are in flight while this step's rows are formed
         if (PF && k + 1 < ke) {
-            L0.fetch(d, x, sig, k + 1, XP, YP, ZP, CP);
-            if (two) L1.fetch(d, x, sig, k + 1, XP, YP, ZP, CP);
+            L0.fetch(d, x, sig, k + 1, XP, YP, ZP, CP, psi, rw);
+            if (two) L1.fetch(d, x, sig, k + 1, XP, YP, ZP, CP, psi, rw);
         }
         __syncthreads();
         if (own) {
             const cx<R> *EXK = EX + sk * NP, *EXN = EX + sn * NP, *EYK = EY + sk * NP, *EYN = EY + sn * NP;
             const cx<R> *EZP = EZ + sn * NP, *EZK = EZ + sk * NP;
             const R *SP = SG + sn * 6 * NS, *SK = SG + sk * 6 * NS;  // component u at u*NS
+            // PS: the mass terms act on u = x + (i/wmu) G psi, the stiffness on x
+            // alone (K G psi = 0): w - A (x + (i/wmu) G psi) = w - K x + i wmu M u
+            const cx<R> *PK = PN + pk * NP, *PM = PN + pm * NP, *PP = PN + pp * NP;
             const bool zk = k >= 1 && k < d.nz;  // both cell planes k-1, k exist
             const R cm = zk ? c[k - 1] : (R)0, cp = zk ? c[k] : (R)0;
             const R ai = TA[cxl + 1], am = TA[cxl], rai = TA[TX + 2 + cxl], ram = TA[TX + 1 + cxl];
@@ -538,8 +563,22 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
                     const cx<R> K = (wkm + wkp + wjm + wjp) * exK - wkm * exP - wkp * exN - wjm * EXK[fo - PX]
                                     - wjp * EXK[fo + PX] + hy * (ey00 - ey10 - eyK + ey11)
                                     + hz * (ezP - ez1 - ezK + ez2);
-                    const cx<R> M = kmd * exK + my1 * (ey00 + ey10) + my2 * (eyK + ey11) + mz1 * (ezP + ez1)
-                                    + mz2 * (ezK + ez2);
+                    cx<R> uK = exK, u00 = ey00, u10 = ey10, uyK = eyK, u11 = ey11;
+                    cx<R> uzP = ezP, uz1 = ez1, uzK = ezK, uz2 = ez2;
+                    if (PS) {
+                        const cx<R> p0 = PK[fo], p1 = PK[fo + 1];
+                        uK = uK + (p1 - p0) * rai;
+                        u00 = u00 + (p0 - PK[fo - PX]) * rbm;
+                        u10 = u10 + (p1 - PK[fo - PX + 1]) * rbm;
+                        uyK = uyK + (PK[fo + PX] - p0) * rbj;
+                        u11 = u11 + (PK[fo + PX + 1] - p1) * rbj;
+                        uzP = uzP + (p0 - PM[fo]) * rcm;
+                        uz1 = uz1 + (p1 - PM[fo + 1]) * rcm;
+                        uzK = uzK + (PP[fo] - p0) * rcp;
+                        uz2 = uz2 + (PP[fo + 1] - p1) * rcp;
+                    }
+                    const cx<R> M = kmd * uK + my1 * (u00 + u10) + my2 * (uyK + u11) + mz1 * (uzP + uz1)
+                                    + mz2 * (uzK + uz2);
                     out = km_combine(K, M, wmu, alpha);
                 }
                 memit<R, DOT>(y, w, ex0 + (int64_t)k * XP, out, in, exK, acc);
@@ -564,8 +603,21 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
                     const cx<R> z1 = EZP[fo + PX], z3 = EZK[fo + PX];
                     const cx<R> K = (wkm + wkp + wim + wip) * eyK - wkm * eyP - wkp * eyN - wim * EYK[fo - 1]
                                     - wip * EYK[fo + 1] + hx * (x0 - exK - x2 + x3) + hz * (ezP - z1 - ezK + z3);
-                    const cx<R> M = kmd * eyK + mx1 * (x0 + x2) + mx2 * (exK + x3) + mz1 * (ezP + z1)
-                                    + mz2 * (ezK + z3);
+                    cx<R> uK = eyK, u0 = x0, u2 = x2, uxK = exK, u3 = x3, uzP = ezP, uz1 = z1, uzK = ezK, uz3 = z3;
+                    if (PS) {
+                        const cx<R> p0 = PK[fo], p2 = PK[fo + PX];
+                        uK = uK + (p2 - p0) * rbj;
+                        u0 = u0 + (p0 - PK[fo - 1]) * ram;
+                        u2 = u2 + (p2 - PK[fo + PX - 1]) * ram;
+                        uxK = uxK + (PK[fo + 1] - p0) * rai;
+                        u3 = u3 + (PK[fo + PX + 1] - p2) * rai;
+                        uzP = uzP + (p0 - PM[fo]) * rcm;
+                        uz1 = uz1 + (p2 - PM[fo + PX]) * rcm;
+                        uzK = uzK + (PP[fo] - p0) * rcp;
+                        uz3 = uz3 + (PP[fo + PX] - p2) * rcp;
+                    }
+                    const cx<R> M = kmd * uK + mx1 * (u0 + u2) + mx2 * (uxK + u3) + mz1 * (uzP + uz1)
+                                    + mz2 * (uzK + uz3);
                     out = km_combine(K, M, wmu, alpha);
                 }
                 memit<R, DOT>(y, w, ey0 + (int64_t)k * YP, out, in, eyK, acc);
@@ -592,8 +644,23 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
                     const cx<R> K = (wjm + wjp + wim + wip) * ezK - wjm * EZK[fo - PX] - wjp * EZK[fo + PX]
                                     - wim * EZK[fo - 1] - wip * EZK[fo + 1] + hx * (exm - exK - exmN + exN)
                                     + hy * (eym - eyK - eymN + eyN);
-                    const cx<R> M = kmd * ezK + mx1 * (exm + exmN) + mx2 * (exK + exN) + my1 * (eym + eymN)
-                                    + my2 * (eyK + eyN);
+                    cx<R> uK = ezK, uxm = exm, uxK = exK, uxmN = exmN, uxN = exN;
+                    cx<R> uym = eym, uyK = eyK, uymN = eymN, uyN = eyN;
+                    if (PS) {
+                        const R rck = rc[k];
+                        const cx<R> p0 = PK[fo], q0 = PP[fo];
+                        uK = uK + (q0 - p0) * rck;
+                        uxm = uxm + (p0 - PK[fo - 1]) * ram;
+                        uxK = uxK + (PK[fo + 1] - p0) * rai;
+                        uxmN = uxmN + (q0 - PP[fo - 1]) * ram;
+                        uxN = uxN + (PP[fo + 1] - q0) * rai;
+                        uym = uym + (p0 - PK[fo - PX]) * rbm;
+                        uyK = uyK + (PK[fo + PX] - p0) * rbj;
+                        uymN = uymN + (q0 - PP[fo - PX]) * rbm;
+                        uyN = uyN + (PP[fo + PX] - q0) * rbj;
+                    }
+                    const cx<R> M = kmd * uK + mx1 * (uxm + uxmN) + mx2 * (uxK + uxN) + my1 * (uym + uymN)
+                                    + my2 * (uyK + uyN);
                     out = km_combine(K, M, wmu, alpha);
                 }
                 memit<R, DOT>(y, w, ez0 + (int64_t)k * ZP, out, in, ezK, acc);
@@ -615,8 +682,8 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
             eyP = eyK;
         }
         if (!PF && k + 1 < ke) {
-            L0.fetch(d, x, sig, k + 1, XP, YP, ZP, CP);
-            if (two) L1.fetch(d, x, sig, k + 1, XP, YP, ZP, CP);
+            L0.fetch(d, x, sig, k + 1, XP, YP, ZP, CP, psi, rw);
+            if (two) L1.fetch(d, x, sig, k + 1, XP, YP, ZP, CP, psi, rw);
         }
     }
     if (DOT >= 1 && DOT <= 3) block_reduce_store<4, G::NT>(acc, partials);
@@ -1385,22 +1452,6 @@ void launch_cocr_upd_grad(const DevGrid<R> &g, RedBuf &rb, cx<R> *x, cx<R> *pv, 
                        phi, cx<R>((R)0, (R)1 / wmu), rb.partials);
 }
 
-// g_e = (G phi)_e on every edge (0 on boundary edges when phi is 0 on boundary nodes)
-template <typename R>
-__global__ __launch_bounds__(TPB) void k_grad(Dims d, const R *__restrict__ a, const R *__restrict__ b,
-                                              const R *__restrict__ c, const cx<R> *__restrict__ phi,
-                                              cx<R> *__restrict__ g)
-{
-    const Tile TL = tile_id();
-    const Slice S = edge_slice(d, TL.y);
-    const int p = TL.x * TPB + threadIdx.x;
-    if (p >= S.nI * S.nJ) return;
-    const int j = p / S.nI, i = p - j * S.nI;
-    int64_t n0, n1;
-    const R l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
-    g[S.base + p] = (phi[n1] - phi[n0]) * ((R)1 / l);
-}
-
 // Mixed-precision recombination of the hybrid preconditioner:
 // z64_e = z32_e + s * (G phi)_e (phi in complex64 or complex128), with G
 // applied in FP64 so that later FP64 operator applications see the gradient
@@ -1724,7 +1775,7 @@ int spmv_chunk(const Dims &d, int ty, int64_t slots)
 
 template <typename R, int TY, bool PF>
 static int64_t march_go(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode,
-                        const cx<R> *w, RedBuf &rb, hipStream_t st, int kc)
+                        const cx<R> *w, RedBuf &rb, hipStream_t st, int kc, const cx<R> *psi = nullptr)
 {
     using G = mt::Geo<TY>;
     const Dims &d = g.d;
@@ -1750,13 +1801,14 @@ static int64_t march_go(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R 
         throw std::runtime_error("reduction buffer smaller than the stencil grid");
 #define SFDX_MARCH(M)                                                                                      \
     hipLaunchKernelGGL((k_spmv_march<R, M, TY, PF>), grid, block, 0, st, d, g.a, g.b, g.c, g.ra, g.rb, g.rc, \
-                       g.sig, wmu, x, y, alpha, w, rb.partials, ntx, kc)
+                       g.sig, wmu, x, y, alpha, w, rb.partials, ntx, kc, psi)
     switch (dot_mode) {
     case 0: SFDX_MARCH(0); break;
     case 1: SFDX_MARCH(1); break;
     case 2: SFDX_MARCH(2); break;
     case 3: SFDX_MARCH(3); break;
     case 4: SFDX_MARCH(4); break;
+    case 5: SFDX_MARCH(5); break;
     default: throw std::runtime_error("launch_spmv_march: bad mode");
     }
 #undef SFDX_MARCH
@@ -1777,19 +1829,19 @@ int64_t launch_spmv_march(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, 
     }
 }
 
-// One row per thread (13-neighbour gather); kept for the split residual and for
-// comparison with the k-marching form.
+// One row per thread (13-neighbour gather); kept for comparison with the
+// k-marching form.
 template <typename R>
 int64_t launch_spmv_gather(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode,
                            const cx<R> *w, RedBuf &rb, hipStream_t st)
 {
     SliceLaunch L = edge_launch(g.d);
     switch (dot_mode) {
-    case 0: hipLaunchKernelGGL((k_spmv<R, 0>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
-    case 1: hipLaunchKernelGGL((k_spmv<R, 1>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
-    case 2: hipLaunchKernelGGL((k_spmv<R, 2>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
-    case 3: hipLaunchKernelGGL((k_spmv<R, 3>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
-    default: hipLaunchKernelGGL((k_spmv<R, 4>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials, nullptr); break;
+    case 0: hipLaunchKernelGGL((k_spmv<R, 0>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials); break;
+    case 1: hipLaunchKernelGGL((k_spmv<R, 1>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials); break;
+    case 2: hipLaunchKernelGGL((k_spmv<R, 2>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials); break;
+    case 3: hipLaunchKernelGGL((k_spmv<R, 3>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials); break;
+    default: hipLaunchKernelGGL((k_spmv<R, 4>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, x, y, alpha, w, rb.partials); break;
     }
     return L.nblocks();
 }
@@ -1840,15 +1892,22 @@ int tune_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, RedBuf &rb, 
     return best;
 }
 
-// t = r - A z - M (G psi): residual of z + (i/wmu) G psi without forming A G psi
+// t = r - A (z + (i/wmu) G psi) on interior edges, 0 on boundary edges: the
+// k-marching stencil with the node potential in a third LDS ring, forming
+// r - K z + i wmu M (z + (i/wmu) G psi) so that the curl part never acts on
+// G psi (K G = 0; in resistive cells psi is orders of magnitude larger than z
+// and K G psi would be rounding noise of that size)
 template <typename R>
-void launch_resid_split(const DevGrid<R> &g, R wmu, const cx<R> *z, const cx<R> *psi, const cx<R> *r, cx<R> *gtmp,
-                        cx<R> *t, hipStream_t st)
+void launch_resid_split(const DevGrid<R> &g, R wmu, const cx<R> *z, const cx<R> *psi, const cx<R> *r, cx<R> *t,
+                        hipStream_t st)
 {
-    SliceLaunch L = edge_launch(g.d);
-    hipLaunchKernelGGL((k_grad<R>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, psi, gtmp);
-    hipLaunchKernelGGL((k_spmv<R, 5>), L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sig, wmu, z, t, (R)-1, r,
-                       (double *)nullptr, (const cx<R> *)gtmp);
+    RedBuf none;
+    switch (g.spmv_variant) {
+    case 1: march_go<R, 8, false>(g, wmu, z, t, (R)-1, 5, r, none, st, 0, psi); break;
+    case 2: march_go<R, 4, true>(g, wmu, z, t, (R)-1, 5, r, none, st, 0, psi); break;
+    case 3: march_go<R, 4, false>(g, wmu, z, t, (R)-1, 5, r, none, st, 0, psi); break;
+    default: march_go<R, 8, true>(g, wmu, z, t, (R)-1, 5, r, none, st, 0, psi); break;
+    }
 }
 
 template <typename R>
@@ -2428,8 +2487,8 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
                                           cx<R> *, const cx<R> *, const cx<R> *, R, hipStream_t);                 \
     template void launch_hybrid_gs<R>(const DevGrid<R> &, R, const R *, const cx<R> *, cx<R> *, cx<R> *, cx<R> *,  \
                                       int, hipStream_t, bool, int);                                                    \
-    template void launch_resid_split<R>(const DevGrid<R> &, R, const cx<R> *, const cx<R> *, const cx<R> *,       \
-                                        cx<R> *, cx<R> *, hipStream_t);                                           \
+    template void launch_resid_split<R>(const DevGrid<R> &, R, const cx<R> *, const cx<R> *, const cx<R> *, cx<R> *, \
+                                        hipStream_t);                                           \
     template void launch_prolong_node_add<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *,    \
                                              cx<R> *, hipStream_t);                                               \
     template void launch_grad_add<R>(const DevGrid<R> &, cx<R>, const cx<R> *, cx<R> *, hipStream_t, const cx<R> *, \

@@ -83,7 +83,7 @@ EXPORTS = ["sfdx_create", "sfdx_destroy", "sfdx_last_error", "sfdx_default_opts"
            "sfdx_post_process", "sfdx_mt1d_anl", "sfdx_apply_operator", "sfdx_ssor_apply", "sfdx_hybrid_apply", "sfdx_div_correct",
            "sfdx_extint_v", "sfdx_bench_kernels", "sfdx_direct_solve", "sfdx_solve_batch", "sfdx_misfit",
            "sfdx_adjoint", "sfdx_fwd_response", "sfdx_post_process_topo", "sfdx_surface_planes",
-           "sfdx_mesh_coarsen", "sfdx_apply_operator_noalloc"]
+           "sfdx_mesh_coarsen", "sfdx_apply_operator_noalloc", "sfdx_split_residual"]
 
 _lib = None
 
@@ -109,6 +109,7 @@ def load(path: str = LIB_PATH):
     L.sfdx_mt1d_anl.argtypes = [I, P, P, D, P, P, P]
     L.sfdx_apply_operator.argtypes = [P, ctypes.POINTER(Grid), P, D, I, P, P]
     L.sfdx_apply_operator_noalloc.argtypes = [P, ctypes.POINTER(Grid), P, D, P, P, P]
+    L.sfdx_split_residual.argtypes = [P, ctypes.POINTER(Grid), P, D, I, P, P, P, P]
     L.sfdx_ssor_apply.argtypes = [P, ctypes.POINTER(Grid), P, D, D, I, P, P]
     L.sfdx_hybrid_apply.argtypes = [P, ctypes.POINTER(Grid), P, D, D, I, I, P, P]
     L.sfdx_div_correct.argtypes = [P, ctypes.POINTER(Grid), P, P, D, ctypes.POINTER(I)]
@@ -275,6 +276,20 @@ class Context:
         _check(self.lib.sfdx_apply_operator(self.h, ctypes.byref(G.g), _ptr(sigma_flat(sigma)), float(freq),
                                             int(precision), _ptr(x), _ptr(y)))
         return y
+
+    def split_residual(self, a, b, c, sigma, freq, r, z, psi, precision=PREC_FP64):
+        """r - A_s (z + (i/wmu) G psi) on interior edges (0 on boundary edges)."""
+        G = _GridHolder(a, b, c)
+        r = np.ascontiguousarray(r, np.complex128)
+        z = np.ascontiguousarray(z, np.complex128)
+        psi = np.ascontiguousarray(psi, np.complex128)
+        nn = (len(a) + 1) * (len(b) + 1) * (len(c) + 1)
+        if r.shape != z.shape or psi.shape != (nn,):
+            raise ValueError("split_residual: r, z need nl entries and psi one per node")
+        t = np.zeros_like(r)
+        _check(self.lib.sfdx_split_residual(self.h, ctypes.byref(G.g), _ptr(sigma_flat(sigma)), float(freq),
+                                            int(precision), _ptr(r), _ptr(z), _ptr(psi), _ptr(t)))
+        return t
 
     def ssor_apply(self, a, b, c, sigma, freq, x, omega=1.0, precision=PREC_FP64):
         G = _GridHolder(a, b, c)

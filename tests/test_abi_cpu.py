@@ -27,6 +27,11 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(sfdx.EXPORTS)
+    # every entry point with parameters has its ctypes prototype (floats would
+    # otherwise not convert to double)
+    for n in names:
+        if n != "sfdx_last_error":
+            assert getattr(lib, n).argtypes is not None, n
 
 
 def test_default_options_match_reference_parameters():

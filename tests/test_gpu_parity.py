@@ -55,6 +55,44 @@ def test_operator_fp64_matches_oracle_csr(ctx, aniso, shape):
     assert np.abs(y2 - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("shape", [(9, 7, 8), (33, 9, 5), (40, 17, 26)])
+@pytest.mark.parametrize("aniso", [False, True])
+def test_split_residual_matches_oracle_operator(ctx, aniso, shape):
+    """sfdx_split_residual, the multigrid V-cycle's residual of a correction
+    held as z + (i/wmu) G psi (formed without the curl part acting on G psi),
+    against the oracle's ve-scaled CSR applied to the assembled field
+    u = z + (i/wmu) G psi, G from the oracle's gradient; psi vanishes on
+    boundary nodes, as in the V-cycle.  complex128 to 1e-12, complex64 to
+    2e-6 of the largest entry.  The psi part must move the result by far more
+    than the tolerance, so a dropped or misplaced gradient term fails."""
+    nx, ny, nz = shape
+    a, b, c, nair, sig = tiny_model(nx, ny, nz, 2, aniso=aniso)
+    f = 0.37
+    wmu = 2 * np.pi * f * O.MU0
+    S = O.build_system(a, b, c, sig, f)
+    ve = O.row_scale(a, b, c, S.bnd)
+    nn = O.nnodes(nx, ny, nz)
+    r = _rand(len(S.p1), 3)
+    z = _rand(len(S.p1), 4)
+    r[S.bnd] = 0
+    z[S.bnd] = 0
+    psi = 0.1 * _rand(nn, 5)   # G psi part ~10-50% of the result; oracle rounding < 2e-13
+    idx = np.arange(nn)
+    ii, jj, kk = idx % (nx + 1), (idx // (nx + 1)) % (ny + 1), idx // ((nx + 1) * (ny + 1))
+    psi[(ii == 0) | (ii == nx) | (jj == 0) | (jj == ny) | (kk == 0) | (kk == nz)] = 0
+    u = z + (1j / wmu) * O.grad(a, b, c, psi)
+    assert np.all(u[S.bnd] == 0)
+    ref = r - ve * (S.A @ u)
+    ref[S.bnd] = 0
+    ref0 = r - ve * (S.A @ z)
+    ref0[S.bnd] = 0
+    assert np.abs(ref - ref0).max() > 2e-2 * np.abs(ref).max()
+    for prec, tol in ((sfdx.PREC_FP64, 1e-12), (sfdx.PREC_MIXED, 2e-6)):
+        t = ctx.split_residual(a, b, c, sig, f, r, z, psi, precision=prec)
+        assert np.all(t[S.bnd] == 0)
+        assert np.abs(t - ref).max() <= tol * np.abs(ref).max(), prec
+
+
 @pytest.mark.parametrize("model", ["halfspace32", "aniso64"])
 @pytest.mark.parametrize("f", [1.0, 0.1])
 def test_operator_fp32_matches_fp64(ctx, model, f):
