@@ -13,12 +13,13 @@ smoothing: lumped-mass multicolour edge SSOR sweeps plus the nodal correction
 by a two-step Jacobi polynomial, residuals in split edge + nodal form):
   * FP64: everything in complex128;
   * mixed: complex128 COCR, operator and residual; the V-cycle in complex64
-    (the complex128 true residual is recomputed after every 1e-6 drop and on
+    (the complex128 true residual is recomputed after every 1e-5 drop and on
     stagnation, and once more before convergence is accepted).
 The roofline object is for the stencil SpMV (k_spmv_march), timed with HIP events on
 the library's stream.  The CPU baseline is the FP64 oracle (CSR, lexicographic
 SSOR, COCR) on one core, timed on a bounded number of fine-level iterations and
-extrapolated to the FP64 GPU iteration count.
+extrapolated to the FP64 GPU iteration count; beside it, the oracle's measured
+time to solution on the reference's CPU case (32x32x32) against the GPU's.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--small] [--no-cpu]
     python bench.py --kernels-only     # stencil/SSOR/copy kernels only (profiling)
@@ -37,7 +38,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-RESTART_DROP = 1e-6
+# Mixed precision restarts the FP64 COCR recurrences from the true residual
+# after every drop by this factor.  With the complex64 V-cycle, measured at
+# 128x128x96 (fine-level s, C2 flat / C3 topography): 1e-2: 1.197 / 3.336,
+# 1e-3: 0.796 / 1.032, 1e-4: 0.755 / 1.004, 1e-5: 0.740 / 1.004, 1e-6:
+# 0.763 / 1.291 (the complex64 preconditioner makes the recurrences drift;
+# the second polarisation of C3 stalled for ~300 iterations between restarts).
+# The one-level hybrid preconditioner (the batch) keeps 1e-6.
+RESTART_DROP_MG, RESTART_DROP = 1e-5, 1e-6
 
 
 def profiled_traffic(kernel):
@@ -86,11 +94,12 @@ def solve_opts(m, precision, eps, verbose=0, precond=None):
     # more per iteration and, inside the V-cycle, takes more iterations).
     # In the mixed path the whole V-cycle runs in complex64 inside complex128 COCR.
     nodal = 1
+    precond = sfdx.PRECOND_MG if precond is None else precond
+    drop = RESTART_DROP_MG if precond == sfdx.PRECOND_MG else RESTART_DROP
     return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0,
                              norm=sfdx.NORM_INTERIOR, precision=precision, verbose=verbose,
-                             precond=sfdx.PRECOND_MG if precond is None else precond,
-                             mg_levels=MG_LEVELS, mg_coarse_pairs=MG_COARSE_PAIRS,
-                             sigma_floor32=0.0, restart_drop=RESTART_DROP, maxit_excmg=8000, use_graph=USE_GRAPH,
+                             precond=precond, mg_levels=MG_LEVELS, mg_coarse_pairs=MG_COARSE_PAIRS,
+                             sigma_floor32=0.0, restart_drop=drop, maxit_excmg=8000, use_graph=USE_GRAPH,
                              nodal_smoother=nodal)
 
 
@@ -175,10 +184,12 @@ def cpu_c1_measured(ctx):
             "fine_speedup": round(cpu_fine / st.fine_seconds, 1) if st.fine_seconds > 0 else None}
 
 
-def impedances(m, ex):
+def impedances(m, ex, surf=None):
+    """Impedances at the stations; with surf (per-column surface node planes),
+    on the ground surface of a model with topography, else on the flat datum."""
     from sfd_excmg_amd import sfdx
 
-    r, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex)
+    r, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, surf=surf)
     return r[:, [0, 2]] + 1j * r[:, [1, 3]]
 
 
@@ -247,22 +258,24 @@ def run_c3(ctx, args):
     from sfd_excmg_amd import sfdx, synth
 
     m = synth.make_aniso_topography_128() if not args.small else synth.make_aniso_topography_128(nx=64, nz_earth=40, nair=8, nstations=20)
+    # stations on the ground: each column's first earth node plane
+    surf = sfdx.surface_planes(m.a, m.b, m.c, m.sigma, 1e-6)
     ref_opts = solve_opts(m, sfdx.PREC_FP64, 1e-11)
     ref_opts.maxit_excmg = 8000
     ex_ref, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, ref_opts)
-    z_ref = impedances(m, ex_ref)
+    z_ref = impedances(m, ex_ref, surf)
     out = {}
     for name, prec in (("fp64", sfdx.PREC_FP64), ("mixed", sfdx.PREC_MIXED)):
         ex, r = run_gpu(ctx, m, 1, 1, solve_opts(m, prec, 1e-8))
-        err = np.abs(impedances(m, ex) - z_ref) / np.abs(z_ref)
+        err = np.abs(impedances(m, ex, surf) - z_ref) / np.abs(z_ref)
         out[name] = {"fine_s": round(r["fine"], 4), "wall_s": round(r["wall"], 3), "fine_iters": r["fine_iters"],
                      "status": r["status"],
                      "impedance_rel_err_vs_fp64_1e-11": {"max": float(err.max()), "p99": float(np.quantile(err, 0.99)),
                                                           "median": float(np.median(err)), "count": int(err.size)}}
         if name == "fp64":
-            z64 = impedances(m, ex)
+            z64 = impedances(m, ex, surf)
         else:
-            d = np.abs(impedances(m, ex) - z64) / np.abs(z64)
+            d = np.abs(impedances(m, ex, surf) - z64) / np.abs(z64)
             out[name]["impedance_rel_err_vs_fp64_1e-8"] = {"max": float(d.max()), "median": float(np.median(d))}
         print(f"{name} done {r['fine']:.3f} s", file=sys.stderr, flush=True)
     print(json.dumps({"metric": "mixed vs FP64 on the 128x128x96 topography model: fine-grid solve s and impedance error",
@@ -270,6 +283,7 @@ def run_c3(ctx, args):
                       "fp64": out["fp64"], "mixed": out["mixed"],
                       "speedup_mixed_vs_fp64": round(out["fp64"]["fine_s"] / out["mixed"]["fine_s"], 3),
                       "topography_max_depth_m": m.meta.get("topography_max_depth"),
+                      "stations": "on the ground surface (first earth node plane of each column)",
                       "dtype": "complex128 Krylov/operator + complex64 preconditioner"}))
 
 
@@ -428,8 +442,6 @@ def main():
         "headline_converged": bool(converged),
         "speedup_mixed_vs_fp64": round(f64["fine"] / mx["fine"], 3) if mx["fine"] > 0 else None,
         "max_rel_impedance_err_mixed_vs_fp64": zerr,
-        "fine_solve_hbm_gbs_algorithmic": {"mixed": round(mx["fine_bytes"] / mx["fine"] / 1e9, 1) if mx["fine"] else None,
-                                          "fp64": round(f64["fine_bytes"] / f64["fine"] / 1e9, 1) if f64["fine"] else None},
         "roofline": {"kernel": "k_spmv_march<double,0,8,true> (complex128 k-marching stencil, 32x8 tile)", "bound": "hbm",
                      "achieved": round(ach64, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),

@@ -118,12 +118,13 @@ typedef struct {
                            the device and keep the fastest (cached in the context).  The
                            tile sets the grouping of the fused reductions, so a fixed tile
                            keeps repeated runs bit-identical */
-    int ssor_cross_mass;  /* edge SSOR inside the hybrid preconditioner: 0 (default) K plus the
-                             lumped (diagonal) mass -- the cross-orientation mass couplings of
-                             anisotropic cells are left out, so each colour sweep reads one
-                             conductivity component instead of three; 1: the full operator.
-                             The plain SSOR preconditioner (the reference's) and the multigrid
-                             smoothers always use the full operator */
+    int ssor_cross_mass;  /* edge SSOR inside the hybrid preconditioner and the multigrid
+                             smoother: 0 (default) K plus the lumped (diagonal) mass -- the
+                             cross-orientation mass couplings of anisotropic cells are left out,
+                             so each colour sweep reads one conductivity component instead of
+                             three; 1: the full operator.  The plain SSOR preconditioner (the
+                             reference's) always uses the full operator, and so do the
+                             multigrid residuals and every operator application */
     double surface_sigma; /* sfdx_fwd_response and sfdx_adjoint: 0 (default) take the station
                              fields on the flat datum, node plane nair (fdfe_main.f90:597-640);
                              > 0: topography carved into sigma -- the surface of each cell column
@@ -136,7 +137,9 @@ typedef struct {
     int status;
     double total_seconds;
     double fine_seconds;            /* fine-level Krylov time, all frequencies and both polarisations */
-    double fine_bytes;              /* algorithmic HBM bytes of the fine-level kernels */
+    double fine_bytes;              /* algorithmic HBM bytes of the fine-level Krylov kernels (stencil,
+                                       vector updates) and of an SSOR or hybrid preconditioner's edge
+                                       SSOR; the multigrid V-cycle is not counted */
     int iters[SFDX_MAX_LEVELS][2];  /* last frequency: Krylov iterations per level and polarisation */
     double relres[SFDX_MAX_LEVELS][2];  /* final relative residual of each Krylov solve, recomputed from
                                            the returned iterate (b - A x), not the recurrence residual */

@@ -479,7 +479,9 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     SliceLaunch L = edge_launch(d);
     const int64_t nbe = L.nblocks();
     const double sc = 2.0 * sizeof(R), vecb = (double)d.NL * sc;
-    const double b_spmv = 2 * vecb + (double)d.NC * 6 * sizeof(R), b_ssor = 2 * (b_spmv + vecb);
+    // preconditioner bytes: the edge SSOR model (two sweeps of stencil traffic);
+    // the multigrid V-cycle is not counted
+    const double b_spmv = 2 * vecb + (double)d.NC * 6 * sizeof(R), b_ssor = mg ? 0.0 : 2 * (b_spmv + vecb);
     SolveInfo info;
     if (ko.dc_at_start) {
         info.pcg_iters += div_correct<R>(ctx, D, tag, x, ko.corr_tol);

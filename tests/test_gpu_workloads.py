@@ -46,10 +46,12 @@ def ctx():
 def headline_opts(m, precision, eps, precond=sfdx.PRECOND_MG):
     """The benchmark's solver setup: COCR preconditioned by the V-cycle over the
     3 finest grids with one coarsest smoothing pair, no periodic divergence
-    correction, interior residual (precond=PRECOND_HYBRID: the one-level
-    hybrid preconditioner instead)."""
+    correction, interior residual, mixed-precision restarts after every 1e-5
+    drop (precond=PRECOND_HYBRID: the one-level hybrid preconditioner, restarts
+    every 1e-6)."""
+    drop = 1e-5 if precond == sfdx.PRECOND_MG else 1e-6
     return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0, norm=sfdx.NORM_INTERIOR,
-                             precision=precision, sigma_floor32=0.0, restart_drop=1e-6, maxit_excmg=8000,
+                             precision=precision, sigma_floor32=0.0, restart_drop=drop, maxit_excmg=8000,
                              precond=precond, mg_levels=3, mg_coarse_pairs=1)
 
 
