@@ -16,6 +16,8 @@
 ! 1 BiCGStab, 3 GPBiCG, 5 COCR), SFDX_PRECOND (0 SSOR, 1 hybrid, 2 multigrid),
 ! SFDX_COARSE (0 direct solves of EXCMG levels 1-2, 1 device Krylov solves),
 ! SFDX_FULL_BG (1: full-tensor 1-D boundary with cross-polar components),
+! SFDX_MG_LEVELS and SFDX_MG_PAIRS (multigrid grids and coarsest smoothing
+! pairs), SFDX_RESTART_DROP (mixed precision: restart factor), SFDX_SPMV_TILE,
 ! SFDX_VERBOSE, SFDX_RES_FULL (also write <name>.resx with 17 significant digits).
 module fwd3d
   use iso_c_binding
@@ -92,6 +94,9 @@ contains
     o%coarse_solver = env_int('SFDX_COARSE', 0)
     o%full_background = env_int('SFDX_FULL_BG', 0)
     o%spmv_tile = env_int('SFDX_SPMV_TILE', 0)
+    o%mg_levels = env_int('SFDX_MG_LEVELS', int(o%mg_levels))
+    o%mg_coarse_pairs = env_int('SFDX_MG_PAIRS', int(o%mg_coarse_pairs))
+    o%restart_drop = env_real('SFDX_RESTART_DROP', o%restart_drop)
     rc = sfdx_fwd_solve(sfdx_handle, g, sigma, int(nf, c_int), freq, o, ex, st)
     if (rc > 0) then
       write(*,*) 'Error in the GPU forward solve: ', sfdx_error_text()
