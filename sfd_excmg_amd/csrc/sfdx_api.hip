@@ -204,6 +204,21 @@ void ensure_P(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag)
     launch_pbuild<R>(D.g.d, D.a64, D.b64, D.c64, D.sig64, D.P, ctx->st);
 }
 
+// Helper context of the second polarisation (its own stream, reduction
+// buffers, pinned slots and work-vector pool), created on first use and kept
+// as the first of the context's worker contexts (which the batch also uses).
+sfdx_ctx *pol_helper(sfdx_ctx *ctx)
+{
+    if (ctx->workers.empty()) {
+        sfdx_ctx *w = nullptr;
+        if (sfdx_create(ctx->device, &w) != SFDX_OK) throw std::runtime_error("helper context: " + g_err);
+        ctx->workers.push_back(w);
+    }
+    sfdx_ctx *w = ctx->workers[0];
+    ensure_partials(w, ctx->rb.max_blocks);
+    return w;
+}
+
 struct SolveInfo {
     int iters = 0, dc_calls = 0, pcg_iters = 0, status = 0, restarts = 0, fallback = 0;
     int stalled = 0;   // COCR stopped on unproductive restarts (not on maxit)
@@ -274,6 +289,25 @@ struct MGHier {
     int coarse_pairs = 4;
     int nodal = 0;  // nodal part of the smoother: 0 8-colour Gauss-Seidel on P, 1 Jacobi polynomial
 };
+
+// The level data the solvers build on first use (the nodal operators P of the
+// level and of every multigrid grid, the FP32 copy of the symmetric form),
+// built here so that two polarisations solved at the same time only read it.
+void prebuild_level(sfdx_ctx *ctx, LevelDev<double> &D64, LevelDev<float> *D32, MGHier<double> *h64,
+                    MGHier<float> *h32, const std::string &tag, bool mixed)
+{
+    ensure_P(ctx, D64, tag);
+    if (mixed && !D64.Ps32) {
+        D64.Ps32 = pool_vec<float>(ctx, tag + "/Ps32", (size_t)10 * D64.g.d.NP);
+        launch_psym32(D64.g.d.NP, D64.P, D64.Ps32, ctx->st);
+    }
+    if (D32) ensure_P(ctx, *D32, tag);
+    for (size_t l = 1; h64 && l < h64->lev.size(); l++)
+        if (h64->lev[l]) ensure_P(ctx, *h64->lev[l], "mg" + std::to_string(l) + "d");
+    for (size_t l = 1; h32 && l < h32->lev.size(); l++)
+        if (h32->lev[l]) ensure_P(ctx, *h32->lev[l], "mg" + std::to_string(l) + "f");
+    SFDX_CHECK(hipStreamSynchronize(ctx->st));
+}
 
 std::vector<double> mg_weights_1d(const std::vector<double> &wf, const std::vector<double> &wc)
 {
@@ -1322,24 +1356,76 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
             zd *edc = pool_vec<zd>(ctx, "edc" + std::to_string(i), edc_h.size());
             SFDX_CHECK(hipMemcpyAsync(edc, edc_h.data(), edc_h.size() * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
             const std::string tag = "L" + std::to_string(i);
+            // initial guesses of both polarisations on the context's stream
+            zd *xs[2];
             for (int pol = 0; pol < 2; pol++) {
-                zd *x = ubuf(i, pol);
-                if (o->use_excmg && i >= 3) extint_device(ctx, lev[i], lev[i - 1], u[i - 2][pol], u[i - 1][pol], x, tag);
-                else SFDX_CHECK(hipMemsetAsync(x, 0, d.NL * sizeof(zd), ctx->st));
-                const int maxit = o->use_excmg ? o->maxit_excmg : o->maxit_plain;
-                SFDX_CHECK(hipStreamSynchronize(ctx->st));
+                xs[pol] = ubuf(i, pol);
+                if (o->use_excmg && i >= 3)
+                    extint_device(ctx, lev[i], lev[i - 1], u[i - 2][pol], u[i - 1][pol], xs[pol], tag);
+                else SFDX_CHECK(hipMemsetAsync(xs[pol], 0, d.NL * sizeof(zd), ctx->st));
+            }
+            const int maxit = o->use_excmg ? o->maxit_excmg : o->maxit_plain;
+            // levels below the multigrid hierarchy's coarsest grid: hybrid
+            const bool mg_here = use_mg && i + off >= 1;
+            sfdx_opts oi = *o;
+            if (use_mg && !mg_here) oi.precond = SFDX_PRECOND_HYBRID;
+            LevelDev<float> *d32 = (lo_fine && i == Lv) ? &D32[i] : nullptr;
+            MGHier<double> *h64 = mg_here ? &H64 : nullptr;
+            MGHier<float> *h32 = (mg_here && lo_fine && i == Lv) ? &H32 : nullptr;
+            const bool two = o->pol_streams == 2;
+            if (two) prebuild_level(ctx, D64[i], d32, h64, h32, tag, lo_fine && i == Lv);
+            SFDX_CHECK(hipStreamSynchronize(ctx->st));
+            auto solve_pol = [&](sfdx_ctx *c, int pol) {
+                LevelSolveResult R = solve_level(c, oi, D64[i], d32, tag, ff, edc, pol, xs[pol], tol, maxit, h64, h32,
+                                                 i + off);
+                SFDX_CHECK(hipStreamSynchronize(c->st));
+                return R;
+            };
+            LevelSolveResult Rs[2];
+            double secs[2] = {0, 0}, level_secs = 0;
+            if (two) {
+                // the second polarisation on the helper context's stream, in a
+                // host thread of its own; both only read the shared level data
+                sfdx_ctx *w = pol_helper(ctx);
+                std::exception_ptr err;
+                std::string err_text;
                 Timer tl;
-                // levels below the multigrid hierarchy's coarsest grid: hybrid
-                const bool mg_here = use_mg && i + off >= 1;
-                sfdx_opts oi = *o;
-                if (use_mg && !mg_here) oi.precond = SFDX_PRECOND_HYBRID;
-                LevelSolveResult R = solve_level(ctx, oi, D64[i], (lo_fine && i == Lv) ? &D32[i] : nullptr,
-                                                 tag, ff, edc, pol, x, tol, maxit, mg_here ? &H64 : nullptr,
-                                                 (mg_here && lo_fine && i == Lv) ? &H32 : nullptr, i + off);
-                SFDX_CHECK(hipStreamSynchronize(ctx->st));
-                const double secs = tl.s();
+                std::thread th([&]() {
+                    try {
+                        SFDX_CHECK(hipSetDevice(ctx->device));
+                        Timer t1;
+                        Rs[1] = solve_pol(w, 1);
+                        secs[1] = t1.s();
+                    } catch (...) {
+                        err = std::current_exception();
+                        err_text = g_err;
+                    }
+                });
+                try {
+                    Rs[0] = solve_pol(ctx, 0);
+                    secs[0] = tl.s();
+                } catch (...) {
+                    th.join();
+                    throw;
+                }
+                th.join();
+                if (err) {
+                    if (!err_text.empty()) g_err = err_text;
+                    std::rethrow_exception(err);
+                }
+                level_secs = tl.s();
+            } else {
+                for (int pol = 0; pol < 2; pol++) {
+                    Timer tl;
+                    Rs[pol] = solve_pol(ctx, pol);
+                    secs[pol] = tl.s();
+                    level_secs += secs[pol];
+                }
+            }
+            if (i == Lv) S.fine_seconds += level_secs;
+            for (int pol = 0; pol < 2; pol++) {
+                const LevelSolveResult &R = Rs[pol];
                 if (i == Lv) {
-                    S.fine_seconds += secs;
                     S.fine_bytes += R.info.bytes;
                     S.fine_iters_total += R.info.iters;
                     S.refine_steps[pol] = R.refine;
@@ -1352,13 +1438,14 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 S.pcg_iters[i - 1][pol] = R.info.pcg_iters;
                 if (o->verbose)
                     fprintf(stderr, "f=%g level %d (%dx%dx%d) pol %d: %d its relres %.3e dc %d pcg %d %.3fs\n", ff, i,
-                            d.nx, d.ny, d.nz, pol, R.info.iters, R.info.relres, R.info.dc_calls, R.info.pcg_iters, secs);
+                            d.nx, d.ny, d.nz, pol, R.info.iters, R.info.relres, R.info.dc_calls, R.info.pcg_iters,
+                            secs[pol]);
                 if (R.info.status > 0) {
                     g_err = "Krylov solve failed on level " + std::to_string(i);
                     return R.info.status;
                 }
                 if (R.info.status < 0) worst = R.info.status;
-                u[i][pol] = x;
+                u[i][pol] = xs[pol];
             }
         }
         if (ex)
@@ -1547,6 +1634,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->spmv_tile = 0;
     o->surface_sigma = 0;
     o->ssor_cross_mass = 0;
+    o->pol_streams = 2;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)
@@ -1641,6 +1729,7 @@ int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_
                 const sfdx_job &J = jobs[it.j];
                 sfdx_opts oj = *o;
                 if (J.level > 0) oj.level = J.level;
+                oj.pol_streams = 1;  // the items already run concurrently
                 const int64_t nl = make_dims(J.grid.nx, J.grid.ny, J.grid.nz).NL;
                 double *exf = J.ex + (size_t)it.f * 2 * nl * 2;
                 std::memset(&sts[k], 0, sizeof(sfdx_stats));

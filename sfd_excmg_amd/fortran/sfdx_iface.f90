@@ -26,6 +26,7 @@ module sfdx_iface
     real(c_double) :: restart_drop
     integer(c_int) :: use_graph, nodal_smoother, coarse_solver, full_background, spmv_tile, ssor_cross_mass
     real(c_double) :: surface_sigma
+    integer(c_int) :: pol_streams
   end type
 
   type, bind(C) :: sfdx_stats_t

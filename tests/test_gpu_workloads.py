@@ -101,6 +101,24 @@ def test_c1_mixed_solves_repeat_bit_for_bit(ctx):
 
 
 # ------------------------------------------------------------------ C2
+@pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
+def test_concurrent_polarisations_equal_sequential_solves(ctx, precision):
+    """The two polarisations of each Krylov level solved at the same time, the
+    second on a helper stream (pol_streams 2, the default), give the same bits
+    and iteration counts as one after the other on one stream: the level data
+    is shared read-only and every reduction has a fixed order.  C2's generator
+    at 64x64x48, V-cycle, 4 EXCMG levels."""
+    m = synth.make_aniso_blocks_128(nx=64, nz_earth=40, nair=8, nstations=20)
+    o = headline_opts(m, precision, 1e-8)
+    o.pol_streams = 1
+    e1, s1 = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, o)
+    o.pol_streams = 2
+    e2, s2 = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, o)
+    assert s1.status == 0 and s2.status == 0
+    assert [list(r) for r in s1.iters] == [list(r) for r in s2.iters]
+    assert np.array_equal(e1.view(np.uint64), e2.view(np.uint64))
+
+
 def test_c2_full_size_fp64_and_mixed_residuals_and_impedances(ctx):
     m = synth.make_aniso_blocks_128()
     eps = 1e-8
