@@ -455,7 +455,13 @@ def main():
                      "ssor_apply_ms": {"fp32": round(ms32[1], 4), "fp64": round(ms64[1], 4)},
                      "hybrid_apply_ms": {"fp32": round(ms32[5], 4), "fp64": round(ms64[5], 4)},
                      "copy_ms": {"fp32": round(ms32[4], 5), "fp64": round(ms64[4], 5),
-                                 "gbs_fp64": round(2 * nl * 16 / (ms64[4] * 1e-3) / 1e9, 1)}},
+                                 "gbs_fp64": round(2 * nl * 16 / (ms64[4] * 1e-3) / 1e9, 1)},
+                     # the same bytes through a streaming copy tuned for bandwidth:
+                     # the rate this box attains, and the stencil's share of it
+                     "attainable": {"tuned_copy_ms_fp64": round(ms64[3], 5),
+                                    "gbs": round(2 * nl * 16 / (ms64[3] * 1e-3) / 1e9, 1),
+                                    "stencil_frac_of_attainable":
+                                        round(ach64 / (2 * nl * 16 / (ms64[3] * 1e-3) / 1e9), 4)}},
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["extrapolated"], 2), "unit": "s", "cores": 1, "kind": "port",
             "sample": f"FP64 oracle COCR+lexicographic SSOR, {cpu['iters']} fine-level iterations timed "

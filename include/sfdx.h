@@ -332,7 +332,9 @@ int sfdx_div_correct(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, dou
 int sfdx_extint_v(sfdx_ctx *ctx, const sfdx_grid *g, const double *u0, const double *u1, double *out);
 /* Times the fine-level kernels on the context's stream with HIP events,
  * averaged over reps: ms[0] SpMV, ms[1] SSOR apply (12 colour sweeps),
- * ms[2] COCR vector update, ms[3] = ms[0]+ms[1]+ms[2], ms[4] plain copy of
+ * ms[2] COCR vector update, ms[3] the same bytes as ms[4] moved by a
+ * bandwidth-tuned streaming copy (several 16-byte loads in flight per thread,
+ * non-temporal stores: the attainable streaming rate), ms[4] plain copy of
  * one edge vector, ms[5] hybrid preconditioner apply, ms[6] SpMV in the
  * one-row-per-thread gather form, ms[7..10] the k-marching SpMV with tiles of
  * 32x8 columns with and without prefetch, then 32x4 with and without,

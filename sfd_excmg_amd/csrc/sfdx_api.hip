@@ -2178,6 +2178,12 @@ int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, d
             // plain copy of one edge vector (2 * NL * sizeof(cx) bytes): calibrates
             // hardware byte counters against a kernel of known traffic
             ms[4] = timed([&](int q) { launch_copy<R>(d.NL, V(2 * q), V(2 * q + 1), ctx->st); });
+            // the same bytes with the bandwidth-tuned streaming copy (4 or 8
+            // loads in flight per thread, non-temporal stores; the faster one):
+            // the attainable streaming rate
+            const size_t cb = (size_t)d.NL * sizeof(cx<R>) / 16 * 16;
+            ms[3] = std::min(timed([&](int q) { launch_stream_copy(cb, V(2 * q), V(2 * q + 1), 4, ctx->st); }),
+                             timed([&](int q) { launch_stream_copy(cb, V(2 * q), V(2 * q + 1), 8, ctx->st); }));
             // hybrid preconditioner (SSOR + nodal auxiliary correction)
             ensure_P(ctx, D, "bench");
             cx<R> *tn = pool_vec<cx<R>>(ctx, std::string("bench/tn") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
@@ -2197,7 +2203,6 @@ int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, d
             });
         };
         if (precision == SFDX_PREC_FP64) run(0.0); else run(0.0f);
-        ms[3] = ms[0] + ms[1] + ms[2];
         SFDX_CHECK(hipEventDestroy(e0));
         SFDX_CHECK(hipEventDestroy(e1));
         return SFDX_OK;

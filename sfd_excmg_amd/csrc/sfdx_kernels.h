@@ -185,6 +185,9 @@ template <typename R>
 void launch_dot(int mode, int64_t n, const cx<R> *u, const cx<R> *v, RedBuf &rb, hipStream_t st, int64_t &nblk);
 template <typename R>
 void launch_copy(int64_t n, const cx<R> *s, cx<R> *t, hipStream_t st);
+// bandwidth-tuned streaming copy of `bytes` (a multiple of 16), units = 4 or 8
+// 16-byte loads in flight per thread (measurement only)
+void launch_stream_copy(size_t bytes, const void *s, void *t, int units, hipStream_t st);
 template <typename RS, typename RD>
 void launch_convert(int64_t n, const cx<RS> *s, cx<RD> *t, hipStream_t st);
 void launch_add_lo(int64_t n, zd *x, const zf *d, hipStream_t st);

@@ -1055,6 +1055,27 @@ __global__ __launch_bounds__(TPB) void k_copy(int64_t n, const cx<R> *__restrict
     if (q < n) t[q] = s[q];
 }
 
+// Streaming copy tuned for bandwidth (measurement only): 16-byte units, U loads
+// in flight per thread before the U non-temporal stores; the block's U x TPB
+// units are contiguous and each of the U load instructions is coalesced.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <int U>
+__global__ __launch_bounds__(TPB) void k_stream_copy(int64_t n16, const u32x4 *__restrict__ s, u32x4 *__restrict__ t)
+{
+    const int64_t base = (int64_t)blockIdx.x * (TPB * U) + threadIdx.x;
+    u32x4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int64_t q = base + (int64_t)u * TPB;
+        if (q < n16) v[u] = s[q];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int64_t q = base + (int64_t)u * TPB;
+        if (q < n16) __builtin_nontemporal_store(v[u], &t[q]);
+    }
+}
+
 template <typename RS, typename RD>
 __global__ __launch_bounds__(TPB) void k_convert(int64_t n, const cx<RS> *__restrict__ s, cx<RD> *__restrict__ t)
 {
@@ -1986,6 +2007,19 @@ template <typename R>
 void launch_copy(int64_t n, const cx<R> *s, cx<R> *t, hipStream_t st)
 {
     hipLaunchKernelGGL((k_copy<R>), grid1(n), dim3(TPB), 0, st, n, s, t);
+}
+
+void launch_stream_copy(size_t bytes, const void *s, void *t, int units, hipStream_t st)
+{
+    if (bytes % 16) throw std::runtime_error("launch_stream_copy: size must be a multiple of 16 bytes");
+    const int64_t n16 = (int64_t)(bytes / 16);
+    if (units == 8) {
+        const dim3 g((unsigned)((n16 + TPB * 8 - 1) / (TPB * 8)));
+        hipLaunchKernelGGL((k_stream_copy<8>), g, dim3(TPB), 0, st, n16, (const u32x4 *)s, (u32x4 *)t);
+    } else {
+        const dim3 g((unsigned)((n16 + TPB * 4 - 1) / (TPB * 4)));
+        hipLaunchKernelGGL((k_stream_copy<4>), g, dim3(TPB), 0, st, n16, (const u32x4 *)s, (u32x4 *)t);
+    }
 }
 
 template <typename RS, typename RD>
