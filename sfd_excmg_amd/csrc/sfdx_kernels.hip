@@ -1592,7 +1592,33 @@ __device__ __forceinline__ void node_split(const double *wl, int q, int &lo, int
     else { hi = lo; w = 1.0; }
 }
 
-// zf_e += (P zc)_e on interior fine edges
+// Layout of one edge orientation seen along the edge: edge (a along o; q1, q2
+// across, in the dimensions t1 < t2 other than o) has index
+// base + a*sa + q1*s1 + q2*s2, and it is interior when 0 < q1 < n1 and
+// 0 < q2 < n2.
+struct EdgeAxes {
+    int64_t base, sa, s1, s2;
+    int n1, n2;
+};
+
+__device__ __forceinline__ EdgeAxes edge_axes(const Dims &d, int o)
+{
+    EdgeAxes A;
+    if (o == 0) {
+        A.base = 0; A.sa = 1; A.s1 = d.nx; A.s2 = (int64_t)d.nx * (d.ny + 1); A.n1 = d.ny; A.n2 = d.nz;
+    } else if (o == 1) {
+        A.base = d.XN; A.sa = d.nx + 1; A.s1 = 1; A.s2 = (int64_t)d.ny * (d.nx + 1); A.n1 = d.nx; A.n2 = d.nz;
+    } else {
+        A.base = d.XN + d.YN; A.sa = (int64_t)(d.nx + 1) * (d.ny + 1); A.s1 = 1; A.s2 = d.nx + 1; A.n1 = d.nx;
+        A.n2 = d.ny;
+    }
+    return A;
+}
+
+// zf_e += (P zc)_e on interior fine edges: the coarse edge containing e along
+// its direction, linear weights over the (one or two) bracketing coarse nodes
+// in each transverse direction; terms in the order (lo,lo), (lo,hi), (hi,lo),
+// (hi,hi), zero weights and boundary coarse edges skipped
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_prolong_add(Dims df, Dims dc, MGTables T, const cx<R> *__restrict__ zc,
                                                      cx<R> *__restrict__ zf)
@@ -1604,33 +1630,31 @@ __global__ __launch_bounds__(TPB) void k_prolong_add(Dims df, Dims dc, MGTables 
     const int j = p / S.nI, i = p - j * S.nI, k = S.k;
     if (!edge_interior(df, S.o, i, j, k)) return;
     const int64_t e = S.base + p;
-    const int idx[3] = {i, j, k};
     const int o = S.o, t1 = (o == 0) ? 1 : 0, t2 = (o == 2) ? 1 : 2;
-    int l1, h1, l2, h2;
-    double w1, w2;
-    node_split(T.wl[t1], idx[t1], l1, h1, w1);
-    node_split(T.wl[t2], idx[t2], l2, h2, w2);
-    const int al = idx[o] >> 1;
+    const int ia = (o == 0) ? i : (o == 1) ? j : k;
+    const int q1 = (o == 0) ? j : i, q2 = (o == 2) ? j : k;
+    const EdgeAxes C = edge_axes(dc, o);
+    const int l1 = q1 >> 1, l2 = q2 >> 1;
+    const bool h1 = q1 & 1, h2 = q2 & 1;
+    const double w1 = h1 ? T.wl[t1][q1] : 1.0, w2 = h2 ? T.wl[t2][q2] : 1.0;
+    const int64_t c00 = C.base + (int64_t)(ia >> 1) * C.sa + (int64_t)l1 * C.s1 + (int64_t)l2 * C.s2;
     double sre = 0, sim = 0;
-    for (int a1 = 0; a1 < 2; a1++)
-        for (int a2 = 0; a2 < 2; a2++) {
-            const double w = (a1 ? 1.0 - w1 : w1) * (a2 ? 1.0 - w2 : w2);
-            if (w == 0) continue;
-            int ci[3];
-            ci[o] = al;
-            ci[t1] = a1 ? h1 : l1;
-            ci[t2] = a2 ? h2 : l2;
-            const int64_t ce = (o == 0) ? ex_id(dc, ci[0], ci[1], ci[2]) : (o == 1) ? ey_id(dc, ci[0], ci[1], ci[2])
-                                                                               : ez_id(dc, ci[0], ci[1], ci[2]);
-            if (!edge_interior(dc, o, ci[0], ci[1], ci[2])) continue;
-            const cx<R> v = zc[ce];
-            sre += w * (double)v.re;
-            sim += w * (double)v.im;
-        }
+    auto term = [&](int c1, int c2, double w, int64_t ce) {
+        if (w == 0 || c1 <= 0 || c1 >= C.n1 || c2 <= 0 || c2 >= C.n2) return;
+        const cx<R> v = zc[ce];
+        sre += w * (double)v.re;
+        sim += w * (double)v.im;
+    };
+    term(l1, l2, w1 * w2, c00);
+    if (h2) term(l1, l2 + 1, w1 * (1.0 - w2), c00 + C.s2);
+    if (h1) term(l1 + 1, l2, (1.0 - w1) * w2, c00 + C.s1);
+    if (h1 && h2) term(l1 + 1, l2 + 1, (1.0 - w1) * (1.0 - w2), c00 + C.s1 + C.s2);
     zf[e] = zf[e] + cx<R>((R)sre, (R)sim);
 }
 
-// rc = P^T rf on interior coarse edges (0 on the coarse boundary)
+// rc = P^T rf on interior coarse edges (0 on the coarse boundary): the 2 fine
+// edges along the coarse edge times the 3 x 3 fine node offsets across it,
+// summed transverse offset 1 outer, offset 2, then along (interior fine edges)
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_restrict(Dims df, Dims dc, MGTables T, const cx<R> *__restrict__ rf,
                                                   cx<R> *__restrict__ rc)
@@ -1642,27 +1666,26 @@ __global__ __launch_bounds__(TPB) void k_restrict(Dims df, Dims dc, MGTables T, 
     const int J = p / S.nI, I = p - J * S.nI, K = S.k;
     const int64_t ec = S.base + p;
     if (!edge_interior(dc, S.o, I, J, K)) { rc[ec] = cx<R>(0); return; }
-    const int idx[3] = {I, J, K};
     const int o = S.o, t1 = (o == 0) ? 1 : 0, t2 = (o == 2) ? 1 : 2;
-    const int nf[3] = {df.nx, df.ny, df.nz};
+    const int A = (o == 0) ? I : (o == 1) ? J : K;
+    const int C1 = (o == 0) ? J : I, C2 = (o == 2) ? J : K;
+    const EdgeAxes F = edge_axes(df, o);
+    const int64_t fa0 = F.base + (int64_t)(2 * A) * F.sa;
     double sre = 0, sim = 0;
+#pragma unroll
     for (int f1 = -1; f1 <= 1; f1++) {
-        const int q1 = 2 * idx[t1] + f1;
-        if (q1 < 0 || q1 > nf[t1]) continue;
+        const int q1 = 2 * C1 + f1;
+        if (q1 <= 0 || q1 >= F.n1) continue;
         const double w1 = f1 == 0 ? 1.0 : (f1 < 0 ? 1.0 - T.wl[t1][q1] : T.wl[t1][q1]);
+#pragma unroll
         for (int f2 = -1; f2 <= 1; f2++) {
-            const int q2 = 2 * idx[t2] + f2;
-            if (q2 < 0 || q2 > nf[t2]) continue;
+            const int q2 = 2 * C2 + f2;
+            if (q2 <= 0 || q2 >= F.n2) continue;
             const double w2 = f2 == 0 ? 1.0 : (f2 < 0 ? 1.0 - T.wl[t2][q2] : T.wl[t2][q2]);
+            const int64_t fe = fa0 + (int64_t)q1 * F.s1 + (int64_t)q2 * F.s2;
+#pragma unroll
             for (int fa = 0; fa < 2; fa++) {
-                int fi[3];
-                fi[o] = 2 * idx[o] + fa;
-                fi[t1] = q1;
-                fi[t2] = q2;
-                if (!edge_interior(df, o, fi[0], fi[1], fi[2])) continue;
-                const int64_t fe = (o == 0) ? ex_id(df, fi[0], fi[1], fi[2]) : (o == 1) ? ey_id(df, fi[0], fi[1], fi[2])
-                                                                                      : ez_id(df, fi[0], fi[1], fi[2]);
-                const cx<R> v = rf[fe];
+                const cx<R> v = rf[fe + fa * F.sa];
                 sre += w1 * w2 * (double)v.re;
                 sim += w1 * w2 * (double)v.im;
             }
@@ -1672,7 +1695,9 @@ __global__ __launch_bounds__(TPB) void k_restrict(Dims df, Dims dc, MGTables T, 
 }
 
 // psi_f += P_n psi_c on interior fine nodes: trilinear nodal interpolation with
-// the same node weights as the edge transfers, so that P_e G_c = G_f P_n.
+// the same node weights as the edge transfers, so that P_e G_c = G_f P_n;
+// terms in the order of the corner bits (x, y, z), zero weights and boundary
+// coarse nodes skipped
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_prolong_node_add(Dims df, Dims dc, MGTables T, const cx<R> *__restrict__ pc,
                                                           cx<R> *__restrict__ pf)
@@ -1684,18 +1709,27 @@ __global__ __launch_bounds__(TPB) void k_prolong_node_add(Dims df, Dims dc, MGTa
     if (p >= nI * nJ) return;
     const int j = p / nI, i = p - j * nI;
     if (node_boundary(df, i, j, k)) return;
-    int lo[3], hi[3];
+    const int q[3] = {i, j, k};
+    int lo[3];
+    bool h[3];
     double w[3];
-    node_split(T.wl[0], i, lo[0], hi[0], w[0]);
-    node_split(T.wl[1], j, lo[1], hi[1], w[1]);
-    node_split(T.wl[2], k, lo[2], hi[2], w[2]);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lo[a] = q[a] >> 1;
+        h[a] = q[a] & 1;
+        w[a] = h[a] ? T.wl[a][q[a]] : 1.0;
+    }
     const int cI = dc.nx + 1, cJ = dc.ny + 1;
+    const int64_t c0 = ((int64_t)lo[2] * cJ + lo[1]) * cI + lo[0];
     double sre = 0, sim = 0;
-    for (int q = 0; q < 8; q++) {
-        const int ci = (q & 1) ? hi[0] : lo[0], cj = (q & 2) ? hi[1] : lo[1], ck = (q & 4) ? hi[2] : lo[2];
-        const double wq = ((q & 1) ? 1.0 - w[0] : w[0]) * ((q & 2) ? 1.0 - w[1] : w[1]) * ((q & 4) ? 1.0 - w[2] : w[2]);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const bool bx = b & 1, by = b & 2, bz = b & 4;
+        if ((bx && !h[0]) || (by && !h[1]) || (bz && !h[2])) continue;  // weight 1 - 1 = 0
+        const double wq = (bx ? 1.0 - w[0] : w[0]) * (by ? 1.0 - w[1] : w[1]) * (bz ? 1.0 - w[2] : w[2]);
+        const int ci = lo[0] + bx, cj = lo[1] + by, ck = lo[2] + bz;
         if (wq == 0 || node_boundary(dc, ci, cj, ck)) continue;
-        const cx<R> v = pc[((int64_t)ck * cJ + cj) * cI + ci];
+        const cx<R> v = pc[c0 + (bx ? 1 : 0) + (by ? (int64_t)cI : 0) + (bz ? (int64_t)cI * cJ : 0)];
         sre += wq * (double)v.re;
         sim += wq * (double)v.im;
     }
