@@ -1807,8 +1807,9 @@ void reduce_final(RedBuf &rb, int64_t nblk, int K, int dst, int op, hipStream_t 
 // Chunk of node planes per block of the k-marching stencil.  A block re-loads
 // one plane at the start of its chunk, so long chunks are cheaper, but the grid
 // should fill the device in whole rounds: with `slots` blocks resident at once,
-// the time goes as rounds * (kc + 1).  kc is taken in [4, 16] minimising that
-// (the longer chunk on ties); slots <= 0 falls back to ~1000 blocks.
+// the time goes as rounds * (kc + 1).  kc is taken in [1, 16] minimising that
+// (the longer chunk on ties), so grids too small to fill the device get short
+// chunks and more blocks; slots <= 0 falls back to ~1000 blocks.
 int spmv_chunk(const Dims &d, int ty, int64_t slots)
 {
     const int64_t ntile = (int64_t)((d.nx + 31) / 32) * ((d.ny + ty - 1) / ty);
@@ -1820,7 +1821,7 @@ int spmv_chunk(const Dims &d, int ty, int64_t slots)
     }
     int best = 16;
     int64_t best_cost = INT64_MAX;
-    for (int kc = 16; kc >= 4; kc--) {
+    for (int kc = 16; kc >= 1; kc--) {
         const int64_t nblk = ntile * ((np + kc - 1) / kc);
         const int64_t cost = ((nblk + slots - 1) / slots) * (kc + 1);
         if (cost < best_cost) { best_cost = cost; best = kc; }
