@@ -15,8 +15,8 @@ by a two-step Jacobi polynomial, residuals in split edge + nodal form):
   * mixed: complex128 COCR, operator and residual; the V-cycle in complex64
     (the complex128 true residual is recomputed after every 1e-5 drop and on
     stagnation, and once more before convergence is accepted).
-The roofline object is for the stencil SpMV (k_spmv_march), timed with HIP events on
-the library's stream.  The CPU baseline is the FP64 oracle (CSR, lexicographic
+The roofline object is for the stencil SpMV (k_spmv_cell, the complex128 operator
+every solver applies), timed with HIP events on the library's stream.  The CPU baseline is the FP64 oracle (CSR, lexicographic
 SSOR, COCR) on one core, timed on a bounded number of fine-level iterations and
 extrapolated to the FP64 GPU iteration count; beside it, the oracle's measured
 time to solution on the reference's CPU case (32x32x32) against the GPU's.
@@ -391,8 +391,9 @@ def main():
         ms64, ms32 = kernel_times(ctx, m, reps=20)
         print(json.dumps({"spmv_ms": {"fp64": ms64[0], "fp32": ms32[0]},
                           "spmv_gather_ms": {"fp64": ms64[6], "fp32": ms32[6]},
-                          "spmv_march_ms_by_tile": {v: [ms64[7 + u], ms32[7 + u]] for u, v in
-                                                    enumerate(("32x8pf", "32x8", "32x4pf", "32x4"))},
+                          "spmv_ms_by_variant": {v: [ms64[7 + u], ms32[7 + u]] for u, v in
+                                                 enumerate(("default", "32x8", "32x4pf", "32x4"))},
+                          "spmv_march_32x8pf_ms": [ms64[12], ms32[12]],
                           "spmv_march_32x8pf_kc4_ms": [ms64[11], ms32[11]],
                           "copy_ms": {"fp64": ms64[4], "fp32": ms32[4]},
                           "copy_bytes": {"fp64": 2 * nl * 16, "fp32": 2 * nl * 8},
@@ -442,10 +443,11 @@ def main():
         "headline_converged": bool(converged),
         "speedup_mixed_vs_fp64": round(f64["fine"] / mx["fine"], 3) if mx["fine"] > 0 else None,
         "max_rel_impedance_err_mixed_vs_fp64": zerr,
-        "roofline": {"kernel": "k_spmv_march<double,0,8,true> (complex128 k-marching stencil, 32x8 tile)", "bound": "hbm",
+        "roofline": {"kernel": "k_spmv_cell<0> (complex128 stencil: cell-plane march over 32x8 node columns, "
+                               "planes loaded straight into LDS)", "bound": "hbm",
                      "achieved": round(ach64, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),
-                     "traffic": None if args.small else profiled_traffic("k_spmv_march<double,0>"),
+                     "traffic": None if args.small else profiled_traffic("k_spmv_cell<0>"),
                      "avg_ms": round(ms64[0], 5), "algorithmic_bytes": b64,
                      "complex64": {"kernel": "k_spmv_march<float,0,8,true>", "achieved": round(ach32, 1),
                                    "traffic": None if args.small else profiled_traffic("k_spmv_march<float,0>"),

@@ -112,12 +112,15 @@ typedef struct {
                              cross-polar boundary components 0; fdfe_main.f90:938-942,
                              470-494); 1: full tensor of that column, with the cross-polar
                              components of the 1-D field on the boundary (Ez stays 0) */
-    int spmv_tile;      /* k-marching SpMV tile of the Krylov levels: 0 (default) 32x8 node
-                           columns with prefetch, 1 32x8 without, 2 32x4 with, 3 32x4 without;
-                           -1 autotune: time the four once per grid shape and precision on
-                           the device and keep the fastest (cached in the context).  The
-                           tile sets the grouping of the fused reductions, so a fixed tile
-                           keeps repeated runs bit-identical */
+    int spmv_tile;      /* SpMV kernel of the Krylov levels: 0 (default) in complex128 the
+                           cell-plane march (32x8 node columns, planes loaded straight into
+                           LDS), in complex64 the k-marching 32x8 tile with prefetch; then
+                           the k-marching tiles 1 32x8 without prefetch, 2 32x4 with, 3 32x4
+                           without, 4 32x8 with prefetch; -1 autotune: time the five once per
+                           grid shape and precision on the device and keep the fastest
+                           (cached in the context).  The kernel sets the grouping of the
+                           fused reductions, so a fixed choice keeps repeated runs
+                           bit-identical */
     int ssor_cross_mass;  /* edge SSOR inside the hybrid preconditioner and the multigrid
                              smoother: 0 (default) K plus the lumped (diagonal) mass -- the
                              cross-orientation mass couplings of anisotropic cells are left out,
@@ -336,9 +339,12 @@ int sfdx_extint_v(sfdx_ctx *ctx, const sfdx_grid *g, const double *u0, const dou
  * bandwidth-tuned streaming copy (several 16-byte loads in flight per thread,
  * non-temporal stores: the attainable streaming rate), ms[4] plain copy of
  * one edge vector, ms[5] hybrid preconditioner apply, ms[6] SpMV in the
- * one-row-per-thread gather form, ms[7..10] the k-marching SpMV with tiles of
- * 32x8 columns with and without prefetch, then 32x4 with and without,
- * ms[11] the first of these over 4-plane chunks.  ms holds 12 doubles.
+ * one-row-per-thread gather form, ms[7..10] the SpMV variants 0-3 (0: the
+ * default, in complex128 the cell-plane march, in complex64 the k-marching
+ * tile of 32x8 columns with prefetch; 1: 32x8 without prefetch; 2, 3: 32x4
+ * with and without), ms[11] the k-marching 32x8 tile over 4-plane chunks,
+ * ms[12] the k-marching 32x8 tile with prefetch (variant 4).  ms holds 13
+ * doubles.
  * No counterpart in the reference (measurement only). */
 int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, int precision,
                        int reps, double *ms);

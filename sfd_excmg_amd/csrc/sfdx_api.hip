@@ -100,7 +100,7 @@ void ensure_partials(sfdx_ctx *ctx, int64_t nblk)
     }
 }
 
-// SpMV tile of a level from the option spmv_tile: fixed (0..3), or (-1) timed
+// SpMV kernel of a level from the option spmv_tile: fixed (0..4), or (-1) timed
 // once per grid shape and precision by tune_spmv and cached in the context.
 // Needs the reduction buffer sized for the level (ensure_partials).
 template <typename R>
@@ -116,11 +116,11 @@ void choose_tile(sfdx_ctx *ctx, DevGrid<R> &g, const std::string &tag, int opt, 
     if (it == ctx->tiles.end()) {
         cx<R> *x = pool_vec<cx<R>>(ctx, tag + "/tune_x", d.NL), *y = pool_vec<cx<R>>(ctx, tag + "/tune_y", d.NL);
         SFDX_CHECK(hipMemsetAsync(x, 0, d.NL * sizeof(cx<R>), ctx->st));
-        float ms[4];
+        float ms[5];
         const int v = tune_spmv<R>(g, (R)1, x, y, ctx->rb, ctx->st, 5, ms);
         if (verbose)
-            fprintf(stderr, "  spmv tile %dx%dx%d %s: %.4f %.4f %.4f %.4f ms -> variant %d\n", d.nx, d.ny, d.nz,
-                    sizeof(R) == 8 ? "c128" : "c64", ms[0], ms[1], ms[2], ms[3], v);
+            fprintf(stderr, "  spmv kernel %dx%dx%d %s: %.4f %.4f %.4f %.4f %.4f ms -> variant %d\n", d.nx, d.ny,
+                    d.nz, sizeof(R) == 8 ? "c128" : "c64", ms[0], ms[1], ms[2], ms[3], ms[4], v);
         it = ctx->tiles.emplace(key, v).first;
     }
     g.spmv_variant = it->second;
@@ -173,6 +173,21 @@ LevelDev<R> upload_level(sfdx_ctx *ctx, const HostLevel &L, const std::string &t
     SFDX_CHECK(hipMemcpyAsync(D.g.b, b.data(), b.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
     SFDX_CHECK(hipMemcpyAsync(D.g.c, c.data(), c.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
     SFDX_CHECK(hipMemcpyAsync(D.g.sig, s.data(), s.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
+    // complex128: sigma also as (xx,yy), (zz,xy), (xz,yz) pieces per cell for the
+    // cell-plane march (kept alive until the stream synchronisation below)
+    std::vector<double> sp;
+    if constexpr (sizeof(R) == 8) {
+        const size_t nc = L.ncell();
+        sp.resize(6 * nc);
+        for (size_t u = 0; u < 3; u++)
+            for (size_t q = 0; q < nc; q++) {
+                sp[(u * nc + q) * 2] = (double)s[2 * u * nc + q];
+                sp[(u * nc + q) * 2 + 1] = (double)s[(2 * u + 1) * nc + q];
+            }
+        double *dp = pool_vec<double>(ctx, p + "sp", sp.size());
+        SFDX_CHECK(hipMemcpyAsync(dp, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+        D.g.sigp = dp;
+    }
     // reciprocal widths (computed in FP64, stored in R) for the stencil rows
     std::vector<R> ra(L.a.size()), rb(L.b.size()), rc(L.c.size());
     for (size_t q = 0; q < ra.size(); q++) ra[q] = (R)(1.0 / L.a[q]);
@@ -1193,8 +1208,8 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         g_err = "solver not supported on the device path";
         return SFDX_ERR_UNSUPPORTED;
     }
-    if (o->spmv_tile < -1 || o->spmv_tile > 3) {
-        g_err = "spmv_tile must be -1 (autotune) or 0..3";
+    if (o->spmv_tile < -1 || o->spmv_tile > 4) {
+        g_err = "spmv_tile must be -1 (autotune) or 0..4";
         return SFDX_ERR_ARG;
     }
     // level hierarchy: lev[i] for i = 1..Lv (Lv = fine)
@@ -2190,16 +2205,20 @@ int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, d
             cx<R> *ph = pool_vec<cx<R>>(ctx, std::string("bench/ph") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
             // (with the nodal Jacobi polynomial, the solver's default)
             ms[5] = timed([&](int q) { launch_hybrid<R>(D.g, wmu, D.P, V(2 * q), V(2 * q + 1), tn, ph, (R)1, ctx->st, 1); });
-            // the one-row-per-thread gather form of the SpMV, and the k-marching
-            // form in its four tile variants (default chunk) and the 32x8 tile
-            // over 4-plane chunks
+            // the one-row-per-thread gather form of the SpMV, the default kernel
+            // (variant 0), the k-marching tile variants 1-3 (default chunk) and
+            // the 32x8 k-marching tile over 4-plane chunks
             ms[6] = timed([&](int q) { launch_spmv_gather<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, 0, nullptr, ctx->rb, ctx->st); });
             for (int u = 0; u < 4; u++)
                 ms[7 + u] = timed([&](int q) {
                     launch_spmv_march<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, 0, nullptr, ctx->rb, ctx->st, 0, u);
                 });
             ms[11] = timed([&](int q) {
-                launch_spmv_march<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, 0, nullptr, ctx->rb, ctx->st, 4, 0);
+                launch_spmv_march<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, 0, nullptr, ctx->rb, ctx->st, 4, 4);
+            });
+            // the k-marching 32x8 tile with prefetch (in complex128 the default is the cell-plane march)
+            ms[12] = timed([&](int q) {
+                launch_spmv_march<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, 0, nullptr, ctx->rb, ctx->st, 0, 4);
             });
         };
         if (precision == SFDX_PREC_FP64) run(0.0); else run(0.0f);

@@ -60,7 +60,10 @@ struct DevGrid {
     R *a = nullptr, *b = nullptr, *c = nullptr;     // widths (device)
     R *ra = nullptr, *rb = nullptr, *rc = nullptr;  // reciprocal widths (device)
     R *sig = nullptr;                               // 6*NC (device)
-    int spmv_variant = 0;                           // k-marching SpMV tile (launch_spmv_march)
+    // complex128 levels: sigma as three 16-byte pieces per cell, (xx,yy),
+    // (zz,xy), (xz,yz), each a plane of NC pieces (the cell-plane march reads it)
+    const double *sigp = nullptr;
+    int spmv_variant = 0;                           // SpMV kernel variant (launch_spmv_march)
     bool ssor_xmass = true;                         // launch_ssor: with the cross-orientation mass terms
 };
 

@@ -326,12 +326,12 @@ __device__ __forceinline__ cx<R> km_combine(cx<R> K, cx<R> M, R wmu, R alpha)
     return cx<R>(alpha * (K.re + wmu * M.im), alpha * (K.im - wmu * M.re));
 }
 
+// the value of row e that memit stores, with the fused dot partial sums
 template <typename R, int DOT>
-__device__ __forceinline__ void memit(cx<R> *__restrict__ y, const cx<R> *__restrict__ w, int64_t e, cx<R> out,
-                                      bool interior, cx<R> xe, double (&acc)[4])
+__device__ __forceinline__ cx<R> mvalue(const cx<R> *__restrict__ w, int64_t e, cx<R> out, bool interior, cx<R> xe,
+                                        double (&acc)[4])
 {
     if ((DOT == 4 || DOT == 5) && interior) out = w[e] + out;
-    y[e] = out;
     if (DOT == 1) {
         acc[0] += (double)xe.re * out.re - (double)xe.im * out.im;
         acc[1] += (double)xe.re * out.im + (double)xe.im * out.re;
@@ -346,6 +346,14 @@ __device__ __forceinline__ void memit(cx<R> *__restrict__ y, const cx<R> *__rest
             acc[2] += (double)out.re * out.re + (double)out.im * out.im;
         }
     }
+    return out;
+}
+
+template <typename R, int DOT>
+__device__ __forceinline__ void memit(cx<R> *__restrict__ y, const cx<R> *__restrict__ w, int64_t e, cx<R> out,
+                                      bool interior, cx<R> xe, double (&acc)[4])
+{
+    y[e] = mvalue<R, DOT>(w, e, out, interior, xe, acc);
 }
 
 // one footprint column of the k-marching stencil: where it lives, which of
@@ -687,6 +695,284 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
         }
     }
     if (DOT >= 1 && DOT <= 3) block_reduce_store<4, G::NT>(acc, partials);
+}
+
+// ------------------------------------------------------------------ K1b
+// Cell-plane march of the same complex128 operator.  A block owns a 32x8 tile
+// of node columns and walks up the cell planes c of its chunk of node planes.
+// Cell plane c lies between node planes c and c+1: it couples the Ex/Ey edges
+// of both planes and the Ez edges between them.  Step c completes the rows of
+// node plane c (their part from cell plane c-1 was formed at step c-1 and is
+// carried in registers), forms the Ez rows of plane c and the part of the
+// Ex/Ey rows of plane c+1 that comes from cell plane c.  It reads Ex/Ey of
+// planes c and c+1, Ez of plane c and the cell tensors of plane c only, so the
+// LDS rings hold 3 planes of Ex/Ey (one in flight) and 2 of Ez and sigma.
+// Buffer loads write the planes straight into LDS one step ahead of their use
+// (no registers stage them, one barrier per step); lanes outside the grid load
+// zeros (offset beyond the buffer).  sigma comes from a pair layout of three
+// 16-byte pieces per cell, (xx,yy), (zz,xy), (xz,yz), so that every load moves
+// 16 bytes per lane.  LDS: 79.9 KB, two blocks per CU.
+namespace cm {
+constexpr int TX = 32, TY = 8, NT = TX * TY;
+constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY;  // edge footprint: i0-1..i0+TX, j0-1..j0+TY
+constexpr int SX = TX + 1, SY = TY + 1, NS = SX * SY;  // cell footprint: i0-1..i0+TX-1, j0-1..j0+TY-1
+constexpr int NPP = 384, NSP = 320;  // footprints padded to whole wavefronts of loads (a wave fills 64 slots)
+static_assert(NP <= NPP && NPP % 64 == 0 && NS <= NSP && NSP % 64 == 0, "footprint padding");
+static_assert(NPP <= 2 * NT && NSP <= 2 * NT, "two load rounds per footprint");
+// LDS in 16-byte units: Ex ring [3][NPP], Ey ring [3][NPP], Ez ring [2][NPP], sigma ring [2][3][NSP]
+constexpr int EXo = 0, EYo = 3 * NPP, EZo = 6 * NPP, SGo = 8 * NPP, N16 = 8 * NPP + 6 * NSP;
+constexpr unsigned OOB = 0x80000000u;  // a byte offset past every buffer: the load returns zeros
+}  // namespace cm
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// 64 lanes x 16 bytes from a raw buffer into LDS at `dst` (wave-uniform; lane l
+// writes dst + 16 l); offset = voff + soff bytes
+__device__ __forceinline__ void lds_load16(__amdgpu_buffer_rsrc_t rs, void *dst, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void_t *)dst, 16, voff, soff, 0, 0);
+}
+
+template <int DOT>
+__global__ __launch_bounds__(cm::NT) void k_spmv_cell(
+    Dims d, const double *__restrict__ a, const double *__restrict__ b, const double *__restrict__ c,
+    const double *__restrict__ ra, const double *__restrict__ rb, const double *__restrict__ rc,
+    const double *__restrict__ sigp, double wmu, const zd *__restrict__ x, zd *__restrict__ y, double alpha,
+    const zd *__restrict__ w, double *__restrict__ partials, int ntx, int kc)
+{
+    using namespace cm;
+    __shared__ __align__(16) double lds[2 * N16];
+    zd *const EX = reinterpret_cast<zd *>(lds) + EXo;
+    zd *const EY = reinterpret_cast<zd *>(lds) + EYo;
+    zd *const EZ = reinterpret_cast<zd *>(lds) + EZo;
+    double2 *const SG = reinterpret_cast<double2 *>(lds) + SGo;
+    const Tile TL = tile_id();
+    const int tj = TL.x / ntx, ti = TL.x - tj * ntx;
+    const int i0 = ti * TX, j0 = tj * TY;
+    const int kb = TL.y * kc, ke = min(kb + kc, d.nz + 1);
+    const int t = threadIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const unsigned XP16 = (unsigned)((d.ny + 1) * d.nx) * 16u, YP16 = (unsigned)(d.ny * (d.nx + 1)) * 16u;
+    const unsigned ZP16 = (unsigned)((d.ny + 1) * (d.nx + 1)) * 16u, CP16 = (unsigned)(d.nx * d.ny) * 16u;
+    const unsigned NC16 = (unsigned)d.NC * 16u;
+    const __amdgpu_buffer_rsrc_t rsx = __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, (int)(d.NL * 16), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rss = __builtin_amdgcn_make_buffer_rsrc((void *)sigp, (short)0, (int)(3 * NC16), 0x00020000);
+    // loader role: footprint slots t and NT + t (the second only where a wave's 64 slots start inside the footprint)
+    unsigned ox[2], oy[2], oz[2], os[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int f = q * NT + t;
+        {
+            const int lj = f / PX, li = f - lj * PX, gi = i0 - 1 + li, gj = j0 - 1 + lj;
+            const bool on = f < NP && gi >= 0 && gi <= d.nx && gj >= 0 && gj <= d.ny;
+            ox[q] = on && gi < d.nx ? (unsigned)(gj * d.nx + gi) * 16u : OOB;
+            oy[q] = on && gj < d.ny ? (unsigned)(d.XN + gj * (d.nx + 1) + gi) * 16u : OOB;
+            oz[q] = on ? (unsigned)(d.XN + d.YN + gj * (d.nx + 1) + gi) * 16u : OOB;
+        }
+        {
+            const int lj = f / SX, li = f - lj * SX, gi = i0 - 1 + li, gj = j0 - 1 + lj;
+            const bool on = f < NS && gi >= 0 && gi < d.nx && gj >= 0 && gj < d.ny;
+            os[q] = on ? (unsigned)(gj * d.nx + gi) * 16u : OOB;
+        }
+    }
+    const bool e2 = NT + 64 * wv < NP, s2 = NT + 64 * wv < NS;  // wave-uniform
+    // Ex, Ey of node plane p into ring slot p mod 3
+    auto load_xy = [&](int p) {
+        zd *ex = EX + (p % 3) * NPP + 64 * wv, *ey = EY + (p % 3) * NPP + 64 * wv;
+        lds_load16(rsx, ex, ox[0], (unsigned)p * XP16);
+        lds_load16(rsx, ey, oy[0], (unsigned)p * YP16);
+        if (e2) {
+            lds_load16(rsx, ex + NT, ox[1], (unsigned)p * XP16);
+            lds_load16(rsx, ey + NT, oy[1], (unsigned)p * YP16);
+        }
+    };
+    // Ez of node plane p and the cell tensors of cell plane p into slot p mod 2
+    auto load_zs = [&](int p) {
+        zd *ez = EZ + (p & 1) * NPP + 64 * wv;
+        double2 *sg = SG + (p & 1) * 3 * NSP + 64 * wv;
+        lds_load16(rsx, ez, oz[0], (unsigned)p * ZP16);
+        if (e2) lds_load16(rsx, ez + NT, oz[1], (unsigned)p * ZP16);
+#pragma unroll
+        for (int u = 0; u < 3; u++) {
+            lds_load16(rss, sg + u * NSP, os[0], (unsigned)p * CP16 + u * NC16);
+            if (s2) lds_load16(rss, sg + u * NSP + NT, os[1], (unsigned)p * CP16 + u * NC16);
+        }
+    };
+    // compute role: tile column (i, j)
+    const int cy = t / TX, cxl = t - cy * TX;
+    const int i = i0 + cxl, j = j0 + cy;
+    const bool own = i < d.nx && j < d.ny;
+    const int fo = (cy + 1) * PX + cxl + 1, so = (cy + 1) * SX + cxl + 1;
+    const int64_t XP = (int64_t)(d.ny + 1) * d.nx, YP = (int64_t)d.ny * (d.nx + 1), ZP = (int64_t)(d.ny + 1) * (d.nx + 1);
+    const int64_t ex0 = (int64_t)j * d.nx + i, ey0 = d.XN + (int64_t)j * (d.nx + 1) + i;
+    const int64_t ez0 = d.XN + d.YN + (int64_t)j * (d.nx + 1) + i;
+    double ai = 0, am = 0, bj = 0, bm = 0, rai = 0, ram = 0, rbj = 0, rbm = 0;
+    if (own) {
+        ai = a[i]; rai = ra[i]; bj = b[j]; rbj = rb[j];
+        if (i >= 1) { am = a[i - 1]; ram = ra[i - 1]; }
+        if (j >= 1) { bm = b[j - 1]; rbm = rb[j - 1]; }
+    }
+    double acc[4] = {0, 0, 0, 0};
+    if (kb < ke) {
+        // zero the rings (the padding slots and any slot a load does not fill stay 0)
+        for (int u = t; u < N16; u += NT) reinterpret_cast<zd *>(lds)[u] = zd(0);
+        __syncthreads();
+        const int cs = kb > 0 ? kb - 1 : 0;  // first cell plane: the one under the chunk's first node plane
+        load_xy(cs);
+        if (cs + 1 <= d.nz) load_xy(cs + 1);
+        if (cs < d.nz) load_zs(cs);
+        const double hE = 0.5, q4 = 0.25, q8 = 0.125;
+        zd pEx(0), pEy(0);  // part of the Ex, Ey rows of node plane c from cell plane c-1
+        // The rows of a step are stored at the start of the next one, after its
+        // loads are issued: the wait at the top of a step (for the loads) then
+        // does not also wait for the stores issued just before it.
+        zd sEx(0), sEy(0), sEz(0);
+        bool pX = false, pY = false, pZ = false;
+        int pc = 0;
+        auto flush = [&]() {
+            if (pX) y[ex0 + (int64_t)pc * XP] = sEx;
+            if (pY) y[ey0 + (int64_t)pc * YP] = sEy;
+            if (pZ) y[ez0 + (int64_t)pc * ZP] = sEz;
+            pX = pY = pZ = false;
+        };
+        for (int cp = cs; cp < ke; cp++) {
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of step cp are in LDS
+            __syncthreads();                // every wave's; the reads of step cp-1 are done
+            if (cp + 1 < ke) {
+                if (cp + 2 <= d.nz) load_xy(cp + 2);
+                if (cp + 1 < d.nz) load_zs(cp + 1);
+            }
+            flush();
+            if (!own) continue;
+            pc = cp;
+            const zd *X0 = EX + (cp % 3) * NPP, *X1 = EX + ((cp + 1) % 3) * NPP;
+            const zd *Y0 = EY + (cp % 3) * NPP, *Y1 = EY + ((cp + 1) % 3) * NPP;
+            const zd *Z = EZ + (cp & 1) * NPP;
+            const double2 *S0 = SG + (cp & 1) * 3 * NSP, *S1 = S0 + NSP, *S2 = S1 + NSP;
+            const bool row = cp >= kb;       // the rows of plane cp are this block's
+            const bool nxt = cp + 1 < ke;    // so are those of plane cp + 1
+            const bool cell = cp < d.nz;     // cell plane cp exists
+            const bool inc = cp >= 1 && cp <= d.nz - 1, inn = cp + 1 <= d.nz - 1;  // interior in k
+            const double cc = cell ? c[cp] : 0.0, rcc = cell ? rc[cp] : 0.0;
+            // ---- Ex rows: plane cp (above part, completes the row) and plane cp+1 (below part)
+            {
+                const bool up = row && inc && j >= 1, dn = nxt && inn && j >= 1;
+                zd out = pEx;
+                if (cell && (up || dn)) {
+                    const double2 p1 = S0[so - SX], p2 = S0[so], q1 = S1[so - SX], q2 = S1[so], r1 = S2[so - SX], r2 = S2[so];
+                    const double acc_ = ai * cc;
+                    const double mxx = acc_ * (bm * p1.x + bj * p2.x) * q4;   // xx of cells (i,j-1), (i,j)
+                    const double my1 = acc_ * bm * q1.y * q8, my2 = acc_ * bj * q2.y * q8;  // xy
+                    const double mxz = acc_ * (bm * r1.x + bj * r2.x) * q8;  // xz
+                    const double wk = hE * ai * (bm + bj) * rcc, hm = hE * acc_ * rbm, hp = hE * acc_ * rbj;
+                    const double hc = hE * cc, hz = hE * (bm + bj);
+                    const zd z0 = Z[fo], z1 = Z[fo + 1];
+                    const zd mz = mxz * (z0 + z1);
+                    if (up) {
+                        const zd e = X0[fo], en = X1[fo], em = X0[fo - PX], ep = X0[fo + PX];
+                        const zd y00 = Y0[fo - PX], y10 = Y0[fo - PX + 1], yk = Y0[fo], y11 = Y0[fo + 1];
+                        const zd K = wk * (e - en) + hm * (e - em) + hp * (e - ep) + hc * (y00 - y10 - yk + y11)
+                                     + hz * (z1 - z0);
+                        const zd M = mxx * e + my1 * (y00 + y10) + my2 * (yk + y11) + mz;
+                        out = out + km_combine(K, M, wmu, alpha);
+                    }
+                    if (dn) {
+                        const zd e = X1[fo], eb = X0[fo], em = X1[fo - PX], ep = X1[fo + PX];
+                        const zd y00 = Y1[fo - PX], y10 = Y1[fo - PX + 1], yk = Y1[fo], y11 = Y1[fo + 1];
+                        const zd K = wk * (e - eb) + hm * (e - em) + hp * (e - ep) + hc * (y00 - y10 - yk + y11)
+                                     + hz * (z0 - z1);
+                        const zd M = mxx * e + my1 * (y00 + y10) + my2 * (yk + y11) + mz;
+                        pEx = km_combine(K, M, wmu, alpha);
+                    }
+                }
+                if (row) {
+                    sEx = mvalue<double, DOT>(w, ex0 + (int64_t)cp * XP, up ? out : zd(0), up, X0[fo], acc);
+                    pX = true;
+                }
+            }
+            // ---- Ey rows
+            {
+                const bool up = row && inc && i >= 1, dn = nxt && inn && i >= 1;
+                zd out = pEy;
+                if (cell && (up || dn)) {
+                    const double2 p1 = S0[so - 1], p2 = S0[so], q1 = S1[so - 1], q2 = S1[so], r1 = S2[so - 1], r2 = S2[so];
+                    const double bcc = bj * cc;
+                    const double myy = bcc * (am * p1.y + ai * p2.y) * q4;  // yy of cells (i-1,j), (i,j)
+                    const double mx1 = bcc * am * q1.y * q8, mx2 = bcc * ai * q2.y * q8;  // xy
+                    const double myz = bcc * (am * r1.y + ai * r2.y) * q8;  // yz
+                    const double wk = hE * bj * (am + ai) * rcc, hm = hE * bcc * ram, hp = hE * bcc * rai;
+                    const double hc = hE * cc, hz = hE * (am + ai);
+                    const zd z0 = Z[fo], z3 = Z[fo + PX];
+                    const zd mz = myz * (z0 + z3);
+                    if (up) {
+                        const zd e = Y0[fo], en = Y1[fo], em = Y0[fo - 1], ep = Y0[fo + 1];
+                        const zd x0 = X0[fo - 1], xk = X0[fo], x2 = X0[fo + PX - 1], x3 = X0[fo + PX];
+                        const zd K = wk * (e - en) + hm * (e - em) + hp * (e - ep) + hc * (x0 - xk - x2 + x3)
+                                     + hz * (z3 - z0);
+                        const zd M = myy * e + mx1 * (x0 + x2) + mx2 * (xk + x3) + mz;
+                        out = out + km_combine(K, M, wmu, alpha);
+                    }
+                    if (dn) {
+                        const zd e = Y1[fo], eb = Y0[fo], em = Y1[fo - 1], ep = Y1[fo + 1];
+                        const zd x0 = X1[fo - 1], xk = X1[fo], x2 = X1[fo + PX - 1], x3 = X1[fo + PX];
+                        const zd K = wk * (e - eb) + hm * (e - em) + hp * (e - ep) + hc * (x0 - xk - x2 + x3)
+                                     + hz * (z0 - z3);
+                        const zd M = myy * e + mx1 * (x0 + x2) + mx2 * (xk + x3) + mz;
+                        pEy = km_combine(K, M, wmu, alpha);
+                    }
+                }
+                if (row) {
+                    sEy = mvalue<double, DOT>(w, ey0 + (int64_t)cp * YP, up ? out : zd(0), up, Y0[fo], acc);
+                    pY = true;
+                }
+            }
+            // ---- Ez row of plane cp (cell plane cp only)
+            if (row && cell) {
+                zd out(0);
+                const bool in = i >= 1 && j >= 1;
+                if (in) {
+                    const double ha = hE * cc * (am + ai), hb = hE * cc * (bm + bj);
+                    const double wjm = ha * rbm, wjp = ha * rbj, wim = hb * ram, wip = hb * rai;
+                    const double hx = hE * (bm + bj), hy = hE * (am + ai);
+                    const int o00 = so - SX - 1, o01 = so - 1, o10 = so - SX;
+                    const double2 u00 = S1[o00], u01 = S1[o01], u10 = S1[o10], u11 = S1[so];  // .x = zz
+                    const double2 v00 = S2[o00], v01 = S2[o01], v10 = S2[o10], v11 = S2[so];  // xz, yz
+                    const double kmd = cc * (am * (bm * u00.x + bj * u01.x) + ai * (bm * u10.x + bj * u11.x)) * q4;
+                    const double mx1 = cc * am * (bm * v00.x + bj * v01.x) * q8;
+                    const double mx2 = cc * ai * (bm * v10.x + bj * v11.x) * q8;
+                    const double my1 = cc * bm * (am * v00.y + ai * v10.y) * q8;
+                    const double my2 = cc * bj * (am * v01.y + ai * v11.y) * q8;
+                    const zd ez = Z[fo];
+                    const zd exm = X0[fo - 1], exk = X0[fo], exmN = X1[fo - 1], exN = X1[fo];
+                    const zd eym = Y0[fo - PX], eyk = Y0[fo], eymN = Y1[fo - PX], eyN = Y1[fo];
+                    const zd K = (wjm + wjp + wim + wip) * ez - wjm * Z[fo - PX] - wjp * Z[fo + PX] - wim * Z[fo - 1]
+                                 - wip * Z[fo + 1] + hx * (exm - exk - exmN + exN) + hy * (eym - eyk - eymN + eyN);
+                    const zd M = kmd * ez + mx1 * (exm + exmN) + mx2 * (exk + exN) + my1 * (eym + eymN)
+                                 + my2 * (eyk + eyN);
+                    out = km_combine(K, M, wmu, alpha);
+                }
+                sEz = mvalue<double, DOT>(w, ez0 + (int64_t)cp * ZP, out, in, Z[fo], acc);
+                pZ = true;
+            }
+            if (row) {
+                // boundary edges of the last column i = nx and the last row j = ny
+                const zd zero(0);
+                if (i == d.nx - 1) {
+                    y[ey0 + 1 + (int64_t)cp * YP] = zero;
+                    if (cell) y[ez0 + 1 + (int64_t)cp * ZP] = zero;
+                }
+                if (j == d.ny - 1) {
+                    y[ex0 + d.nx + (int64_t)cp * XP] = zero;
+                    if (cell) {
+                        y[ez0 + (d.nx + 1) + (int64_t)cp * ZP] = zero;
+                        if (i == d.nx - 1) y[ez0 + (d.nx + 2) + (int64_t)cp * ZP] = zero;
+                    }
+                }
+            }
+        }
+        flush();
+    }
+    if (DOT >= 1 && DOT <= 3) block_reduce_store<4, cm::NT>(acc, partials);
 }
 
 // w = M_sigma x on interior edges
@@ -1829,6 +2115,52 @@ int spmv_chunk(const Dims &d, int ty, int64_t slots)
     return best;
 }
 
+// The cell-plane march (complex128): its buffer loads take 32-bit byte
+// offsets, so the edge vector and the sigma pairs must stay below 2 GiB
+// (about 130 M edges); the modes are the plain product, its dots and the
+// residual form (0-4).
+static bool cell_march_fits(const DevGrid<double> &g, int dot_mode)
+{
+    return g.sigp && dot_mode >= 0 && dot_mode <= 4 && g.d.NL * 16 < ((int64_t)1 << 31) &&
+           g.d.NC * 48 < ((int64_t)1 << 31);
+}
+
+static int64_t cell_go(const DevGrid<double> &g, double wmu, const zd *x, zd *y, double alpha, int dot_mode,
+                       const zd *w, RedBuf &rb, hipStream_t st, int kc)
+{
+    const Dims &d = g.d;
+    if (kc <= 0) {
+        static const int64_t slots = []() -> int64_t {
+            int per_cu = 0, dev = 0, ncu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)&k_spmv_cell<0>, cm::NT, 0) !=
+                    hipSuccess ||
+                hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+                return 0;
+            return (int64_t)per_cu * ncu;
+        }();
+        kc = spmv_chunk(d, cm::TY, slots);
+    }
+    const int ntx = (d.nx + cm::TX - 1) / cm::TX, nty = (d.ny + cm::TY - 1) / cm::TY;
+    const unsigned gx = (unsigned)(ntx * nty), nch = (unsigned)((d.nz + kc) / kc);
+    const dim3 grid(gx, pad_y(gx, nch), 1), block(cm::NT, 1, 1);
+    const int64_t nblk = (int64_t)grid.x * grid.y;
+    if (dot_mode >= 1 && dot_mode <= 3 && nblk > rb.max_blocks)
+        throw std::runtime_error("reduction buffer smaller than the stencil grid");
+#define SFDX_CELL(M)                                                                                         \
+    hipLaunchKernelGGL((k_spmv_cell<M>), grid, block, 0, st, d, g.a, g.b, g.c, g.ra, g.rb, g.rc, g.sigp, wmu, x, \
+                       y, alpha, w, rb.partials, ntx, kc)
+    switch (dot_mode) {
+    case 0: SFDX_CELL(0); break;
+    case 1: SFDX_CELL(1); break;
+    case 2: SFDX_CELL(2); break;
+    case 3: SFDX_CELL(3); break;
+    default: SFDX_CELL(4); break;
+    }
+#undef SFDX_CELL
+    return nblk;
+}
+
 template <typename R, int TY, bool PF>
 static int64_t march_go(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode,
                         const cx<R> *w, RedBuf &rb, hipStream_t st, int kc, const cx<R> *psi = nullptr)
@@ -1878,6 +2210,10 @@ int64_t launch_spmv_march(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, 
                           const cx<R> *w, RedBuf &rb, hipStream_t st, int kc, int variant)
 {
     switch (variant) {
+    case 0:
+        if constexpr (sizeof(R) == 8)
+            if (cell_march_fits(g, dot_mode)) return cell_go(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
+        return march_go<R, 8, true>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
     case 1: return march_go<R, 8, false>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
     case 2: return march_go<R, 4, true>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
     case 3: return march_go<R, 4, false>(g, wmu, x, y, alpha, dot_mode, w, rb, st, kc);
@@ -1906,16 +2242,16 @@ template <typename R>
 int64_t launch_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R alpha, int dot_mode, const cx<R> *w,
                     RedBuf &rb, hipStream_t st)
 {
-    // the level's tile: 0 unless set by tune_spmv or the caller's option; the
-    // variant sweep of sfdx_bench_kernels on 128x128x96 found 32x8 columns with
-    // prefetch fastest (0.058 ms in complex128, 32x4: 0.059; 0.035 ms in
-    // complex64, 32x4: 0.037)
+    // the level's kernel: 0 unless set by tune_spmv or the caller's option; among
+    // the k-marching tiles the sweep of sfdx_bench_kernels on 128x128x96 found
+    // 32x8 columns with prefetch fastest (0.058 ms in complex128, 32x4: 0.059;
+    // 0.035 ms in complex64, 32x4: 0.037)
     return launch_spmv_march<R>(g, wmu, x, y, alpha, dot_mode, w, rb, st, 0, g.spmv_variant);
 }
 
-// Tile autotuner: time the four k-marching variants on this grid (as the
-// Krylov solvers launch them, with the fused dot, chunk from the occupancy
-// rule) and return the fastest.  One untimed launch per variant, then the
+// Kernel autotuner: time the five SpMV variants on this grid (as the Krylov
+// solvers launch them, with the fused dot, chunk from the occupancy rule) and
+// return the fastest.  One untimed launch per variant, then the
 // median of `reps` timed ones (HIP events on st).  Host-synchronising: call
 // it at set-up, never inside a stream capture.  x must hold NL finite values,
 // y is overwritten.
@@ -1929,7 +2265,7 @@ int tune_spmv(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, RedBuf &rb, 
     int best = 0;
     float best_ms = 0;
     std::vector<float> t(reps > 0 ? reps : 1);
-    for (int v = 0; v < 4; v++) {
+    for (int v = 0; v < 5; v++) {
         launch_spmv_march<R>(g, wmu, x, y, (R)1, 1, nullptr, rb, st, 0, v);
         for (auto &ms : t) {
             SFDX_CHECK(hipEventRecord(e0, st));

@@ -378,20 +378,21 @@ def test_device_coarse_levels_match_direct_solves(ctx, precision):
 
 
 def test_spmv_tiles_and_autotuner_give_the_same_solve(ctx):
-    """Every k-marching SpMV tile (0..3) and the autotuner (-1) solve the same
-    system: impedances agree to 1e-9 (only the grouping of the fused
-    reductions differs between tiles); a fixed tile is reported as used, the
-    autotuned one is a valid tile, and a second autotuned solve on the same
-    context reuses the cached choice."""
+    """Every SpMV kernel variant (0: the cell-plane march in complex128, 1..4
+    the k-marching tiles) and the autotuner (-1) solve the same system:
+    impedances agree to 1e-9 (only the rounding order of the rows and the
+    grouping of the fused reductions differ); a fixed variant is reported as
+    used, the autotuned one is a valid variant, a second autotuned solve on
+    the same context reuses the cached choice, and variant 5 is rejected."""
     m = small_excmg_model(True)
     kw = dict(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR)
     ref, st0 = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(**kw))
     assert st0.status == 0 and st0.spmv_tile == 0
     zref = _impedances(m, ref)
-    for tile in (1, 2, 3, -1):
+    for tile in (1, 2, 3, 4, -1):
         ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(spmv_tile=tile, **kw))
         assert st.status == 0
-        assert st.spmv_tile == tile if tile >= 0 else 0 <= st.spmv_tile <= 3
+        assert st.spmv_tile == tile if tile >= 0 else 0 <= st.spmv_tile <= 4
         for p, q in zip(_impedances(m, ex), zref):
             assert np.max(np.abs(p - q) / np.abs(q)) < 1e-9, tile
     ex2, st2 = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(spmv_tile=-1, **kw))
@@ -399,7 +400,7 @@ def test_spmv_tiles_and_autotuner_give_the_same_solve(ctx):
     for p, q in zip(_impedances(m, ex2), zref):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-9
     with pytest.raises(sfdx.SfdxError):
-        ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(spmv_tile=4, **kw))
+        ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(spmv_tile=5, **kw))
 
 
 def test_complex64_krylov_option_reports_nonconvergence_with_air(ctx):
