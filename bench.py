@@ -2,7 +2,7 @@
 
 Workload (default): 128x128x96 cells (16 air + 80 earth layers), full
 anisotropic sigma in 16 seeded blocks over 100 ohm-m, 0.1 Hz, both
-polarisations, 4 EXCMG levels (levels 1-2 direct, 3-4 Krylov), COCR to a
+polarisations, 4 EXCMG levels (all solved by COCR on the device), COCR to a
 fine-level relative residual of 1e-8 (interior rows).
 
 A step is one full forward solve (all levels, both polarisations).  `value`

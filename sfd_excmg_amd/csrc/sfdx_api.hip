@@ -35,6 +35,8 @@ static thread_local std::string g_err;
 // stand-in for the symbol: its device address is looked up once per device
 // (noalloc_scratch_init, from sfdx_create) and only that address is used.
 static constexpr int kNoallocMaxDim = 4096;
+// tightest relative residual asked of the device solves of EXCMG levels 1-2
+static constexpr double kCoarseTolFloor = 1e-12;
 static __device__ __align__(256) double scratch[3 * kNoallocMaxDim];
 static constexpr int kMaxDevices = 64;
 static double *g_scratch_dev[kMaxDevices] = {};
@@ -217,6 +219,32 @@ void ensure_P(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag)
     if (D.P) return;
     D.P = pool_vec<R>(ctx, tag + (sizeof(R) == 8 ? "/P64" : "/P32"), (size_t)29 * D.g.d.NP);
     launch_pbuild<R>(D.g.d, D.a64, D.b64, D.c64, D.sig64, D.P, ctx->st);
+}
+
+// The grid of a 2^k coarsening (mesh_coarsen, src/fdfe_main.f90:18-84): widths
+// summed on the host, sigma averaged on the device from the fine level's
+// resident FP64 copy (k_mesh_coarsen, the same bits as the host coarsen) and
+// copied back for the host-side users (1-D background, direct solves).
+HostLevel coarsen_device(sfdx_ctx *ctx, const HostLevel &F, const LevelDev<double> &DF, int k, const std::string &tag)
+{
+    const int f = 1 << k;
+    HostLevel C;
+    C.nx = F.nx / f; C.ny = F.ny / f; C.nz = F.nz / f; C.nair = F.nair / f;
+    C.a.assign(C.nx, 0.0); C.b.assign(C.ny, 0.0); C.c.assign(C.nz, 0.0);
+    for (int i = 0; i < C.nx; i++) for (int t = 0; t < f; t++) C.a[i] += F.a[i * f + t];
+    for (int i = 0; i < C.ny; i++) for (int t = 0; t < f; t++) C.b[i] += F.b[i * f + t];
+    for (int i = 0; i < C.nz; i++) for (int t = 0; t < f; t++) C.c[i] += F.c[i * f + t];
+    const Dims df = make_dims(F.nx, F.ny, F.nz), dc = make_dims(C.nx, C.ny, C.nz);
+    double *ac = pool_vec<double>(ctx, tag + "/ca", C.nx), *bc = pool_vec<double>(ctx, tag + "/cb", C.ny);
+    double *cc = pool_vec<double>(ctx, tag + "/cc", C.nz), *sc = pool_vec<double>(ctx, tag + "/cs", 6 * dc.NC);
+    SFDX_CHECK(hipMemcpyAsync(ac, C.a.data(), C.nx * 8, hipMemcpyHostToDevice, ctx->st));
+    SFDX_CHECK(hipMemcpyAsync(bc, C.b.data(), C.ny * 8, hipMemcpyHostToDevice, ctx->st));
+    SFDX_CHECK(hipMemcpyAsync(cc, C.c.data(), C.nz * 8, hipMemcpyHostToDevice, ctx->st));
+    launch_mesh_coarsen(df, dc, f, DF.a64, DF.b64, DF.c64, ac, bc, cc, DF.sig64, sc, ctx->st);
+    C.sig.assign(6 * dc.NC, 0.0);
+    SFDX_CHECK(hipMemcpyAsync(C.sig.data(), sc, 6 * dc.NC * 8, hipMemcpyDeviceToHost, ctx->st));
+    SFDX_CHECK(hipStreamSynchronize(ctx->st));
+    return C;
 }
 
 // Helper context of the second polarisation (its own stream, reduction
@@ -1212,12 +1240,14 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         g_err = "spmv_tile must be -1 (autotune) or 0..4";
         return SFDX_ERR_ARG;
     }
-    // level hierarchy: lev[i] for i = 1..Lv (Lv = fine)
+    // level hierarchy: lev[i] for i = 1..Lv (Lv = fine); the fine level is
+    // uploaded first and the coarser ones are averaged from it on the device
     std::vector<HostLevel> lev(Lv + 1);
     lev[Lv] = fine;
-    for (int i = Lv - 1; i >= 1; i--) lev[i] = coarsen(fine, Lv - i);
     std::vector<LevelDev<double>> D64(Lv + 1);
     std::vector<LevelDev<float>> D32(Lv + 1);
+    D64[Lv] = upload_level<double>(ctx, fine, "L" + std::to_string(Lv));
+    for (int i = Lv - 1; i >= 1; i--) lev[i] = coarsen_device(ctx, fine, D64[Lv], Lv - i, "crs" + std::to_string(i));
     // levels solved by the Krylov method on the device: with coarse_solver = 1
     // also the two coarsest EXCMG levels (instead of the host direct solves)
     const bool coarse_dev = o->use_excmg && o->coarse_solver == 1;
@@ -1230,7 +1260,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     // grid, so with multigrid all levels get their complex64 data.
     const bool lo_fine = o->precision != SFDX_PREC_FP64;
     for (int i = first_up; i <= Lv; i++) {
-        D64[i] = upload_level<double>(ctx, lev[i], "L" + std::to_string(i));
+        if (i < Lv) D64[i] = upload_level<double>(ctx, lev[i], "L" + std::to_string(i));
         if (lo_fine && (i == Lv || use_mg))
             D32[i] = upload_level<float>(ctx, lev[i], "L" + std::to_string(i), o->sigma_floor32);
     }
@@ -1266,7 +1296,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 H64.lev[j] = &D64[j - off];
                 if (o->precision != SFDX_PREC_FP64) H32.lev[j] = &D32[j - off];
             } else {
-                mgh[j] = coarsen(fine, Lmg - j);
+                mgh[j] = coarsen_device(ctx, fine, D64[Lv], Lmg - j, "crsm" + std::to_string(j));
                 X64[j] = upload_level<double>(ctx, mgh[j], "M" + std::to_string(j));
                 H64.lev[j] = &X64[j];
                 if (o->precision != SFDX_PREC_FP64) {
@@ -1366,6 +1396,11 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         for (int i = first_dev; i <= Lv; i++) {
             tol *= 10;                                   // level tolerance eps*10^(i-L) (:1331)
             if (!o->use_excmg) tol = o->eps;
+            // The two coarsest levels replace direct solves, which are exact; the
+            // Krylov solves stop at 1e-12 at the most, below which their FP64
+            // recurrences stall (a 5-level solve to eps 1e-10 asks 1e-14 of
+            // level 1), and the EXCMG guess gains nothing from more digits.
+            const double tol_i = (coarse_dev && i <= 2) ? std::max(tol, kCoarseTolFloor) : tol;
             const std::vector<zc> edc_h = background_field(lev[i], ff, o->full_background != 0);
             const Dims d = make_dims(lev[i].nx, lev[i].ny, lev[i].nz);
             zd *edc = pool_vec<zd>(ctx, "edc" + std::to_string(i), edc_h.size());
@@ -1391,7 +1426,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
             if (two) prebuild_level(ctx, D64[i], d32, h64, h32, tag, lo_fine && i == Lv);
             SFDX_CHECK(hipStreamSynchronize(ctx->st));
             auto solve_pol = [&](sfdx_ctx *c, int pol) {
-                LevelSolveResult R = solve_level(c, oi, D64[i], d32, tag, ff, edc, pol, xs[pol], tol, maxit, h64, h32,
+                LevelSolveResult R = solve_level(c, oi, D64[i], d32, tag, ff, edc, pol, xs[pol], tol_i, maxit, h64, h32,
                                                  i + off);
                 SFDX_CHECK(hipStreamSynchronize(c->st));
                 return R;
@@ -1644,7 +1679,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->restart_drop = 1e-6;
     o->use_graph = 1;
     o->nodal_smoother = 1;
-    o->coarse_solver = 0;
+    o->coarse_solver = 1;
     o->full_background = 0;
     o->spmv_tile = 0;
     o->surface_sigma = 0;
@@ -1891,6 +1926,29 @@ int sfdx_mesh_coarsen(const sfdx_grid *g, const double *sigma, int k, double *a_
     }
     return guarded([&]() {
         const HostLevel C = coarsen(host_level(g, sigma), k);
+        std::copy(C.a.begin(), C.a.end(), a_c);
+        std::copy(C.b.begin(), C.b.end(), b_c);
+        std::copy(C.c.begin(), C.c.end(), c_c);
+        std::copy(C.sig.begin(), C.sig.end(), sigma_c);
+        return (int)SFDX_OK;
+    });
+}
+
+int sfdx_mesh_coarsen_device(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int k, double *a_c,
+                             double *b_c, double *c_c, double *sigma_c)
+{
+    if (!ctx || !g || !g->a || !g->b || !g->c || !sigma || !a_c || !b_c || !c_c || !sigma_c || k < 0 || k > 16)
+        return SFDX_ERR_ARG;
+    const int f = 1 << k;
+    if (g->nx % f || g->ny % f || g->nz % f) {
+        g_err = "grid dimensions must be divisible by 2^k";
+        return SFDX_ERR_MESH;
+    }
+    return guarded([&]() {
+        SFDX_CHECK(hipSetDevice(ctx->device));
+        const HostLevel F = host_level(g, sigma);
+        const LevelDev<double> D = upload_level<double>(ctx, F, "mc");
+        const HostLevel C = coarsen_device(ctx, F, D, k, "mc");
         std::copy(C.a.begin(), C.a.end(), a_c);
         std::copy(C.b.begin(), C.b.end(), b_c);
         std::copy(C.c.begin(), C.c.end(), c_c);

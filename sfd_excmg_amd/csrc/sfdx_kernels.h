@@ -220,6 +220,11 @@ void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R
 template <typename R>
 void launch_cocr_upd_grad(const DevGrid<R> &g, RedBuf &rb, cx<R> *x, cx<R> *pv, const cx<R> *Ap, cx<R> *r, cx<R> *z,
                           const cx<R> *zs, const cx<R> *phi, R wmu, hipStream_t st);
+// sigma of the 2^k-coarsened grid (f = 2^k) from the fine sigma, all on the
+// device; ac, bc, cc are the coarse widths
+void launch_mesh_coarsen(const Dims &df, const Dims &dc, int f, const double *a, const double *b, const double *c,
+                         const double *ac, const double *bc, const double *cc, const double *sig, double *sigc,
+                         hipStream_t st);
 void launch_psym32(int64_t np, const double *P, float *Ps32, hipStream_t st);
 void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *Ps32,
                           const zd *in, zd *out, zf *in32, zf *z32, zd *tnode, zd *phi, float omega,

@@ -2023,6 +2023,46 @@ __global__ __launch_bounds__(TPB) void k_prolong_node_add(Dims df, Dims dc, MGTa
     pf[n] = pf[n] + cx<R>((R)sre, (R)sim);
 }
 
+// ------------------------------------------------------------------ coarsening
+// sigma of a 2^k coarsening: each coarse cell is the volume-weighted mean of
+// its 8^k fine cells (mesh_coarsen, src/fdfe_main.f90:18-84, with the fine
+// index taken inside the block).  One coarse cell per thread; the products
+// and sums come in the order of the host restatement (z outer, x inner,
+// a*b*c, then the division by the coarse volume) and are not contracted into
+// fused multiply-adds, so both give the same bits.
+__global__ __launch_bounds__(TPB) void k_mesh_coarsen(Dims df, Dims dc, int f, const double *__restrict__ a,
+                                                      const double *__restrict__ b, const double *__restrict__ c,
+                                                      const double *__restrict__ ac, const double *__restrict__ bc,
+                                                      const double *__restrict__ cc, const double *__restrict__ sig,
+                                                      double *__restrict__ sigc)
+{
+#pragma clang fp contract(off)
+    const int64_t ic = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (ic >= dc.NC) return;
+    const int I = (int)(ic % dc.nx), J = (int)((ic / dc.nx) % dc.ny), K = (int)(ic / ((int64_t)dc.nx * dc.ny));
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int r = 0; r < f; r++)
+        for (int s = 0; s < f; s++)
+            for (int t = 0; t < f; t++) {
+                const int fi = I * f + t, fj = J * f + s, fk = K * f + r;
+                const int64_t ie = ((int64_t)fk * df.ny + fj) * df.nx + fi;
+                const double v = a[fi] * b[fj] * c[fk];
+#pragma unroll
+                for (int q = 0; q < 6; q++) acc[q] += sig[q * df.NC + ie] * v;
+            }
+    const double V = ac[I] * bc[J] * cc[K];
+#pragma unroll
+    for (int q = 0; q < 6; q++) sigc[q * dc.NC + ic] = acc[q] / V;
+}
+
+void launch_mesh_coarsen(const Dims &df, const Dims &dc, int f, const double *a, const double *b, const double *c,
+                         const double *ac, const double *bc, const double *cc, const double *sig, double *sigc,
+                         hipStream_t st)
+{
+    const unsigned nb = (unsigned)((dc.NC + TPB - 1) / TPB);
+    hipLaunchKernelGGL(k_mesh_coarsen, dim3(nb), dim3(TPB), 0, st, df, dc, f, a, b, c, ac, bc, cc, sig, sigc);
+}
+
 // ------------------------------------------------------------------ K4 prolongation
 // Fine edge value = sum over the block's coarse edges of separable Lagrange
 // weights: along-edge table (m weights) x two transverse node tables (m+1).

@@ -83,7 +83,7 @@ EXPORTS = ["sfdx_create", "sfdx_destroy", "sfdx_last_error", "sfdx_default_opts"
            "sfdx_post_process", "sfdx_mt1d_anl", "sfdx_apply_operator", "sfdx_ssor_apply", "sfdx_hybrid_apply", "sfdx_div_correct",
            "sfdx_extint_v", "sfdx_bench_kernels", "sfdx_direct_solve", "sfdx_solve_batch", "sfdx_misfit",
            "sfdx_adjoint", "sfdx_fwd_response", "sfdx_post_process_topo", "sfdx_surface_planes",
-           "sfdx_mesh_coarsen", "sfdx_apply_operator_noalloc", "sfdx_split_residual"]
+           "sfdx_mesh_coarsen", "sfdx_apply_operator_noalloc", "sfdx_split_residual", "sfdx_mesh_coarsen_device"]
 
 _lib = None
 
@@ -106,6 +106,7 @@ def load(path: str = LIB_PATH):
     L.sfdx_post_process_topo.argtypes = [ctypes.POINTER(Grid), P, I, P, I, P, P, I, I, P, P]
     L.sfdx_surface_planes.argtypes = [ctypes.POINTER(Grid), P, D, P]
     L.sfdx_mesh_coarsen.argtypes = [ctypes.POINTER(Grid), P, I, P, P, P, P]
+    L.sfdx_mesh_coarsen_device.argtypes = [P, ctypes.POINTER(Grid), P, I, P, P, P, P]
     L.sfdx_mt1d_anl.argtypes = [I, P, P, D, P, P, P]
     L.sfdx_apply_operator.argtypes = [P, ctypes.POINTER(Grid), P, D, I, P, P]
     L.sfdx_apply_operator_noalloc.argtypes = [P, ctypes.POINTER(Grid), P, D, P, P, P]
@@ -268,6 +269,17 @@ class Context:
         if return_fields:
             return phi.value, grad, ex.T.copy(), st
         return phi.value, grad, st
+
+    def mesh_coarsen(self, a, b, c, sigma, k):
+        """mesh_coarsen with sigma averaged on the device (sfdx_mesh_coarsen_device):
+        widths and sigma(ncell, 6) of the grid coarsened by 2^k."""
+        G = _GridHolder(a, b, c)
+        f = 1 << int(k)
+        ac, bc, cc = np.zeros(len(G.a) // f), np.zeros(len(G.b) // f), np.zeros(len(G.c) // f)
+        sc = np.zeros(6 * len(ac) * len(bc) * len(cc))
+        _check(self.lib.sfdx_mesh_coarsen_device(self.h, ctypes.byref(G.g), _ptr(sigma_flat(sigma)), int(k), _ptr(ac),
+                                                 _ptr(bc), _ptr(cc), _ptr(sc)))
+        return ac, bc, cc, sc.reshape(6, -1).T.copy()
 
     def apply_operator(self, a, b, c, sigma, freq, x, precision=PREC_FP64):
         G = _GridHolder(a, b, c)

@@ -292,6 +292,21 @@ def _impedances(m, ex, rd=2):
     return res[:, 0] + 1j * res[:, 1], res[:, 2] + 1j * res[:, 3]
 
 
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_device_mesh_coarsen_equals_host_and_oracle(ctx, k):
+    """sfdx_mesh_coarsen_device (k_mesh_coarsen: sigma averaged on the device,
+    as sfdx_fwd_solve builds its coarse levels) gives the host coarsening's
+    bits and the oracle's mesh_coarsen (fdfe_main.f90:18-84, index fixed) to
+    1e-13, on a full-tensor model with non-uniform widths."""
+    m = synth.make_aniso_blocks_128(nx=32, nz_earth=20, nair=4, nstations=8)
+    a1, b1, c1, s1 = ctx.mesh_coarsen(m.a, m.b, m.c, m.sigma, k)
+    a0, b0, c0, s0 = sfdx.mesh_coarsen(m.a, m.b, m.c, m.sigma, k)
+    a2, b2, c2, s2 = O.mesh_coarsen(m.a, m.b, m.c, m.sigma, k)
+    assert np.array_equal(a1, a0) and np.array_equal(b1, b0) and np.array_equal(c1, c0)
+    assert np.array_equal(s1, s0)
+    assert np.allclose(s1, s2, rtol=1e-13, atol=1e-15 * np.abs(s2).max())
+
+
 def small_excmg_model(aniso=False):
     m = synth.make_iso_block_model("s16", 16, 16, 12, 4, seed=11, nblocks=3, freq=1.0, level=3, nstations=6)
     if aniso:
@@ -363,16 +378,19 @@ def test_fwd_solve_multigrid_matches_oracle(ctx, use_excmg, mg_levels):
 
 @pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
 def test_device_coarse_levels_match_direct_solves(ctx, precision):
-    """EXCMG levels 1-2 solved by the device Krylov method (coarse_solver 1)
-    instead of the host direct solves: same impedances to 1e-6, and the two
-    coarse levels report Krylov iterations."""
+    """EXCMG levels 1-2 solved by the device Krylov method (coarse_solver 1,
+    the default) and by the host direct solves (0, the reference's u1u2):
+    same impedances to 1e-6; only the device path reports Krylov iterations
+    on the two coarse levels."""
     m = small_excmg_model(True)
     kw = dict(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR,
               precision=precision, sigma_floor32=0.0)
-    exd, _ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(**kw))
-    exk, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(coarse_solver=1, **kw))
-    assert st.status == 0
+    assert sfdx.default_opts().coarse_solver == 1
+    exd, sd = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(coarse_solver=0, **kw))
+    exk, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, sfdx.default_opts(**kw))
+    assert st.status == 0 and sd.status == 0
     assert min(st.iters[0][0], st.iters[0][1], st.iters[1][0], st.iters[1][1]) > 0
+    assert max(sd.iters[0][0], sd.iters[0][1], sd.iters[1][0], sd.iters[1][1]) == 0
     for p, q in zip(_impedances(m, exk), _impedances(m, exd)):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 

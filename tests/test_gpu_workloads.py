@@ -143,6 +143,16 @@ def test_c2_full_size_fp64_and_mixed_residuals_and_impedances(ctx):
     rmx, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, exmx)
     for p, q in zip(impedances(rmx), impedances(r64)):
         assert zrel(p, q) < 1e-6
+    # the EXCMG levels 1-2 by the host direct solves (the reference's u1u2)
+    # instead of the device Krylov solves: the same impedances
+    print("C2 128x128x96: mixed solve with host direct coarse levels", flush=True)
+    o = headline_opts(m, sfdx.PREC_MIXED, eps)
+    o.coarse_solver = 0
+    exd, std_ = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, o)
+    assert std_.status == 0 and std_.relres[m.level - 1][0] <= eps and std_.relres[m.level - 1][1] <= eps
+    rd, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, exd)
+    for p, q in zip(impedances(rd), impedances(r64)):
+        assert zrel(p, q) < 1e-6
 
 
 # ------------------------------------------------------------------ C3
