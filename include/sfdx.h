@@ -98,7 +98,11 @@ typedef struct {
     int mg_coarse_pairs; /* multigrid: symmetric smoothing pairs on the coarsest grid */
     double restart_drop; /* SFDX_PREC_MIXED: restart the FP64 Krylov recurrences from
                             the true residual whenever the residual has dropped by
-                            this factor since the last restart (0 = never) */
+                            this factor since the last restart (0 = never).  Default
+                            1e-3: with the hybrid preconditioner it converged at all 16
+                            sounding frequencies (1e-3..1e3 Hz) of the 128x128x96
+                            model, where 1e-5 and 1e-6 drifted for 8000 iterations at
+                            398 Hz; the multigrid V-cycle at 0.1 Hz is fastest at 1e-5 */
     int use_graph;      /* 1: replay the steady-state COCR iteration as a captured HIP
                            graph (only without periodic divergence correction) */
     int nodal_smoother; /* nodal part of the hybrid preconditioner and of the multigrid
@@ -353,7 +357,8 @@ int sfdx_extint_v(sfdx_ctx *ctx, const sfdx_grid *g, const double *u0, const dou
  * default, in complex128 the cell-plane march, in complex64 the k-marching
  * tile of 32x8 columns with prefetch; 1: 32x8 without prefetch; 2, 3: 32x4
  * with and without), ms[11] the k-marching 32x8 tile over 4-plane chunks,
- * ms[12] the k-marching 32x8 tile with prefetch (variant 4).  ms holds 13
+ * ms[12] the k-marching 32x8 tile with prefetch (variant 4), ms[13] the
+ * multigrid V-cycle's split residual r - A (z + (i/wmu) G psi).  ms holds 14
  * doubles.
  * No counterpart in the reference (measurement only). */
 int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, double freq, int precision,

@@ -1676,7 +1676,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->precond = SFDX_PRECOND_HYBRID;
     o->mg_levels = 0;
     o->mg_coarse_pairs = 4;
-    o->restart_drop = 1e-6;
+    o->restart_drop = 1e-3;
     o->use_graph = 1;
     o->nodal_smoother = 1;
     o->coarse_solver = 1;
@@ -2278,6 +2278,15 @@ int sfdx_bench_kernels(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, d
             ms[12] = timed([&](int q) {
                 launch_spmv_march<R>(D.g, wmu, V(2 * q), V(2 * q + 1), (R)1, 0, nullptr, ctx->rb, ctx->st, 0, 4);
             });
+            // the V-cycle's split residual t = r - A (z + (i/wmu) G psi) (psi: node values)
+            {
+                std::vector<cx<R>> hp(d.NP);
+                for (int64_t n = 0; n < d.NP; n++) hp[n] = cx<R>((R)std::cos(0.003 * n), (R)std::sin(0.001 * n));
+                cx<R> *psi = pool_vec<cx<R>>(ctx, std::string("bench/psi") + (sizeof(R) == 8 ? "d" : "f"), d.NP);
+                SFDX_CHECK(hipMemcpyAsync(psi, hp.data(), d.NP * sizeof(cx<R>), hipMemcpyHostToDevice, ctx->st));
+                ms[13] = timed([&](int q) { launch_resid_split<R>(D.g, wmu, V(3 * q), psi, V(3 * q + 1), V(3 * q + 2), ctx->st); });
+                SFDX_CHECK(hipStreamSynchronize(ctx->st));
+            }
         };
         if (precision == SFDX_PREC_FP64) run(0.0); else run(0.0f);
         SFDX_CHECK(hipEventDestroy(e0));

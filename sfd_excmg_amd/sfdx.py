@@ -338,7 +338,7 @@ class Context:
 
     def bench_kernels(self, a, b, c, sigma, freq, precision=PREC_FP64, reps=20):
         G = _GridHolder(a, b, c)
-        ms = np.zeros(13)
+        ms = np.zeros(14)
         _check(self.lib.sfdx_bench_kernels(self.h, ctypes.byref(G.g), _ptr(sigma_flat(sigma)), float(freq),
                                            int(precision), int(reps), _ptr(ms)))
         return ms

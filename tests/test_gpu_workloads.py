@@ -48,8 +48,8 @@ def headline_opts(m, precision, eps, precond=sfdx.PRECOND_MG):
     3 finest grids with one coarsest smoothing pair, no periodic divergence
     correction, interior residual, mixed-precision restarts after every 1e-5
     drop (precond=PRECOND_HYBRID: the one-level hybrid preconditioner, restarts
-    every 1e-6)."""
-    drop = 1e-5 if precond == sfdx.PRECOND_MG else 1e-6
+    every 1e-3)."""
+    drop = 1e-5 if precond == sfdx.PRECOND_MG else 1e-3
     return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0, norm=sfdx.NORM_INTERIOR,
                              precision=precision, sigma_floor32=0.0, restart_drop=drop, maxit_excmg=8000,
                              precond=precond, mg_levels=3, mg_coarse_pairs=1)
