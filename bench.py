@@ -415,8 +415,14 @@ def main():
     print(f"fp64 done: fine {f64['fine']:.3f} s, {f64['fine_iters']} iterations", file=sys.stderr, flush=True)
     ex_mx, mx = run_gpu(ctx, m, args.steps, 0, solve_opts(m, sfdx.PREC_MIXED, eps, args.verbose))
     print(f"mixed done: fine {mx['fine']:.3f} s, {mx['fine_iters']} iterations", file=sys.stderr, flush=True)
+    # complex64 COCR directions in split edge + nodal form (restarts every 1e-4:
+    # 1e-5 stalls with complex64 directions); reported beside the headline
+    o_sp = solve_opts(m, sfdx.PREC_MIXED_SPLIT, eps, args.verbose)
+    o_sp.restart_drop = 1e-4
+    ex_sp, sp = run_gpu(ctx, m, args.steps, 0, o_sp)
+    print(f"mixed split done: fine {sp['fine']:.3f} s, {sp['fine_iters']} iterations", file=sys.stderr, flush=True)
     # the headline is the faster of the paths that converged (status 0)
-    converged = [r for r in (f64, mx) if r["status"] == 0]
+    converged = [r for r in (f64, mx, sp) if r["status"] == 0]
     head = min(converged, key=lambda r: r["fine"]) if converged else f64
     z64, zmx = impedances(m, ex_64), impedances(m, ex_mx)
     zerr = float(np.max(np.abs(zmx - z64) / np.abs(z64)))
@@ -440,7 +446,9 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("complex128 Krylov/operator + complex64 preconditioner" if head is mx else "complex128"),
+        "dtype": ("complex128 Krylov/operator + complex64 preconditioner" if head is mx else
+                  "complex128 solution/residual + complex64 split directions and preconditioner" if head is sp
+                  else "complex128"),
         "data": "synthetic (seeded anisotropic block model)",
         "config": {"workload": ("mt3d_fwd_128x128x96_aniso_0.1Hz_2pol_excmg4_cocr" if not args.small
                                 else "mt3d_fwd_64x64x48_aniso_small"), "cells": [m.nx, m.ny, m.nz], "edges": nl,
@@ -451,6 +459,11 @@ def main():
                      "status": f64["status"]},
         "mixed_gpu": {"fine_s": round(mx["fine"], 4), "wall_s": round(mx["wall"], 3), "fine_iters": mx["fine_iters"],
                       "status": mx["status"], "restarts": mx["stats"]["refine_steps"] if mx["stats"] else None},
+        "mixed_split_gpu": {"fine_s": round(sp["fine"], 4), "wall_s": round(sp["wall"], 3),
+                            "fine_iters": sp["fine_iters"], "status": sp["status"],
+                            "restarts": sp["stats"]["refine_steps"] if sp["stats"] else None,
+                            "max_rel_impedance_err_vs_fp64":
+                                float(np.max(np.abs(impedances(m, ex_sp) - z64) / np.abs(z64)))},
         "headline_converged": bool(converged),
         "speedup_mixed_vs_fp64": round(f64["fine"] / mx["fine"], 3) if mx["fine"] > 0 else None,
         "max_rel_impedance_err_mixed_vs_fp64": zerr,

@@ -46,6 +46,12 @@ extern "C" {
                                    Experimental: with resistive air (sigma ~1e-8 S/m) the
                                    complex64 product K(G phi) cancels to noise, the inner
                                    solves diverge and the solve returns SFDX_WARN_NOCONV */
+#define SFDX_PREC_MIXED_SPLIT 3 /* complex128 solution and residual; COCR directions, their
+                                   operator products and the multigrid V-cycle in complex64,
+                                   the directions in split edge + nodal form (v_e + (i/wmu)
+                                   G v_psi) so that the curl part never acts on gradients.
+                                   With precond SFDX_PRECOND_MG and solver COCR on the fine
+                                   level; otherwise as SFDX_PREC_MIXED */
 
 #define SFDX_PRECOND_SSOR 0
 #define SFDX_PRECOND_HYBRID 1  /* SSOR + G (G^T A G)^-1 G^T, G^T A G = -i w mu P */

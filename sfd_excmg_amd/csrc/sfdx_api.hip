@@ -776,6 +776,144 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     return info;
 }
 
+// COCR with complex64 directions in split form (SFDX_PREC_MIXED_SPLIT, with
+// the multigrid V-cycle).  The solution x and the residual r stay complex128;
+// the directions z and p (each an edge part and node values, v = v_e + (i/wmu)
+// G v_psi, which is the form the complex64 V-cycle returns), Ap and Az are
+// complex64.  Az = A (z_e + (i/wmu) G z_psi) is formed with the curl part
+// acting on z_e alone (K G = 0), so the large gradient components of
+// resistive cells never meet the curl operator in complex64, and the fused
+// dot (z, Az) takes the assembled u = z_e + (i/wmu) G z_psi.  Restarts from
+// the true FP64 residual after every restart_drop reduction, on stagnation and
+// on divergence; convergence is accepted only on a recomputed residual.
+// After 3 restarts in a row that did not halve the best residual it returns
+// with fallback set and SFDX_WARN_NOCONV, leaving x as the iterate.
+SolveInfo cocr_split(sfdx_ctx *ctx, LevelDev<double> &D, LevelDev<float> &F, MGHier<float> &H, int mg_level,
+                     const std::string &tag, double wmu, zd *x, double bnorm, double eps, int maxit, const KOpts &ko)
+{
+    const Dims &d = D.g.d;
+    const std::string p = tag + "/ks";
+    zd *r = pool_vec<zd>(ctx, p + "r", d.NL);
+    zf *r32 = pool_vec<zf>(ctx, p + "r32", d.NL);
+    zf *ze = pool_vec<zf>(ctx, p + "ze", d.NL), *pe = pool_vec<zf>(ctx, p + "pe", d.NL);
+    zf *qe = pool_vec<zf>(ctx, p + "qe", d.NL), *Ap = pool_vec<zf>(ctx, p + "Ap", d.NL);
+    zf *Az = pool_vec<zf>(ctx, p + "Az", d.NL);
+    zf *zpsi = pool_vec<zf>(ctx, p + "zp", d.NP), *ppsi = pool_vec<zf>(ctx, p + "pp", d.NP);
+    zf *qpsi = pool_vec<zf>(ctx, p + "qp", d.NP);
+    const float w32 = (float)wmu;
+    const zf s32(0.0f, (float)(1.0 / wmu));
+    const int64_t nbe = edge_launch(d).nblocks();
+    const double v16 = (double)d.NL * 16, v8 = (double)d.NL * 8, n8 = (double)d.NP * 8;
+    const double b_prod = 3 * v8 + n8 + (double)d.NC * 24;           // split product: z_e, z_psi, sigma in, Az out
+    const double b_iter = b_prod + (2 * v8 + n8)                       // the dot (q, Ap)
+                          + (4 * v16 + 6 * v8 + 6 * n8)                // split update (x, r; p_e, z_e, q_e, Ap; nodes)
+                          + 3 * v8;                                    // Ap = Az + beta Ap
+    SolveInfo info;
+    int state = 0;
+    bool fresh = false;
+    double rel0 = 0, cur[SL_COUNT];
+    bool pending = false;
+    int pend_slot = 0;
+    StreamGraph graph;
+    double best = 1e300, best_at_restart = 1e300;
+    int since_best = 0, idle_restarts = 0;
+    while (true) {
+        if (state != 2) {
+            fresh = true;
+            pending = false;
+            launch_spmv<double>(D.g, wmu, x, r, -1.0, 0, nullptr, ctx->rb, ctx->st);
+            launch_convert<double, float>(d.NL, r, r32, ctx->st);
+            mg_vcycle<float>(ctx, H, mg_level, w32, r32, ze, zpsi);
+            const int64_t nbs = launch_split_product(F.g, w32, ze, zpsi, Az, ctx->rb, ctx->st);
+            reduce_final(ctx->rb, nbs, 2, SL_TMP, OP_COCR_RHO, ctx->st);  // also beta = 0
+            launch_copy<float>(d.NL, ze, pe, ctx->st);
+            launch_copy<float>(d.NP, zpsi, ppsi, ctx->st);
+            launch_cocr_ap<float>(d.NL, ctx->rb, Ap, Az, nullptr, true, ctx->st);
+            launch_vec<double>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
+            reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+            read_slots(ctx);
+            std::memcpy(cur, ctx->hslots, sizeof(cur));
+            info.bytes += (2 * v16 + (double)d.NC * 48) + 2 * v8 + b_prod + 2 * (v8 + n8) + 3 * v8 + v16;
+            state = 2;
+        }
+        info.relres = std::sqrt(cur[SL_RN]) / bnorm;
+        const bool bad = !(info.relres == info.relres) || std::isinf(info.relres);
+        const bool fresh_now = fresh;
+        if (fresh) { rel0 = info.relres; fresh = false; best = info.relres; since_best = 0; }
+        if (info.relres < best) { best = info.relres; since_best = 0; }
+        const bool diverging = info.relres > 100 * best;
+        const bool stalled = since_best > 400;
+        const bool restart = (ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0) || diverging || stalled;
+        if (bad || info.relres < eps || info.iters >= maxit || cur[SL_BREAK] != 0 || restart) {
+            if (pending) {  // decide on the newest iteration instead
+                std::memcpy(cur, wait_slots(ctx, pend_slot), sizeof(cur));
+                pending = false;
+                continue;
+            }
+            if (!bad && (info.relres < eps || info.iters >= maxit)) {
+                if (!fresh_now) {  // accept or stop only on a recomputed residual
+                    state = 0;
+                    continue;
+                }
+                info.status = info.relres < eps ? 0 : SFDX_WARN_NOCONV;
+                break;
+            }
+            const bool planned = !bad && cur[SL_BREAK] == 0 && !stalled && !diverging;
+            if (!planned) {
+                idle_restarts = (best < 0.5 * best_at_restart) ? 0 : idle_restarts + 1;
+                best_at_restart = std::min(best_at_restart, best);
+                if (idle_restarts >= 3) {
+                    info.fallback = 1;
+                    info.status = SFDX_WARN_NOCONV;
+                    if (!fresh_now) {  // leave with the residual of the returned iterate
+                        launch_spmv<double>(D.g, wmu, x, r, -1.0, 0, nullptr, ctx->rb, ctx->st);
+                        launch_vec<double>(5, D.g, ctx->rb, r, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
+                        reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+                        read_slots(ctx);
+                        info.relres = std::sqrt(ctx->hslots[SL_RN]) / bnorm;
+                    }
+                    break;
+                }
+            } else {
+                best_at_restart = std::min(best_at_restart, best);
+            }
+            state = 0;
+            info.restarts++;
+            continue;
+        }
+        if (ko.verbose > 1 && info.iters % 50 == 0) fprintf(stderr, "  cocr32 it %d relres %.3e\n", info.iters, info.relres);
+        auto part1 = [&]() {
+            mg_vcycle<float>(ctx, H, mg_level, w32, Ap, qe, qpsi);
+            launch_grad_add<float>(F.g, s32, qpsi, qe, ctx->st, Ap, ctx->rb.partials, false);  // (q, Ap) only
+            reduce_final(ctx->rb, nbe, 2, SL_TMP, OP_COCR_ALPHA, ctx->st);
+            launch_cocr_split_upd(D.g, ctx->rb, x, r, Ap, ze, qe, pe, ppsi, zpsi, qpsi, wmu, ctx->st);
+            reduce_final(ctx->rb, nbe, 1, SL_TMP, OP_RN, ctx->st);
+        };
+        auto part2 = [&]() {
+            const int64_t nbs = launch_split_product(F.g, w32, ze, zpsi, Az, ctx->rb, ctx->st);
+            reduce_final(ctx->rb, nbs, 2, SL_TMP, OP_COCR_BETA, ctx->st);
+            launch_cocr_ap<float>(d.NL, ctx->rb, Ap, Az, nullptr, false, ctx->st);
+        };
+        since_best++;
+        if (ko.use_graph && pending) {
+            // steady state: the whole iteration is one captured graph, replayed
+            if (!graph.exec) graph.capture(ctx->st, [&]() { part1(); part2(); });
+            graph.launch(ctx->st);
+        } else {
+            part1();
+            part2();
+        }
+        info.iters++;
+        info.bytes += b_iter;
+        const int slot = info.iters & 1;
+        post_slots(ctx, slot);
+        if (pending) std::memcpy(cur, wait_slots(ctx, pend_slot), sizeof(cur));
+        pending = true;
+        pend_slot = slot;
+    }
+    return info;
+}
+
 // Right-preconditioned BiCGStab (Bicg_stab_p, src/solvers.f90:522-719) on A_s.
 // Divergence correction every corr_intv iterations followed by r = f - Ax,
 // rhat = r; with dc_restart = 0 rho, alpha, omega, p, v carry over as in the
@@ -1108,7 +1246,30 @@ LevelSolveResult solve_level(sfdx_ctx *ctx, const sfdx_opts &o, LevelDev<double>
         res.info = krylov<double>(ctx, o.solver, D64, tag, wmu, nullptr, x, bnorm, eps, maxit, ko);
         return res;
     }
-    if (o.precision == SFDX_PREC_MIXED) {
+    if (o.precision == SFDX_PREC_MIXED_SPLIT && h32 && ko.precond == SFDX_PRECOND_MG &&
+        o.solver == SFDX_SOLVER_COCR) {
+        KOpts ks = ko;
+        ks.restart_drop = o.restart_drop;
+        res.info = cocr_split(ctx, D64, *D32, *h32, level, tag, wmu, x, bnorm, eps, maxit, ks);
+        res.refine = res.info.restarts;
+        if (res.info.status != 0 && res.info.fallback && res.info.iters < maxit) {
+            // continue from the iterate with complex128 directions (SFDX_PREC_MIXED)
+            KOpts km = ko;
+            km.lo = D32;
+            km.restart_drop = o.restart_drop;
+            const SolveInfo s2 =
+                krylov<double>(ctx, o.solver, D64, tag, wmu, nullptr, x, bnorm, eps, maxit - res.info.iters, km);
+            res.info.iters += s2.iters;
+            res.info.restarts += s2.restarts;
+            res.info.bytes += s2.bytes;
+            res.info.status = s2.status;
+            res.info.relres = s2.relres;
+            res.info.switched += s2.switched;
+            res.refine = res.info.restarts;
+        }
+        return res;
+    }
+    if (o.precision == SFDX_PREC_MIXED || o.precision == SFDX_PREC_MIXED_SPLIT) {
         KOpts km = ko;
         km.lo = D32;
         km.restart_drop = o.restart_drop;
@@ -1542,10 +1703,11 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     }
     LevelDev<double> D64 = upload_level<double>(ctx, fine, "ADJ");
     LevelDev<float> D32;
-    if (o->precision == SFDX_PREC_MIXED) D32 = upload_level<float>(ctx, fine, "ADJ", o->sigma_floor32);
+    const bool adj_mixed = o->precision == SFDX_PREC_MIXED || o->precision == SFDX_PREC_MIXED_SPLIT;
+    if (adj_mixed) D32 = upload_level<float>(ctx, fine, "ADJ", o->sigma_floor32);
     ensure_partials(ctx, edge_launch(d).nblocks() + 64);
     choose_tile(ctx, D64.g, "ADJ/d", o->spmv_tile, o->verbose);
-    if (o->precision == SFDX_PREC_MIXED) choose_tile(ctx, D32.g, "ADJ/f", o->spmv_tile, o->verbose);
+    if (adj_mixed) choose_tile(ctx, D32.g, "ADJ/f", o->spmv_tile, o->verbose);
     const bool xmass = o->precond == SFDX_PRECOND_SSOR || o->ssor_cross_mass != 0;
     D64.g.ssor_xmass = D32.g.ssor_xmass = xmass;
     double *gdev = pool_vec<double>(ctx, "adj/g", d.NC);
@@ -1559,7 +1721,7 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     ko.corr_intv = 0;
     ko.dc_at_start = 0;
     if (ko.precond == SFDX_PRECOND_MG) ko.precond = SFDX_PRECOND_HYBRID;
-    if (o->precision == SFDX_PREC_MIXED) {
+    if (adj_mixed) {
         ko.lo = &D32;
         ko.restart_drop = o->restart_drop;
     }

@@ -225,6 +225,15 @@ void launch_cocr_upd_grad(const DevGrid<R> &g, RedBuf &rb, cx<R> *x, cx<R> *pv, 
 void launch_mesh_coarsen(const Dims &df, const Dims &dc, int f, const double *a, const double *b, const double *c,
                          const double *ac, const double *bc, const double *cc, const double *sig, double *sigc,
                          hipStream_t st);
+// y = A_s (z + (i/wmu) G psi), with the partial sums of u^T y (u = z + (i/wmu) G psi);
+// returns the number of partial sums
+int64_t launch_split_product(const DevGrid<float> &g, float wmu, const zf *z, const zf *psi, zf *y, RedBuf &rb,
+                             hipStream_t st);
+// COCR update with complex64 split directions: node part (p_psi = z_psi + beta
+// p_psi, z_psi -= alpha q_psi), then edge part (p_e = z_e + beta p_e; x +=
+// alpha (p_e + s G p_psi); r -= alpha Ap; z_e -= alpha q_e; |r/ve|^2 partials)
+void launch_cocr_split_upd(const DevGrid<double> &g, RedBuf &rb, zd *x, zd *r, const zf *Ap, zf *ze, const zf *qe,
+                           zf *pe, zf *ppsi, zf *zpsi, const zf *qpsi, double wmu, hipStream_t st);
 void launch_psym32(int64_t np, const double *P, float *Ps32, hipStream_t st);
 void launch_hybrid_edge32(const DevGrid<double> &g64, const DevGrid<float> &g32, double wmu, const float *Ps32,
                           const zd *in, zd *out, zf *in32, zf *z32, zd *tnode, zd *phi, float omega,

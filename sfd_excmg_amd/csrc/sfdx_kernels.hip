@@ -332,7 +332,7 @@ __device__ __forceinline__ cx<R> mvalue(const cx<R> *__restrict__ w, int64_t e, 
                                         double (&acc)[4])
 {
     if ((DOT == 4 || DOT == 5) && interior) out = w[e] + out;
-    if (DOT == 1) {
+    if (DOT == 1 || DOT == 6) {
         acc[0] += (double)xe.re * out.re - (double)xe.im * out.im;
         acc[1] += (double)xe.re * out.im + (double)xe.im * out.re;
     } else if ((DOT == 2 || DOT == 3) && interior) {
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
 {
     using G = mt::Geo<TY>;
     constexpr int TX = G::TX, PX = G::PX, SX = G::SX, SY = G::SY, NP = G::NP, NS = G::NS;
-    constexpr bool PS = DOT == 5;
+    constexpr bool PS = DOT == 5 || DOT == 6;
     __shared__ __align__(16) R lds[G::LDS_R + (PS ? 6 * NP : 0)];
     cx<R> *const EX = reinterpret_cast<cx<R> *>(lds);  // [2][NP], slot k&1 holds plane k
     cx<R> *const EY = EX + 2 * NP;
@@ -555,7 +555,7 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
             auto S2 = [&](int u, int o) { return cm * SP[u * NS + o] + cp * SK[u * NS + o]; };
             // ---- Ex(i,j,k)
             {
-                cx<R> out(0);
+                cx<R> out(0), xo = exK;  // xo: the own value the fused dot takes (mode 6: u)
                 const bool in = j >= 1 && zk;
                 if (in) {
                     const R rcm = rc[k - 1], rcp = rc[k];
@@ -588,15 +588,16 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
                     const cx<R> M = kmd * uK + my1 * (u00 + u10) + my2 * (uyK + u11) + mz1 * (uzP + uz1)
                                     + mz2 * (uzK + uz2);
                     out = km_combine(K, M, wmu, alpha);
+                    if (DOT == 6) xo = uK;
                 }
-                memit<R, DOT>(y, w, ex0 + (int64_t)k * XP, out, in, exK, acc);
+                memit<R, DOT>(y, w, ex0 + (int64_t)k * XP, out, in, xo, acc);
             }
             // rows one after another: the scheduler may not hoist the next row's
             // LDS reads above this point, which bounds the registers live at once
             __builtin_amdgcn_sched_barrier(0);
             // ---- Ey(i,j,k)
             {
-                cx<R> out(0);
+                cx<R> out(0), xo = eyK;
                 const bool in = i >= 1 && zk;
                 if (in) {
                     const R rcm = rc[k - 1], rcp = rc[k];
@@ -627,13 +628,14 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
                     const cx<R> M = kmd * uK + mx1 * (u0 + u2) + mx2 * (uxK + u3) + mz1 * (uzP + uz1)
                                     + mz2 * (uzK + uz3);
                     out = km_combine(K, M, wmu, alpha);
+                    if (DOT == 6) xo = uK;
                 }
-                memit<R, DOT>(y, w, ey0 + (int64_t)k * YP, out, in, eyK, acc);
+                memit<R, DOT>(y, w, ey0 + (int64_t)k * YP, out, in, xo, acc);
             }
             __builtin_amdgcn_sched_barrier(0);
             // ---- Ez(i,j,k), between node planes k and k+1
             if (k < d.nz) {
-                cx<R> out(0);
+                cx<R> out(0), xo = ezK;
                 const bool in = i >= 1 && j >= 1;
                 if (in) {
                     const R ck = c[k];
@@ -670,8 +672,9 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
                     const cx<R> M = kmd * uK + mx1 * (uxm + uxmN) + mx2 * (uxK + uxN) + my1 * (uym + uymN)
                                     + my2 * (uyK + uyN);
                     out = km_combine(K, M, wmu, alpha);
+                    if (DOT == 6) xo = uK;
                 }
-                memit<R, DOT>(y, w, ez0 + (int64_t)k * ZP, out, in, ezK, acc);
+                memit<R, DOT>(y, w, ez0 + (int64_t)k * ZP, out, in, xo, acc);
             }
             // boundary edges of the last column i = nx and the last row j = ny
             const cx<R> zero(0);
@@ -694,7 +697,7 @@ __global__ __launch_bounds__(mt::Geo<TY>::NT) void k_spmv_march(
             if (two) L1.fetch(d, x, sig, k + 1, XP, YP, ZP, CP, psi, rw);
         }
     }
-    if (DOT >= 1 && DOT <= 3) block_reduce_store<4, G::NT>(acc, partials);
+    if ((DOT >= 1 && DOT <= 3) || DOT == 6) block_reduce_store<4, G::NT>(acc, partials);
 }
 
 // ------------------------------------------------------------------ K1b
@@ -1860,6 +1863,80 @@ void launch_cocr_upd_hyb(const DevGrid<double> &g, RedBuf &rb, zd *x, zd *pv, co
                        z32, phi, zd(0.0, 1.0 / wmu), rb.partials);
 }
 
+// COCR with complex64 directions in split form (edge part and node values:
+// v = v_e + s G v_psi, s = i/wmu), complex128 solution and residual.  The node
+// part runs first: p_psi = z_psi + beta p_psi (the direction kept one step
+// behind), z_psi -= alpha q_psi.
+__global__ __launch_bounds__(TPB) void k_cocr_split_node(int64_t np, const double *__restrict__ slots,
+                                                         zf *__restrict__ ppsi, zf *__restrict__ zpsi,
+                                                         const zf *__restrict__ qpsi)
+{
+    const int64_t n = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (n >= np || slots[SL_BREAK] != 0) return;
+    const zd al(slots[SL_ALPHA], slots[SL_ALPHA + 1]), be(slots[SL_BETA], slots[SL_BETA + 1]);
+    const zf zv = zpsi[n], pv = ppsi[n], qv = qpsi[n];
+    const zd pn = fma_c(zd(zv.re, zv.im), be, zd(pv.re, pv.im));
+    const zd zn = zd(zv.re, zv.im) - al * zd(qv.re, qv.im);
+    ppsi[n] = zf((float)pn.re, (float)pn.im);
+    zpsi[n] = zf((float)zn.re, (float)zn.im);
+}
+
+// The edge part: p_e = z_e + beta p_e; x += alpha (p_e + s G p_psi) (in FP64,
+// from the stored complex64 direction and the already updated p_psi);
+// r -= alpha Ap; z_e -= alpha q_e; partial sums of |r/ve|^2 on interior edges.
+__global__ __launch_bounds__(TPB) void k_cocr_split_upd(Dims d, const double *__restrict__ a,
+                                                        const double *__restrict__ b, const double *__restrict__ c,
+                                                        const double *__restrict__ slots, zd *__restrict__ x,
+                                                        zd *__restrict__ r, const zf *__restrict__ Ap,
+                                                        zf *__restrict__ ze, const zf *__restrict__ qe,
+                                                        zf *__restrict__ pe, const zf *__restrict__ ppsi, zd s,
+                                                        double *__restrict__ partials)
+{
+    const Tile TL = tile_id();
+    const Slice S = edge_slice(d, TL.y);
+    const int p = TL.x * TPB + threadIdx.x;
+    double acc[4] = {0, 0, 0, 0};
+    if (p < S.nI * S.nJ) {
+        const int j = p / S.nI, i = p - j * S.nI;
+        const int64_t e = S.base + p;
+        const bool inner = edge_interior(d, S.o, i, j, S.k);
+        zd rr = r[e];
+        if (slots[SL_BREAK] == 0) {
+            const zd al(slots[SL_ALPHA], slots[SL_ALPHA + 1]), be(slots[SL_BETA], slots[SL_BETA + 1]);
+            const zf zv = ze[e], pv = pe[e], av = Ap[e], qv = qe[e];
+            const zd pn = fma_c(zd(zv.re, zv.im), be, zd(pv.re, pv.im));
+            const zf pn32((float)pn.re, (float)pn.im);
+            pe[e] = pn32;
+            int64_t n0, n1;
+            const double l = edge_len_nodes(d, a, b, c, S.o, i, j, S.k, n0, n1);
+            const zf f1 = ppsi[n1], f0 = ppsi[n0];
+            const zd g(((double)f1.re - (double)f0.re) / l, ((double)f1.im - (double)f0.im) / l);
+            const zd pf = fma_c(zd(pn32.re, pn32.im), s, g);
+            x[e] = fma_c(x[e], al, pf);
+            rr = rr - al * zd(av.re, av.im);
+            r[e] = rr;
+            const zd zn = zd(zv.re, zv.im) - al * zd(qv.re, qv.im);
+            ze[e] = zf((float)zn.re, (float)zn.im);
+        }
+        if (inner) {
+            const double ve = dual_volume(a, b, c, S.o, i, j, S.k);
+            acc[0] = (rr.re * rr.re + rr.im * rr.im) / (ve * ve);
+        }
+    }
+    block_reduce_store<4>(acc, partials);
+}
+
+void launch_cocr_split_upd(const DevGrid<double> &g, RedBuf &rb, zd *x, zd *r, const zf *Ap, zf *ze, const zf *qe,
+                           zf *pe, zf *ppsi, zf *zpsi, const zf *qpsi, double wmu, hipStream_t st)
+{
+    const int64_t np = g.d.NP;
+    hipLaunchKernelGGL(k_cocr_split_node, dim3((unsigned)((np + TPB - 1) / TPB)), dim3(TPB), 0, st, np, rb.slots,
+                       ppsi, zpsi, qpsi);
+    SliceLaunch L = edge_launch(g.d);
+    hipLaunchKernelGGL(k_cocr_split_upd, L.grid, L.block, 0, st, g.d, g.a, g.b, g.c, rb.slots, x, r, Ap, ze, qe, pe,
+                       (const zf *)ppsi, zd(0.0, 1.0 / wmu), rb.partials);
+}
+
 // ------------------------------------------------------------------ K8 multigrid transfers
 // Edge prolongation between nested grids (fine = 2 x coarse in every
 // direction): the tangential field is constant along a coarse edge and linear
@@ -2225,7 +2302,7 @@ static int64_t march_go(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R 
     const unsigned gx = (unsigned)(ntx * nty), nch = (unsigned)((d.nz + kc) / kc);
     const dim3 grid(gx, pad_y(gx, nch), 1), block(G::NT, 1, 1);
     const int64_t nblk = (int64_t)grid.x * grid.y;
-    if (dot_mode >= 1 && dot_mode <= 3 && nblk > rb.max_blocks)
+    if (((dot_mode >= 1 && dot_mode <= 3) || dot_mode == 6) && nblk > rb.max_blocks)
         throw std::runtime_error("reduction buffer smaller than the stencil grid");
 #define SFDX_MARCH(M)                                                                                      \
     hipLaunchKernelGGL((k_spmv_march<R, M, TY, PF>), grid, block, 0, st, d, g.a, g.b, g.c, g.ra, g.rb, g.rc, \
@@ -2237,6 +2314,12 @@ static int64_t march_go(const DevGrid<R> &g, R wmu, const cx<R> *x, cx<R> *y, R 
     case 3: SFDX_MARCH(3); break;
     case 4: SFDX_MARCH(4); break;
     case 5: SFDX_MARCH(5); break;
+    case 6:
+        if constexpr (sizeof(R) == 4) {
+            SFDX_MARCH(6);
+            break;
+        }
+        throw std::runtime_error("launch_spmv_march: mode 6 is complex64 only");
     default: throw std::runtime_error("launch_spmv_march: bad mode");
     }
 #undef SFDX_MARCH
@@ -2340,6 +2423,15 @@ void launch_resid_split(const DevGrid<R> &g, R wmu, const cx<R> *z, const cx<R> 
     case 3: march_go<R, 4, false>(g, wmu, z, t, (R)-1, 5, r, none, st, 0, psi); break;
     default: march_go<R, 8, true>(g, wmu, z, t, (R)-1, 5, r, none, st, 0, psi); break;
     }
+}
+
+// y = A_s (z + (i/wmu) G psi) on interior edges (0 on boundary edges) with the
+// partial sums of u^T y, u = z + (i/wmu) G psi (unconjugated): the operator on
+// the split directions of the complex64 COCR, the curl part acting on z alone
+int64_t launch_split_product(const DevGrid<float> &g, float wmu, const zf *z, const zf *psi, zf *y, RedBuf &rb,
+                             hipStream_t st)
+{
+    return march_go<float, 8, true>(g, wmu, z, y, 1.0f, 6, nullptr, rb, st, 0, psi);
 }
 
 template <typename R>

@@ -4,7 +4,8 @@ module sfdx_iface
   implicit none
 
   integer(c_int), parameter :: SFDX_SOLVER_BICGSTAB = 1, SFDX_SOLVER_GPBICG = 3, SFDX_SOLVER_COCR = 5
-  integer(c_int), parameter :: SFDX_PREC_FP64 = 0, SFDX_PREC_MIXED = 1, SFDX_PREC_MIXED_IR = 2
+  integer(c_int), parameter :: SFDX_PREC_FP64 = 0, SFDX_PREC_MIXED = 1, SFDX_PREC_MIXED_IR = 2, &
+                                SFDX_PREC_MIXED_SPLIT = 3
   integer(c_int), parameter :: SFDX_PRECOND_SSOR = 0, SFDX_PRECOND_HYBRID = 1, SFDX_PRECOND_MG = 2
   integer(c_int), parameter :: SFDX_NORM_REFERENCE = 0, SFDX_NORM_INTERIOR = 1
 

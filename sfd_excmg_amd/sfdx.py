@@ -22,7 +22,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "lib", "libsfdx.so")
 
 SOLVER_BICGSTAB, SOLVER_GPBICG, SOLVER_COCR = 1, 3, 5
-PREC_FP64, PREC_MIXED, PREC_MIXED_IR = 0, 1, 2
+PREC_FP64, PREC_MIXED, PREC_MIXED_IR, PREC_MIXED_SPLIT = 0, 1, 2, 3
 NORM_REFERENCE, NORM_INTERIOR = 0, 1
 PRECOND_SSOR, PRECOND_HYBRID, PRECOND_MG = 0, 1, 2
 MAX_LEVELS = 8

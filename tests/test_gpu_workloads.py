@@ -143,6 +143,22 @@ def test_c2_full_size_fp64_and_mixed_residuals_and_impedances(ctx):
     rmx, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, exmx)
     for p, q in zip(impedances(rmx), impedances(r64)):
         assert zrel(p, q) < 1e-6
+    # complex64 COCR directions in split edge + nodal form (PREC_MIXED_SPLIT)
+    print("C2 128x128x96: mixed solve with complex64 split directions", flush=True)
+    o = headline_opts(m, sfdx.PREC_MIXED_SPLIT, eps)
+    o.restart_drop = 1e-4
+    exs, sts = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, o)
+    print(f"  status {sts.status} fine iterations {sts.fine_iters_total} ({sts.fine_seconds:.3f} s)", flush=True)
+    assert sts.status == 0 and sts.fallback_fp64 == 0
+    S = O.build_system(m.a, m.b, m.c, m.sigma, float(m.freqs[0]))
+    for pol, rhs in ((0, S.p1), (1, S.p2)):
+        rr = interior_relres(S, exs[pol * nl:(pol + 1) * nl, 0], rhs)
+        print(f"  split pol {pol}: reported {sts.relres[m.level - 1][pol]:.3e} recomputed {rr:.3e}", flush=True)
+        assert sts.relres[m.level - 1][pol] <= eps and rr <= 1.01 * eps
+    del S
+    rs, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, exs)
+    for p, q in zip(impedances(rs), impedances(r64)):
+        assert zrel(p, q) < 1e-6
     # the EXCMG levels 1-2 by the host direct solves (the reference's u1u2)
     # instead of the device Krylov solves: the same impedances
     print("C2 128x128x96: mixed solve with host direct coarse levels", flush=True)
@@ -153,6 +169,24 @@ def test_c2_full_size_fp64_and_mixed_residuals_and_impedances(ctx):
     rd, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, exd)
     for p, q in zip(impedances(rd), impedances(r64)):
         assert zrel(p, q) < 1e-6
+
+
+def test_mixed_split_directions_match_oracle_fixture(ctx):
+    """PREC_MIXED_SPLIT (complex64 COCR directions in split edge + nodal form,
+    complex64 V-cycle, complex128 solution and residual) on the benchmark
+    model's generator at 64x64x48: converged on the recomputed residual, no
+    fallback to complex128 directions, impedances within 1e-6 of the oracle
+    fixture (tests/golden/aniso64_oracle_impedances.npz)."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "aniso64_oracle_impedances.npz"))
+    m = synth.make_aniso_blocks_128(nx=64, nz_earth=40, nair=8, nstations=20)
+    o = headline_opts(m, sfdx.PREC_MIXED_SPLIT, 1e-10)
+    o.restart_drop = 1e-4
+    ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, o)
+    assert st.status == 0 and st.fallback_fp64 == 0
+    assert max(st.relres[m.level - 1]) <= 1e-10
+    r, _ = sfdx.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv, ex, out_type=1, rd=2)
+    for p, q in ((r[:, 0] + 1j * r[:, 1], g["zxy"]), (r[:, 2] + 1j * r[:, 3], g["zyx"])):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
 # ------------------------------------------------------------------ C3
