@@ -225,9 +225,12 @@ def run_c4(ctx, args):
         big.freqs = big.freqs[::4]
     models = [big] + small
     prec = sfdx.PREC_FP64 if args.batch_fp64 else sfdx.PREC_MIXED
-    # the V-cycle's coarse grids stop resolving the skin depth at the upper
-    # sounding frequencies, so the batch keeps the one-level hybrid preconditioner
-    opts = solve_opts(big, prec, 1e-8, precond=sfdx.PRECOND_HYBRID)
+    # the preconditioner per item: the V-cycle where its first coarse grid
+    # resolves the skin depth and the field is not quasi-static over the mesh,
+    # the one-level hybrid preconditioner elsewhere (SFDX_PRECOND_AUTO); on the
+    # 128x128x96 model the V-cycle fails from ~150 Hz up and stalls in
+    # complex64 at 1-2.5 mHz, and is ~1.6-1.9x faster from 6 mHz to 1.6 Hz
+    opts = solve_opts(big, prec, 1e-8, precond=sfdx.PRECOND_AUTO)
     nrhs = 2 * sum(len(m.freqs) for m in models)
     t0 = time.perf_counter()
     exb, stb = ctx.solve_batch(models, opts, nstreams=args.nstreams)
@@ -240,7 +243,7 @@ def run_c4(ctx, args):
     zerr = None
     if not args.batch_fp64:
         # impedances of the mixed batch against a complex128 batch of the same items
-        ex64, _ = ctx.solve_batch(models, solve_opts(big, sfdx.PREC_FP64, 1e-8, precond=sfdx.PRECOND_HYBRID),
+        ex64, _ = ctx.solve_batch(models, solve_opts(big, sfdx.PREC_FP64, 1e-8, precond=sfdx.PRECOND_AUTO),
                                   nstreams=args.nstreams)
         zerr = max(float(np.max(np.abs(impedances(m, p) - impedances(m, q)) / np.abs(impedances(m, q))))
                    for m, p, q in zip(models, exb, ex64))
@@ -253,6 +256,7 @@ def run_c4(ctx, args):
                       "max_rel_field_diff_batch_vs_one_stream": diff,
                       "max_rel_impedance_err_vs_complex128_batch": zerr,
                       "fallback_fp64": [int(s.fallback_fp64) for s in stb],
+                      "multigrid_items": [int(s.mg_freqs) for s in stb],
                       "dtype": "complex128" if args.batch_fp64 else "complex128 Krylov/operator + complex64 preconditioner"}))
 
 

@@ -56,6 +56,12 @@ extern "C" {
 #define SFDX_PRECOND_SSOR 0
 #define SFDX_PRECOND_HYBRID 1  /* SSOR + G (G^T A G)^-1 G^T, G^T A G = -i w mu P */
 #define SFDX_PRECOND_MG 2      /* V-cycle over the EXCMG grid hierarchy, hybrid smoothing */
+#define SFDX_PRECOND_AUTO 3    /* per frequency: the V-cycle where its first coarse grid resolves the
+                                  skin depth of the most conductive earth cell and the skin depth at
+                                  the median earth conductivity is within the mesh's horizontal
+                                  extent, on meshes whose cells are at most 200 times longer than
+                                  wide; the hybrid preconditioner elsewhere (sfdx_stats.mg_freqs
+                                  counts the frequencies that ran the V-cycle) */
 
 #define SFDX_NORM_REFERENCE 0   /* ||b-Ax||/||b||, b incl. Dirichlet rows (solvers.f90:615-625) */
 #define SFDX_NORM_INTERIOR 1    /* interior rows only */
@@ -173,6 +179,7 @@ typedef struct {
     int solver_switches;            /* COCR solves that stagnated (restarts without progress) and
                                        were continued by GPBiCG from the current iterate */
     int spmv_tile;                  /* SpMV tile used on the fine level (complex128 operator) */
+    int mg_freqs;                   /* frequencies whose fine level ran the multigrid V-cycle */
 } sfdx_stats;
 
 /* Context: device selection, stream, workspace. */

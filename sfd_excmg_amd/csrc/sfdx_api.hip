@@ -1365,6 +1365,79 @@ bool mesh_ok(const HostLevel &F, int L)
     return F.nx % f == 0 && F.ny % f == 0 && F.nz % f == 0 && F.nair % (1 << (L - 1)) == 0 && F.nx / f >= 1;
 }
 
+// Grids of the V-cycle hierarchy over a fine grid (the fine grid included):
+// halve while every dimension divides and keeps at least 2 cells, capped by
+// mg_levels when that is set.
+int mg_grid_count(const HostLevel &F, const sfdx_opts *o)
+{
+    int m = 1;
+    while (m < SFDX_MAX_LEVELS && F.nx % (1 << m) == 0 && F.ny % (1 << m) == 0 && F.nz % (1 << m) == 0 &&
+           std::min(std::min(F.nx, F.ny), F.nz) >> m >= 2)
+        m++;
+    return o->mg_levels > 0 ? std::min(o->mg_levels, m) : m;
+}
+
+// SFDX_PRECOND_AUTO: the preconditioner of each frequency's solves.  The
+// V-cycle where
+//  (a) its first coarse grid resolves the skin depth of the most conductive
+//      earth cell, delta(sigma_max, f) >= the smallest cell width of that grid
+//      (twice the fine grid's); below it the rediscretised coarse operators no
+//      longer describe the fine grid's diffusion and the V-cycle stops
+//      converging (on the 128x128x96 block model from ~150 Hz up);
+//  (b) the field is not quasi-static over the model: delta at the median earth
+//      conductivity is at most the larger horizontal extent of the mesh; in
+//      that regime (the 128x128x96 block model at 1-2.5 mHz) the complex64
+//      V-cycle was measured to stall COCR's recurrences between restarts;
+//  (c) no cell is more than kAutoMaxAspect times longer than its shortest
+//      side: the V-cycle's point smoothers do not smooth error that varies
+//      along the long side of a stretched cell; on a 96^3 model with 354:1
+//      cells it took 1.8x the iterations of the hybrid preconditioner, on the
+//      128x128x96 model (133:1) 2.9x fewer;
+// and the one-level hybrid preconditioner elsewhere.  sigma_max and the median
+// are taken over the diagonal tensor components of the cells below the air
+// layers.
+constexpr double kAutoMaxAspect = 200.0;
+struct AutoPrecond {
+    double sig_max = 0, sig_med = 0, extent = 0, h_coarse = 0, aspect = 0;
+    bool usable = false;  // the fine grid coarsens at least once
+    AutoPrecond(const HostLevel &F, int grids)
+    {
+        usable = grids >= 2;
+        // largest cell aspect ratio: a cell takes one width per axis, so it is
+        // the largest width of one axis over the smallest of another
+        const std::vector<double> *w[3] = {&F.a, &F.b, &F.c};
+        for (int p = 0; p < 3; p++)
+            for (int q = 0; q < 3; q++)
+                if (p != q)
+                    aspect = std::max(aspect, *std::max_element(w[p]->begin(), w[p]->end()) /
+                                                  *std::min_element(w[q]->begin(), w[q]->end()));
+        const int64_t nc = F.ncell(), nce = (int64_t)F.nx * F.ny * (F.nz - F.nair);
+        std::vector<double> mean(std::max<int64_t>(nce, 1), 0.0);
+        for (int64_t q = 0; q < nce; q++) {
+            const int64_t ie = (int64_t)F.nx * F.ny * F.nair + q;
+            const double sx = F.sig[ie], sy = F.sig[nc + ie], sz = F.sig[2 * nc + ie];
+            sig_max = std::max(sig_max, std::max(sx, std::max(sy, sz)));
+            mean[q] = (sx + sy + sz) / 3;
+        }
+        std::nth_element(mean.begin(), mean.begin() + mean.size() / 2, mean.end());
+        sig_med = mean[mean.size() / 2];
+        double wa = 0, wb = 0;
+        for (double v : F.a) wa += v;
+        for (double v : F.b) wb += v;
+        extent = std::max(wa, wb);
+        // smallest width of the first coarse grid: sums of fine width pairs
+        h_coarse = 1e300;
+        for (const std::vector<double> *w : {&F.a, &F.b, &F.c})
+            for (size_t s = 0; s + 2 <= w->size(); s += 2) h_coarse = std::min(h_coarse, (*w)[s] + (*w)[s + 1]);
+    }
+    static double skin_depth(double sigma, double f) { return std::sqrt(2.0 / (2 * M_PI * f * 4e-7 * M_PI * sigma)); }
+    bool use_mg(double f) const
+    {
+        return usable && aspect <= kAutoMaxAspect && sig_max > 0 && sig_med > 0 && skin_depth(sig_max, f) >= h_coarse &&
+               skin_depth(sig_med, f) <= extent;
+    }
+};
+
 HostLevel host_level(const sfdx_grid *g, const double *sigma)
 {
     HostLevel L;
@@ -1387,7 +1460,27 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     const HostLevel fine = host_level(g, sigma);
     for (size_t q = 0; q < fine.sig.size(); q++)
         if (!std::isfinite(fine.sig[q])) { g_err = "non-finite sigma"; return SFDX_ERR_NAN; }
-    const bool use_mg = o->precond == SFDX_PRECOND_MG;
+    if (o->precond < SFDX_PRECOND_SSOR || o->precond > SFDX_PRECOND_AUTO) {
+        g_err = "precond must be 0..3";
+        return SFDX_ERR_ARG;
+    }
+    // the V-cycle per frequency (SFDX_PRECOND_MG: all; SFDX_PRECOND_AUTO: by
+    // the skin-depth rule); its grids are built when any frequency uses it
+    const bool auto_pc = o->precond == SFDX_PRECOND_AUTO;
+    std::vector<char> mg_freq(nf, o->precond == SFDX_PRECOND_MG);
+    if (auto_pc) {
+        const AutoPrecond rule(fine, mg_grid_count(fine, o));
+        for (int fi = 0; fi < nf; fi++) {
+            mg_freq[fi] = rule.use_mg(freq[fi]);
+            if (o->verbose)
+                fprintf(stderr, "f=%g: %s (skin depth %.4g m at sigma_max %.4g S/m, first coarse width %.4g m; %.4g m at "
+                                "median sigma %.4g S/m, extent %.4g m; cell aspect %.4g)\n",
+                        freq[fi], mg_freq[fi] ? "multigrid V-cycle" : "hybrid", AutoPrecond::skin_depth(rule.sig_max, freq[fi]),
+                        rule.sig_max, rule.h_coarse, AutoPrecond::skin_depth(rule.sig_med, freq[fi]), rule.sig_med,
+                        rule.extent, rule.aspect);
+        }
+    }
+    const bool use_mg = std::find(mg_freq.begin(), mg_freq.end(), 1) != mg_freq.end();
     const int Lv = o->use_excmg ? o->level : 1;
     if (o->use_excmg && (Lv < 3 || Lv > SFDX_MAX_LEVELS || !mesh_ok(fine, Lv))) {
         g_err = "grid dimensions must be divisible by 2^level and nair by 2^(level-1)";
@@ -1431,14 +1524,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     // coarsest EXCMG level, and EXCMG levels below its coarsest grid are
     // solved with the hybrid preconditioner.
     int Lmg = Lv;
-    if (use_mg) {
-        int m = 1;
-        while (m < SFDX_MAX_LEVELS && fine.nx % (1 << m) == 0 && fine.ny % (1 << m) == 0 && fine.nz % (1 << m) == 0 &&
-               std::min(std::min(fine.nx, fine.ny), fine.nz) >> m >= 2)
-            m++;
-        Lmg = o->mg_levels > 0 ? std::min(o->mg_levels, m) : m;
-        Lmg = std::max(Lmg, 2);
-    }
+    if (use_mg) Lmg = std::max(mg_grid_count(fine, o), 2);
     const int off = Lmg - Lv;
     std::vector<HostLevel> mgh(Lmg + 1);
     std::vector<LevelDev<double>> X64(std::max(off, 0) + 1);
@@ -1577,9 +1663,10 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
             }
             const int maxit = o->use_excmg ? o->maxit_excmg : o->maxit_plain;
             // levels below the multigrid hierarchy's coarsest grid: hybrid
-            const bool mg_here = use_mg && i + off >= 1;
+            const bool mg_here = mg_freq[fi] && i + off >= 1;
             sfdx_opts oi = *o;
-            if (use_mg && !mg_here) oi.precond = SFDX_PRECOND_HYBRID;
+            if ((use_mg || auto_pc) && !mg_here) oi.precond = SFDX_PRECOND_HYBRID;
+            if (auto_pc && mg_here) oi.precond = SFDX_PRECOND_MG;
             LevelDev<float> *d32 = (lo_fine && i == Lv) ? &D32[i] : nullptr;
             MGHier<double> *h64 = mg_here ? &H64 : nullptr;
             MGHier<float> *h32 = (mg_here && lo_fine && i == Lv) ? &H32 : nullptr;
@@ -1633,7 +1720,10 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                     level_secs += secs[pol];
                 }
             }
-            if (i == Lv) S.fine_seconds += level_secs;
+            if (i == Lv) {
+                S.fine_seconds += level_secs;
+                S.mg_freqs += mg_here;
+            }
             for (int pol = 0; pol < 2; pol++) {
                 const LevelSolveResult &R = Rs[pol];
                 if (i == Lv) {
@@ -1720,7 +1810,7 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     KOpts ko = kopts_of(*o);
     ko.corr_intv = 0;
     ko.dc_at_start = 0;
-    if (ko.precond == SFDX_PRECOND_MG) ko.precond = SFDX_PRECOND_HYBRID;
+    if (ko.precond == SFDX_PRECOND_MG || ko.precond == SFDX_PRECOND_AUTO) ko.precond = SFDX_PRECOND_HYBRID;
     if (adj_mixed) {
         ko.lo = &D32;
         ko.restart_drop = o->restart_drop;
@@ -1971,6 +2061,7 @@ int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_
                 S.fine_iters_total += T.fine_iters_total;
                 S.fallback_fp64 += T.fallback_fp64;
                 S.solver_switches += T.solver_switches;
+                S.mg_freqs += T.mg_freqs;
                 S.spmv_tile = T.spmv_tile;
                 if (T.status < 0) S.status = T.status;
                 if (items[k].f == jobs[items[k].j].nf - 1) {

@@ -6,7 +6,8 @@ module sfdx_iface
   integer(c_int), parameter :: SFDX_SOLVER_BICGSTAB = 1, SFDX_SOLVER_GPBICG = 3, SFDX_SOLVER_COCR = 5
   integer(c_int), parameter :: SFDX_PREC_FP64 = 0, SFDX_PREC_MIXED = 1, SFDX_PREC_MIXED_IR = 2, &
                                 SFDX_PREC_MIXED_SPLIT = 3
-  integer(c_int), parameter :: SFDX_PRECOND_SSOR = 0, SFDX_PRECOND_HYBRID = 1, SFDX_PRECOND_MG = 2
+  integer(c_int), parameter :: SFDX_PRECOND_SSOR = 0, SFDX_PRECOND_HYBRID = 1, SFDX_PRECOND_MG = 2, &
+                                SFDX_PRECOND_AUTO = 3
   integer(c_int), parameter :: SFDX_NORM_REFERENCE = 0, SFDX_NORM_INTERIOR = 1
 
   type, bind(C) :: sfdx_grid_t
@@ -36,7 +37,7 @@ module sfdx_iface
     integer(c_int) :: iters(2, 8)
     real(c_double) :: relres(2, 8)
     integer(c_int) :: dc_calls(2, 8), pcg_iters(2, 8), refine_steps(2), fine_iters_total, fallback_fp64, &
-                      solver_switches, spmv_tile
+                      solver_switches, spmv_tile, mg_freqs
   end type
 
   interface

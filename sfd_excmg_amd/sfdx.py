@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(_PKG, "lib", "libsfdx.so")
 SOLVER_BICGSTAB, SOLVER_GPBICG, SOLVER_COCR = 1, 3, 5
 PREC_FP64, PREC_MIXED, PREC_MIXED_IR, PREC_MIXED_SPLIT = 0, 1, 2, 3
 NORM_REFERENCE, NORM_INTERIOR = 0, 1
-PRECOND_SSOR, PRECOND_HYBRID, PRECOND_MG = 0, 1, 2
+PRECOND_SSOR, PRECOND_HYBRID, PRECOND_MG, PRECOND_AUTO = 0, 1, 2, 3
 MAX_LEVELS = 8
 WARN_NOCONV = -1
 
@@ -60,7 +60,7 @@ class Stats(ctypes.Structure):
                 ("relres", (ctypes.c_double * 2) * MAX_LEVELS), ("dc_calls", (ctypes.c_int * 2) * MAX_LEVELS),
                 ("pcg_iters", (ctypes.c_int * 2) * MAX_LEVELS), ("refine_steps", ctypes.c_int * 2),
                 ("fine_iters_total", ctypes.c_int), ("fallback_fp64", ctypes.c_int),
-                ("solver_switches", ctypes.c_int), ("spmv_tile", ctypes.c_int)]
+                ("solver_switches", ctypes.c_int), ("spmv_tile", ctypes.c_int), ("mg_freqs", ctypes.c_int)]
 
     def to_dict(self, levels=MAX_LEVELS):
         return dict(status=self.status, total_seconds=self.total_seconds, fine_seconds=self.fine_seconds,
@@ -71,7 +71,7 @@ class Stats(ctypes.Structure):
                     pcg_iters=[[self.pcg_iters[i][p] for p in range(2)] for i in range(levels)],
                     refine_steps=[self.refine_steps[0], self.refine_steps[1]],
                     fine_iters_total=self.fine_iters_total, fallback_fp64=self.fallback_fp64,
-                    solver_switches=self.solver_switches, spmv_tile=self.spmv_tile)
+                    solver_switches=self.solver_switches, spmv_tile=self.spmv_tile, mg_freqs=self.mg_freqs)
 
 
 class Job(ctypes.Structure):

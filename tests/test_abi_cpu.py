@@ -34,6 +34,32 @@ def test_library_exports_every_declared_symbol():
             assert getattr(lib, n).argtypes is not None, n
 
 
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """Size and every field offset of sfdx_grid, sfdx_opts, sfdx_stats and
+    sfdx_job as gcc lays them out from include/sfdx.h equal the ctypes
+    structures of the binding."""
+    import ctypes
+    import subprocess
+
+    structs = {"sfdx_grid": sfdx.Grid, "sfdx_opts": sfdx.Opts, "sfdx_stats": sfdx.Stats, "sfdx_job": sfdx.Job}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "sfdx.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
 def test_default_options_match_reference_parameters():
     o = sfdx.default_opts()
     assert o.corr_intv == 200 and o.corr_tol == 1e-3 and o.ssor_omega == 1.0   # global_para.f90:9-10, solvers.f90:14

@@ -395,6 +395,36 @@ def test_device_coarse_levels_match_direct_solves(ctx, precision):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
+@pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
+def test_auto_preconditioner_chooses_per_frequency(ctx, precision):
+    """SFDX_PRECOND_AUTO on a 32x32x24 full-tensor block model (4 km wide,
+    first coarse cells 100 m, most conductive cell 0.84 S/m, median 0.01 S/m):
+    the V-cycle at 4 Hz only; at 1 mHz the skin depth at the median
+    conductivity (159 km) exceeds the mesh and at 1 kHz the one in the most
+    conductive cell (17 m) is below the first coarse grid's cells, so both take
+    the hybrid preconditioner.  Every frequency's impedances equal those of a
+    solve with the preconditioner fixed accordingly, to 1e-6."""
+    m = synth.make_aniso_blocks_128(nx=32, nz_earth=20, nair=4, nstations=8)
+    m.freqs = np.array([1e-3, 4.0, 1e3])
+    kw = dict(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR,
+              precision=precision, sigma_floor32=0.0, restart_drop=1e-3)
+    exa, sa = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs,
+                            sfdx.default_opts(precond=sfdx.PRECOND_AUTO, **kw))
+    assert sa.status == 0 and sa.mg_freqs == 1
+    exh, sh = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs,
+                            sfdx.default_opts(precond=sfdx.PRECOND_HYBRID, **kw))
+    exm, sm = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs[1:2],
+                            sfdx.default_opts(precond=sfdx.PRECOND_MG, **kw))
+    assert sh.status == 0 and sh.mg_freqs == 0 and sm.status == 0 and sm.mg_freqs == 1
+    for p, q in zip(_impedances(m, exa), _impedances(m, exh)):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+    # the 4 Hz column also against the solve with the V-cycle fixed
+    m1 = synth.make_aniso_blocks_128(nx=32, nz_earth=20, nair=4, nstations=8)
+    m1.freqs = m.freqs[1:2]
+    for p, q in zip(_impedances(m1, exa[:, 1:2]), _impedances(m1, exm)):
+        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
+
+
 def test_spmv_tiles_and_autotuner_give_the_same_solve(ctx):
     """Every SpMV kernel variant (0: the cell-plane march in complex128, 1..4
     the k-marching tiles) and the autotuner (-1) solve the same system:

@@ -47,8 +47,8 @@ def headline_opts(m, precision, eps, precond=sfdx.PRECOND_MG):
     """The benchmark's solver setup: COCR preconditioned by the V-cycle over the
     3 finest grids with one coarsest smoothing pair, no periodic divergence
     correction, interior residual, mixed-precision restarts after every 1e-5
-    drop (precond=PRECOND_HYBRID: the one-level hybrid preconditioner, restarts
-    every 1e-3)."""
+    drop (precond=PRECOND_HYBRID, the one-level hybrid preconditioner, or
+    PRECOND_AUTO, the choice per frequency: restarts every 1e-3)."""
     drop = 1e-5 if precond == sfdx.PRECOND_MG else 1e-3
     return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0, norm=sfdx.NORM_INTERIOR,
                              precision=precision, sigma_floor32=0.0, restart_drop=drop, maxit_excmg=8000,
@@ -224,21 +224,26 @@ def test_c3_topography_matches_oracle_fixture(ctx, precision):
 # ------------------------------------------------------------------ C4
 def test_c4_batch_of_uneven_models_and_sounding_frequencies(ctx):
     """The 8 uneven isotropic models (1 Hz) and the 128x128x96 model at the
-    first and last of the 16 sounding frequencies, as one mixed-precision
-    batch over 8 streams: every item agrees with its single solve, and the
+    first, sixth and last of the 16 sounding frequencies (1 mHz, 0.1 Hz,
+    1 kHz), as one mixed-precision batch over 8 streams with the preconditioner
+    chosen per item (the V-cycle at 0.1 Hz and for the 1 Hz models but the
+    96^3 one, whose deepest cells are 354 times thicker than wide; the hybrid
+    one at 1 mHz and 1 kHz): every item agrees with its single solve, and the
     48^3 and 64x64x48 models with the oracle."""
     g = np.load(os.path.join(GOLDEN, "batch_oracle_impedances.npz"))
     small = synth.make_uneven_iso_models()
     big = synth.make_aniso_blocks_128()
-    big.freqs = synth.sounding_frequencies_16()[[0, 15]]
+    big.freqs = synth.sounding_frequencies_16()[[0, 5, 15]]
     models = [big] + small
-    hyb = sfdx.PRECOND_HYBRID   # the batch's preconditioner (up to 1 kHz)
+    hyb = sfdx.PRECOND_AUTO   # the batch's preconditioner choice (1 mHz to 1 kHz)
     opts = headline_opts(big, sfdx.PREC_MIXED, 1e-8, hyb)
-    print("C4 batch: 9 models, 10 frequency items", flush=True)
+    print("C4 batch: 9 models, 11 frequency items", flush=True)
     exb, stb = ctx.solve_batch(models, opts, nstreams=8)
     for k, (m, ex, st) in enumerate(zip(models, exb, stb)):
-        print(f"  model {k} {m.nx}x{m.ny}x{m.nz}: status {st.status} fine iterations {st.fine_iters_total}", flush=True)
+        print(f"  model {k} {m.nx}x{m.ny}x{m.nz}: status {st.status} fine iterations {st.fine_iters_total} "
+              f"V-cycle items {st.mg_freqs}", flush=True)
         assert st.status == 0
+        assert st.mg_freqs == (0 if m.name == "iso_96x96x96" else 1)
         # the batch takes each model's own level count; so does the single solve
         ex1, st1 = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, headline_opts(m, sfdx.PREC_MIXED, 1e-8, hyb))
         assert st1.status == 0
