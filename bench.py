@@ -8,12 +8,14 @@ fine-level relative residual of 1e-8 (interior rows).
 A step is one full forward solve (all levels, both polarisations).  `value`
 is the fine-grid solve time (both polarisations) of the faster converged GPU
 path, averaged over the timed steps.  Both paths run COCR preconditioned by a
-geometric multigrid V-cycle over the 3 finest grids (hybrid Gauss-Seidel
-smoothing: lumped-mass multicolour edge SSOR sweeps plus the nodal correction
-by a two-step Jacobi polynomial, residuals in split edge + nodal form):
+geometric multigrid V-cycle over merge-ratio grids (each coarser grid merges
+two cells of an axis only where both are at most 3x its smallest width; up to
+8 grids per level; hybrid Gauss-Seidel smoothing: lumped-mass multicolour edge
+SSOR sweeps plus the nodal correction by a two-step Jacobi polynomial,
+residuals in split edge + nodal form):
   * FP64: everything in complex128;
   * mixed: complex128 COCR, operator and residual; the V-cycle in complex64
-    (the complex128 true residual is recomputed after every 1e-5 drop and on
+    (the complex128 true residual is recomputed after every 1e-3 drop and on
     stagnation, and once more before convergence is accepted).
 The roofline object is for the stencil SpMV (k_spmv_cell, the complex128 operator
 every solver applies), timed with HIP events on the library's stream.  The CPU baseline is the FP64 oracle (CSR, lexicographic
@@ -39,16 +41,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # Mixed precision restarts the FP64 COCR recurrences from the true residual
-# after every drop by this factor.  With the complex64 V-cycle, measured at
-# 128x128x96 (fine-level s, C2 flat / C3 topography): 1e-2: 1.197 / 3.336,
-# 1e-3: 0.796 / 1.032, 1e-4: 0.755 / 1.004, 1e-5: 0.740 / 1.004, 1e-6:
-# 0.763 / 1.291 (the complex64 preconditioner makes the recurrences drift;
-# the second polarisation of C3 stalled for ~300 iterations between restarts).
-# The one-level hybrid preconditioner (the batch) restarts after every 1e-3
-# drop: over the 16 sounding frequencies of the 128x128x96 model the fine
-# levels took 25.2 s (1e-3), 26.6 s (1e-4) and 29.2 s (1e-5, one solve at
-# 398 Hz not converged in 8000 iterations; 1e-6 likewise).
-RESTART_DROP_MG, RESTART_DROP = 1e-5, 1e-3
+# after every drop by this factor (the complex64 preconditioner makes the
+# recurrences drift).  With the merge-ratio V-cycle, measured at 128x128x96
+# (fine-level s, 0.1 Hz): 1e-5 0.153, 1e-4 0.159, 1e-3 0.156.  The one-level
+# hybrid preconditioner over the 16 sounding frequencies of that model: 25.2 s
+# (1e-3), 26.6 s (1e-4), 29.2 s (1e-5, one solve at 398 Hz not converged in
+# 8000 iterations).
+RESTART_DROP_MG, RESTART_DROP = 1e-3, 1e-3
 
 
 def profiled_traffic(kernel):
@@ -78,14 +77,15 @@ USE_GRAPH = 1
 POL_STREAMS = 2
 
 
-# Preconditioner of the headline solves: a multigrid V-cycle over 3 grids
-# (128x128x96, 64x64x48, 32x32x24) with one smoothing pair on the coarsest,
-# hybrid Gauss-Seidel smoothing (lumped-mass edge SSOR sweeps plus the
-# Jacobi-polynomial nodal correction).  Measured at this workload, fine level,
-# FP64 / mixed: 1.167 / 0.836 s (about 420 iterations per polarisation),
-# against 1.36 / 1.23 s with the one-level hybrid preconditioner (about 1240);
-# 4 grids 1.216 / 0.878 s, 2 grids with 2 pairs 1.280 / 0.853 s.
-MG_LEVELS, MG_COARSE_PAIRS = 3, 1
+# Preconditioner of the headline solves: a multigrid V-cycle with hybrid
+# Gauss-Seidel smoothing (lumped-mass edge SSOR sweeps plus the
+# Jacobi-polynomial nodal correction) over merge-ratio grids (two cells of an
+# axis merged only where both are at most 3x the grid's smallest width) down to
+# 8 grids, one smoothing pair on the coarsest.  Measured at this workload, fine
+# level, mixed: 0.156 s (about 65 iterations per polarisation), against 0.61 s
+# (about 390) with 3 grids of 2:1 coarsening and 1.23 s (about 1240) with the
+# one-level hybrid preconditioner.
+MG_LEVELS, MG_COARSE_PAIRS, MG_MERGE_RATIO = 0, 1, 3.0
 
 
 def solve_opts(m, precision, eps, verbose=0, precond=None):
@@ -105,6 +105,7 @@ def solve_opts(m, precision, eps, verbose=0, precond=None):
     return sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=eps, corr_intv=0,
                              norm=sfdx.NORM_INTERIOR, precision=precision, verbose=verbose,
                              precond=precond, mg_levels=MG_LEVELS, mg_coarse_pairs=MG_COARSE_PAIRS,
+                             mg_merge_ratio=MG_MERGE_RATIO,
                              sigma_floor32=0.0, restart_drop=drop, maxit_excmg=8000, use_graph=USE_GRAPH,
                              pol_streams=POL_STREAMS,
                              nodal_smoother=nodal)
@@ -457,8 +458,9 @@ def main():
         "config": {"workload": ("mt3d_fwd_128x128x96_aniso_0.1Hz_2pol_excmg4_cocr" if not args.small
                                 else "mt3d_fwd_64x64x48_aniso_small"), "cells": [m.nx, m.ny, m.nz], "edges": nl,
                    "levels": m.level, "residual": "interior rows, relative 1e-8",
-                   "preconditioner": f"multigrid V-cycle, {MG_LEVELS} grids, {MG_COARSE_PAIRS} coarsest smoothing "
-                                     "pair(s), hybrid Gauss-Seidel smoothing"},
+                   "preconditioner": f"multigrid V-cycle over merge-ratio grids (ratio {MG_MERGE_RATIO:g}, up to "
+                                     f"{MG_LEVELS or 8} grids), {MG_COARSE_PAIRS} coarsest smoothing pair(s), hybrid "
+                                     "Gauss-Seidel smoothing"},
         "fp64_gpu": {"fine_s": round(f64["fine"], 4), "wall_s": round(f64["wall"], 3), "fine_iters": f64["fine_iters"],
                      "status": f64["status"]},
         "mixed_gpu": {"fine_s": round(mx["fine"], 4), "wall_s": round(mx["wall"], 3), "fine_iters": mx["fine_iters"],

@@ -56,12 +56,14 @@ extern "C" {
 #define SFDX_PRECOND_SSOR 0
 #define SFDX_PRECOND_HYBRID 1  /* SSOR + G (G^T A G)^-1 G^T, G^T A G = -i w mu P */
 #define SFDX_PRECOND_MG 2      /* V-cycle over the EXCMG grid hierarchy, hybrid smoothing */
-#define SFDX_PRECOND_AUTO 3    /* per frequency: the V-cycle where its first coarse grid resolves the
-                                  skin depth of the most conductive earth cell and the skin depth at
-                                  the median earth conductivity is within the mesh's horizontal
-                                  extent, on meshes whose cells are at most 200 times longer than
-                                  wide; the hybrid preconditioner elsewhere (sfdx_stats.mg_freqs
-                                  counts the frequencies that ran the V-cycle) */
+#define SFDX_PRECOND_AUTO 3    /* per frequency: with mg_merge_ratio > 0 the V-cycle unless the skin
+                                  depth in the most conductive earth cell is below half the smallest
+                                  cell width; with mg_merge_ratio = 0 the V-cycle only where its first
+                                  coarse grid resolves that skin depth, the skin depth at the median
+                                  earth conductivity is within the mesh's horizontal extent and no
+                                  cell is more than 200 times longer than wide; the hybrid
+                                  preconditioner elsewhere (sfdx_stats.mg_freqs counts the
+                                  frequencies that ran the V-cycle) */
 
 #define SFDX_NORM_REFERENCE 0   /* ||b-Ax||/||b||, b incl. Dirichlet rows (solvers.f90:615-625) */
 #define SFDX_NORM_INTERIOR 1    /* interior rows only */
@@ -157,6 +159,14 @@ typedef struct {
                              level data is shared and only read; results are bit-identical to
                              the sequential solves); 0 or 1: one after the other on the context's
                              stream.  sfdx_solve_batch runs its items with 1 */
+    double mg_merge_ratio; /* grids of the multigrid V-cycle: r >= 1 (default 3) builds them per
+                              Krylov level, merging two consecutive cells of an axis only where
+                              both are at most r times the grid's smallest width (never across
+                              the air-earth interface), so the long cells of stretched padding
+                              stay unmerged until the short ones have grown to their size;
+                              mg_levels then caps the number of grids (0: until no axis merges,
+                              an axis would keep fewer than 2 cells, or 8 grids).  0: the EXCMG
+                              levels and 2:1 coarsenings of the fine grid */
 } sfdx_opts;
 
 typedef struct {

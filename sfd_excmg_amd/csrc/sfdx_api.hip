@@ -221,30 +221,77 @@ void ensure_P(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag)
     launch_pbuild<R>(D.g.d, D.a64, D.b64, D.c64, D.sig64, D.P, ctx->st);
 }
 
-// The grid of a 2^k coarsening (mesh_coarsen, src/fdfe_main.f90:18-84): widths
-// summed on the host, sigma averaged on the device from the fine level's
-// resident FP64 copy (k_mesh_coarsen, the same bits as the host coarsen) and
-// copied back for the host-side users (1-D background, direct solves).
-HostLevel coarsen_device(sfdx_ctx *ctx, const HostLevel &F, const LevelDev<double> &DF, int k, const std::string &tag)
+// Fine node index of each coarse node of a 2^k coarsening of n cells.
+std::vector<int> uniform_starts(int n, int k)
 {
-    const int f = 1 << k;
+    std::vector<int> st((size_t)(n >> k) + 1);
+    for (size_t c = 0; c < st.size(); c++) st[c] = (int)(c << k);
+    return st;
+}
+
+template <typename T>
+T *upload_vec(sfdx_ctx *ctx, const std::string &tag, const std::vector<T> &v)
+{
+    T *d = pool_vec<T>(ctx, tag, v.size());
+    SFDX_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->st));
+    return d;
+}
+
+using Starts = std::array<std::vector<int>, 3>;
+
+// The grid whose cell c along axis a merges the fine cells [st[a][c],
+// st[a][c+1]): widths summed on the host, sigma averaged on the device from
+// the fine grid's resident FP64 copy (k_mesh_coarsen; for a 2^k coarsening,
+// mesh_coarsen of src/fdfe_main.f90:18-84, the same bits as the host coarsen)
+// and copied back for the host-side users (1-D background, direct solves).
+HostLevel coarsen_by_starts(sfdx_ctx *ctx, const HostLevel &F, const LevelDev<double> &DF, const Starts &st,
+                            const std::string &tag)
+{
     HostLevel C;
-    C.nx = F.nx / f; C.ny = F.ny / f; C.nz = F.nz / f; C.nair = F.nair / f;
-    C.a.assign(C.nx, 0.0); C.b.assign(C.ny, 0.0); C.c.assign(C.nz, 0.0);
-    for (int i = 0; i < C.nx; i++) for (int t = 0; t < f; t++) C.a[i] += F.a[i * f + t];
-    for (int i = 0; i < C.ny; i++) for (int t = 0; t < f; t++) C.b[i] += F.b[i * f + t];
-    for (int i = 0; i < C.nz; i++) for (int t = 0; t < f; t++) C.c[i] += F.c[i * f + t];
+    const std::vector<double> *wf[3] = {&F.a, &F.b, &F.c};
+    std::vector<double> *wc[3] = {&C.a, &C.b, &C.c};
+    for (int a = 0; a < 3; a++) {
+        wc[a]->assign(st[a].size() - 1, 0.0);
+        for (size_t c = 0; c + 1 < st[a].size(); c++)
+            for (int q = st[a][c]; q < st[a][c + 1]; q++) (*wc[a])[c] += (*wf[a])[q];
+    }
+    C.nx = (int)C.a.size(); C.ny = (int)C.b.size(); C.nz = (int)C.c.size();
+    C.nair = 0;  // coarse cells wholly in the air
+    while (C.nair < C.nz && st[2][C.nair + 1] <= F.nair) C.nair++;
     const Dims df = make_dims(F.nx, F.ny, F.nz), dc = make_dims(C.nx, C.ny, C.nz);
-    double *ac = pool_vec<double>(ctx, tag + "/ca", C.nx), *bc = pool_vec<double>(ctx, tag + "/cb", C.ny);
-    double *cc = pool_vec<double>(ctx, tag + "/cc", C.nz), *sc = pool_vec<double>(ctx, tag + "/cs", 6 * dc.NC);
-    SFDX_CHECK(hipMemcpyAsync(ac, C.a.data(), C.nx * 8, hipMemcpyHostToDevice, ctx->st));
-    SFDX_CHECK(hipMemcpyAsync(bc, C.b.data(), C.ny * 8, hipMemcpyHostToDevice, ctx->st));
-    SFDX_CHECK(hipMemcpyAsync(cc, C.c.data(), C.nz * 8, hipMemcpyHostToDevice, ctx->st));
-    launch_mesh_coarsen(df, dc, f, DF.a64, DF.b64, DF.c64, ac, bc, cc, DF.sig64, sc, ctx->st);
+    const int *sx = upload_vec(ctx, tag + "/sx", st[0]), *sy = upload_vec(ctx, tag + "/sy", st[1]),
+              *sz = upload_vec(ctx, tag + "/sz", st[2]);
+    const double *ac = upload_vec(ctx, tag + "/ca", C.a), *bc = upload_vec(ctx, tag + "/cb", C.b),
+                 *cc = upload_vec(ctx, tag + "/cc", C.c);
+    double *sc = pool_vec<double>(ctx, tag + "/cs", 6 * dc.NC);
+    launch_mesh_coarsen(df, dc, sx, sy, sz, DF.a64, DF.b64, DF.c64, ac, bc, cc, DF.sig64, sc, ctx->st);
     C.sig.assign(6 * dc.NC, 0.0);
     SFDX_CHECK(hipMemcpyAsync(C.sig.data(), sc, 6 * dc.NC * 8, hipMemcpyDeviceToHost, ctx->st));
     SFDX_CHECK(hipStreamSynchronize(ctx->st));
     return C;
+}
+
+// The grid of a 2^k coarsening (mesh_coarsen, src/fdfe_main.f90:18-84).
+HostLevel coarsen_device(sfdx_ctx *ctx, const HostLevel &F, const LevelDev<double> &DF, int k, const std::string &tag)
+{
+    const Starts st = {uniform_starts(F.nx, k), uniform_starts(F.ny, k), uniform_starts(F.nz, k)};
+    return coarsen_by_starts(ctx, F, DF, st, tag);
+}
+
+// Coarse node starts of one axis of the merge-ratio coarsening: two
+// consecutive cells (scanning from index 0) are merged when both are at most
+// thr wide and node `keep` (the air-earth interface; -1: none) does not lie
+// between them; every other cell stays as it is.
+std::vector<int> merge_starts(const std::vector<double> &w, double thr, int keep)
+{
+    std::vector<int> st{0};
+    const int n = (int)w.size();
+    for (int i = 0; i < n;) {
+        const bool pair = i + 1 < n && w[i] <= thr && w[i + 1] <= thr && i + 1 != keep;
+        i += pair ? 2 : 1;
+        st.push_back(i);
+    }
+    return st;
 }
 
 // Helper context of the second polarisation (its own stream, reduction
@@ -260,6 +307,20 @@ sfdx_ctx *pol_helper(sfdx_ctx *ctx)
     sfdx_ctx *w = ctx->workers[0];
     ensure_partials(w, ctx->rb.max_blocks);
     return w;
+}
+
+// Stagnation window of the restarted COCR with a complex64 preconditioner:
+// iterations without a new best residual after which the recurrences are
+// taken to have stalled.  Two decades at the reduction rate reached since the
+// last restart (it_best - it_restart iterations took the residual from rel0 to
+// best), between 20 and 400: a V-cycle that gains a decade every ~10
+// iterations is restarted after ~20 idle ones, the one-level preconditioner
+// (a decade per ~150 iterations) only after 300-400, as its residual wanders
+// that long between new bests.
+inline double stall_window(double rel0, double best, int it_restart, int it_best)
+{
+    if (!(best < rel0)) return 400;
+    return std::min(400.0, std::max(20.0, 2.0 * (it_best - it_restart) / std::log10(rel0 / best)));
 }
 
 struct SolveInfo {
@@ -327,10 +388,11 @@ int div_correct(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, cx<R> *x,
 // (P_e G_c = G_f P_n) so psi is carried through the hierarchy as nodes.
 template <typename R>
 struct MGHier {
-    std::vector<LevelDev<R> *> lev;           // index = level (1-based), lev[0] unused
-    std::vector<std::array<double *, 3>> wl;  // wl[l]: node weights of level l vs l-1
+    std::vector<LevelDev<R> *> lev;  // index = level (1-based), lev[0] unused
+    std::vector<MGTables> tr;        // tr[l]: transfer tables of level l vs l-1
     int coarse_pairs = 4;
     int nodal = 0;  // nodal part of the smoother: 0 8-colour Gauss-Seidel on P, 1 Jacobi polynomial
+    std::string name = "mg";  // prefix of the work-vector and operator pool tags
 };
 
 // The level data the solvers build on first use (the nodal operators P of the
@@ -346,22 +408,40 @@ void prebuild_level(sfdx_ctx *ctx, LevelDev<double> &D64, LevelDev<float> *D32, 
     }
     if (D32) ensure_P(ctx, *D32, tag);
     for (size_t l = 1; h64 && l < h64->lev.size(); l++)
-        if (h64->lev[l]) ensure_P(ctx, *h64->lev[l], "mg" + std::to_string(l) + "d");
+        if (h64->lev[l]) ensure_P(ctx, *h64->lev[l], h64->name + std::to_string(l) + "d");
     for (size_t l = 1; h32 && l < h32->lev.size(); l++)
-        if (h32->lev[l]) ensure_P(ctx, *h32->lev[l], "mg" + std::to_string(l) + "f");
+        if (h32->lev[l]) ensure_P(ctx, *h32->lev[l], h32->name + std::to_string(l) + "f");
     SFDX_CHECK(hipStreamSynchronize(ctx->st));
 }
 
-std::vector<double> mg_weights_1d(const std::vector<double> &wf, const std::vector<double> &wc)
+// Transfer tables between grid F and the coarser grid C whose node starts are
+// st (MGTables): the coarse node at or below each fine node and its linear
+// weight from the node positions, uploaded under `tag`.
+MGTables mg_tables(sfdx_ctx *ctx, const HostLevel &F, const HostLevel &C, const Starts &st, const std::string &tag)
 {
-    std::vector<double> xf(wf.size() + 1, 0.0), xc(wc.size() + 1, 0.0), w(wf.size() + 1, 1.0);
-    for (size_t i = 0; i < wf.size(); i++) xf[i + 1] = xf[i] + wf[i];
-    for (size_t i = 0; i < wc.size(); i++) xc[i + 1] = xc[i] + wc[i];
-    for (size_t q = 1; q < xf.size(); q += 2) {
-        const size_t lo = q / 2, hi = lo + 1;
-        w[q] = (xc[hi] - xf[q]) / (xc[hi] - xc[lo]);
+    MGTables T;
+    const std::vector<double> *wf[3] = {&F.a, &F.b, &F.c}, *wc[3] = {&C.a, &C.b, &C.c};
+    std::vector<int> lo[3];
+    std::vector<double> wl[3];
+    for (int a = 0; a < 3; a++) {
+        const size_t nf = wf[a]->size(), nc = wc[a]->size();
+        std::vector<double> xf(nf + 1, 0.0), xc(nc + 1, 0.0);
+        for (size_t i = 0; i < nf; i++) xf[i + 1] = xf[i] + (*wf[a])[i];
+        for (size_t i = 0; i < nc; i++) xc[i + 1] = xc[i] + (*wc[a])[i];
+        lo[a].assign(nf + 1, (int)nc);
+        wl[a].assign(nf + 1, 1.0);
+        for (size_t c = 0; c < nc; c++)
+            for (int q = st[a][c]; q < st[a][c + 1]; q++) {
+                lo[a][q] = (int)c;
+                if (q != st[a][c]) wl[a][q] = (xc[c + 1] - xf[q]) / (xc[c + 1] - xc[c]);
+            }
+        const std::string t = tag + "_" + std::to_string(a);
+        T.wl[a] = upload_vec(ctx, t + "w", wl[a]);
+        T.lo[a] = upload_vec(ctx, t + "l", lo[a]);
+        T.st[a] = upload_vec(ctx, t + "s", st[a]);
     }
-    return w;
+    SFDX_CHECK(hipStreamSynchronize(ctx->st));  // the host tables above end here
+    return T;
 }
 
 // (z, psi) ~= A^-1 r on level l (split form, see above)
@@ -370,7 +450,7 @@ void mg_vcycle(sfdx_ctx *ctx, MGHier<R> &H, int l, R wmu, const cx<R> *r, cx<R> 
 {
     LevelDev<R> &D = *H.lev[l];
     const Dims &d = D.g.d;
-    const std::string tg = "mg" + std::to_string(l) + (sizeof(R) == 8 ? "d" : "f");
+    const std::string tg = H.name + std::to_string(l) + (sizeof(R) == 8 ? "d" : "f");
     cx<R> *t = pool_vec<cx<R>>(ctx, tg + "t", d.NL), *w = pool_vec<cx<R>>(ctx, tg + "w", d.NL);
     cx<R> *tn = pool_vec<cx<R>>(ctx, tg + "tn", d.NP), *ph = pool_vec<cx<R>>(ctx, tg + "ph", d.NP);
     ensure_P(ctx, D, tg);
@@ -391,14 +471,14 @@ void mg_vcycle(sfdx_ctx *ctx, MGHier<R> &H, int l, R wmu, const cx<R> *r, cx<R> 
         return;
     }
     LevelDev<R> &C = *H.lev[l - 1];
-    const std::string tc = "mg" + std::to_string(l - 1) + (sizeof(R) == 8 ? "d" : "f");
+    const std::string tc = H.name + std::to_string(l - 1) + (sizeof(R) == 8 ? "d" : "f");
     cx<R> *rc = pool_vec<cx<R>>(ctx, tc + "r", C.g.d.NL), *zc = pool_vec<cx<R>>(ctx, tc + "z", C.g.d.NL);
     cx<R> *pc = pool_vec<cx<R>>(ctx, tc + "p", C.g.d.NP);
     launch_resid_split<R>(D.g, wmu, z, psi, r, t, st);
-    launch_restrict<R>(d, C.g.d, H.wl[l].data(), t, rc, st);
+    launch_restrict<R>(d, C.g.d, H.tr[l], t, rc, st);
     mg_vcycle<R>(ctx, H, l - 1, wmu, rc, zc, pc);
-    launch_prolong_add<R>(d, C.g.d, H.wl[l].data(), zc, z, st);
-    launch_prolong_node_add<R>(d, C.g.d, H.wl[l].data(), pc, psi, st);
+    launch_prolong_add<R>(d, C.g.d, H.tr[l], zc, z, st);
+    launch_prolong_node_add<R>(d, C.g.d, H.tr[l], pc, psi, st);
     smooth_add(1);
 }
 
@@ -578,18 +658,21 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
     StreamGraph graph;
     // Safeguards: restart from the true residual on a breakdown (non-finite
     // scalars; the update of x is skipped on the device), on stagnation (no new
-    // best residual for 200 iterations, 400 with the complex64 preconditioner:
-    // the recurrence residual has reached its attainable accuracy or the
-    // recurrences drift; with complex64 the residual wanders longer between new
-    // bests at high frequencies, and a 200-iteration window restarted it up to
-    // 12 times per solve on the 128x128x96 model above 1 Hz) and, with the
+    // best residual for 200 iterations; with the complex64 preconditioner for
+    // stall_window's count, 20-400 by the convergence rate: the recurrence
+    // residual has reached its attainable accuracy or the recurrences drift;
+    // with the one-level complex64 preconditioner the residual wanders for a
+    // few hundred iterations between new bests at high frequencies, and a
+    // 200-iteration window restarted it up to 12 times per solve on the
+    // 128x128x96 model above 1 Hz, while a merge-ratio V-cycle's recurrences
+    // froze at 6.5e-8 after ~100 iterations of a solve that takes ~70) and, with the
     // complex64 preconditioner (restart_drop > 0), on divergence (residual 100x
     // above the best since the last restart).  After 3 such restarts in a row that did not
     // halve the best residual, the complex64 preconditioner is replaced by the
     // complex128 one (a fixed linear operator); without it the solve stops with
     // SFDX_WARN_NOCONV and the last iterate.
     double best = 1e300, best_at_restart = 1e300;
-    int since_best = 0, idle_restarts = 0;
+    int since_best = 0, idle_restarts = 0, it_restart = 0, it_best = 0;
     const bool guard = ko.restart_drop > 0;
     while (true) {
         if (state != 2) {
@@ -619,10 +702,10 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
         info.relres = std::sqrt(cur[SL_RN]) / bnorm;
         const bool bad = !(info.relres == info.relres) || std::isinf(info.relres);
         const bool fresh_now = fresh;  // this residual was recomputed from x
-        if (fresh) { rel0 = info.relres; fresh = false; best = info.relres; since_best = 0; }
-        if (info.relres < best) { best = info.relres; since_best = 0; }
+        if (fresh) { rel0 = info.relres; fresh = false; best = info.relres; since_best = 0; it_restart = it_best = info.iters; }
+        if (info.relres < best) { best = info.relres; since_best = 0; it_best = info.iters; }
         const bool diverging = guard && info.relres > 100 * best;
-        const bool stalled = since_best > (guard ? 400 : 200);
+        const bool stalled = since_best > (guard ? stall_window(rel0, best, it_restart, it_best) : 200);
         const bool restart = (ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0) || diverging || stalled;
         if (bad || info.relres < eps || info.iters >= maxit || cur[SL_BREAK] != 0 || restart) {
             if (pending) {  // decide on the newest iteration instead
@@ -650,6 +733,10 @@ SolveInfo cocr(sfdx_ctx *ctx, LevelDev<R> &D, const std::string &tag, R wmu, con
                 break;
             }
             const bool planned = !bad && cur[SL_BREAK] == 0 && !stalled && !diverging;
+            if (ko.verbose > 1)
+                fprintf(stderr, "  cocr restart at it %d relres %.3e best %.3e |b| %.3e (%s)\n", info.iters,
+                        info.relres, best, bnorm,
+                        planned ? "planned" : stalled ? "stalled" : diverging ? "diverging" : "breakdown");
             if (!planned) {
                 // no progress since the last restart: count it
                 idle_restarts = (best < 0.5 * best_at_restart) ? 0 : idle_restarts + 1;
@@ -816,7 +903,7 @@ SolveInfo cocr_split(sfdx_ctx *ctx, LevelDev<double> &D, LevelDev<float> &F, MGH
     int pend_slot = 0;
     StreamGraph graph;
     double best = 1e300, best_at_restart = 1e300;
-    int since_best = 0, idle_restarts = 0;
+    int since_best = 0, idle_restarts = 0, it_restart = 0, it_best = 0;
     while (true) {
         if (state != 2) {
             fresh = true;
@@ -839,10 +926,10 @@ SolveInfo cocr_split(sfdx_ctx *ctx, LevelDev<double> &D, LevelDev<float> &F, MGH
         info.relres = std::sqrt(cur[SL_RN]) / bnorm;
         const bool bad = !(info.relres == info.relres) || std::isinf(info.relres);
         const bool fresh_now = fresh;
-        if (fresh) { rel0 = info.relres; fresh = false; best = info.relres; since_best = 0; }
-        if (info.relres < best) { best = info.relres; since_best = 0; }
+        if (fresh) { rel0 = info.relres; fresh = false; best = info.relres; since_best = 0; it_restart = it_best = info.iters; }
+        if (info.relres < best) { best = info.relres; since_best = 0; it_best = info.iters; }
         const bool diverging = info.relres > 100 * best;
-        const bool stalled = since_best > 400;
+        const bool stalled = since_best > stall_window(rel0, best, it_restart, it_best);
         const bool restart = (ko.restart_drop > 0 && info.relres < ko.restart_drop * rel0) || diverging || stalled;
         if (bad || info.relres < eps || info.iters >= maxit || cur[SL_BREAK] != 0 || restart) {
             if (pending) {  // decide on the newest iteration instead
@@ -1365,6 +1452,65 @@ bool mesh_ok(const HostLevel &F, int L)
     return F.nx % f == 0 && F.ny % f == 0 && F.nz % f == 0 && F.nair % (1 << (L - 1)) == 0 && F.nx / f >= 1;
 }
 
+// V-cycle grids of the merge-ratio coarsening over one level (mg_merge_ratio
+// > 0): grid L is the level itself (g[0]); each coarser grid merges two
+// consecutive cells of an axis only where both are at most mg_merge_ratio
+// times the smallest width of the grid above it, and never across the
+// air-earth interface, until no axis merges, an axis would keep fewer than 2
+// cells, or mg_levels grids exist (0: 8).  On stretched meshes the long cells
+// of the padding stay unmerged until the short ones have grown to their size,
+// so every coarse grid represents the error that point smoothing leaves along
+// their long sides.
+struct MergeHier {
+    std::vector<HostLevel> g;                                     // g[n]: n coarsenings below the level
+    std::vector<Starts> st;                                       // st[n]: starts of g[n + 1] over g[n]
+    std::vector<LevelDev<double>> X64 = std::vector<LevelDev<double>>(SFDX_MAX_LEVELS + 1);  // n >= 1
+    std::vector<LevelDev<float>> X32 = std::vector<LevelDev<float>>(SFDX_MAX_LEVELS + 1);
+    MGHier<double> H64;
+    MGHier<float> H32;
+    int L = 0;
+
+    // D32: the level's complex64 copy when its V-cycle runs in complex64
+    void build(sfdx_ctx *ctx, const HostLevel &top, LevelDev<double> &D64, LevelDev<float> *D32, const sfdx_opts *o,
+               const std::string &tag)
+    {
+        g.assign(1, top);
+        const int cap = o->mg_levels > 0 ? std::min(o->mg_levels, SFDX_MAX_LEVELS) : SFDX_MAX_LEVELS;
+        while ((int)g.size() < cap) {
+            const HostLevel &G = g.back();
+            double hmin = 1e300;
+            for (const std::vector<double> *w : {&G.a, &G.b, &G.c})
+                hmin = std::min(hmin, *std::min_element(w->begin(), w->end()));
+            const double thr = o->mg_merge_ratio * hmin;
+            Starts s = {merge_starts(G.a, thr, -1), merge_starts(G.b, thr, -1), merge_starts(G.c, thr, G.nair)};
+            if (s[0].size() == G.a.size() + 1 && s[1].size() == G.b.size() + 1 && s[2].size() == G.c.size() + 1) break;
+            if (s[0].size() < 3 || s[1].size() < 3 || s[2].size() < 3) break;
+            const int n = (int)g.size();
+            const std::string t = tag + "_" + std::to_string(n);
+            HostLevel C = coarsen_by_starts(ctx, G, n == 1 ? D64 : X64[n - 1], s, "crs" + t);
+            g.push_back(std::move(C));
+            st.push_back(s);
+            X64[n] = upload_level<double>(ctx, g.back(), t);
+            if (D32) X32[n] = upload_level<float>(ctx, g.back(), t, o->sigma_floor32);
+        }
+        L = (int)g.size();
+        H64.name = H32.name = "fm" + tag + "_";
+        H64.lev.assign(L + 1, nullptr);
+        H32.lev.assign(L + 1, nullptr);
+        H64.tr.assign(L + 1, MGTables{});
+        H64.coarse_pairs = H32.coarse_pairs = o->mg_coarse_pairs > 0 ? o->mg_coarse_pairs : 4;
+        H64.nodal = H32.nodal = o->nodal_smoother;
+        for (int n = 0; n < L; n++) {
+            H64.lev[L - n] = n == 0 ? &D64 : &X64[n];
+            if (D32) H32.lev[L - n] = n == 0 ? D32 : &X32[n];
+            if (n + 1 < L) H64.tr[L - n] = mg_tables(ctx, g[n], g[n + 1], st[n], "fmt" + tag + "_" + std::to_string(n));
+            if (o->verbose)
+                fprintf(stderr, "%s merge-ratio grid %d: %dx%dx%d\n", tag.c_str(), L - n, g[n].nx, g[n].ny, g[n].nz);
+        }
+        H32.tr = H64.tr;
+    }
+};
+
 // Grids of the V-cycle hierarchy over a fine grid (the fine grid included):
 // halve while every dimension divides and keeps at least 2 cells, capped by
 // mg_levels when that is set.
@@ -1377,32 +1523,40 @@ int mg_grid_count(const HostLevel &F, const sfdx_opts *o)
     return o->mg_levels > 0 ? std::min(o->mg_levels, m) : m;
 }
 
-// SFDX_PRECOND_AUTO: the preconditioner of each frequency's solves.  The
-// V-cycle where
-//  (a) its first coarse grid resolves the skin depth of the most conductive
-//      earth cell, delta(sigma_max, f) >= the smallest cell width of that grid
-//      (twice the fine grid's); below it the rediscretised coarse operators no
-//      longer describe the fine grid's diffusion and the V-cycle stops
-//      converging (on the 128x128x96 block model from ~150 Hz up);
+// SFDX_PRECOND_AUTO: the preconditioner of each frequency's solves.
+// With the merge-ratio V-cycle (mg_merge_ratio > 0) the V-cycle unless the
+// skin depth in the most conductive earth cell is below half the fine grid's
+// smallest width (the fine grid itself then under-resolves the field; on the
+// 128x128x96 block model at 1 kHz, 17 m against 25 m, the hybrid
+// preconditioner was 1.4x faster; from 1 mHz to 398 Hz the V-cycle was
+// 1.9-6.7x faster).  With the 2:1 V-cycle (mg_merge_ratio = 0) the V-cycle
+// only where
+//  (a) its first coarse grid resolves that skin depth, delta(sigma_max, f) >=
+//      the smallest cell width of that grid (twice the fine grid's); below it
+//      the rediscretised coarse operators no longer describe the fine grid's
+//      diffusion and the V-cycle stops converging (on the 128x128x96 block
+//      model from ~150 Hz up);
 //  (b) the field is not quasi-static over the model: delta at the median earth
 //      conductivity is at most the larger horizontal extent of the mesh; in
 //      that regime (the 128x128x96 block model at 1-2.5 mHz) the complex64
 //      V-cycle was measured to stall COCR's recurrences between restarts;
 //  (c) no cell is more than kAutoMaxAspect times longer than its shortest
-//      side: the V-cycle's point smoothers do not smooth error that varies
-//      along the long side of a stretched cell; on a 96^3 model with 354:1
-//      cells it took 1.8x the iterations of the hybrid preconditioner, on the
-//      128x128x96 model (133:1) 2.9x fewer;
+//      side: the 2:1 grids then cannot represent the error that point
+//      smoothing leaves along the long side of a stretched cell; on a 96^3
+//      model with 354:1 cells it took 1.8x the iterations of the hybrid
+//      preconditioner, on the 128x128x96 model (133:1) 2.9x fewer;
 // and the one-level hybrid preconditioner elsewhere.  sigma_max and the median
 // are taken over the diagonal tensor components of the cells below the air
 // layers.
 constexpr double kAutoMaxAspect = 200.0;
 struct AutoPrecond {
-    double sig_max = 0, sig_med = 0, extent = 0, h_coarse = 0, aspect = 0;
+    double sig_max = 0, sig_med = 0, extent = 0, h_min = 0, h_coarse = 0, aspect = 0;
     bool usable = false;  // the fine grid coarsens at least once
-    AutoPrecond(const HostLevel &F, int grids)
+    bool merge = false;   // merge-ratio grids
+    AutoPrecond(const HostLevel &F, int grids, bool merge_grids)
     {
         usable = grids >= 2;
+        merge = merge_grids;
         // largest cell aspect ratio: a cell takes one width per axis, so it is
         // the largest width of one axis over the smallest of another
         const std::vector<double> *w[3] = {&F.a, &F.b, &F.c};
@@ -1425,15 +1579,19 @@ struct AutoPrecond {
         for (double v : F.a) wa += v;
         for (double v : F.b) wb += v;
         extent = std::max(wa, wb);
-        // smallest width of the first coarse grid: sums of fine width pairs
-        h_coarse = 1e300;
-        for (const std::vector<double> *w : {&F.a, &F.b, &F.c})
-            for (size_t s = 0; s + 2 <= w->size(); s += 2) h_coarse = std::min(h_coarse, (*w)[s] + (*w)[s + 1]);
+        // smallest width of the fine grid and of its first 2:1 coarse grid
+        h_min = h_coarse = 1e300;
+        for (const std::vector<double> *v : {&F.a, &F.b, &F.c}) {
+            for (size_t s = 0; s < v->size(); s++) h_min = std::min(h_min, (*v)[s]);
+            for (size_t s = 0; s + 2 <= v->size(); s += 2) h_coarse = std::min(h_coarse, (*v)[s] + (*v)[s + 1]);
+        }
     }
     static double skin_depth(double sigma, double f) { return std::sqrt(2.0 / (2 * M_PI * f * 4e-7 * M_PI * sigma)); }
     bool use_mg(double f) const
     {
-        return usable && aspect <= kAutoMaxAspect && sig_max > 0 && sig_med > 0 && skin_depth(sig_max, f) >= h_coarse &&
+        if (!usable || !(sig_max > 0)) return false;
+        if (merge) return skin_depth(sig_max, f) >= 0.5 * h_min;
+        return aspect <= kAutoMaxAspect && sig_med > 0 && skin_depth(sig_max, f) >= h_coarse &&
                skin_depth(sig_med, f) <= extent;
     }
 };
@@ -1469,15 +1627,15 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     const bool auto_pc = o->precond == SFDX_PRECOND_AUTO;
     std::vector<char> mg_freq(nf, o->precond == SFDX_PRECOND_MG);
     if (auto_pc) {
-        const AutoPrecond rule(fine, mg_grid_count(fine, o));
+        const AutoPrecond rule(fine, mg_grid_count(fine, o), o->mg_merge_ratio > 0);
         for (int fi = 0; fi < nf; fi++) {
             mg_freq[fi] = rule.use_mg(freq[fi]);
             if (o->verbose)
-                fprintf(stderr, "f=%g: %s (skin depth %.4g m at sigma_max %.4g S/m, first coarse width %.4g m; %.4g m at "
-                                "median sigma %.4g S/m, extent %.4g m; cell aspect %.4g)\n",
+                fprintf(stderr, "f=%g: %s (skin depth %.4g m at sigma_max %.4g S/m, smallest width %.4g m, first coarse "
+                                "width %.4g m; %.4g m at median sigma %.4g S/m, extent %.4g m; cell aspect %.4g)\n",
                         freq[fi], mg_freq[fi] ? "multigrid V-cycle" : "hybrid", AutoPrecond::skin_depth(rule.sig_max, freq[fi]),
-                        rule.sig_max, rule.h_coarse, AutoPrecond::skin_depth(rule.sig_med, freq[fi]), rule.sig_med,
-                        rule.extent, rule.aspect);
+                        rule.sig_max, rule.h_min, rule.h_coarse, AutoPrecond::skin_depth(rule.sig_med, freq[fi]),
+                        rule.sig_med, rule.extent, rule.aspect);
         }
     }
     const bool use_mg = std::find(mg_freq.begin(), mg_freq.end(), 1) != mg_freq.end();
@@ -1523,18 +1681,23 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     // fewer grids than EXCMG levels (off < 0) the V-cycle stops above the
     // coarsest EXCMG level, and EXCMG levels below its coarsest grid are
     // solved with the hybrid preconditioner.
+    const bool flex = use_mg && o->mg_merge_ratio > 0;
+    if (flex && o->mg_merge_ratio < 1) {
+        g_err = "mg_merge_ratio must be 0 or >= 1";
+        return SFDX_ERR_ARG;
+    }
     int Lmg = Lv;
-    if (use_mg) Lmg = std::max(mg_grid_count(fine, o), 2);
+    if (use_mg && !flex) Lmg = std::max(mg_grid_count(fine, o), 2);
     const int off = Lmg - Lv;
     std::vector<HostLevel> mgh(Lmg + 1);
     std::vector<LevelDev<double>> X64(std::max(off, 0) + 1);
     std::vector<LevelDev<float>> X32(std::max(off, 0) + 1);
     MGHier<double> H64;
     MGHier<float> H32;
-    if (use_mg) {
+    if (use_mg && !flex) {
         H64.lev.assign(Lmg + 1, nullptr);
         H32.lev.assign(Lmg + 1, nullptr);
-        H64.wl.assign(Lmg + 1, {nullptr, nullptr, nullptr});
+        H64.tr.assign(Lmg + 1, MGTables{});
         H64.coarse_pairs = H32.coarse_pairs = o->mg_coarse_pairs > 0 ? o->mg_coarse_pairs : 4;
         H64.nodal = H32.nodal = o->nodal_smoother;
         for (int j = 1; j <= Lmg; j++) {
@@ -1552,22 +1715,19 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 }
             }
         }
-        // the host weight tables stay alive until the stream has drained
-        std::vector<std::vector<double>> wtabs;
-        wtabs.reserve((size_t)3 * Lmg);
         for (int j = 2; j <= Lmg; j++) {
-            const std::vector<double> *wf[3] = {&mgh[j].a, &mgh[j].b, &mgh[j].c};
-            const std::vector<double> *wc[3] = {&mgh[j - 1].a, &mgh[j - 1].b, &mgh[j - 1].c};
-            for (int q = 0; q < 3; q++) {
-                wtabs.push_back(mg_weights_1d(*wf[q], *wc[q]));
-                const std::vector<double> &w = wtabs.back();
-                double *dv = pool_vec<double>(ctx, "mgwl" + std::to_string(j) + "_" + std::to_string(q), w.size());
-                SFDX_CHECK(hipMemcpyAsync(dv, w.data(), w.size() * 8, hipMemcpyHostToDevice, ctx->st));
-                H64.wl[j][q] = dv;
-            }
+            const Starts st = {uniform_starts(mgh[j].nx, 1), uniform_starts(mgh[j].ny, 1), uniform_starts(mgh[j].nz, 1)};
+            H64.tr[j] = mg_tables(ctx, mgh[j], mgh[j - 1], st, "mgt" + std::to_string(j));
         }
-        H32.wl = H64.wl;
-        SFDX_CHECK(hipStreamSynchronize(ctx->st));
+        H32.tr = H64.tr;
+    }
+    // With mg_merge_ratio > 0 every Krylov level runs the V-cycle over grids of
+    // its own (MergeHier): coarsenings that merge cells only where they are short.
+    std::vector<MergeHier> MH(flex ? Lv + 1 : 0);
+    for (int i = flex ? first_dev : Lv + 1; i <= Lv; i++) {
+        const bool lo_i = lo_fine && i == Lv;
+        // a level that merges no cells gets a one-grid V-cycle (the smoother)
+        MH[i].build(ctx, lev[i], D64[i], lo_i ? &D32[i] : nullptr, o, "F" + std::to_string(i));
     }
     ensure_partials(ctx, edge_launch(make_dims(fine.nx, fine.ny, fine.nz)).nblocks() + 64);
     for (int i = first_up; i <= Lv; i++) {
@@ -1586,6 +1746,11 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         X64[j].g.ssor_xmass = xmass;
         X32[j].g.ssor_xmass = xmass;
     }
+    for (MergeHier &M : MH)
+        for (int n = 1; n < M.L; n++) {
+            M.X64[n].g.ssor_xmass = xmass;
+            M.X32[n].g.ssor_xmass = xmass;
+        }
     S.spmv_tile = D64[Lv].g.spmv_variant;
     const int64_t NLf = make_dims(fine.nx, fine.ny, fine.nz).NL;
     // surface node plane per column for the device post-process (topography
@@ -1605,6 +1770,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
         SFDX_CHECK(hipStreamSynchronize(ctx->st));
     }
     int worst = 0;
+    if (o->verbose) fprintf(stderr, "set-up (levels, grids, tables) %.3fs\n", total.s());
     for (int fi = 0; fi < nf; fi++) {
         const double ff = freq[fi];
         // device solution history: u[pol][level]
@@ -1663,19 +1829,20 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
             }
             const int maxit = o->use_excmg ? o->maxit_excmg : o->maxit_plain;
             // levels below the multigrid hierarchy's coarsest grid: hybrid
-            const bool mg_here = mg_freq[fi] && i + off >= 1;
+            const bool mg_here = mg_freq[fi] && (flex || i + off >= 1);
             sfdx_opts oi = *o;
             if ((use_mg || auto_pc) && !mg_here) oi.precond = SFDX_PRECOND_HYBRID;
             if (auto_pc && mg_here) oi.precond = SFDX_PRECOND_MG;
             LevelDev<float> *d32 = (lo_fine && i == Lv) ? &D32[i] : nullptr;
-            MGHier<double> *h64 = mg_here ? &H64 : nullptr;
-            MGHier<float> *h32 = (mg_here && lo_fine && i == Lv) ? &H32 : nullptr;
+            MGHier<double> *h64 = mg_here ? (flex ? &MH[i].H64 : &H64) : nullptr;
+            MGHier<float> *h32 = (mg_here && lo_fine && i == Lv) ? (flex ? &MH[i].H32 : &H32) : nullptr;
+            const int mgl = flex ? MH[i].L : i + off;
             const bool two = o->pol_streams == 2;
             if (two) prebuild_level(ctx, D64[i], d32, h64, h32, tag, lo_fine && i == Lv);
             SFDX_CHECK(hipStreamSynchronize(ctx->st));
             auto solve_pol = [&](sfdx_ctx *c, int pol) {
                 LevelSolveResult R = solve_level(c, oi, D64[i], d32, tag, ff, edc, pol, xs[pol], tol_i, maxit, h64, h32,
-                                                 i + off);
+                                                 mgl);
                 SFDX_CHECK(hipStreamSynchronize(c->st));
                 return R;
             };
@@ -1724,6 +1891,7 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 S.fine_seconds += level_secs;
                 S.mg_freqs += mg_here;
             }
+            if (o->verbose) fprintf(stderr, "level %d done at %.3fs\n", i, total.s());
             for (int pol = 0; pol < 2; pol++) {
                 const LevelSolveResult &R = Rs[pol];
                 if (i == Lv) {
@@ -1937,6 +2105,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->surface_sigma = 0;
     o->ssor_cross_mass = 0;
     o->pol_streams = 2;
+    o->mg_merge_ratio = 3;
 }
 
 int sfdx_create(int device, sfdx_ctx **out)

@@ -220,11 +220,13 @@ void launch_hybrid(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, cx<R
 template <typename R>
 void launch_cocr_upd_grad(const DevGrid<R> &g, RedBuf &rb, cx<R> *x, cx<R> *pv, const cx<R> *Ap, cx<R> *r, cx<R> *z,
                           const cx<R> *zs, const cx<R> *phi, R wmu, hipStream_t st);
-// sigma of the 2^k-coarsened grid (f = 2^k) from the fine sigma, all on the
-// device; ac, bc, cc are the coarse widths
-void launch_mesh_coarsen(const Dims &df, const Dims &dc, int f, const double *a, const double *b, const double *c,
-                         const double *ac, const double *bc, const double *cc, const double *sig, double *sigc,
-                         hipStream_t st);
+// sigma of a coarsened grid from the fine sigma, all on the device: coarse
+// cell (I, J, K) is the volume-weighted mean of the fine cells
+// [sx[I], sx[I+1]) x [sy[J], sy[J+1]) x [sz[K], sz[K+1]); ac, bc, cc are the
+// coarse widths (the sums of the fine widths of each range)
+void launch_mesh_coarsen(const Dims &df, const Dims &dc, const int *sx, const int *sy, const int *sz, const double *a,
+                         const double *b, const double *c, const double *ac, const double *bc, const double *cc,
+                         const double *sig, double *sigc, hipStream_t st);
 // y = A_s (z + (i/wmu) G psi), with the partial sums of u^T y (u = z + (i/wmu) G psi);
 // returns the number of partial sums
 int64_t launch_split_product(const DevGrid<float> &g, float wmu, const zf *z, const zf *psi, zf *y, RedBuf &rb,
@@ -264,8 +266,19 @@ void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, c
 template <typename R>
 void launch_resid_split(const DevGrid<R> &g, R wmu, const cx<R> *z, const cx<R> *psi, const cx<R> *r, cx<R> *t,
                         hipStream_t st);
+// Transfers between a grid and a coarser one whose cells each merge one or two
+// consecutive fine cells along every axis a: st[a][c] is the fine node index
+// of coarse node c (nc + 1 entries), lo[a][q] the coarse node at or below fine
+// node q and wl[a][q] the linear weight of that coarse node (1 where fine node
+// q is a coarse node), nf + 1 entries each.  Standard 2:1 coarsening is
+// st[c] = 2c.
+struct MGTables {
+    const double *wl[3];
+    const int *lo[3];
+    const int *st[3];
+};
 template <typename R>
-void launch_prolong_node_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *pc, cx<R> *pf,
+void launch_prolong_node_add(const Dims &df, const Dims &dc, const MGTables &T, const cx<R> *pc, cx<R> *pf,
                              hipStream_t st);
 // z += s G phi; with dotw the partial sums of dotw^T (z + s G phi), and with
 // store false z is left unchanged (only the dot)
@@ -276,10 +289,9 @@ void launch_grad_add(const DevGrid<R> &g, cx<R> s, const cx<R> *phi, cx<R> *z, h
 void launch_hyb_combine(const DevGrid<double> &g64, zd s, const zf *phi, const zf *z32, zd *z64, hipStream_t st,
                         const zd *dotw = nullptr, double *partials = nullptr, bool store = true);
 template <typename R>
-void launch_restrict(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *rf, cx<R> *rc,
-                     hipStream_t st);
+void launch_restrict(const Dims &df, const Dims &dc, const MGTables &T, const cx<R> *rf, cx<R> *rc, hipStream_t st);
 template <typename R>
-void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *zc, cx<R> *zf,
+void launch_prolong_add(const Dims &df, const Dims &dc, const MGTables &T, const cx<R> *zc, cx<R> *zf,
                         hipStream_t st);
 // GPBiCG work vectors (pgpbicg, src/solvers.f90:137-315)
 template <typename R>

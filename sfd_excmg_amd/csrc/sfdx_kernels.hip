@@ -1938,22 +1938,16 @@ void launch_cocr_split_upd(const DevGrid<double> &g, RedBuf &rb, zd *x, zd *r, c
 }
 
 // ------------------------------------------------------------------ K8 multigrid transfers
-// Edge prolongation between nested grids (fine = 2 x coarse in every
-// direction): the tangential field is constant along a coarse edge and linear
-// across it, so a fine edge takes the coarse edge containing it (along) with
-// linear weights over the two bracketing coarse nodes in each transverse
-// direction.  wl[d][q] is the weight of the lower coarse node for fine node q
-// of dimension d (1 for q even).  Restriction is the exact transpose.
-struct MGTables {
-    const double *wl[3];
-};
+// Edge prolongation between nested grids (each coarse cell merges one or two
+// fine cells along every axis, MGTables): the tangential field is constant
+// along a coarse edge and linear across it, so a fine edge takes the coarse
+// edge containing it (along) with linear weights over the one or two
+// bracketing coarse nodes in each transverse direction.  Restriction is the
+// exact transpose.
 
-__device__ __forceinline__ void node_split(const double *wl, int q, int &lo, int &hi, double &w)
-{
-    lo = q >> 1;
-    if (q & 1) { hi = lo + 1; w = wl[q]; }
-    else { hi = lo; w = 1.0; }
-}
+// fine node q of axis a lies strictly inside a coarse cell (two bracketing
+// coarse nodes) rather than on a coarse node
+__device__ __forceinline__ bool node_between(const MGTables &T, int a, int q) { return T.st[a][T.lo[a][q]] != q; }
 
 // Layout of one edge orientation seen along the edge: edge (a along o; q1, q2
 // across, in the dimensions t1 < t2 other than o) has index
@@ -1997,10 +1991,10 @@ __global__ __launch_bounds__(TPB) void k_prolong_add(Dims df, Dims dc, MGTables 
     const int ia = (o == 0) ? i : (o == 1) ? j : k;
     const int q1 = (o == 0) ? j : i, q2 = (o == 2) ? j : k;
     const EdgeAxes C = edge_axes(dc, o);
-    const int l1 = q1 >> 1, l2 = q2 >> 1;
-    const bool h1 = q1 & 1, h2 = q2 & 1;
+    const int l1 = T.lo[t1][q1], l2 = T.lo[t2][q2];
+    const bool h1 = node_between(T, t1, q1), h2 = node_between(T, t2, q2);
     const double w1 = h1 ? T.wl[t1][q1] : 1.0, w2 = h2 ? T.wl[t2][q2] : 1.0;
-    const int64_t c00 = C.base + (int64_t)(ia >> 1) * C.sa + (int64_t)l1 * C.s1 + (int64_t)l2 * C.s2;
+    const int64_t c00 = C.base + (int64_t)T.lo[o][ia] * C.sa + (int64_t)l1 * C.s1 + (int64_t)l2 * C.s2;
     double sre = 0, sim = 0;
     auto term = [&](int c1, int c2, double w, int64_t ce) {
         if (w == 0 || c1 <= 0 || c1 >= C.n1 || c2 <= 0 || c2 >= C.n2) return;
@@ -2015,8 +2009,9 @@ __global__ __launch_bounds__(TPB) void k_prolong_add(Dims df, Dims dc, MGTables 
     zf[e] = zf[e] + cx<R>((R)sre, (R)sim);
 }
 
-// rc = P^T rf on interior coarse edges (0 on the coarse boundary): the 2 fine
-// edges along the coarse edge times the 3 x 3 fine node offsets across it,
+// rc = P^T rf on interior coarse edges (0 on the coarse boundary): the 1 or 2
+// fine edges along the coarse edge times the fine nodes across it (the coarse
+// node's own and, where they lie inside a coarse cell, its two neighbours),
 // summed transverse offset 1 outer, offset 2, then along (interior fine edges)
 template <typename R>
 __global__ __launch_bounds__(TPB) void k_restrict(Dims df, Dims dc, MGTables T, const cx<R> *__restrict__ rf,
@@ -2033,21 +2028,23 @@ __global__ __launch_bounds__(TPB) void k_restrict(Dims df, Dims dc, MGTables T, 
     const int A = (o == 0) ? I : (o == 1) ? J : K;
     const int C1 = (o == 0) ? J : I, C2 = (o == 2) ? J : K;
     const EdgeAxes F = edge_axes(df, o);
-    const int64_t fa0 = F.base + (int64_t)(2 * A) * F.sa;
+    const int a0 = T.st[o][A], na = T.st[o][A + 1] - a0;
+    const int64_t fa0 = F.base + (int64_t)a0 * F.sa;
     double sre = 0, sim = 0;
 #pragma unroll
     for (int f1 = -1; f1 <= 1; f1++) {
-        const int q1 = 2 * C1 + f1;
-        if (q1 <= 0 || q1 >= F.n1) continue;
+        const int q1 = T.st[t1][C1] + f1;
+        if (q1 <= 0 || q1 >= F.n1 || (f1 != 0 && !node_between(T, t1, q1))) continue;
         const double w1 = f1 == 0 ? 1.0 : (f1 < 0 ? 1.0 - T.wl[t1][q1] : T.wl[t1][q1]);
 #pragma unroll
         for (int f2 = -1; f2 <= 1; f2++) {
-            const int q2 = 2 * C2 + f2;
-            if (q2 <= 0 || q2 >= F.n2) continue;
+            const int q2 = T.st[t2][C2] + f2;
+            if (q2 <= 0 || q2 >= F.n2 || (f2 != 0 && !node_between(T, t2, q2))) continue;
             const double w2 = f2 == 0 ? 1.0 : (f2 < 0 ? 1.0 - T.wl[t2][q2] : T.wl[t2][q2]);
             const int64_t fe = fa0 + (int64_t)q1 * F.s1 + (int64_t)q2 * F.s2;
 #pragma unroll
             for (int fa = 0; fa < 2; fa++) {
+                if (fa >= na) break;
                 const cx<R> v = rf[fe + fa * F.sa];
                 sre += w1 * w2 * (double)v.re;
                 sim += w1 * w2 * (double)v.im;
@@ -2078,8 +2075,8 @@ __global__ __launch_bounds__(TPB) void k_prolong_node_add(Dims df, Dims dc, MGTa
     double w[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        lo[a] = q[a] >> 1;
-        h[a] = q[a] & 1;
+        lo[a] = T.lo[a][q[a]];
+        h[a] = node_between(T, a, q[a]);
         w[a] = h[a] ? T.wl[a][q[a]] : 1.0;
     }
     const int cI = dc.nx + 1, cJ = dc.ny + 1;
@@ -2101,27 +2098,28 @@ __global__ __launch_bounds__(TPB) void k_prolong_node_add(Dims df, Dims dc, MGTa
 }
 
 // ------------------------------------------------------------------ coarsening
-// sigma of a 2^k coarsening: each coarse cell is the volume-weighted mean of
-// its 8^k fine cells (mesh_coarsen, src/fdfe_main.f90:18-84, with the fine
-// index taken inside the block).  One coarse cell per thread; the products
-// and sums come in the order of the host restatement (z outer, x inner,
-// a*b*c, then the division by the coarse volume) and are not contracted into
-// fused multiply-adds, so both give the same bits.
-__global__ __launch_bounds__(TPB) void k_mesh_coarsen(Dims df, Dims dc, int f, const double *__restrict__ a,
-                                                      const double *__restrict__ b, const double *__restrict__ c,
-                                                      const double *__restrict__ ac, const double *__restrict__ bc,
-                                                      const double *__restrict__ cc, const double *__restrict__ sig,
-                                                      double *__restrict__ sigc)
+// sigma of a coarsening: each coarse cell is the volume-weighted mean of its
+// fine cells (mesh_coarsen, src/fdfe_main.f90:18-84, with the fine index taken
+// inside the block, for 2^k coarsenings; the multigrid grids also merge one or
+// two cells per axis).  One coarse cell per thread; the products and sums come
+// in the order of the host restatement (z outer, x inner, a*b*c, then the
+// division by the coarse volume) and are not contracted into fused
+// multiply-adds, so both give the same bits.
+__global__ __launch_bounds__(TPB) void k_mesh_coarsen(Dims df, Dims dc, const int *__restrict__ sx,
+                                                      const int *__restrict__ sy, const int *__restrict__ sz,
+                                                      const double *__restrict__ a, const double *__restrict__ b,
+                                                      const double *__restrict__ c, const double *__restrict__ ac,
+                                                      const double *__restrict__ bc, const double *__restrict__ cc,
+                                                      const double *__restrict__ sig, double *__restrict__ sigc)
 {
 #pragma clang fp contract(off)
     const int64_t ic = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (ic >= dc.NC) return;
     const int I = (int)(ic % dc.nx), J = (int)((ic / dc.nx) % dc.ny), K = (int)(ic / ((int64_t)dc.nx * dc.ny));
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int r = 0; r < f; r++)
-        for (int s = 0; s < f; s++)
-            for (int t = 0; t < f; t++) {
-                const int fi = I * f + t, fj = J * f + s, fk = K * f + r;
+    for (int fk = sz[K]; fk < sz[K + 1]; fk++)
+        for (int fj = sy[J]; fj < sy[J + 1]; fj++)
+            for (int fi = sx[I]; fi < sx[I + 1]; fi++) {
                 const int64_t ie = ((int64_t)fk * df.ny + fj) * df.nx + fi;
                 const double v = a[fi] * b[fj] * c[fk];
 #pragma unroll
@@ -2132,12 +2130,13 @@ __global__ __launch_bounds__(TPB) void k_mesh_coarsen(Dims df, Dims dc, int f, c
     for (int q = 0; q < 6; q++) sigc[q * dc.NC + ic] = acc[q] / V;
 }
 
-void launch_mesh_coarsen(const Dims &df, const Dims &dc, int f, const double *a, const double *b, const double *c,
-                         const double *ac, const double *bc, const double *cc, const double *sig, double *sigc,
-                         hipStream_t st)
+void launch_mesh_coarsen(const Dims &df, const Dims &dc, const int *sx, const int *sy, const int *sz, const double *a,
+                         const double *b, const double *c, const double *ac, const double *bc, const double *cc,
+                         const double *sig, double *sigc, hipStream_t st)
 {
     const unsigned nb = (unsigned)((dc.NC + TPB - 1) / TPB);
-    hipLaunchKernelGGL(k_mesh_coarsen, dim3(nb), dim3(TPB), 0, st, df, dc, f, a, b, c, ac, bc, cc, sig, sigc);
+    hipLaunchKernelGGL(k_mesh_coarsen, dim3(nb), dim3(TPB), 0, st, df, dc, sx, sy, sz, a, b, c, ac, bc, cc, sig,
+                       sigc);
 }
 
 // ------------------------------------------------------------------ K4 prolongation
@@ -2893,11 +2892,8 @@ void launch_hybrid_gs(const DevGrid<R> &g, R wmu, const R *P, const cx<R> *in, c
 }
 
 template <typename R>
-void launch_restrict(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *rf, cx<R> *rc,
-                     hipStream_t st)
+void launch_restrict(const Dims &df, const Dims &dc, const MGTables &T, const cx<R> *rf, cx<R> *rc, hipStream_t st)
 {
-    MGTables T;
-    for (int q = 0; q < 3; q++) T.wl[q] = wl[q];
     SliceLaunch L = edge_launch(dc);
     hipLaunchKernelGGL((k_restrict<R>), L.grid, L.block, 0, st, df, dc, T, rf, rc);
 }
@@ -2939,11 +2935,9 @@ void launch_gpbicg(int mode, const DevGrid<R> &g, RedBuf &rb, const GPVecs<R> &v
 }
 
 template <typename R>
-void launch_prolong_node_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *pc, cx<R> *pf,
+void launch_prolong_node_add(const Dims &df, const Dims &dc, const MGTables &T, const cx<R> *pc, cx<R> *pf,
                              hipStream_t st)
 {
-    MGTables T;
-    for (int q = 0; q < 3; q++) T.wl[q] = wl[q];
     SliceLaunch L = node_launch(df);
     hipLaunchKernelGGL((k_prolong_node_add<R>), L.grid, L.block, 0, st, df, dc, T, pc, pf);
 }
@@ -2979,11 +2973,9 @@ void launch_hyb_combine(const DevGrid<double> &g64, zd s, const zf *phi, const z
 }
 
 template <typename R>
-void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3], const cx<R> *zc, cx<R> *zf,
+void launch_prolong_add(const Dims &df, const Dims &dc, const MGTables &T, const cx<R> *zc, cx<R> *zf,
                         hipStream_t st)
 {
-    MGTables T;
-    for (int q = 0; q < 3; q++) T.wl[q] = wl[q];
     SliceLaunch L = edge_launch(df);
     hipLaunchKernelGGL((k_prolong_add<R>), L.grid, L.block, 0, st, df, dc, T, zc, zf);
 }
@@ -3026,14 +3018,14 @@ void launch_prolong_add(const Dims &df, const Dims &dc, const double *const wl[3
                                       int, hipStream_t, bool, int);                                                    \
     template void launch_resid_split<R>(const DevGrid<R> &, R, const cx<R> *, const cx<R> *, const cx<R> *, cx<R> *, \
                                         hipStream_t);                                           \
-    template void launch_prolong_node_add<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *,    \
+    template void launch_prolong_node_add<R>(const Dims &, const Dims &, const MGTables &, const cx<R> *,             \
                                              cx<R> *, hipStream_t);                                               \
     template void launch_grad_add<R>(const DevGrid<R> &, cx<R>, const cx<R> *, cx<R> *, hipStream_t, const cx<R> *, \
                                      double *, bool);            \
     template void launch_gpbicg<R>(int, const DevGrid<R> &, RedBuf &, const GPVecs<R> &, hipStream_t);           \
-    template void launch_restrict<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *, cx<R> *,   \
+    template void launch_restrict<R>(const Dims &, const Dims &, const MGTables &, const cx<R> *, cx<R> *,            \
                                      hipStream_t);                                                                \
-    template void launch_prolong_add<R>(const Dims &, const Dims &, const double *const[3], const cx<R> *,         \
+    template void launch_prolong_add<R>(const Dims &, const Dims &, const MGTables &, const cx<R> *,                  \
                                         cx<R> *, hipStream_t);
 
 SFDX_INST(double)

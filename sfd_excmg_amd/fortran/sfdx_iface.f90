@@ -29,6 +29,7 @@ module sfdx_iface
     integer(c_int) :: use_graph, nodal_smoother, coarse_solver, full_background, spmv_tile, ssor_cross_mass
     real(c_double) :: surface_sigma
     integer(c_int) :: pol_streams
+    real(c_double) :: mg_merge_ratio
   end type
 
   type, bind(C) :: sfdx_stats_t

@@ -51,7 +51,8 @@ class Opts(ctypes.Structure):
                 ("restart_drop", ctypes.c_double), ("use_graph", ctypes.c_int),
                 ("nodal_smoother", ctypes.c_int), ("coarse_solver", ctypes.c_int),
                 ("full_background", ctypes.c_int), ("spmv_tile", ctypes.c_int), ("ssor_cross_mass", ctypes.c_int),
-                ("surface_sigma", ctypes.c_double), ("pol_streams", ctypes.c_int)]
+                ("surface_sigma", ctypes.c_double), ("pol_streams", ctypes.c_int),
+                ("mg_merge_ratio", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):

@@ -361,13 +361,16 @@ def test_fwd_solve_mixed_bicg_family_matches_fp64(ctx, solver):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
+@pytest.mark.parametrize("merge", [0.0, 3.0])
 @pytest.mark.parametrize("use_excmg,mg_levels", [(1, 0), (0, 2), (0, 0)])
-def test_fwd_solve_multigrid_matches_oracle(ctx, use_excmg, mg_levels):
+def test_fwd_solve_multigrid_matches_oracle(ctx, use_excmg, mg_levels, merge):
     """Multigrid-preconditioned COCR (split edge/nodal V-cycle) reaches the
-    oracle's solution; the hierarchy may be deeper than the EXCMG levels."""
+    oracle's solution, over 2:1 grids (merge 0; the hierarchy may be deeper
+    than the EXCMG levels) and over merge-ratio grids (3, the default)."""
     m = small_excmg_model(True)
     opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR,
-                             precond=sfdx.PRECOND_MG, use_excmg=use_excmg, mg_levels=mg_levels)
+                             precond=sfdx.PRECOND_MG, use_excmg=use_excmg, mg_levels=mg_levels,
+                             mg_merge_ratio=merge)
     ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, opts)
     assert st.status == 0
     oopt = O.Options(solver=O.SOLVER_COCR, corr_intv=50, norm="interior")
@@ -395,33 +398,36 @@ def test_device_coarse_levels_match_direct_solves(ctx, precision):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
+@pytest.mark.parametrize("merge", [0.0, 3.0])
 @pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
-def test_auto_preconditioner_chooses_per_frequency(ctx, precision):
+def test_auto_preconditioner_chooses_per_frequency(ctx, precision, merge):
     """SFDX_PRECOND_AUTO on a 32x32x24 full-tensor block model (4 km wide,
-    first coarse cells 100 m, most conductive cell 0.84 S/m, median 0.01 S/m):
-    the V-cycle at 4 Hz only; at 1 mHz the skin depth at the median
-    conductivity (159 km) exceeds the mesh and at 1 kHz the one in the most
-    conductive cell (17 m) is below the first coarse grid's cells, so both take
-    the hybrid preconditioner.  Every frequency's impedances equal those of a
-    solve with the preconditioner fixed accordingly, to 1e-6."""
+    smallest cells 50 m, first 2:1 coarse cells 100 m, most conductive cell
+    0.84 S/m, median 0.01 S/m) at 1 mHz, 4 Hz and 1 kHz.  With merge-ratio
+    grids the V-cycle everywhere but at 1 kHz, where the skin depth in the most
+    conductive cell (17 m) is below half the smallest width; with 2:1 grids
+    (merge 0) only at 4 Hz, since at 1 mHz the skin depth at the median
+    conductivity (159 km) also exceeds the mesh.  Each frequency's field is
+    bit-identical to the solve with its chosen preconditioner fixed, and at
+    4 Hz the V-cycle and hybrid impedances agree to 1e-6."""
     m = synth.make_aniso_blocks_128(nx=32, nz_earth=20, nair=4, nstations=8)
     m.freqs = np.array([1e-3, 4.0, 1e3])
     kw = dict(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR,
-              precision=precision, sigma_floor32=0.0, restart_drop=1e-3)
+              precision=precision, sigma_floor32=0.0, restart_drop=1e-3, mg_merge_ratio=merge)
     exa, sa = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs,
                             sfdx.default_opts(precond=sfdx.PRECOND_AUTO, **kw))
-    assert sa.status == 0 and sa.mg_freqs == 1
+    choice = [True, True, False] if merge > 0 else [False, True, False]   # V-cycle per frequency
+    assert sa.status == 0 and sa.mg_freqs == sum(choice)
     exh, sh = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs,
                             sfdx.default_opts(precond=sfdx.PRECOND_HYBRID, **kw))
-    exm, sm = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs[1:2],
+    exm, sm = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs,
                             sfdx.default_opts(precond=sfdx.PRECOND_MG, **kw))
-    assert sh.status == 0 and sh.mg_freqs == 0 and sm.status == 0 and sm.mg_freqs == 1
-    for p, q in zip(_impedances(m, exa), _impedances(m, exh)):
-        assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
-    # the 4 Hz column also against the solve with the V-cycle fixed
+    assert sh.mg_freqs == 0 and sm.mg_freqs == 3
+    for k, mg in enumerate(choice):
+        assert np.array_equal(exa[:, k], (exm if mg else exh)[:, k]), k
     m1 = synth.make_aniso_blocks_128(nx=32, nz_earth=20, nair=4, nstations=8)
     m1.freqs = m.freqs[1:2]
-    for p, q in zip(_impedances(m1, exa[:, 1:2]), _impedances(m1, exm)):
+    for p, q in zip(_impedances(m1, exm[:, 1:2]), _impedances(m1, exh[:, 1:2])):
         assert np.max(np.abs(p - q) / np.abs(q)) < 1e-6
 
 
