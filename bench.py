@@ -309,8 +309,11 @@ def run_c5(ctx, args):
     kw = dict(nx=48, ny=48, nz_earth=24, nstations=8) if args.small else {}
     m = synth.make_random_field_96(**kw)
     d = synth.make_random_field_96(seed=6, **kw)
+    # forward and adjoint solves with the V-cycle over merge-ratio grids where
+    # the per-frequency rule picks it (all 6 frequencies of this model)
     opts = sfdx.default_opts(level=m.level, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0,
-                             norm=sfdx.NORM_INTERIOR, maxit_excmg=5000, maxit_plain=5000)
+                             norm=sfdx.NORM_INTERIOR, maxit_excmg=5000, maxit_plain=5000,
+                             precond=sfdx.PRECOND_AUTO, mg_coarse_pairs=MG_COARSE_PAIRS)
     exd, _ = ctx.fwd_solve(d.a, d.b, d.c, d.nair, d.sigma, d.freqs, opts)
     obs, _ = sfdx.post_process(d.a, d.b, d.c, d.nair, d.freqs, d.recv, exd, out_type=2)
     ctx.adjoint(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, obs, opts)   # warm-up
@@ -346,7 +349,7 @@ def run_c5(ctx, args):
                       "value": round(t_adj, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
                       "cells": [m.nx, m.ny, m.nz], "ncell": int(m.nx * m.ny * m.nz), "frequencies": len(m.freqs),
                       "stations": int(len(m.recv)), "misfit": phi, "status": int(st.status),
-                      "forward_fine_iters": int(st.fine_iters_total),
+                      "forward_fine_iters": int(st.fine_iters_total), "multigrid_frequencies": int(st.mg_freqs),
                       "fd_cells": cells, "fd_step_log_sigma": h, "fd_values": fds,
                       "adjoint_values": [float(grad[c]) for c in cells],
                       "max_abs_err_vs_fd_over_max_grad": float(max(errs) / scale), "fd_wall_s": round(t_fd, 2),

@@ -1974,15 +1974,25 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     zd *lam = pool_vec<zd>(ctx, "adj/lam", d.NL);
     std::vector<zc> exf((size_t)2 * d.NL), wf((size_t)2 * d.NL);
     // Adjoint solves: no divergence correction (the adjoint source is not
-    // divergence-free); the multigrid option falls back to the hybrid smoother.
+    // divergence-free).  With the multigrid options and merge-ratio grids they
+    // run the V-cycle over grids of their own (SFDX_PRECOND_AUTO: at the
+    // frequencies its rule picks), else the hybrid preconditioner.
     KOpts ko = kopts_of(*o);
     ko.corr_intv = 0;
     ko.dc_at_start = 0;
+    const bool adj_mg = (ko.precond == SFDX_PRECOND_MG || ko.precond == SFDX_PRECOND_AUTO) && o->mg_merge_ratio > 0;
+    const bool adj_auto = ko.precond == SFDX_PRECOND_AUTO;
     if (ko.precond == SFDX_PRECOND_MG || ko.precond == SFDX_PRECOND_AUTO) ko.precond = SFDX_PRECOND_HYBRID;
     if (adj_mixed) {
         ko.lo = &D32;
         ko.restart_drop = o->restart_drop;
     }
+    MergeHier MA;
+    if (adj_mg) {
+        MA.build(ctx, fine, D64, adj_mixed ? &D32 : nullptr, o, "ADJ");
+        for (int n = 1; n < MA.L; n++) MA.X64[n].g.ssor_xmass = MA.X32[n].g.ssor_xmass = xmass;
+    }
+    const AutoPrecond rule(fine, mg_grid_count(fine, o), true);
     sfdx_stats S;
     std::memset(&S, 0, sizeof(S));
     int worst = SFDX_OK;
@@ -1998,6 +2008,7 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
         S.fine_iters_total += sf.fine_iters_total;
         S.fallback_fp64 += sf.fallback_fp64;
         S.solver_switches += sf.solver_switches;
+        S.mg_freqs += sf.mg_freqs;
         S.spmv_tile = sf.spmv_tile;
         if (ex_out) std::memcpy(ex_out + (size_t)f * 2 * d.NL * 2, exf.data(), (size_t)2 * d.NL * sizeof(zc));
         double phf = 0;
@@ -2020,7 +2031,14 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
             const double bnorm = std::sqrt(ctx->hslots[SL_RN]);
             if (!(bnorm > 0)) continue;
             SFDX_CHECK(hipMemsetAsync(lam, 0, d.NL * sizeof(zd), ctx->st));
-            SolveInfo si = krylov<double>(ctx, o->solver, D64, "ADJ", wmu, w, lam, bnorm, o->eps, o->maxit_plain, ko);
+            KOpts kf = ko;
+            if (adj_mg && (!adj_auto || rule.use_mg(freq[f]))) {
+                kf.precond = SFDX_PRECOND_MG;
+                kf.mg64 = &MA.H64;
+                kf.mg32 = adj_mixed ? &MA.H32 : nullptr;
+                kf.mg_level = MA.L;
+            }
+            SolveInfo si = krylov<double>(ctx, o->solver, D64, "ADJ", wmu, w, lam, bnorm, o->eps, o->maxit_plain, kf);
             S.fallback_fp64 += si.fallback;
             S.solver_switches += si.switched;
             if (si.status > 0) {

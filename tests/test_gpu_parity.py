@@ -663,14 +663,16 @@ def _adjoint_case():
     return m, obs, cells
 
 
-def test_adjoint_gradient_matches_oracle_sensitivity_and_fd(ctx):
+@pytest.mark.parametrize("precond", [sfdx.PRECOND_HYBRID, sfdx.PRECOND_MG])
+def test_adjoint_gradient_matches_oracle_sensitivity_and_fd(ctx, precond):
     """d misfit / d log(s_c) from the device adjoint (forward + adjoint COCR
-    solves with the forward kernels, cell kernel over the 12 edges) against the
-    FP64 CPU tangent-linear sensitivity of the oracle and against central
-    differences of device forward solves."""
+    solves with the forward kernels, cell kernel over the 12 edges; the
+    hybrid preconditioner, or the V-cycle over merge-ratio grids in the forward
+    and the adjoint solves) against the FP64 CPU tangent-linear sensitivity of
+    the oracle and against central differences of device forward solves."""
     m, obs, cells = _adjoint_case()
     opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-12, corr_intv=0, norm=sfdx.NORM_INTERIOR,
-                             maxit_plain=5000, maxit_excmg=5000)
+                             maxit_plain=5000, maxit_excmg=5000, precond=precond)
     phi, grad, st = ctx.adjoint(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, obs, opts)
     assert st.status == 0
     sens = O.misfit_sensitivity(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, obs, cells)
