@@ -157,60 +157,67 @@ struct LevelDev {
     float *Ps32 = nullptr;  // 10 * NP: the symmetric form in FP32 (complex64-edge hybrid)
 };
 
+template <typename T>
+T *upload_vec(sfdx_ctx *ctx, const std::string &tag, const std::vector<T> &v)
+{
+    T *d = pool_vec<T>(ctx, tag, v.size());
+    SFDX_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->st));
+    return d;
+}
+
+// Device copy of one level in precision R from its widths (host) and its FP64
+// sigma already on the device (sig64, 6 x ncell component-major, which stays
+// the level's FP64 copy): sigma in R (FP32: the diagonal raised to
+// sigma_floor, 0 = unchanged) or, in FP64, sig64 itself plus the pair layout of
+// the cell-plane march, both formed on the device.
 template <typename R>
-LevelDev<R> upload_level(sfdx_ctx *ctx, const HostLevel &L, const std::string &tag, double sigma_floor = 0)
+LevelDev<R> level_from_device(sfdx_ctx *ctx, const HostLevel &L, double *sig64, const std::string &tag,
+                              double sigma_floor = 0)
 {
     LevelDev<R> D;
     D.g.d = make_dims(L.nx, L.ny, L.nz);
-    std::vector<R> a(L.a.begin(), L.a.end()), b(L.b.begin(), L.b.end()), c(L.c.begin(), L.c.end()),
-        s(L.sig.begin(), L.sig.end());
-    if (sigma_floor > 0)
-        for (size_t q = 0; q < 3 * L.ncell(); q++) s[q] = std::max(s[q], (R)sigma_floor);
+    const int64_t nc = (int64_t)L.nx * L.ny * L.nz;
+    std::vector<R> a(L.a.begin(), L.a.end()), b(L.b.begin(), L.b.end()), c(L.c.begin(), L.c.end());
     const std::string p = tag + (sizeof(R) == 8 ? "/d" : "/f");
-    D.g.a = pool_vec<R>(ctx, p + "a", a.size());
-    D.g.b = pool_vec<R>(ctx, p + "b", b.size());
-    D.g.c = pool_vec<R>(ctx, p + "c", c.size());
-    D.g.sig = pool_vec<R>(ctx, p + "s", s.size());
-    SFDX_CHECK(hipMemcpyAsync(D.g.a, a.data(), a.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
-    SFDX_CHECK(hipMemcpyAsync(D.g.b, b.data(), b.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
-    SFDX_CHECK(hipMemcpyAsync(D.g.c, c.data(), c.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
-    SFDX_CHECK(hipMemcpyAsync(D.g.sig, s.data(), s.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
-    // complex128: sigma also as (xx,yy), (zz,xy), (xz,yz) pieces per cell for the
-    // cell-plane march (kept alive until the stream synchronisation below)
-    std::vector<double> sp;
+    D.g.a = upload_vec(ctx, p + "a", a);
+    D.g.b = upload_vec(ctx, p + "b", b);
+    D.g.c = upload_vec(ctx, p + "c", c);
     if constexpr (sizeof(R) == 8) {
-        const size_t nc = L.ncell();
-        sp.resize(6 * nc);
-        for (size_t u = 0; u < 3; u++)
-            for (size_t q = 0; q < nc; q++) {
-                sp[(u * nc + q) * 2] = (double)s[2 * u * nc + q];
-                sp[(u * nc + q) * 2 + 1] = (double)s[(2 * u + 1) * nc + q];
-            }
-        double *dp = pool_vec<double>(ctx, p + "sp", sp.size());
-        SFDX_CHECK(hipMemcpyAsync(dp, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+        if (sigma_floor > 0) throw std::runtime_error("sigma_floor applies to FP32 levels only");
+        D.g.sig = sig64;
+        double *dp = pool_vec<double>(ctx, p + "sp", 6 * nc);
+        launch_sigma_pairs(nc, sig64, dp, ctx->st);
         D.g.sigp = dp;
+    } else {
+        float *sf = pool_vec<float>(ctx, p + "s", 6 * nc);
+        launch_sigma_f32(nc, sigma_floor, sig64, sf, ctx->st);
+        D.g.sig = sf;
     }
     // reciprocal widths (computed in FP64, stored in R) for the stencil rows
     std::vector<R> ra(L.a.size()), rb(L.b.size()), rc(L.c.size());
     for (size_t q = 0; q < ra.size(); q++) ra[q] = (R)(1.0 / L.a[q]);
     for (size_t q = 0; q < rb.size(); q++) rb[q] = (R)(1.0 / L.b[q]);
     for (size_t q = 0; q < rc.size(); q++) rc[q] = (R)(1.0 / L.c[q]);
-    D.g.ra = pool_vec<R>(ctx, p + "ra", ra.size());
-    D.g.rb = pool_vec<R>(ctx, p + "rb", rb.size());
-    D.g.rc = pool_vec<R>(ctx, p + "rc", rc.size());
-    SFDX_CHECK(hipMemcpyAsync(D.g.ra, ra.data(), ra.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
-    SFDX_CHECK(hipMemcpyAsync(D.g.rb, rb.data(), rb.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
-    SFDX_CHECK(hipMemcpyAsync(D.g.rc, rc.data(), rc.size() * sizeof(R), hipMemcpyHostToDevice, ctx->st));
-    D.a64 = pool_vec<double>(ctx, tag + "/a64", L.a.size());
-    D.b64 = pool_vec<double>(ctx, tag + "/b64", L.b.size());
-    D.c64 = pool_vec<double>(ctx, tag + "/c64", L.c.size());
-    D.sig64 = pool_vec<double>(ctx, tag + "/s64", L.sig.size());
-    SFDX_CHECK(hipMemcpyAsync(D.a64, L.a.data(), L.a.size() * 8, hipMemcpyHostToDevice, ctx->st));
-    SFDX_CHECK(hipMemcpyAsync(D.b64, L.b.data(), L.b.size() * 8, hipMemcpyHostToDevice, ctx->st));
-    SFDX_CHECK(hipMemcpyAsync(D.c64, L.c.data(), L.c.size() * 8, hipMemcpyHostToDevice, ctx->st));
-    SFDX_CHECK(hipMemcpyAsync(D.sig64, L.sig.data(), L.sig.size() * 8, hipMemcpyHostToDevice, ctx->st));
-    SFDX_CHECK(hipStreamSynchronize(ctx->st));
+    D.g.ra = upload_vec(ctx, p + "ra", ra);
+    D.g.rb = upload_vec(ctx, p + "rb", rb);
+    D.g.rc = upload_vec(ctx, p + "rc", rc);
+    D.a64 = upload_vec(ctx, tag + "/a64", L.a);
+    D.b64 = upload_vec(ctx, tag + "/b64", L.b);
+    D.c64 = upload_vec(ctx, tag + "/c64", L.c);
+    D.sig64 = sig64;
+    SFDX_CHECK(hipStreamSynchronize(ctx->st));  // the host vectors above end here
     return D;
+}
+
+// Device copy of one grid level in precision R (plus FP64 copies for P
+// assembly) from a host level.
+template <typename R>
+LevelDev<R> upload_level(sfdx_ctx *ctx, const HostLevel &L, const std::string &tag, double sigma_floor = 0)
+{
+    const size_t n = 6 * L.ncell();
+    double *s64 = pool_vec<double>(ctx, tag + "/s64", n);
+    SFDX_CHECK(hipMemcpyAsync(s64, L.S(), n * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    return level_from_device<R>(ctx, L, s64, tag, sigma_floor);
 }
 
 template <typename R>
@@ -229,23 +236,16 @@ std::vector<int> uniform_starts(int n, int k)
     return st;
 }
 
-template <typename T>
-T *upload_vec(sfdx_ctx *ctx, const std::string &tag, const std::vector<T> &v)
-{
-    T *d = pool_vec<T>(ctx, tag, v.size());
-    SFDX_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->st));
-    return d;
-}
-
 using Starts = std::array<std::vector<int>, 3>;
 
 // The grid whose cell c along axis a merges the fine cells [st[a][c],
 // st[a][c+1]): widths summed on the host, sigma averaged on the device from
 // the fine grid's resident FP64 copy (k_mesh_coarsen; for a 2^k coarsening,
 // mesh_coarsen of src/fdfe_main.f90:18-84, the same bits as the host coarsen)
-// and copied back for the host-side users (1-D background, direct solves).
+// and copied back for the host-side users (1-D background, direct solves)
+// unless host_sig is false; *dev_sig (when given) receives the device sigma.
 HostLevel coarsen_by_starts(sfdx_ctx *ctx, const HostLevel &F, const LevelDev<double> &DF, const Starts &st,
-                            const std::string &tag)
+                            const std::string &tag, double **dev_sig = nullptr, bool host_sig = true)
 {
     HostLevel C;
     const std::vector<double> *wf[3] = {&F.a, &F.b, &F.c};
@@ -265,17 +265,21 @@ HostLevel coarsen_by_starts(sfdx_ctx *ctx, const HostLevel &F, const LevelDev<do
                  *cc = upload_vec(ctx, tag + "/cc", C.c);
     double *sc = pool_vec<double>(ctx, tag + "/cs", 6 * dc.NC);
     launch_mesh_coarsen(df, dc, sx, sy, sz, DF.a64, DF.b64, DF.c64, ac, bc, cc, DF.sig64, sc, ctx->st);
-    C.sig.assign(6 * dc.NC, 0.0);
-    SFDX_CHECK(hipMemcpyAsync(C.sig.data(), sc, 6 * dc.NC * 8, hipMemcpyDeviceToHost, ctx->st));
+    if (host_sig) {
+        C.sig.assign(6 * dc.NC, 0.0);
+        SFDX_CHECK(hipMemcpyAsync(C.sig.data(), sc, 6 * dc.NC * 8, hipMemcpyDeviceToHost, ctx->st));
+    }
     SFDX_CHECK(hipStreamSynchronize(ctx->st));
+    if (dev_sig) *dev_sig = sc;
     return C;
 }
 
 // The grid of a 2^k coarsening (mesh_coarsen, src/fdfe_main.f90:18-84).
-HostLevel coarsen_device(sfdx_ctx *ctx, const HostLevel &F, const LevelDev<double> &DF, int k, const std::string &tag)
+HostLevel coarsen_device(sfdx_ctx *ctx, const HostLevel &F, const LevelDev<double> &DF, int k, const std::string &tag,
+                         double **dev_sig = nullptr)
 {
     const Starts st = {uniform_starts(F.nx, k), uniform_starts(F.ny, k), uniform_starts(F.nz, k)};
-    return coarsen_by_starts(ctx, F, DF, st, tag);
+    return coarsen_by_starts(ctx, F, DF, st, tag, dev_sig);
 }
 
 // Coarse node starts of one axis of the merge-ratio coarsening: two
@@ -1487,11 +1491,12 @@ struct MergeHier {
             if (s[0].size() < 3 || s[1].size() < 3 || s[2].size() < 3) break;
             const int n = (int)g.size();
             const std::string t = tag + "_" + std::to_string(n);
-            HostLevel C = coarsen_by_starts(ctx, G, n == 1 ? D64 : X64[n - 1], s, "crs" + t);
+            double *sd = nullptr;  // the grid's sigma stays on the device
+            HostLevel C = coarsen_by_starts(ctx, G, n == 1 ? D64 : X64[n - 1], s, "crs" + t, &sd, false);
             g.push_back(std::move(C));
             st.push_back(s);
-            X64[n] = upload_level<double>(ctx, g.back(), t);
-            if (D32) X32[n] = upload_level<float>(ctx, g.back(), t, o->sigma_floor32);
+            X64[n] = level_from_device<double>(ctx, g.back(), sd, t);
+            if (D32) X32[n] = level_from_device<float>(ctx, g.back(), sd, t, o->sigma_floor32);
         }
         L = (int)g.size();
         H64.name = H32.name = "fm" + tag + "_";
@@ -1569,7 +1574,7 @@ struct AutoPrecond {
         std::vector<double> mean(std::max<int64_t>(nce, 1), 0.0);
         for (int64_t q = 0; q < nce; q++) {
             const int64_t ie = (int64_t)F.nx * F.ny * F.nair + q;
-            const double sx = F.sig[ie], sy = F.sig[nc + ie], sz = F.sig[2 * nc + ie];
+            const double sx = F.S()[ie], sy = F.S()[nc + ie], sz = F.S()[2 * nc + ie];
             sig_max = std::max(sig_max, std::max(sx, std::max(sy, sz)));
             mean[q] = (sx + sy + sz) / 3;
         }
@@ -1603,7 +1608,7 @@ HostLevel host_level(const sfdx_grid *g, const double *sigma)
     L.a.assign(g->a, g->a + g->nx);
     L.b.assign(g->b, g->b + g->ny);
     L.c.assign(g->c, g->c + g->nz);
-    L.sig.assign(sigma, sigma + 6 * L.ncell());
+    L.ext = sigma;  // read in place for the duration of the call
     return L;
 }
 
@@ -1616,8 +1621,9 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     sfdx_stats S;
     std::memset(&S, 0, sizeof(S));
     const HostLevel fine = host_level(g, sigma);
-    for (size_t q = 0; q < fine.sig.size(); q++)
-        if (!std::isfinite(fine.sig[q])) { g_err = "non-finite sigma"; return SFDX_ERR_NAN; }
+    double probe = 0;  // NaN if any entry is NaN or infinite (a branch-free loop)
+    for (size_t q = 0; q < 6 * fine.ncell(); q++) probe += fine.S()[q] * 0.0;
+    if (probe != 0) { g_err = "non-finite sigma"; return SFDX_ERR_NAN; }
     if (o->precond < SFDX_PRECOND_SSOR || o->precond > SFDX_PRECOND_AUTO) {
         g_err = "precond must be 0..3";
         return SFDX_ERR_ARG;
@@ -1658,8 +1664,12 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     lev[Lv] = fine;
     std::vector<LevelDev<double>> D64(Lv + 1);
     std::vector<LevelDev<float>> D32(Lv + 1);
+    if (o->verbose) fprintf(stderr, "set-up: host copy and checks %.3fs\n", total.s());
     D64[Lv] = upload_level<double>(ctx, fine, "L" + std::to_string(Lv));
-    for (int i = Lv - 1; i >= 1; i--) lev[i] = coarsen_device(ctx, fine, D64[Lv], Lv - i, "crs" + std::to_string(i));
+    if (o->verbose) fprintf(stderr, "set-up: fine level uploaded %.3fs\n", total.s());
+    std::vector<double *> levsig(Lv + 1, nullptr);  // device sigma of the coarser levels
+    for (int i = Lv - 1; i >= 1; i--)
+        lev[i] = coarsen_device(ctx, fine, D64[Lv], Lv - i, "crs" + std::to_string(i), &levsig[i]);
     // levels solved by the Krylov method on the device: with coarse_solver = 1
     // also the two coarsest EXCMG levels (instead of the host direct solves)
     const bool coarse_dev = o->use_excmg && o->coarse_solver == 1;
@@ -1671,21 +1681,22 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     // iterations.  A complex64 V-cycle from the fine level still visits every
     // grid, so with multigrid all levels get their complex64 data.
     const bool lo_fine = o->precision != SFDX_PREC_FP64;
-    for (int i = first_up; i <= Lv; i++) {
-        if (i < Lv) D64[i] = upload_level<double>(ctx, lev[i], "L" + std::to_string(i));
-        if (lo_fine && (i == Lv || use_mg))
-            D32[i] = upload_level<float>(ctx, lev[i], "L" + std::to_string(i), o->sigma_floor32);
-    }
-    // Multigrid hierarchy mg[1..Lmg] (mg[Lmg] = fine); the EXCMG level i is
-    // mg[i + off].  Grids coarser than the EXCMG levels are uploaded here; with
-    // fewer grids than EXCMG levels (off < 0) the V-cycle stops above the
-    // coarsest EXCMG level, and EXCMG levels below its coarsest grid are
-    // solved with the hybrid preconditioner.
     const bool flex = use_mg && o->mg_merge_ratio > 0;
     if (flex && o->mg_merge_ratio < 1) {
         g_err = "mg_merge_ratio must be 0 or >= 1";
         return SFDX_ERR_ARG;
     }
+    for (int i = first_up; i <= Lv; i++) {
+        if (i < Lv) D64[i] = level_from_device<double>(ctx, lev[i], levsig[i], "L" + std::to_string(i));
+        if (lo_fine && (i == Lv || (use_mg && !flex)))
+            D32[i] = level_from_device<float>(ctx, lev[i], D64[i].sig64, "L" + std::to_string(i), o->sigma_floor32);
+    }
+    // Multigrid hierarchy mg[1..Lmg] (mg[Lmg] = fine) of 2:1 grids
+    // (mg_merge_ratio = 0); the EXCMG level i is mg[i + off].  Grids coarser
+    // than the EXCMG levels are uploaded here; with fewer grids than EXCMG
+    // levels (off < 0) the V-cycle stops above the coarsest EXCMG level, and
+    // EXCMG levels below its coarsest grid are solved with the hybrid
+    // preconditioner.
     int Lmg = Lv;
     if (use_mg && !flex) Lmg = std::max(mg_grid_count(fine, o), 2);
     const int off = Lmg - Lv;
@@ -1706,11 +1717,12 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
                 H64.lev[j] = &D64[j - off];
                 if (o->precision != SFDX_PREC_FP64) H32.lev[j] = &D32[j - off];
             } else {
-                mgh[j] = coarsen_device(ctx, fine, D64[Lv], Lmg - j, "crsm" + std::to_string(j));
-                X64[j] = upload_level<double>(ctx, mgh[j], "M" + std::to_string(j));
+                double *sd = nullptr;
+                mgh[j] = coarsen_device(ctx, fine, D64[Lv], Lmg - j, "crsm" + std::to_string(j), &sd);
+                X64[j] = level_from_device<double>(ctx, mgh[j], sd, "M" + std::to_string(j));
                 H64.lev[j] = &X64[j];
                 if (o->precision != SFDX_PREC_FP64) {
-                    X32[j] = upload_level<float>(ctx, mgh[j], "M" + std::to_string(j), o->sigma_floor32);
+                    X32[j] = level_from_device<float>(ctx, mgh[j], sd, "M" + std::to_string(j), o->sigma_floor32);
                     H32.lev[j] = &X32[j];
                 }
             }
@@ -1723,11 +1735,13 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     }
     // With mg_merge_ratio > 0 every Krylov level runs the V-cycle over grids of
     // its own (MergeHier): coarsenings that merge cells only where they are short.
+    if (o->verbose) fprintf(stderr, "set-up: EXCMG levels %.3fs\n", total.s());
     std::vector<MergeHier> MH(flex ? Lv + 1 : 0);
     for (int i = flex ? first_dev : Lv + 1; i <= Lv; i++) {
         const bool lo_i = lo_fine && i == Lv;
         // a level that merges no cells gets a one-grid V-cycle (the smoother)
         MH[i].build(ctx, lev[i], D64[i], lo_i ? &D32[i] : nullptr, o, "F" + std::to_string(i));
+        if (o->verbose) fprintf(stderr, "set-up: level %d grids %.3fs\n", i, total.s());
     }
     ensure_partials(ctx, edge_launch(make_dims(fine.nx, fine.ny, fine.nz)).nblocks() + 64);
     for (int i = first_up; i <= Lv; i++) {
@@ -1962,7 +1976,7 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     LevelDev<double> D64 = upload_level<double>(ctx, fine, "ADJ");
     LevelDev<float> D32;
     const bool adj_mixed = o->precision == SFDX_PREC_MIXED || o->precision == SFDX_PREC_MIXED_SPLIT;
-    if (adj_mixed) D32 = upload_level<float>(ctx, fine, "ADJ", o->sigma_floor32);
+    if (adj_mixed) D32 = level_from_device<float>(ctx, fine, D64.sig64, "ADJ", o->sigma_floor32);
     ensure_partials(ctx, edge_launch(d).nblocks() + 64);
     choose_tile(ctx, D64.g, "ADJ/d", o->spmv_tile, o->verbose);
     if (adj_mixed) choose_tile(ctx, D32.g, "ADJ/f", o->spmv_tile, o->verbose);

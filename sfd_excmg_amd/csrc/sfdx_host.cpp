@@ -124,7 +124,7 @@ std::vector<zc> background_field(const HostLevel &L, double freq, bool full)
     const int ncomp = full ? 6 : 3;
     for (int k = 0; k < nz; k++)
         for (int q = 0; q < ncomp; q++)
-            conv[(size_t)q * nz + k] = L.sig[(size_t)q * L.ncell() + (size_t)k * L.nx * L.ny];
+            conv[(size_t)q * nz + k] = L.S()[(size_t)q * L.ncell() + (size_t)k * L.nx * L.ny];
     std::vector<zc> e(full ? 4 * nz + 4 : 2 * nz + 2);
     mt1d_anl(nz, conv.data(), L.c.data(), freq, e.data(), nullptr, nullptr, full ? e.data() + 2 * nz + 2 : nullptr);
     return e;
@@ -154,7 +154,7 @@ HostLevel coarsen(const HostLevel &F, int k)
                             const int fi = Ii * f + t, fj = J * f + s, fk = K * f + r;
                             const size_t ie = ((size_t)fk * F.ny + fj) * F.nx + fi;
                             const double v = F.a[fi] * F.b[fj] * F.c[fk];
-                            for (int q = 0; q < 6; q++) acc[q] += F.sig[q * ncf + ie] * v;
+                            for (int q = 0; q < 6; q++) acc[q] += F.S()[q * ncf + ie] * v;
                         }
                 const double V = C.a[Ii] * C.b[J] * C.c[K];
                 const size_t ic = ((size_t)K * C.ny + J) * C.nx + Ii;
@@ -174,7 +174,7 @@ void surface_planes(const HostLevel &L, double air_max, std::vector<int> &surf)
     surf.assign(nxy, L.nz);
     for (size_t s = 0; s < nxy; s++)
         for (int k = 0; k < L.nz; k++)
-            if (L.sig[(size_t)k * nxy + s] > air_max) {
+            if (L.S()[(size_t)k * nxy + s] > air_max) {
                 surf[s] = k;
                 break;
             }
@@ -265,7 +265,7 @@ void coarse_system(const HostLevel &L, double freq, const std::vector<zc> &edc, 
         int i, j, k;
         int o = edge_decode(d, e, i, j, k);
         Row<double> rw;
-        row_coeffs(d, L.a.data(), L.b.data(), L.c.data(), L.sig.data(), o, i, j, k, rw);
+        row_coeffs(d, L.a.data(), L.b.data(), L.c.data(), L.S(), o, i, j, k, rw);
         row.clear();
         row.push_back({r, zc(rw.kgd, -wmu * rw.kmd)});
         for (int s = 0; s < 12; s++) {

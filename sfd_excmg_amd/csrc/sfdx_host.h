@@ -15,8 +15,10 @@ extern const double kMu0;
 struct HostLevel {
     int nx = 0, ny = 0, nz = 0, nair = 0;
     std::vector<double> a, b, c;  // widths
-    std::vector<double> sig;      // 6*ncell, sig[comp*ncell + cell]
+    std::vector<double> sig;      // 6*ncell, sig[comp*ncell + cell] (empty when ext is set)
+    const double *ext = nullptr;  // the caller's sigma in the same layout, not copied
     size_t ncell() const { return (size_t)nx * ny * nz; }
+    const double *S() const { return ext ? ext : sig.data(); }
 };
 
 struct ProlongHost {

@@ -227,6 +227,10 @@ void launch_cocr_upd_grad(const DevGrid<R> &g, RedBuf &rb, cx<R> *x, cx<R> *pv, 
 void launch_mesh_coarsen(const Dims &df, const Dims &dc, const int *sx, const int *sy, const int *sz, const double *a,
                          const double *b, const double *c, const double *ac, const double *bc, const double *cc,
                          const double *sig, double *sigc, hipStream_t st);
+// sigma (6 x nc, component-major) in the cell-plane march's pair layout, and in
+// FP32 with the diagonal components raised to floor (0: unchanged)
+void launch_sigma_pairs(int64_t nc, const double *sig, double *sigp, hipStream_t st);
+void launch_sigma_f32(int64_t nc, double floor, const double *sig, float *out, hipStream_t st);
 // y = A_s (z + (i/wmu) G psi), with the partial sums of u^T y (u = z + (i/wmu) G psi);
 // returns the number of partial sums
 int64_t launch_split_product(const DevGrid<float> &g, float wmu, const zf *z, const zf *psi, zf *y, RedBuf &rb,

@@ -2130,6 +2130,41 @@ __global__ __launch_bounds__(TPB) void k_mesh_coarsen(Dims df, Dims dc, const in
     for (int q = 0; q < 6; q++) sigc[q * dc.NC + ic] = acc[q] / V;
 }
 
+// Device copies of a level's sigma: the (xx,yy), (zz,xy), (xz,yz) pair layout
+// of the complex128 cell-plane march, and FP32 with the diagonal raised to
+// `floor` (0: unchanged).  One cell (pairs) or one value (FP32) per thread.
+__global__ __launch_bounds__(TPB) void k_sigma_pairs(int64_t nc, const double *__restrict__ sig,
+                                                     double *__restrict__ sigp)
+{
+    const int64_t q = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (q >= nc) return;
+#pragma unroll
+    for (int u = 0; u < 3; u++) {
+        sigp[(u * nc + q) * 2] = sig[2 * u * nc + q];
+        sigp[(u * nc + q) * 2 + 1] = sig[(2 * u + 1) * nc + q];
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_sigma_f32(int64_t nc, double floor, const double *__restrict__ sig,
+                                                   float *__restrict__ out)
+{
+    const int64_t q = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (q >= 6 * nc) return;
+    float v = (float)sig[q];
+    if (floor > 0 && q < 3 * nc) v = fmaxf(v, (float)floor);
+    out[q] = v;
+}
+
+void launch_sigma_pairs(int64_t nc, const double *sig, double *sigp, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sigma_pairs, dim3((unsigned)((nc + TPB - 1) / TPB)), dim3(TPB), 0, st, nc, sig, sigp);
+}
+
+void launch_sigma_f32(int64_t nc, double floor, const double *sig, float *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sigma_f32, dim3((unsigned)((6 * nc + TPB - 1) / TPB)), dim3(TPB), 0, st, nc, floor, sig, out);
+}
+
 void launch_mesh_coarsen(const Dims &df, const Dims &dc, const int *sx, const int *sy, const int *sz, const double *a,
                          const double *b, const double *c, const double *ac, const double *bc, const double *cc,
                          const double *sig, double *sigc, hipStream_t st)
