@@ -10,7 +10,7 @@ is the fine-grid solve time (both polarisations) of the faster converged GPU
 path, averaged over the timed steps.  Both paths run COCR preconditioned by a
 geometric multigrid V-cycle over merge-ratio grids (each coarser grid merges
 two cells of an axis only where both are at most 3x its smallest width; up to
-8 grids per level; hybrid Gauss-Seidel smoothing: lumped-mass multicolour edge
+7 grids per level; hybrid Gauss-Seidel smoothing: lumped-mass multicolour edge
 SSOR sweeps plus the nodal correction by a two-step Jacobi polynomial,
 residuals in split edge + nodal form):
   * FP64: everything in complex128;
@@ -81,11 +81,13 @@ POL_STREAMS = 2
 # Gauss-Seidel smoothing (lumped-mass edge SSOR sweeps plus the
 # Jacobi-polynomial nodal correction) over merge-ratio grids (two cells of an
 # axis merged only where both are at most 3x the grid's smallest width) down to
-# 8 grids, one smoothing pair on the coarsest.  Measured at this workload, fine
-# level, mixed: 0.156 s (about 65 iterations per polarisation), against 0.61 s
-# (about 390) with 3 grids of 2:1 coarsening and 1.23 s (about 1240) with the
-# one-level hybrid preconditioner.
-MG_LEVELS, MG_COARSE_PAIRS, MG_MERGE_RATIO = 0, 1, 3.0
+# 7 grids, one smoothing pair on the coarsest.  Measured at this workload, fine
+# level, mixed: 0.15-0.16 s (about 65 iterations per polarisation), against
+# 0.61 s (about 390) with 3 grids of 2:1 coarsening and 1.23 s (about 1240)
+# with the one-level hybrid preconditioner.  Over merge ratios 2.5-4, 6-8
+# grids and 1-2 coarsest pairs the fine levels at 0.1 + 1 + 10 Hz summed to
+# 0.65-0.80 s; ratio 3 with 7 grids and one pair was the fastest (0.654 s).
+MG_LEVELS, MG_COARSE_PAIRS, MG_MERGE_RATIO = 7, 1, 3.0
 
 
 def solve_opts(m, precision, eps, verbose=0, precond=None):
