@@ -56,14 +56,15 @@ extern "C" {
 #define SFDX_PRECOND_SSOR 0
 #define SFDX_PRECOND_HYBRID 1  /* SSOR + G (G^T A G)^-1 G^T, G^T A G = -i w mu P */
 #define SFDX_PRECOND_MG 2      /* V-cycle over the EXCMG grid hierarchy, hybrid smoothing */
-#define SFDX_PRECOND_AUTO 3    /* per frequency: with mg_merge_ratio > 0 the V-cycle unless the skin
-                                  depth in the most conductive earth cell is below half the smallest
-                                  cell width; with mg_merge_ratio = 0 the V-cycle only where its first
+#define SFDX_PRECOND_AUTO 3    /* per frequency: with mg_merge_ratio > 0 the V-cycle on each Krylov
+                                  level unless the skin depth in the most conductive earth cell is
+                                  below half that level's smallest cell width; with
+                                  mg_merge_ratio = 0 the V-cycle only where its first
                                   coarse grid resolves that skin depth, the skin depth at the median
                                   earth conductivity is within the mesh's horizontal extent and no
                                   cell is more than 200 times longer than wide; the hybrid
                                   preconditioner elsewhere (sfdx_stats.mg_freqs counts the
-                                  frequencies that ran the V-cycle) */
+                                  frequencies whose fine level ran the V-cycle) */
 
 #define SFDX_NORM_REFERENCE 0   /* ||b-Ax||/||b||, b incl. Dirichlet rows (solvers.f90:615-625) */
 #define SFDX_NORM_INTERIOR 1    /* interior rows only */

@@ -1529,12 +1529,15 @@ int mg_grid_count(const HostLevel &F, const sfdx_opts *o)
 }
 
 // SFDX_PRECOND_AUTO: the preconditioner of each frequency's solves.
-// With the merge-ratio V-cycle (mg_merge_ratio > 0) the V-cycle unless the
-// skin depth in the most conductive earth cell is below half the fine grid's
-// smallest width (the fine grid itself then under-resolves the field; on the
-// 128x128x96 block model at 1 kHz, 17 m against 25 m, the hybrid
-// preconditioner was 1.4x faster; from 1 mHz to 398 Hz the V-cycle was
-// 1.9-6.7x faster).  With the 2:1 V-cycle (mg_merge_ratio = 0) the V-cycle
+// With the merge-ratio V-cycle (mg_merge_ratio > 0) the V-cycle on each
+// Krylov level unless the skin depth in the most conductive earth cell is
+// below half the level's smallest width: that level's grid under-resolves the
+// field, and there the complex64 V-cycle converged slowly or not at all (a
+// 32x32x24 block model at 1 kHz to 1e-10) while the hybrid preconditioner
+// needs fewer and cheaper iterations on the coarser levels (the 128x128x96
+// block model at 1 kHz, levels 1-3: 164-984 iterations against 757-1253 with
+// the V-cycle; 16 sounding frequencies, whole solves 11.0 s against 13.2 s
+// with the V-cycle on every level).  With the 2:1 V-cycle (mg_merge_ratio = 0) the V-cycle
 // only where
 //  (a) its first coarse grid resolves that skin depth, delta(sigma_max, f) >=
 //      the smallest cell width of that grid (twice the fine grid's); below it
@@ -1554,6 +1557,14 @@ int mg_grid_count(const HostLevel &F, const sfdx_opts *o)
 // are taken over the diagonal tensor components of the cells below the air
 // layers.
 constexpr double kAutoMaxAspect = 200.0;
+
+double min_width(const HostLevel &L)
+{
+    double h = 1e300;
+    for (const std::vector<double> *w : {&L.a, &L.b, &L.c}) h = std::min(h, *std::min_element(w->begin(), w->end()));
+    return h;
+}
+
 struct AutoPrecond {
     double sig_max = 0, sig_med = 0, extent = 0, h_min = 0, h_coarse = 0, aspect = 0;
     bool usable = false;  // the fine grid coarsens at least once
@@ -1592,10 +1603,12 @@ struct AutoPrecond {
         }
     }
     static double skin_depth(double sigma, double f) { return std::sqrt(2.0 / (2 * M_PI * f * 4e-7 * M_PI * sigma)); }
+    // merge-ratio grids: the V-cycle on a level of smallest width h
+    bool use_mg_on(double f, double h) const { return usable && sig_max > 0 && skin_depth(sig_max, f) >= 0.5 * h; }
     bool use_mg(double f) const
     {
         if (!usable || !(sig_max > 0)) return false;
-        if (merge) return skin_depth(sig_max, f) >= 0.5 * h_min;
+        if (merge) return use_mg_on(f, h_min);
         return aspect <= kAutoMaxAspect && sig_med > 0 && skin_depth(sig_max, f) >= h_coarse &&
                skin_depth(sig_med, f) <= extent;
     }
@@ -1632,8 +1645,10 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     // the skin-depth rule); its grids are built when any frequency uses it
     const bool auto_pc = o->precond == SFDX_PRECOND_AUTO;
     std::vector<char> mg_freq(nf, o->precond == SFDX_PRECOND_MG);
+    std::unique_ptr<AutoPrecond> rule_p;
+    if (auto_pc) rule_p.reset(new AutoPrecond(fine, mg_grid_count(fine, o), o->mg_merge_ratio > 0));
     if (auto_pc) {
-        const AutoPrecond rule(fine, mg_grid_count(fine, o), o->mg_merge_ratio > 0);
+        const AutoPrecond &rule = *rule_p;
         for (int fi = 0; fi < nf; fi++) {
             mg_freq[fi] = rule.use_mg(freq[fi]);
             if (o->verbose)
@@ -1843,7 +1858,11 @@ int fwd_solve_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
             }
             const int maxit = o->use_excmg ? o->maxit_excmg : o->maxit_plain;
             // levels below the multigrid hierarchy's coarsest grid: hybrid
-            const bool mg_here = mg_freq[fi] && (flex || i + off >= 1);
+            // with merge-ratio grids SFDX_PRECOND_AUTO decides per level: a coarser
+            // EXCMG level whose own cells under-resolve the skin depth takes
+            // the hybrid preconditioner
+            const bool mg_here = mg_freq[fi] && (flex || i + off >= 1) &&
+                                 !(auto_pc && flex && i < Lv && !rule_p->use_mg_on(ff, min_width(lev[i])));
             sfdx_opts oi = *o;
             if ((use_mg || auto_pc) && !mg_here) oi.precond = SFDX_PRECOND_HYBRID;
             if (auto_pc && mg_here) oi.precond = SFDX_PRECOND_MG;

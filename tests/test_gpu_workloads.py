@@ -251,9 +251,10 @@ def test_c4_batch_of_uneven_models_and_sounding_frequencies(ctx):
     """The 8 uneven isotropic models (1 Hz) and the 128x128x96 model at the
     first, sixth and last of the 16 sounding frequencies (1 mHz, 0.1 Hz,
     1 kHz), as one mixed-precision batch over 8 streams with the preconditioner
-    chosen per item (the V-cycle but for the 128x128x96 model at 1 kHz):
-    every item agrees with its single solve, and the 48^3 and 64x64x48 models
-    with the oracle."""
+    chosen per item and level (the V-cycle but for the 128x128x96 model at
+    1 kHz, whose levels all under-resolve the skin depth): every item agrees
+    with its single solve, and the 48^3 and 64x64x48 models with the
+    oracle."""
     g = np.load(os.path.join(GOLDEN, "batch_oracle_impedances.npz"))
     small = synth.make_uneven_iso_models()
     big = synth.make_aniso_blocks_128()
