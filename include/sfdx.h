@@ -101,7 +101,8 @@ typedef struct {
     int inner_maxit;    /* mixed precision: iteration cap of one inner solve */
     int precond;        /* SFDX_PRECOND_*: 0 multicolour SSOR (the reference's SSOR,
                            re-ordered), 1 SSOR + auxiliary nodal correction,
-                           2 geometric multigrid V-cycle with hybrid smoothing */
+                           2 geometric multigrid V-cycle with hybrid smoothing,
+                           3 (default) 2 or 1 per frequency and level */
     double sigma_floor32; /* complex64 edge smoother only: diagonal conductivities
                              below this floor (S/m) are raised to it, bounding the
                              amplification of near-null air modes; 0 = off */

@@ -2144,7 +2144,7 @@ void sfdx_default_opts(sfdx_opts *o)
     o->dc_restart = 0;
     o->dc_at_start = 0;
     o->inner_maxit = 400;
-    o->precond = SFDX_PRECOND_HYBRID;
+    o->precond = SFDX_PRECOND_AUTO;
     o->mg_levels = 0;
     o->mg_coarse_pairs = 4;
     o->restart_drop = 1e-3;

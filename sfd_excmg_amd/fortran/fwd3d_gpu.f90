@@ -13,7 +13,8 @@
 ! global_para.f90:5-19): SFDX_FWD_ACC (0 = single level, the reference
 ! default; 1 = EXCMG), SFDX_CORR_INTV, SFDX_CORR_TOL, SFDX_PRECISION
 ! (fp64|mixed), SFDX_NORM (reference|interior), SFDX_SOLVER (overrides slv:
-! 1 BiCGStab, 3 GPBiCG, 5 COCR), SFDX_PRECOND (0 SSOR, 1 hybrid, 2 multigrid),
+! 1 BiCGStab, 3 GPBiCG, 5 COCR), SFDX_PRECOND (0 SSOR, 1 hybrid, 2 multigrid,
+! 3 = the library default: multigrid or hybrid per frequency and level),
 ! SFDX_COARSE (0 direct solves of EXCMG levels 1-2, 1 device Krylov solves),
 ! SFDX_FULL_BG (1: full-tensor 1-D boundary with cross-polar components),
 ! SFDX_MG_LEVELS and SFDX_MG_PAIRS (multigrid grids and coarsest smoothing

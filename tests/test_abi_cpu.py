@@ -65,6 +65,7 @@ def test_default_options_match_reference_parameters():
     assert o.corr_intv == 200 and o.corr_tol == 1e-3 and o.ssor_omega == 1.0   # global_para.f90:9-10, solvers.f90:14
     assert o.maxit_plain == 5000 and o.maxit_excmg == 1000                    # fdfe_main.f90:993,1414
     assert o.spmv_tile == 0      # fixed SpMV tile by default: repeated runs stay bit-identical
+    assert o.precond == sfdx.PRECOND_AUTO and o.mg_merge_ratio == 3.0   # the V-cycle over merge-ratio grids
 
 
 @pytest.mark.parametrize("aniso", [False, True])

@@ -472,7 +472,8 @@ def test_complex64_krylov_option_reports_nonconvergence_with_air(ctx):
     converge with 1e-8 S/m air: the solve must say so (SFDX_WARN_NOCONV)
     rather than return its fields as converged."""
     m = small_excmg_model(False)
-    kw = dict(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR)
+    kw = dict(level=3, solver=sfdx.SOLVER_COCR, eps=1e-10, corr_intv=0, norm=sfdx.NORM_INTERIOR,
+              precond=sfdx.PRECOND_HYBRID)
     ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, m.freqs,
                            sfdx.default_opts(precision=sfdx.PREC_MIXED_IR, **kw))
     assert st.status == -1      # SFDX_WARN_NOCONV
