@@ -437,7 +437,11 @@ MGTables mg_tables(sfdx_ctx *ctx, const HostLevel &F, const HostLevel &C, const 
         for (size_t c = 0; c < nc; c++)
             for (int q = st[a][c]; q < st[a][c + 1]; q++) {
                 lo[a][q] = (int)c;
-                if (q != st[a][c]) wl[a][q] = (xc[c + 1] - xf[q]) / (xc[c + 1] - xc[c]);
+                if (q != st[a][c]) {
+                    wl[a][q] = (xc[c + 1] - xf[q]) / (xc[c + 1] - xc[c]);
+                    // the kernels tell a node inside a coarse cell by its weight below 1
+                    if (!(wl[a][q] > 0 && wl[a][q] < 1)) throw std::runtime_error("degenerate multigrid cell widths");
+                }
             }
         const std::string t = tag + "_" + std::to_string(a);
         T.wl[a] = upload_vec(ctx, t + "w", wl[a]);

@@ -273,9 +273,9 @@ void launch_resid_split(const DevGrid<R> &g, R wmu, const cx<R> *z, const cx<R> 
 // Transfers between a grid and a coarser one whose cells each merge one or two
 // consecutive fine cells along every axis a: st[a][c] is the fine node index
 // of coarse node c (nc + 1 entries), lo[a][q] the coarse node at or below fine
-// node q and wl[a][q] the linear weight of that coarse node (1 where fine node
-// q is a coarse node), nf + 1 entries each.  Standard 2:1 coarsening is
-// st[c] = 2c.
+// node q and wl[a][q] the linear weight of that coarse node (exactly 1 where
+// fine node q is a coarse node, below 1 elsewhere), nf + 1 entries each.
+// Standard 2:1 coarsening is st[c] = 2c.
 struct MGTables {
     const double *wl[3];
     const int *lo[3];

@@ -1946,8 +1946,10 @@ void launch_cocr_split_upd(const DevGrid<double> &g, RedBuf &rb, zd *x, zd *r, c
 // exact transpose.
 
 // fine node q of axis a lies strictly inside a coarse cell (two bracketing
-// coarse nodes) rather than on a coarse node
-__device__ __forceinline__ bool node_between(const MGTables &T, int a, int q) { return T.st[a][T.lo[a][q]] != q; }
+// coarse nodes) rather than on a coarse node: its weight is below 1 (a fine
+// node on a coarse node has weight exactly 1, one inside lies a whole fine
+// width past the lower coarse node)
+__device__ __forceinline__ bool node_between(const MGTables &T, int a, int q) { return T.wl[a][q] != 1.0; }
 
 // Layout of one edge orientation seen along the edge: edge (a along o; q1, q2
 // across, in the dimensions t1 < t2 other than o) has index
