@@ -9,8 +9,8 @@ A step is one full forward solve (all levels, both polarisations).  `value`
 is the fine-grid solve time (both polarisations) of the faster converged GPU
 path, averaged over the timed steps.  Both paths run COCR preconditioned by a
 geometric multigrid V-cycle over merge-ratio grids (each coarser grid merges
-two cells of an axis only where both are at most 3x its smallest width; up to
-7 grids per level; hybrid Gauss-Seidel smoothing: lumped-mass multicolour edge
+two cells of an axis only where both are at most 3x its smallest width; down
+to at most 2048 cells, 7 grids here; hybrid Gauss-Seidel smoothing: lumped-mass multicolour edge
 SSOR sweeps plus the nodal correction by a two-step Jacobi polynomial,
 residuals in split edge + nodal form):
   * FP64: everything in complex128;
@@ -81,13 +81,14 @@ POL_STREAMS = 2
 # Gauss-Seidel smoothing (lumped-mass edge SSOR sweeps plus the
 # Jacobi-polynomial nodal correction) over merge-ratio grids (two cells of an
 # axis merged only where both are at most 3x the grid's smallest width) down to
-# 7 grids, one smoothing pair on the coarsest.  Measured at this workload, fine
+# a grid of at most 2048 cells (7 grids here, the coarsest 11x11x11), one
+# smoothing pair on the coarsest.  Measured at this workload, fine
 # level, mixed: 0.15-0.16 s (about 65 iterations per polarisation), against
 # 0.61 s (about 390) with 3 grids of 2:1 coarsening and 1.23 s (about 1240)
 # with the one-level hybrid preconditioner.  Over merge ratios 2.5-4, 6-8
 # grids and 1-2 coarsest pairs the fine levels at 0.1 + 1 + 10 Hz summed to
 # 0.65-0.80 s; ratio 3 with 7 grids and one pair was the fastest (0.654 s).
-MG_LEVELS, MG_COARSE_PAIRS, MG_MERGE_RATIO = 7, 1, 3.0
+MG_LEVELS, MG_COARSE_PAIRS, MG_MERGE_RATIO = 0, 1, 3.0
 
 
 def solve_opts(m, precision, eps, verbose=0, precond=None):
@@ -463,9 +464,9 @@ def main():
         "config": {"workload": ("mt3d_fwd_128x128x96_aniso_0.1Hz_2pol_excmg4_cocr" if not args.small
                                 else "mt3d_fwd_64x64x48_aniso_small"), "cells": [m.nx, m.ny, m.nz], "edges": nl,
                    "levels": m.level, "residual": "interior rows, relative 1e-8",
-                   "preconditioner": f"multigrid V-cycle over merge-ratio grids (ratio {MG_MERGE_RATIO:g}, up to "
-                                     f"{MG_LEVELS or 8} grids), {MG_COARSE_PAIRS} coarsest smoothing pair(s), hybrid "
-                                     "Gauss-Seidel smoothing"},
+                   "preconditioner": f"multigrid V-cycle over merge-ratio grids (ratio {MG_MERGE_RATIO:g}, "
+                                     f"{'down to <= 2048 cells' if not MG_LEVELS else f'up to {MG_LEVELS} grids'}), "
+                                     f"{MG_COARSE_PAIRS} coarsest smoothing pair(s), hybrid Gauss-Seidel smoothing"},
         "fp64_gpu": {"fine_s": round(f64["fine"], 4), "wall_s": round(f64["wall"], 3), "fine_iters": f64["fine_iters"],
                      "status": f64["status"]},
         "mixed_gpu": {"fine_s": round(mx["fine"], 4), "wall_s": round(mx["wall"], 3), "fine_iters": mx["fine_iters"],

@@ -166,9 +166,10 @@ typedef struct {
                               both are at most r times the grid's smallest width (never across
                               the air-earth interface), so the long cells of stretched padding
                               stay unmerged until the short ones have grown to their size;
-                              mg_levels then caps the number of grids (0: until no axis merges,
-                              an axis would keep fewer than 2 cells, or 8 grids).  0: the EXCMG
-                              levels and 2:1 coarsenings of the fine grid */
+                              mg_levels then caps the number of grids (0: down to a grid of at
+                              most 2048 cells, up to 12 grids); a larger coarsest grid gets a
+                              smoothing pair per 2048 cells (at least mg_coarse_pairs).  0: the
+                              EXCMG levels and 2:1 coarsenings of the fine grid */
 } sfdx_opts;
 
 typedef struct {
