@@ -40,7 +40,61 @@ static M4 mul4(const M4 &A, const M4 &B)
     return C;
 }
 
+// The reference's propagator (below) multiplies layer matrices whose cosh and
+// sinh grow like exp(k h); the fields it returns are ratios of those products.
+// A column whose summed decay sqrt(w mu a / 2) h (a: the smaller horizontal
+// conductivity of each layer, after the sigma_z reduction) passes ~700
+// overflows and returns NaN, which models with tens of km of deep layers do
+// at a few hundred Hz.  Below a summed decay of kColumnDecay the fields are
+// zero to double precision relative to the top (e^-40 ~ 4e-18), so the column
+// is cut there: the layer that crosses it is thinned to reach it exactly and
+// continued as the half-space below, the nodes under the cut get zero fields
+// and imp refers to the cut bottom.  Columns that do not reach it (every
+// column of the benchmark workloads) are solved unchanged.
+static const double kColumnDecay = 40.0;
+
+static void mt1d_column(int nz, const double *conv, const double *c, double freq, zc *ex, zc *hx, zc *imp,
+                        zc *exc);
+
 void mt1d_anl(int nz, const double *conv, const double *c, double freq, zc *ex, zc *hx, zc *imp, zc *exc)
+{
+    const double w = 2 * kPi * freq;
+    auto C = [&](int layer, int comp) { return conv[(size_t)comp * nz + layer]; };
+    double decay = 0;
+    int ne = nz;
+    double cut_h = 0;
+    for (int L = 0; L < nz && ne == nz; L++) {
+        const double axx = C(L, 0) - C(L, 4) * C(L, 4) / C(L, 2);
+        const double ayy = C(L, 1) - C(L, 5) * C(L, 5) / C(L, 2);
+        const double axy = C(L, 3) - C(L, 4) * C(L, 5) / C(L, 2);
+        const double amin = (axx + ayy) / 2 - std::sqrt((axx - ayy) * (axx - ayy) / 4 + axy * axy);
+        const double kr = std::sqrt(w * kMu0 * std::max(amin, 0.0) / 2);
+        if (decay + kr * c[L] > kColumnDecay) {
+            ne = L + 1;
+            cut_h = (kColumnDecay - decay) / kr;
+        }
+        decay += kr * c[L];
+    }
+    if (ne == nz) {
+        mt1d_column(nz, conv, c, freq, ex, hx, imp, exc);
+        return;
+    }
+    std::vector<double> cv((size_t)6 * ne), cc(c, c + ne);
+    for (int q = 0; q < 6; q++)
+        for (int L = 0; L < ne; L++) cv[(size_t)q * ne + L] = C(L, q);
+    cc[ne - 1] = cut_h;
+    std::vector<zc> e(2 * ne + 2), h(2 * ne + 2), x(2 * ne + 2);
+    mt1d_column(ne, cv.data(), cc.data(), freq, e.data(), h.data(), imp, x.data());
+    for (int mode = 0; mode < 2; mode++)
+        for (int n = 0; n <= nz; n++) {
+            const size_t o = (size_t)mode * (nz + 1) + n, t = (size_t)mode * (ne + 1) + n;
+            ex[o] = n <= ne ? e[t] : zc(0);
+            if (hx) hx[o] = n <= ne ? h[t] : zc(0);
+            if (exc) exc[o] = n <= ne ? x[t] : zc(0);
+        }
+}
+
+static void mt1d_column(int nz, const double *conv, const double *c, double freq, zc *ex, zc *hx, zc *imp, zc *exc)
 {
     const double w = 2 * kPi * freq;
     const zc iwm(0.0, w * kMu0);
