@@ -2237,16 +2237,21 @@ int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_
         SFDX_CHECK(hipSetDevice(ctx->device));
         struct Item {
             int j, f;
-            double cost;
+            double cost, freq;
         };
         std::vector<Item> items;
         for (int j = 0; j < njobs; j++) {
             const sfdx_grid &g = jobs[j].grid;
             const double nl = (double)make_dims(g.nx, g.ny, g.nz).NL;
-            for (int f = 0; f < jobs[j].nf; f++) items.push_back({j, f, nl});
+            for (int f = 0; f < jobs[j].nf; f++) items.push_back({j, f, nl, jobs[j].freq[f]});
         }
-        // longest processing time first
-        std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
+        // longest processing time first: larger grids first, and on grids of one
+        // size higher frequencies first (the skin depth shrinks towards the cell
+        // widths and the iteration counts grow: on a 128x128x96 model 65
+        // iterations at 0.1 Hz, about 180 at 100 Hz and over 1000 at 1 kHz)
+        std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
+            return x.cost != y.cost ? x.cost > y.cost : x.freq > y.freq;
+        });
         const int ns = std::max(1, std::min({nstreams > 0 ? nstreams : 8, 16, (int)items.size()}));
         while ((int)ctx->workers.size() < ns) {
             sfdx_ctx *w = nullptr;
