@@ -369,7 +369,7 @@ def main():
     ap.add_argument("--kernels-only", action="store_true")
     ap.add_argument("--batch", "--c4", dest="c4", action="store_true", help="batched workload: 16 frequencies on the 128x128x96 model "
                     "plus 8 isotropic models of uneven sizes, batched vs one stream")
-    ap.add_argument("--nstreams", type=int, default=8)
+    ap.add_argument("--nstreams", type=int, default=6)
     ap.add_argument("--adjoint", "--c5", dest="c5", action="store_true", help="adjoint gradient on the 96x96x64 random-field model with a "
                     "central-difference check on --fd-cells random cells")
     ap.add_argument("--fd-cells", type=int, default=20)

@@ -269,7 +269,7 @@ typedef struct {
 
 /* Batched forward solves: every (model, frequency) pair of `jobs` (both
  * polarisations each) is an independent work item; items run concurrently on
- * `nstreams` HIP streams (0 = 8, at most 16), longest first (larger grids, then
+ * `nstreams` HIP streams (0 = 6, at most 16), longest first (larger grids, then
  * higher frequencies), each stream with
  * its own workspace, so small uneven models fill the GPU together.  Results are
  * identical to calling sfdx_fwd_solve per model.  st (optional) receives one

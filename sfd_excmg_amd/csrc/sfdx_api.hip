@@ -2252,7 +2252,10 @@ int sfdx_solve_batch(sfdx_ctx *ctx, int njobs, const sfdx_job *jobs, const sfdx_
         std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
             return x.cost != y.cost ? x.cost > y.cost : x.freq > y.freq;
         });
-        const int ns = std::max(1, std::min({nstreams > 0 ? nstreams : 8, 16, (int)items.size()}));
+        // default 6: on the C4 batch (16 frequencies of a 128x128x96 model and 8
+        // models of 48^3 to 96^3) 6 streams took 8.67-8.78 s, 5 or 7 about 9.0 s,
+        // 8 9.0-9.2 s, 4 10.2 s (the device has 4 hardware queues per process)
+        const int ns = std::max(1, std::min({nstreams > 0 ? nstreams : 6, 16, (int)items.size()}));
         while ((int)ctx->workers.size() < ns) {
             sfdx_ctx *w = nullptr;
             const int rc = sfdx_create(ctx->device, &w);
