@@ -156,7 +156,8 @@ typedef struct {
                              is the top node plane of its first cell, from the top, whose sigma_xx
                              exceeds this value in S/m (as sfdx_surface_planes; the reference's
                              topography branch, fdfe_main.f90:604-614, 656-666) */
-    int pol_streams;      /* Krylov levels: 2 (default) solve the two polarisations at the same time,
+    int pol_streams;      /* Krylov levels and the adjoint solves of sfdx_adjoint: 2 (default) solve
+                             the two polarisations at the same time,
                              the second on a helper HIP stream with its own work vectors (the
                              level data is shared and only read; results are bit-identical to
                              the sequential solves); 0 or 1: one after the other on the context's

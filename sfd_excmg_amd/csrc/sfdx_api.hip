@@ -2018,7 +2018,7 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     double *gdev = pool_vec<double>(ctx, "adj/g", d.NC);
     SFDX_CHECK(hipMemsetAsync(gdev, 0, d.NC * sizeof(double), ctx->st));
     zd *ed = pool_vec<zd>(ctx, "adj/e", 2 * d.NL), *wd = pool_vec<zd>(ctx, "adj/w", 2 * d.NL);
-    zd *lam = pool_vec<zd>(ctx, "adj/lam", d.NL);
+    zd *lam[2] = {pool_vec<zd>(ctx, "adj/lam", d.NL), pool_vec<zd>(ctx, "adj/lam1", d.NL)};
     std::vector<zc> exf((size_t)2 * d.NL), wf((size_t)2 * d.NL);
     // Adjoint solves: no divergence correction (the adjoint source is not
     // divergence-free).  With the multigrid options and merge-ratio grids they
@@ -2069,31 +2069,75 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
         SFDX_CHECK(hipMemcpyAsync(ed, exf.data(), (size_t)2 * d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
         SFDX_CHECK(hipMemcpyAsync(wd, wf.data(), (size_t)2 * d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
         const double wmu = 2 * 3.141592653589793 * freq[f] * kMu0;
+        double bnorm[2];
         for (int pol = 0; pol < 2; pol++) {
             zd *w = wd + (size_t)pol * d.NL;
             launch_zero_boundary(d, w, ctx->st);  // the Dirichlet values are data, not unknowns
             launch_vec<double>(5, D64.g, ctx->rb, w, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
             reduce_final(ctx->rb, L.nblocks(), 1, SL_TMP, OP_RN, ctx->st);
             read_slots(ctx);
-            const double bnorm = std::sqrt(ctx->hslots[SL_RN]);
-            if (!(bnorm > 0)) continue;
-            SFDX_CHECK(hipMemsetAsync(lam, 0, d.NL * sizeof(zd), ctx->st));
-            KOpts kf = ko;
-            if (adj_mg && (!adj_auto || rule.use_mg(freq[f]))) {
-                kf.precond = SFDX_PRECOND_MG;
-                kf.mg64 = &MA.H64;
-                kf.mg32 = adj_mixed ? &MA.H32 : nullptr;
-                kf.mg_level = MA.L;
+            bnorm[pol] = std::sqrt(ctx->hslots[SL_RN]);
+            SFDX_CHECK(hipMemsetAsync(lam[pol], 0, d.NL * sizeof(zd), ctx->st));
+        }
+        KOpts kf = ko;
+        if (adj_mg && (!adj_auto || rule.use_mg(freq[f]))) {
+            kf.precond = SFDX_PRECOND_MG;
+            kf.mg64 = &MA.H64;
+            kf.mg32 = adj_mixed ? &MA.H32 : nullptr;
+            kf.mg_level = MA.L;
+        }
+        const bool two = o->pol_streams == 2 && bnorm[0] > 0 && bnorm[1] > 0;
+        if (two) prebuild_level(ctx, D64, adj_mixed ? &D32 : nullptr, kf.mg64, kf.mg32, "ADJ", adj_mixed);
+        SFDX_CHECK(hipStreamSynchronize(ctx->st));
+        auto solve_pol = [&](sfdx_ctx *c, int pol) {
+            SolveInfo r;
+            if (!(bnorm[pol] > 0)) return r;
+            r = krylov<double>(c, o->solver, D64, "ADJ", wmu, wd + (size_t)pol * d.NL, lam[pol], bnorm[pol], o->eps,
+                               o->maxit_plain, kf);
+            SFDX_CHECK(hipStreamSynchronize(c->st));
+            return r;
+        };
+        SolveInfo si[2];
+        if (two) {
+            // the two polarisations' adjoint solves at once, the second on the
+            // helper context's stream in a host thread of its own (both only
+            // read the level data, built above)
+            sfdx_ctx *hw = pol_helper(ctx);
+            std::exception_ptr err;
+            std::string err_text;
+            std::thread th([&]() {
+                try {
+                    SFDX_CHECK(hipSetDevice(ctx->device));
+                    si[1] = solve_pol(hw, 1);
+                } catch (...) {
+                    err = std::current_exception();
+                    err_text = g_err;
+                }
+            });
+            try {
+                si[0] = solve_pol(ctx, 0);
+            } catch (...) {
+                th.join();
+                throw;
             }
-            SolveInfo si = krylov<double>(ctx, o->solver, D64, "ADJ", wmu, w, lam, bnorm, o->eps, o->maxit_plain, kf);
-            S.fallback_fp64 += si.fallback;
-            S.solver_switches += si.switched;
-            if (si.status > 0) {
+            th.join();
+            if (err) {
+                if (!err_text.empty()) g_err = err_text;
+                std::rethrow_exception(err);
+            }
+        } else {
+            for (int pol = 0; pol < 2; pol++) si[pol] = solve_pol(ctx, pol);
+        }
+        for (int pol = 0; pol < 2; pol++) {
+            if (!(bnorm[pol] > 0)) continue;
+            S.fallback_fp64 += si[pol].fallback;
+            S.solver_switches += si[pol].switched;
+            if (si[pol].status > 0) {
                 g_err = "adjoint solve failed (frequency " + std::to_string(f) + ")";
-                return si.status;
+                return si[pol].status;
             }
-            if (si.status < 0) worst = si.status;
-            launch_cell_grad(D64.g, wmu, ed + (size_t)pol * d.NL, lam, gdev, ctx->st);
+            if (si[pol].status < 0) worst = si[pol].status;
+            launch_cell_grad(D64.g, wmu, ed + (size_t)pol * d.NL, lam[pol], gdev, ctx->st);
         }
     }
     SFDX_CHECK(hipMemcpyAsync(grad, gdev, d.NC * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
