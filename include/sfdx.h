@@ -156,8 +156,8 @@ typedef struct {
                              is the top node plane of its first cell, from the top, whose sigma_xx
                              exceeds this value in S/m (as sfdx_surface_planes; the reference's
                              topography branch, fdfe_main.f90:604-614, 656-666) */
-    int pol_streams;      /* Krylov levels and the adjoint solves of sfdx_adjoint: 2 (default) solve
-                             the two polarisations at the same time,
+    int pol_streams;      /* Krylov levels (and the adjoint solves of a one-frequency sfdx_adjoint):
+                             2 (default) solve the two polarisations at the same time,
                              the second on a helper HIP stream with its own work vectors (the
                              level data is shared and only read; results are bit-identical to
                              the sequential solves); 0 or 1: one after the other on the context's
@@ -307,8 +307,11 @@ int sfdx_misfit(const sfdx_grid *g, int nf, const double *freq, int nrec, const 
  * (sfdx_misfit), adjoint solves A_s^T lam = w with the forward kernels (A_s is
  * complex symmetric) and homogeneous Dirichlet data, then
  * grad_c += Re(i omega mu lam^T M_c e) summed over both polarisations, M_c the
- * cell's corner-lumped edge mass.  Only one frequency's fields live on the
- * device at a time.  The Dirichlet data of the 1-D background are held fixed
+ * cell's corner-lumped edge mass.  The frequencies run as independent items
+ * on up to 6 HIP streams at once (higher frequencies first; each keeps its
+ * fields and adjoint solutions on the device, 4*nl complex128 per frequency),
+ * and the gradient is summed afterwards in frequency and polarisation order,
+ * so the result does not depend on the schedule.  The Dirichlet data of the 1-D background are held fixed
  * (their dependence on the corner column, fdfe_main.f90:938-942, is not
  * differentiated).  grad: ncell doubles; ex (optional): the forward fields
  * 2*nl*nf; st (optional): forward-solve statistics.  No counterpart in the

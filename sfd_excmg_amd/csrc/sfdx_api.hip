@@ -2017,9 +2017,6 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     D64.g.ssor_xmass = D32.g.ssor_xmass = xmass;
     double *gdev = pool_vec<double>(ctx, "adj/g", d.NC);
     SFDX_CHECK(hipMemsetAsync(gdev, 0, d.NC * sizeof(double), ctx->st));
-    zd *ed = pool_vec<zd>(ctx, "adj/e", 2 * d.NL), *wd = pool_vec<zd>(ctx, "adj/w", 2 * d.NL);
-    zd *lam[2] = {pool_vec<zd>(ctx, "adj/lam", d.NL), pool_vec<zd>(ctx, "adj/lam1", d.NL)};
-    std::vector<zc> exf((size_t)2 * d.NL), wf((size_t)2 * d.NL);
     // Adjoint solves: no divergence correction (the adjoint source is not
     // divergence-free).  With the multigrid options and merge-ratio grids they
     // run the V-cycle over grids of their own (SFDX_PRECOND_AUTO: at the
@@ -2045,39 +2042,73 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
     int worst = SFDX_OK;
     double phi = 0;
     SliceLaunch L = edge_launch(d);
-    for (int f = 0; f < nf; f++) {
-        sfdx_stats sf;
-        const int rc = fwd_solve_impl(ctx, g, sigma, 1, freq + f, o, (double *)exf.data(), &sf);
-        if (rc > 0) return rc;
-        if (rc < 0) worst = rc;
-        S.total_seconds += sf.total_seconds;
-        S.fine_seconds += sf.fine_seconds;
-        S.fine_iters_total += sf.fine_iters_total;
-        S.fallback_fp64 += sf.fallback_fp64;
-        S.solver_switches += sf.solver_switches;
-        S.mg_freqs += sf.mg_freqs;
-        S.spmv_tile = sf.spmv_tile;
-        if (ex_out) std::memcpy(ex_out + (size_t)f * 2 * d.NL * 2, exf.data(), (size_t)2 * d.NL * sizeof(zc));
-        double phf = 0;
-        const int bad = misfit_fields(pg, 1, freq + f, nrec, recv, rd, obs + (size_t)f * nrec * 4, exf.data(), phf,
-                                      wf.data());
-        if (bad) {
-            g_err = "station " + std::to_string(bad) + " lies outside the interpolation area";
-            return SFDX_ERR_ARG;
+    // Every frequency is an independent item (forward solve, misfit and its
+    // field derivative, the two adjoint solves), run on up to kAdjWorkers
+    // worker contexts at once, higher frequencies first (they take more
+    // iterations); a single frequency runs on the context itself.  Each item
+    // keeps its fields and adjoint solutions, and the gradient is summed
+    // afterwards on the context's stream in frequency and polarisation order:
+    // the same sums as solving one frequency after the other.
+    constexpr int kAdjWorkers = 6;
+    const int nw = std::min(nf, kAdjWorkers);
+    std::vector<sfdx_ctx *> wk(nw, ctx);
+    if (nw > 1) {
+        while ((int)ctx->workers.size() < nw) {
+            sfdx_ctx *w = nullptr;
+            const int rc = sfdx_create(ctx->device, &w);
+            if (rc != SFDX_OK) return rc;
+            ctx->workers.push_back(w);
         }
-        phi += phf;
-        SFDX_CHECK(hipMemcpyAsync(ed, exf.data(), (size_t)2 * d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
-        SFDX_CHECK(hipMemcpyAsync(wd, wf.data(), (size_t)2 * d.NL * sizeof(zd), hipMemcpyHostToDevice, ctx->st));
+        for (int w = 0; w < nw; w++) {
+            wk[w] = ctx->workers[w];
+            ensure_partials(wk[w], L.nblocks() + 64);
+        }
+    }
+    std::vector<zd *> edf(nf), lamf(nf), wdw(nw);
+    for (int f = 0; f < nf; f++) {
+        edf[f] = pool_vec<zd>(ctx, "adj/e" + std::to_string(f), 2 * d.NL);
+        lamf[f] = pool_vec<zd>(ctx, "adj/l" + std::to_string(f), 2 * d.NL);
+    }
+    for (int w = 0; w < nw; w++) wdw[w] = pool_vec<zd>(ctx, "adj/w" + std::to_string(w), 2 * d.NL);
+    // the level data that the solves build on first use, built once here
+    prebuild_level(ctx, D64, adj_mixed ? &D32 : nullptr, adj_mg ? &MA.H64 : nullptr,
+                   adj_mg && adj_mixed ? &MA.H32 : nullptr, "ADJ", adj_mixed);
+    struct ItemOut {
+        int rc = 0, bad = 0;
+        double phf = 0, bnorm[2] = {0, 0};
+        sfdx_stats sf;
+        SolveInfo si[2];
+        std::string err;
+        std::exception_ptr ex;
+    };
+    std::vector<ItemOut> out(nf);
+    auto run_item = [&](sfdx_ctx *c, zd *wd, int f) {
+        ItemOut &it = out[f];
+        std::memset(&it.sf, 0, sizeof(it.sf));
+        std::vector<zc> exf((size_t)2 * d.NL), wf((size_t)2 * d.NL);
+        sfdx_opts oj = *o;
+        if (nw > 1) oj.pol_streams = 1;  // the items already run concurrently
+        it.rc = fwd_solve_impl(c, g, sigma, 1, freq + f, &oj, (double *)exf.data(), &it.sf);
+        if (it.rc > 0) {
+            it.err = g_err;
+            return;
+        }
+        if (ex_out) std::memcpy(ex_out + (size_t)f * 2 * d.NL * 2, exf.data(), (size_t)2 * d.NL * sizeof(zc));
+        it.bad = misfit_fields(pg, 1, freq + f, nrec, recv, rd, obs + (size_t)f * nrec * 4, exf.data(), it.phf,
+                               wf.data());
+        if (it.bad) return;
+        zd *ed = edf[f], *lam = lamf[f];
+        SFDX_CHECK(hipMemcpyAsync(ed, exf.data(), (size_t)2 * d.NL * sizeof(zd), hipMemcpyHostToDevice, c->st));
+        SFDX_CHECK(hipMemcpyAsync(wd, wf.data(), (size_t)2 * d.NL * sizeof(zd), hipMemcpyHostToDevice, c->st));
         const double wmu = 2 * 3.141592653589793 * freq[f] * kMu0;
-        double bnorm[2];
         for (int pol = 0; pol < 2; pol++) {
             zd *w = wd + (size_t)pol * d.NL;
-            launch_zero_boundary(d, w, ctx->st);  // the Dirichlet values are data, not unknowns
-            launch_vec<double>(5, D64.g, ctx->rb, w, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->st);
-            reduce_final(ctx->rb, L.nblocks(), 1, SL_TMP, OP_RN, ctx->st);
-            read_slots(ctx);
-            bnorm[pol] = std::sqrt(ctx->hslots[SL_RN]);
-            SFDX_CHECK(hipMemsetAsync(lam[pol], 0, d.NL * sizeof(zd), ctx->st));
+            launch_zero_boundary(d, w, c->st);  // the Dirichlet values are data, not unknowns
+            launch_vec<double>(5, D64.g, c->rb, w, nullptr, nullptr, nullptr, nullptr, nullptr, c->st);
+            reduce_final(c->rb, L.nblocks(), 1, SL_TMP, OP_RN, c->st);
+            read_slots(c);
+            it.bnorm[pol] = std::sqrt(c->hslots[SL_RN]);
+            SFDX_CHECK(hipMemsetAsync(lam + (size_t)pol * d.NL, 0, d.NL * sizeof(zd), c->st));
         }
         KOpts kf = ko;
         if (adj_mg && (!adj_auto || rule.use_mg(freq[f]))) {
@@ -2086,36 +2117,33 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
             kf.mg32 = adj_mixed ? &MA.H32 : nullptr;
             kf.mg_level = MA.L;
         }
-        const bool two = o->pol_streams == 2 && bnorm[0] > 0 && bnorm[1] > 0;
-        if (two) prebuild_level(ctx, D64, adj_mixed ? &D32 : nullptr, kf.mg64, kf.mg32, "ADJ", adj_mixed);
-        SFDX_CHECK(hipStreamSynchronize(ctx->st));
-        auto solve_pol = [&](sfdx_ctx *c, int pol) {
+        SFDX_CHECK(hipStreamSynchronize(c->st));  // the host copies above end here
+        auto solve_pol = [&](sfdx_ctx *cc, int pol) {
             SolveInfo r;
-            if (!(bnorm[pol] > 0)) return r;
-            r = krylov<double>(c, o->solver, D64, "ADJ", wmu, wd + (size_t)pol * d.NL, lam[pol], bnorm[pol], o->eps,
-                               o->maxit_plain, kf);
-            SFDX_CHECK(hipStreamSynchronize(c->st));
+            if (!(it.bnorm[pol] > 0)) return r;
+            r = krylov<double>(cc, o->solver, D64, "ADJ", wmu, wd + (size_t)pol * d.NL, lam + (size_t)pol * d.NL,
+                               it.bnorm[pol], o->eps, o->maxit_plain, kf);
+            SFDX_CHECK(hipStreamSynchronize(cc->st));
             return r;
         };
-        SolveInfo si[2];
+        const bool two = nw == 1 && o->pol_streams == 2 && it.bnorm[0] > 0 && it.bnorm[1] > 0;
         if (two) {
             // the two polarisations' adjoint solves at once, the second on the
-            // helper context's stream in a host thread of its own (both only
-            // read the level data, built above)
-            sfdx_ctx *hw = pol_helper(ctx);
+            // helper context's stream in a host thread of its own
+            sfdx_ctx *hw = pol_helper(c);
             std::exception_ptr err;
             std::string err_text;
             std::thread th([&]() {
                 try {
                     SFDX_CHECK(hipSetDevice(ctx->device));
-                    si[1] = solve_pol(hw, 1);
+                    it.si[1] = solve_pol(hw, 1);
                 } catch (...) {
                     err = std::current_exception();
                     err_text = g_err;
                 }
             });
             try {
-                si[0] = solve_pol(ctx, 0);
+                it.si[0] = solve_pol(c, 0);
             } catch (...) {
                 th.join();
                 throw;
@@ -2126,18 +2154,67 @@ int adjoint_impl(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf,
                 std::rethrow_exception(err);
             }
         } else {
-            for (int pol = 0; pol < 2; pol++) si[pol] = solve_pol(ctx, pol);
+            for (int pol = 0; pol < 2; pol++) it.si[pol] = solve_pol(c, pol);
         }
-        for (int pol = 0; pol < 2; pol++) {
-            if (!(bnorm[pol] > 0)) continue;
-            S.fallback_fp64 += si[pol].fallback;
-            S.solver_switches += si[pol].switched;
-            if (si[pol].status > 0) {
-                g_err = "adjoint solve failed (frequency " + std::to_string(f) + ")";
-                return si[pol].status;
+    };
+    if (nw == 1) {
+        for (int f = 0; f < nf; f++) run_item(ctx, wdw[0], f);
+    } else {
+        std::vector<int> order(nf);
+        for (int f = 0; f < nf; f++) order[f] = f;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return freq[x] > freq[y]; });
+        std::atomic<int> next{0};
+        auto worker = [&](int w) {
+            for (int k = next++; k < nf; k = next++) {
+                ItemOut &it = out[order[k]];
+                try {
+                    SFDX_CHECK(hipSetDevice(ctx->device));
+                    run_item(wk[w], wdw[w], order[k]);
+                } catch (...) {
+                    it.ex = std::current_exception();
+                    it.err = g_err;
+                }
             }
-            if (si[pol].status < 0) worst = si[pol].status;
-            launch_cell_grad(D64.g, wmu, ed + (size_t)pol * d.NL, lam[pol], gdev, ctx->st);
+        };
+        std::vector<std::thread> th;
+        for (int w = 0; w < nw; w++) th.emplace_back(worker, w);
+        for (auto &t : th) t.join();
+    }
+    for (int f = 0; f < nf; f++) {
+        ItemOut &it = out[f];
+        if (it.ex) {
+            if (!it.err.empty()) g_err = it.err;
+            std::rethrow_exception(it.ex);
+        }
+        if (it.rc > 0) {
+            g_err = it.err;
+            return it.rc;
+        }
+        if (it.rc < 0) worst = it.rc;
+        const sfdx_stats &sf = it.sf;
+        S.total_seconds += sf.total_seconds;
+        S.fine_seconds += sf.fine_seconds;
+        S.fine_iters_total += sf.fine_iters_total;
+        S.fallback_fp64 += sf.fallback_fp64;
+        S.solver_switches += sf.solver_switches;
+        S.mg_freqs += sf.mg_freqs;
+        S.spmv_tile = sf.spmv_tile;
+        if (it.bad) {
+            g_err = "station " + std::to_string(it.bad) + " lies outside the interpolation area";
+            return SFDX_ERR_ARG;
+        }
+        phi += it.phf;
+        const double wmu = 2 * 3.141592653589793 * freq[f] * kMu0;
+        for (int pol = 0; pol < 2; pol++) {
+            if (!(it.bnorm[pol] > 0)) continue;
+            S.fallback_fp64 += it.si[pol].fallback;
+            S.solver_switches += it.si[pol].switched;
+            if (it.si[pol].status > 0) {
+                g_err = "adjoint solve failed (frequency " + std::to_string(f) + ")";
+                return it.si[pol].status;
+            }
+            if (it.si[pol].status < 0) worst = it.si[pol].status;
+            launch_cell_grad(D64.g, wmu, edf[f] + (size_t)pol * d.NL, lamf[f] + (size_t)pol * d.NL, gdev, ctx->st);
         }
     }
     SFDX_CHECK(hipMemcpyAsync(grad, gdev, d.NC * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
