@@ -709,6 +709,36 @@ def test_adjoint_gradient_matches_oracle_sensitivity_and_fd(ctx, precond):
         assert abs(grad[cell] - fd) <= 1e-3 * scale
 
 
+def test_adjoint_frequencies_as_concurrent_items_sum_single_frequency_calls(ctx):
+    """sfdx_adjoint runs its frequencies as concurrent items on worker streams
+    (higher frequencies first) and sums the gradient afterwards in frequency
+    order: misfit and gradient over four unsorted frequencies equal the sums of
+    one-frequency calls (which run on the context itself), to rounding."""
+    m = small_excmg_model(True)
+    m.freqs = np.array([1.0, 10.0, 3.0, 0.3])
+    res, _ = O.post_process(m.a, m.b, m.c, m.nair, m.freqs, m.recv,
+                            O.fwd_solver(m.a, m.b, m.c, m.nair, 3, 1e-11, m.sigma, m.freqs,
+                                         O.Options(solver=O.SOLVER_COCR, corr_intv=50, norm="interior"))[0],
+                            out_type=2)
+    obs = res * np.array([1.1, 0.95, 1.0, 1.0]) + np.array([0, 0, 1.0, -2.0])
+    opts = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-12, corr_intv=0, norm=sfdx.NORM_INTERIOR,
+                             maxit_plain=5000, maxit_excmg=5000)
+    phi, grad, st = ctx.adjoint(m.a, m.b, m.c, m.nair, m.sigma, m.freqs, m.recv, obs, opts)
+    assert st.status == 0
+    nrec = len(m.recv)
+    phis, grads = [], []
+    for f, fr in enumerate(m.freqs):
+        p1, g1, s1 = ctx.adjoint(m.a, m.b, m.c, m.nair, m.sigma, np.array([fr]), m.recv,
+                                 obs[f * nrec:(f + 1) * nrec], opts)
+        assert s1.status == 0
+        phis.append(p1)
+        grads.append(g1)
+    assert abs(phi - sum(phis)) <= 1e-12 * abs(phi)
+    gsum = np.sum(grads, axis=0)
+    assert np.max(np.abs(grad - gsum)) <= 1e-12 * np.max(np.abs(gsum))
+    assert np.max(np.abs(grad)) > 0
+
+
 def test_autograd_function_returns_adjoint_gradient(ctx):
     import torch
     from sfd_excmg_amd.autograd import mt_misfit
