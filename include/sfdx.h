@@ -203,7 +203,11 @@ const char *sfdx_last_error(void);
 void sfdx_default_opts(sfdx_opts *o);
 
 /* Replaces fwd3d::fwd_solver (sfd_excmg/src/fdfe_main.f90:1565-1594): all
- * frequencies, both polarisations; ex caller-allocated with 2*nl*nf entries. */
+ * frequencies, both polarisations; ex caller-allocated with 2*nl*nf entries.
+ * With several frequencies and pol_streams = 2 (the default) the frequencies
+ * run as concurrent items, as sfdx_solve_batch over this one model (identical
+ * fields; st then sums the items' times); pol_streams 0 or 1 solves them one
+ * after the other on the context's stream. */
 int sfdx_fwd_solve(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int nf, const double *freq,
                    const sfdx_opts *o, double *ex, sfdx_stats *st);
 

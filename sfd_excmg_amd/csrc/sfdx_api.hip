@@ -2338,6 +2338,18 @@ int sfdx_fwd_solve(sfdx_ctx *ctx, const sfdx_grid *g, const double *sigma, int n
     if (!ctx || !g || !sigma || !freq || !ex || nf <= 0) return SFDX_ERR_ARG;
     sfdx_opts def;
     if (!o) { sfdx_default_opts(&def); o = &def; }
+    if (nf > 1 && o->pol_streams == 2) {
+        // several frequencies: independent items on concurrent streams, as a
+        // batch of this one model (results identical to the frequency loop)
+        sfdx_job J;
+        std::memset(&J, 0, sizeof(J));
+        J.grid = *g;
+        J.sigma = sigma;
+        J.nf = nf;
+        J.freq = freq;
+        J.ex = ex;
+        return sfdx_solve_batch(ctx, 1, &J, o, 0, st);
+    }
     return guarded([&]() {
         SFDX_CHECK(hipSetDevice(ctx->device));
         return fwd_solve_impl(ctx, g, sigma, nf, freq, o, ex, st);
