@@ -640,6 +640,24 @@ def test_graph_replay_matches_eager(ctx, precision):
     assert np.array_equal(ex0, ex1)
 
 
+@pytest.mark.parametrize("precision", [sfdx.PREC_FP64, sfdx.PREC_MIXED])
+def test_multi_frequency_solve_runs_items_with_identical_fields(ctx, precision):
+    """sfdx_fwd_solve over several frequencies runs them as concurrent batch
+    items (pol_streams 2, the default): fields bit-identical to the frequency
+    loop on the context's stream (pol_streams 1)."""
+    m = small_excmg_model(True)
+    freqs = np.array([0.3, 30.0, 3.0])
+    out = {}
+    for ps in (2, 1):
+        o = sfdx.default_opts(level=3, solver=sfdx.SOLVER_COCR, eps=1e-9, corr_intv=0, norm=sfdx.NORM_INTERIOR,
+                              precision=precision, pol_streams=ps)
+        ex, st = ctx.fwd_solve(m.a, m.b, m.c, m.nair, m.sigma, freqs, o)
+        assert st.status == 0
+        out[ps] = ex
+    assert out[2].shape[1] == 3
+    assert np.array_equal(out[1], out[2])
+
+
 @pytest.mark.parametrize("use_excmg", [0, 1])
 def test_batched_solve_matches_single_solves(ctx, use_excmg):
     """sfdx_solve_batch over models of uneven sizes and several frequencies,
